@@ -1,0 +1,295 @@
+"""ctypes bindings for the CPU oracle and for the reference's own compiled subroutines.
+
+TEST INFRASTRUCTURE ONLY.  ``oracle`` restates Picard's PIC time step in C
+(FP64, reference evaluation order) so that tests can check the HIP product
+path against it.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+``bench.py``'s ``cpu_baseline`` leg may import this package.
+
+Array conventions (numpy, C order), matching the Fortran column-major layout
+of the reference:
+
+* ``rp``  real_particles(8, n)  -> ``float64[n, 8]``
+* ``sp``  int_particles(n)      -> ``int32[n]`` (species, 1-based)
+* ``U``, ``C``                  -> ``float64[Nz+2, Ny+2, Nx+2]``
+* ``E``                         -> ``float64[Nz+2, Ny+2, Nx+2, 3]``
+* ``N``                         -> ``float64[Nz+2, Ny+2, Nx+2, S]``
+* ``D``                         -> ``float64[2Nz+1, 2Ny+1, 2Nx+1]``
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpicard_oracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libpicard_ref.so")
+
+_dp = ct.POINTER(ct.c_double)
+_ip = ct.POINTER(ct.c_int32)
+_lp = ct.POINTER(ct.c_int64)
+
+
+def build(quiet: bool = True) -> None:
+    """Compile the oracle (and oracle/_ref where the reference is mounted)."""
+    out = subprocess.run(["make", "-C", HERE, "all"], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if not quiet:
+        print(out.stdout)
+
+
+class _Grid(ct.Structure):
+    _fields_ = [
+        ("nx", ct.c_int32), ("ny", ct.c_int32), ("nz", ct.c_int32),
+        ("species", ct.c_int32),
+        ("flow", ct.c_int32 * 3),
+        ("lmin", ct.c_double * 3), ("lmax", ct.c_double * 3),
+        ("gmin", ct.c_double * 3), ("gmax", ct.c_double * 3),
+        ("dxyz", ct.c_double * 3),
+    ]
+
+
+@dataclass
+class Grid:
+    """One-rank domain: the local box equals the global box."""
+    nx: int
+    ny: int
+    nz: int
+    species: int
+    flow: tuple = (True, False, False)
+    gmin: tuple = (-160.0, -160.0, -160.0)
+    gmax: tuple = (160.0, 160.0, 160.0)
+    dxyz: tuple = field(default=None)
+
+    def __post_init__(self):
+        if self.dxyz is None:
+            # picard.f90:339-360 on one rank: Lx_local = Lx, dxyz = Lx_local/Nx_local
+            self.dxyz = tuple((self.gmax[c] - self.gmin[c]) / float(n)
+                              for c, n in enumerate((self.nx, self.ny, self.nz)))
+
+    @property
+    def shape(self):
+        return (self.nz + 2, self.ny + 2, self.nx + 2)
+
+    def cstruct(self) -> _Grid:
+        g = _Grid()
+        g.nx, g.ny, g.nz, g.species = self.nx, self.ny, self.nz, self.species
+        for c in range(3):
+            g.flow[c] = 1 if self.flow[c] else 0
+            g.lmin[c] = self.gmin[c]
+            g.lmax[c] = self.gmax[c]
+            g.gmin[c] = self.gmin[c]
+            g.gmax[c] = self.gmax[c]
+            g.dxyz[c] = self.dxyz[c]
+        return g
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ct.CDLL(LIB_PATH)
+        G = ct.POINTER(_Grid)
+        L.orc_advance_position.argtypes = [_dp, ct.c_double, ct.c_int64]
+        L.orc_bc_outflow.argtypes = [G, _dp, _ip, _lp, _lp]
+        L.orc_bc_outflow.restype = ct.c_int64
+        L.orc_particles_transfer.argtypes = [G, _dp, ct.c_int64]
+        L.orc_particles_gridify.argtypes = [G, _dp, _dp, _dp, _ip, ct.c_int64, _dp, _dp]
+        L.orc_particles_gridify.restype = ct.c_int64
+        L.orc_calc_distance.argtypes = [_dp, ct.c_int32, ct.c_int32, ct.c_int32, _dp, _ip]
+        L.orc_bc_potentials.argtypes = [G, _dp, _dp, _dp]
+        L.orc_calc_epotential.argtypes = [G, _dp, _dp, ct.c_double, ct.c_int32, _dp]
+        L.orc_calc_epotential.restype = ct.c_int64
+        L.orc_calc_epotential_pairwise.argtypes = [G, _dp, _dp, ct.c_double, ct.c_int32]
+        L.orc_calc_epotential_pairwise.restype = ct.c_int64
+        L.orc_calc_efield.argtypes = [G, _dp, _dp]
+        L.orc_bc_fields.argtypes = [G, _dp]
+        L.orc_advance_velocity.argtypes = [G, _dp, _ip, _dp, _dp, ct.c_double, _dp, _dp, _dp,
+                                           _dp, ct.c_int64, _ip, _ip]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+def _f8(v, n=8):
+    a = np.zeros(n, np.float64)
+    v = np.asarray(v, np.float64).ravel()
+    a[: v.size] = v
+    return a
+
+
+def _i8(v, n=8):
+    a = np.zeros(n, np.int32)
+    v = np.asarray(v).astype(np.int32).ravel()
+    a[: v.size] = v
+    return a
+
+
+# ---------------------------------------------------------------------------
+# oracle entry points (in-place on numpy arrays, like the Fortran subroutines)
+
+def advance_position(rp, dt, n=None):
+    n = rp.shape[0] if n is None else n
+    lib().orc_advance_position(_d(rp), dt, n)
+
+
+def bc_outflow(g: Grid, rp, sp, n=None):
+    """Returns (new n, removed slots in loop order (1-based))."""
+    n = rp.shape[0] if n is None else n
+    nn = ct.c_int64(n)
+    removed = np.zeros(max(n, 1), np.int64)
+    cnt = lib().orc_bc_outflow(ct.byref(g.cstruct()), _d(rp), _i(sp), ct.byref(nn),
+                               removed.ctypes.data_as(_lp))
+    return nn.value, removed[:cnt].copy()
+
+
+def particles_transfer(g: Grid, rp, n=None):
+    n = rp.shape[0] if n is None else n
+    lib().orc_particles_transfer(ct.byref(g.cstruct()), _d(rp), n)
+
+
+def particles_gridify(g: Grid, rp, sp, charge, n2p, n=None):
+    n = rp.shape[0] if n is None else n
+    N = np.zeros(g.shape + (g.species,), np.float64)
+    C = np.zeros(g.shape, np.float64)
+    skipped = lib().orc_particles_gridify(ct.byref(g.cstruct()), _d(N), _d(C), _d(rp), _i(sp), n,
+                                          _d(_f8(charge)), _d(_f8(n2p)))
+    return N, C, skipped
+
+
+def calc_distance(nx, ny, nz, dxyz, flow):
+    D = np.zeros((2 * nz + 1, 2 * ny + 1, 2 * nx + 1), np.float64)
+    lib().orc_calc_distance(_d(D), nx, ny, nz, _d(np.asarray(dxyz, np.float64)),
+                            _i(_i8(flow, 3)))
+    return D
+
+
+def bc_potentials(g: Grid, C, D, U=None):
+    U = np.zeros(g.shape, np.float64) if U is None else U
+    lib().orc_bc_potentials(ct.byref(g.cstruct()), _d(U), _d(C), _d(D))
+    return U
+
+
+def calc_epotential(g: Grid, U, C, maxerr, maxit):
+    """In place on U.  Returns (sweeps, final sqrt(r)/sqrt(b))."""
+    ratio = ct.c_double(0.0)
+    sweeps = lib().orc_calc_epotential(ct.byref(g.cstruct()), _d(U), _d(C), maxerr, maxit,
+                                       ct.byref(ratio))
+    return sweeps, ratio.value
+
+
+def calc_epotential_pairwise(g: Grid, U, C, maxerr, maxit):
+    return lib().orc_calc_epotential_pairwise(ct.byref(g.cstruct()), _d(U), _d(C), maxerr, maxit)
+
+
+def calc_efield(g: Grid, U):
+    E = np.zeros(g.shape + (3,), np.float64)
+    lib().orc_calc_efield(ct.byref(g.cstruct()), _d(E), _d(U))
+    return E
+
+
+def bc_fields(g: Grid, E):
+    lib().orc_bc_fields(ct.byref(g.cstruct()), _d(E))
+
+
+def advance_velocity(g: Grid, rp, sp, E, B0, dt, ve, eta, tB, sB, fluid=None, gyration=None, n=None):
+    n = rp.shape[0] if n is None else n
+    fluid = _i8([0] * 8 if fluid is None else fluid)
+    gyration = _i8([1] * 8 if gyration is None else gyration)
+    tB = np.ascontiguousarray(np.asarray(tB, np.float64).reshape(8, 4))
+    sB = np.ascontiguousarray(np.asarray(sB, np.float64).reshape(8, 4))
+    lib().orc_advance_velocity(ct.byref(g.cstruct()), _d(rp), _i(sp), _d(E), _d(_f8(B0, 4)), dt,
+                               _d(_f8(ve, 4)), _d(_f8(eta)), _d(tB), _d(sB), n, _i(fluid),
+                               _i(gyration))
+
+
+# ---------------------------------------------------------------------------
+# the reference's own MPI-free subroutines, compiled from /root/reference by
+# oracle/Makefile into oracle/_ref (absent on machines without the reference)
+
+_ref = None
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = ct.CDLL(REF_PATH)
+    return _ref
+
+
+def _byref_d(x):
+    return ct.byref(ct.c_double(x))
+
+
+def _byref_i(x):
+    return ct.byref(ct.c_int32(int(x)))
+
+
+def ref_advance_position(rp, dt, n):
+    """AdvancePosition(real_particles, dt, max_per_proc, num_local)."""
+    R = ref()
+    R.advanceposition_(_d(rp), _byref_d(dt), _byref_i(rp.shape[0]), _byref_i(n))
+
+
+def ref_bc_outflow(g: Grid, rp, sp, n):
+    """BCoutflow(...) with Lx_min.. = xmin.. on one rank.  Returns the new num_local."""
+    R = ref()
+    nl = ct.c_int32(n)
+    a = [ct.c_double(v) for v in (g.gmin[0], g.gmax[0], g.gmin[1], g.gmax[1], g.gmin[2], g.gmax[2])]
+    R.bcoutflow_(_d(rp), _i(sp), *[ct.byref(x) for x in a], *[ct.byref(ct.c_double(v)) for v in
+                 (g.gmin[0], g.gmax[0], g.gmin[1], g.gmax[1], g.gmin[2], g.gmax[2])],
+                 _byref_i(rp.shape[0]), ct.byref(nl),
+                 *[_byref_i(1 if f else 0) for f in g.flow])
+    return nl.value
+
+
+def ref_calc_distance(nx, ny, nz, dxyz, flow):
+    R = ref()
+    D = np.zeros((2 * nz + 1, 2 * ny + 1, 2 * nx + 1), np.float64)
+    R.calcdistance_(_d(D), _byref_i(nx), _byref_i(ny), _byref_i(nz),
+                    _d(np.asarray(dxyz, np.float64)), *[_byref_i(1 if f else 0) for f in flow])
+    return D
+
+
+def ref_bc_fields(g: Grid, E, U, B0, E0):
+    R = ref()
+    R.bcfields_(_d(E), _d(U), _d(_f8(B0, 4)), _d(_f8(E0, 4)), _byref_i(g.nx), _byref_i(g.ny),
+                _byref_i(g.nz), *[_byref_i(1 if f else 0) for f in g.flow], _byref_i(1),
+                _byref_i(1), _byref_i(1), _byref_i(0))
+
+
+def ref_advance_velocity(g: Grid, rp, sp, E, B0, dt, v0, ve, eta, tB, sB, n, fluid=None,
+                         gyration=None):
+    R = ref()
+    fluid = _i8([0] * 8 if fluid is None else fluid)
+    gyration = _i8([1] * 8 if gyration is None else gyration)
+    tB = np.ascontiguousarray(np.asarray(tB, np.float64).reshape(8, 4))
+    sB = np.ascontiguousarray(np.asarray(sB, np.float64).reshape(8, 4))
+    v0 = np.ascontiguousarray(np.asarray(v0, np.float64).reshape(8, 4))
+    dxyz = np.asarray(g.dxyz, np.float64)
+    R.advancevelocity_(_d(rp), _i(sp), _d(E), _d(_f8(B0, 4)), _byref_d(g.gmin[0]),
+                       _byref_d(g.gmin[1]), _byref_d(g.gmin[2]), _d(dxyz), _byref_d(dt), _d(v0),
+                       _d(_f8(ve, 4)), _d(_f8(eta)), _d(tB), _d(sB), _byref_i(n),
+                       _byref_i(rp.shape[0]), _byref_i(g.species), _byref_i(g.nx),
+                       _byref_i(g.ny), _byref_i(g.nz), _i(fluid), _i(gyration))

@@ -1,0 +1,89 @@
+/*
+ * picard_oracle.h -- CPU restatement of Picard's PIC time step (FP64, reference
+ * evaluation order).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP product path
+ * is compared against.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product (picard_amd) never links, loads
+ * or falls back to it.
+ *
+ * Conventions (single MPI rank, iprocs = jprocs = kprocs = 1, myid = 0):
+ *   - real_particles(8, n) is stored as rp[8*p + c], p = pp-1, c = slot-1.
+ *   - int_particles(n) (species, 1-based) is sp[p].
+ *   - Grid arrays keep the Fortran column-major layout with one guard layer:
+ *       node (i,j,k), 1-based i in 1..Nx+2, is index (i-1) + X*((j-1) + Y*(k-1))
+ *       with X = Nx+2, Y = Ny+2.  E(3,...) and N(S,...) keep their leading
+ *       component/species index fastest.
+ *   - D(2Nx+1, 2Ny+1, 2Nz+1) column-major.
+ *   - Logical flags are int (nonzero = .true.).
+ */
+#ifndef PICARD_ORACLE_H
+#define PICARD_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int32_t nx, ny, nz;      /* Nx_local, Ny_local, Nz_local (= Nx, Ny, Nz on one rank) */
+  int32_t species;         /* number of species S */
+  int32_t flow[3];         /* xflow, yflow, zflow */
+  double lmin[3], lmax[3]; /* Lx_min..Lz_max (local box = global box on one rank) */
+  double gmin[3], gmax[3]; /* xmin..zmax */
+  double dxyz[3];
+} orc_grid;
+
+/* AdvancePosition.f90:4-19 */
+void orc_advance_position(double *rp, double dt, int64_t n);
+
+/* BCoutflow.f90:5-87.  Sequential swap-with-last removal; *n is inout.
+ * removed_slots (optional, may be NULL) receives the 1-based slot numbers
+ * deleted, in loop order; returns the number removed. */
+int64_t orc_bc_outflow(const orc_grid *g, double *rp, int32_t *sp, int64_t *n,
+                       int64_t *removed_slots);
+
+/* ParticlesTransfer.f90:494-519 (all three dimensions on one rank). */
+void orc_particles_transfer(const orc_grid *g, double *rp, int64_t n);
+
+/* ParticlesGridify.f90:56-293.  N(S,G), C(G) are outputs.  Returns the number
+ * of particles whose CIC stencil fell outside the array (skipped; the
+ * reference would write out of bounds). */
+int64_t orc_particles_gridify(const orc_grid *g, double *N, double *C, const double *rp,
+                              const int32_t *sp, int64_t n, const double charge[8],
+                              const double n2p[8]);
+
+/* CalcDistance.f90:1-95 (global Nx, Ny, Nz). */
+void orc_calc_distance(double *D, int32_t Nx, int32_t Ny, int32_t Nz, const double dxyz[3],
+                       const int32_t flow[3]);
+
+/* BCpotentials.f90:36-141 on one rank (bi = bj = bk = 0). */
+void orc_bc_potentials(const orc_grid *g, double *U, const double *C, const double *D);
+
+/* CalcEpotential.f90:28-242.  Returns the number of Jacobi sweeps done.
+ * final_ratio (optional) receives sqrt(r)/sqrt(b) at exit. */
+int64_t orc_calc_epotential(const orc_grid *g, double *U, const double *C, double maxerr,
+                            int32_t maxit, double *final_ratio);
+
+/* CalcEfield.f90:27-228. */
+void orc_calc_efield(const orc_grid *g, double *E, const double *U);
+
+/* BCfields.f90:31-90. */
+void orc_bc_fields(const orc_grid *g, double *E);
+
+/* AdvanceVelocity.f90:34-121 (including the fluid branch). */
+void orc_advance_velocity(const orc_grid *g, double *rp, const int32_t *sp, const double *E,
+                          const double B0[4], double dt, const double ve[4],
+                          const double eta[8], const double tB[32], const double sB[32],
+                          int64_t n, const int32_t fluid[8], const int32_t gyration[8]);
+
+/* The same Jacobi stopping rule with the norm sums taken in a different
+ * association (pairwise); used to show that iteration counts do not hinge
+ * on the summation order. */
+int64_t orc_calc_epotential_pairwise(const orc_grid *g, double *U, const double *C,
+                                     double maxerr, int32_t maxit);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

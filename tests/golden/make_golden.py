@@ -1,0 +1,91 @@
+"""Writes golden input/output vectors for the reference's MPI-free hot-path
+subroutines (AdvancePosition, BCoutflow, CalcDistance, AdvanceVelocity),
+computed by the reference's own Fortran compiled into oracle/_ref.
+
+Run in the container where /root/reference is mounted:
+    make -C oracle && python tests/golden/make_golden.py
+The .npz files are data only (inputs and the reference's outputs)."""
+import itertools
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle  # noqa: E402
+from tests.helpers import boris_tables, random_particles  # noqa: E402
+
+
+def main():
+    if not oracle.ref_available():
+        raise SystemExit("oracle/_ref is not built; run `make -C oracle` where the reference is")
+    rng = np.random.default_rng(20261202)
+
+    # AdvancePosition
+    g = oracle.Grid(6, 6, 6, 2)
+    rp, _ = random_particles(rng, g, 600)
+    out = rp.copy()
+    oracle.ref_advance_position(out, 7.927e-7, 500)
+    np.savez_compressed(os.path.join(HERE, "ref_advance_position.npz"), rp_in=rp, rp_out=out,
+                        dt=np.float64(7.927e-7), n=np.int64(500))
+
+    # BCoutflow, every flow-flag combination
+    g = oracle.Grid(5, 6, 4, 2)
+    rp, sp = random_particles(rng, g, 700, spread=2.0)
+    rp[::53, 2] = g.gmin[2] - 1.6 * (g.gmax[2] - g.gmin[2])
+    d = dict(rp_in=rp, sp_in=sp, dims=np.array([5, 6, 4]))
+    flows = list(itertools.product([0, 1], repeat=3))
+    for t, flow in enumerate(flows):
+        gg = oracle.Grid(5, 6, 4, 2, flow=tuple(bool(f) for f in flow))
+        a, sa = rp.copy(), sp.copy()
+        n = oracle.ref_bc_outflow(gg, a, sa, a.shape[0])
+        d[f"flow{t}"] = np.array(flow, np.int32)
+        d[f"n{t}"] = np.int64(n)
+        d[f"rp_out{t}"] = a[:n]
+        d[f"sp_out{t}"] = sa[:n]
+    d["ncase"] = np.int64(len(flows))
+    np.savez_compressed(os.path.join(HERE, "ref_bc_outflow.npz"), **d)
+
+    # CalcDistance
+    d = {}
+    cases = [((4, 5, 3), (10.0, 10.0, 10.0)), ((6, 4, 5), (3.0, 7.5, 11.25))]
+    t = 0
+    for dims, dx in cases:
+        for flow in flows:
+            d[f"dims{t}"] = np.array(dims, np.int32)
+            d[f"dxyz{t}"] = np.array(dx, np.float64)
+            d[f"flow{t}"] = np.array(flow, np.int32)
+            d[f"D{t}"] = oracle.ref_calc_distance(*dims, dx, flow)
+            t += 1
+    d["ncase"] = np.int64(t)
+    np.savez_compressed(os.path.join(HERE, "ref_calc_distance.npz"), **d)
+
+    # AdvanceVelocity (Boris branch, three species)
+    flow = (1, 0, 0)
+    g = oracle.Grid(5, 4, 6, 3, flow=tuple(bool(f) for f in flow))
+    rp, sp = random_particles(rng, g, 500, species=3)
+    E = rng.normal(0.0, 1e-3, size=g.shape + (3,))
+    dt = 7.927e-7
+    B0 = np.array([1.1e-9, -2.0e-9, 5e-9])
+    B4 = np.array([B0[0], B0[1], B0[2], np.sqrt(np.sum(B0 ** 2))])
+    mass = np.array([9.1093819e-31, 1.6726216e-27, 18 * 1.6726216e-27] + [1.6726216e-27] * 5)
+    charge = np.array([-1.60217646e-19] + [1.60217646e-19] * 7)
+    eta = charge / mass
+    fluid = np.zeros(8, np.int32)
+    gyration = np.ones(8, np.int32)
+    tB, sB = boris_tables(dt, B0, eta, gyration)
+    ve = np.array([4.0e5, 0.0, 0.0, 4.0e5])
+    out = rp.copy()
+    oracle.ref_advance_velocity(g, out, sp, E, B4, dt, np.zeros((8, 4)), ve, eta, tB, sB, 500,
+                                fluid, gyration)
+    np.savez_compressed(os.path.join(HERE, "ref_advance_velocity.npz"), rp_in=rp, rp_out=out,
+                        sp=sp, E=E, B0=B4, dt=np.float64(dt), ve=ve, eta=eta, tB=tB, sB=sB,
+                        fluid=fluid, gyration=gyration, n=np.int64(500),
+                        dims=np.array([5, 4, 6]), flow=np.array(flow, np.int32))
+    print("golden vectors written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
