@@ -1,0 +1,162 @@
+/*
+ * picard_hip.h -- C ABI of the MI355X (gfx950) implementation of Picard's
+ * PIC time step.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns an int status (PICARD_OK = 0, negative = error, positive = warning);
+ * picard_last_error() gives the message of the last failure on the calling
+ * thread.  State (particles, fields) stays resident on the device inside a
+ * picard_ctx between calls; nothing is copied to or from the host unless a
+ * *_read / *_write / *_append entry point is called.
+ *
+ * The Fortran drop-in layer (picard_amd/fortran/picard_dropin.f90) binds these
+ * through ISO_C_BINDING and exposes the reference's subroutine names and
+ * argument lists on top of them.
+ *
+ * Array layouts follow the reference (Fortran column-major, one guard layer):
+ *   real_particles(8, n)  -> double[8*n], slot fastest
+ *   int_particles(n)      -> int32_t[n]  (species, 1-based)
+ *   U, C (Nx+2,Ny+2,Nz+2) -> double[G],  G = (Nx+2)(Ny+2)(Nz+2)
+ *   E(3, ...)             -> double[3*G], component fastest
+ *   N(species, ...)       -> double[S*G], species fastest
+ *   D(2Nx+1,2Ny+1,2Nz+1)  -> double[(2Nx+1)(2Ny+1)(2Nz+1)]
+ */
+#ifndef PICARD_HIP_H
+#define PICARD_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PICARD_OK = 0,
+  PICARD_ERR_ARG = -1,       /* bad argument or shape */
+  PICARD_ERR_HIP = -2,       /* HIP runtime failure */
+  PICARD_ERR_CAPACITY = -3,  /* more particles than max_particles */
+  PICARD_ERR_RANGE = -4,     /* a particle's CIC stencil falls outside the grid arrays */
+  PICARD_ERR_STATE = -5,     /* call out of order (e.g. BCpotentials before CalcDistance) */
+  PICARD_WARN_NOCONV = 1     /* CalcEpotential ran past maxit without converging */
+};
+
+enum { PICARD_FP64 = 0, PICARD_FP32_SPLIT = 1 };
+enum { PICARD_DEPOSIT_REFERENCE_ORDER = 0, PICARD_DEPOSIT_FIXED_POINT = 1 };
+enum {
+  PICARD_FIELD_U = 0, PICARD_FIELD_E = 1, PICARD_FIELD_N = 2, PICARD_FIELD_C = 3,
+  PICARD_FIELD_D = 4
+};
+
+/* One domain (one MPI rank of the reference, here one GPU). */
+typedef struct {
+  int32_t nx, ny, nz;        /* Nx_local, Ny_local, Nz_local (picard.f90:46-48) */
+  int32_t species;           /* species (picard.f90:50), 1..8 */
+  int32_t flow[3];           /* xflow, yflow, zflow (picard.f90:148-150), nonzero = .true. */
+  int32_t precision;         /* PICARD_FP64 or PICARD_FP32_SPLIT */
+  int32_t deposit;           /* PICARD_DEPOSIT_* (FP64 form; FP32-split always fixed point) */
+  int32_t device;            /* HIP device ordinal */
+  int64_t max_particles;     /* max_per_proc (picard.f90:52) */
+  double lmin[3], lmax[3];   /* Lx_min..Lz_max (picard.f90:350-355) */
+  double gmin[3], gmax[3];   /* xmin..zmax (picard.f90:159-164) */
+  double dxyz[3];            /* cell sizes (picard.f90:358-360) */
+} picard_geometry;
+
+/* Per-species constants derived by the host (picard.f90:365-466). */
+typedef struct {
+  double charge[8], n2p[8], eta[8];
+  double tB[32], sB[32];     /* tB(4,8), sB(4,8): element (c,s) at [4*(s-1) + (c-1)] */
+  double ve[4], B0[4], E0[4];
+  int32_t fluid[8], gyration[8];
+} picard_species;
+
+typedef struct picard_ctx picard_ctx;
+
+const char *picard_last_error(void);
+
+/* Allocate a device-resident domain.  No reference counterpart: the
+ * reference keeps these arrays as static locals of the main program
+ * (picard.f90:91-113). */
+int picard_create(const picard_geometry *geo, const picard_species *sp, picard_ctx **out);
+int picard_destroy(picard_ctx *ctx);
+int picard_set_species(picard_ctx *ctx, const picard_species *sp);
+int picard_synchronize(picard_ctx *ctx);
+
+/* Append `count` particles after the current num_local: the range that
+ * BCinflow (BCinflow.f90:2) and ParticlesProduce (ParticlesProduce.f90:1)
+ * write on the host.  real_particles points at the first appended particle,
+ * 8 doubles each. */
+int picard_particles_append(picard_ctx *ctx, const double *real_particles,
+                            const int32_t *int_particles, int64_t count);
+/* Copy particles [first, first+count) (0-based) back to host AoS arrays. */
+int picard_particles_read(picard_ctx *ctx, int64_t first, int64_t count, double *real_particles,
+                          int32_t *int_particles);
+int picard_particles_count(picard_ctx *ctx, int64_t *num_local);
+
+/* ---- the time-step call surface (picard.f90:564-631) ---- */
+
+/* AdvancePosition(real_particles, dt, max_per_proc, num_local)   AdvancePosition.f90:4 */
+int picard_advance_position(picard_ctx *ctx, double dt);
+
+/* BCoutflow(real_particles, int_particles, Lx_min..Lz_max, xmin..zmax, max_per_proc,
+ *           num_local, xflow, yflow, zflow)                        BCoutflow.f90:5
+ * Reference swap-with-last order; num_local (out) and removed (out, optional). */
+int picard_bc_outflow(picard_ctx *ctx, int64_t *num_local, int64_t *removed);
+
+/* ParticlesTransfer(...) on one domain: periodic wrap     ParticlesTransfer.f90:2 */
+int picard_particles_transfer(picard_ctx *ctx);
+
+/* ParticlesGridify(N, C, real_particles, int_particles, ...)   ParticlesGridify.f90:1 */
+int picard_particles_gridify(picard_ctx *ctx);
+
+/* CalcDistance(D, Nx, Ny, Nz, dxyz, xflow, yflow, zflow)        CalcDistance.f90:1 */
+int picard_calc_distance(picard_ctx *ctx);
+
+/* BCpotentials(U, C, D, xmin..zmax, Nx_local.., Nx.., flows, ...)  BCpotentials.f90:1 */
+int picard_bc_potentials(picard_ctx *ctx);
+
+/* CalcEpotential(U, C, ..., dxyz, maxerr, maxit, ...)        CalcEpotential.f90:4
+ * sweeps (out) = Jacobi sweeps performed.  Returns PICARD_WARN_NOCONV when the
+ * loop ran to 2*maxit+1 sweeps. */
+int picard_calc_epotential(picard_ctx *ctx, double maxerr, int32_t maxit, int64_t *sweeps);
+
+/* CalcEfield(E, U, Nx_local.., dxyz, flows, ...)                 CalcEfield.f90:3 */
+int picard_calc_efield(picard_ctx *ctx);
+
+/* BCfields(E, U, B0, E0, Nx_local.., flows, ...)                 BCfields.f90:3 */
+int picard_bc_fields(picard_ctx *ctx);
+
+/* AdvanceVelocity(real_particles, int_particles, E, B0, Lx_min.., dxyz, dt, v0, ve,
+ *                 eta, tB, sB, num_local, ...)                   AdvanceVelocity.f90:6 */
+int picard_advance_velocity(picard_ctx *ctx, double dt);
+
+/* Field transfer between device and host arrays of the layouts above;
+ * `count` is the number of doubles. */
+int picard_field_read(picard_ctx *ctx, int32_t which, double *host, int64_t count);
+int picard_field_write(picard_ctx *ctx, int32_t which, const double *host, int64_t count);
+
+/* Kernel timing of the Jacobi sweep kernel, for roofline measurement; no
+ * reference counterpart (the reference has no timers).  While enabled every
+ * sweep launch carries start/stop events stamped by its dispatch packet;
+ * picard_timing_read returns the summed kernel time and the launch count
+ * since the last enable. */
+int picard_timing_enable(picard_ctx *ctx, int32_t enable);
+int picard_timing_read(picard_ctx *ctx, double *jacobi_ms, int64_t *jacobi_launches);
+
+/* One whole main-loop body after BCinflow (picard.f90:564-631, no output
+ * step): AdvancePosition, append `produced` (ParticlesProduce's particles,
+ * may be 0), BCoutflow, ParticlesTransfer, ParticlesGridify, BCpotentials,
+ * CalcEpotential, CalcEfield, BCfields, AdvanceVelocity. */
+typedef struct {
+  int64_t num_local;         /* after BCoutflow */
+  int64_t removed;           /* BCoutflow removals */
+  int64_t sweeps;            /* Jacobi sweeps */
+  int64_t pushed;            /* particles given to AdvanceVelocity */
+} picard_step_stats;
+
+int picard_step(picard_ctx *ctx, double dt, double maxerr, int32_t maxit,
+                const double *produced, const int32_t *produced_species, int64_t nproduced,
+                picard_step_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

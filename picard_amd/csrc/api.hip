@@ -1,0 +1,429 @@
+// api.hip -- the extern "C" boundary of libpicard_hip.so (include/picard_hip.h).
+//
+// A picard_ctx owns one domain's device-resident state.  Each entry point
+// validates its arguments on the host, launches on the context's stream and
+// returns a status; kernels never see a shape that was not checked here.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "picard_internal.h"
+
+namespace picard {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+namespace {
+
+template <typename T>
+int dalloc(T **p, int64_t count) {
+  *p = nullptr;
+  if (count <= 0) count = 1;
+  PICARD_HIP(hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * (size_t)count));
+  return PICARD_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+int check_geometry(const picard_geometry *g) {
+  if (!g) {
+    set_error("geometry is NULL");
+    return PICARD_ERR_ARG;
+  }
+  if (g->nx < 2 || g->ny < 2 || g->nz < 2) {
+    set_error("Nx_local, Ny_local, Nz_local must each be >= 2");
+    return PICARD_ERR_ARG;
+  }
+  if (g->species < 1 || g->species > 8) {
+    set_error("species must be in 1..8");
+    return PICARD_ERR_ARG;
+  }
+  if (g->max_particles < 1 || g->max_particles >= (int64_t(1) << 31)) {
+    set_error("max_particles must be in 1..2^31-1");
+    return PICARD_ERR_ARG;
+  }
+  for (int c = 0; c < 3; ++c) {
+    if (!(g->dxyz[c] > 0.0) || !(g->gmax[c] > g->gmin[c]) || !(g->lmax[c] > g->lmin[c])) {
+      set_error("cell sizes and box extents must be positive");
+      return PICARD_ERR_ARG;
+    }
+  }
+  if (g->precision != PICARD_FP64) {
+    set_error("this context type supports precision PICARD_FP64");
+    return PICARD_ERR_ARG;
+  }
+  if (g->deposit != PICARD_DEPOSIT_REFERENCE_ORDER) {
+    set_error("the FP64 form deposits in reference order (PICARD_DEPOSIT_REFERENCE_ORDER)");
+    return PICARD_ERR_ARG;
+  }
+  const int64_t K = (int64_t)(g->nx + 1) * (g->ny + 1) * (g->nz + 1);
+  if ((int64_t)g->species * K >= (int64_t(1) << 31)) {
+    set_error("grid too large for 32-bit deposit keys");
+    return PICARD_ERR_ARG;
+  }
+  return PICARD_OK;
+}
+
+int upload_species(picard_ctx *c) {
+  double h[16];
+  for (int s = 0; s < 8; ++s) {
+    h[s] = c->spc.n2p[s];
+    h[8 + s] = c->spc.charge[s];
+  }
+  PICARD_HIP(hipMemcpyAsync(c->consts, h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+  PICARD_HIP(hipStreamSynchronize(c->stream));
+  return PICARD_OK;
+}
+
+void release(picard_ctx *c) {
+  for (int q = 0; q < 8; ++q) dfree(c->p.r[q]);
+  dfree(c->p.sp);
+  dfree(c->U);
+  dfree(c->E);
+  dfree(c->N);
+  dfree(c->Nraw);
+  dfree(c->C);
+  dfree(c->D);
+  for (int q = 0; q < 3; ++q) dfree(c->jb[q]);
+  dfree(c->b);
+  dfree(c->consts);
+  dfree(c->js);
+  dfree(c->s.flags);
+  dfree(c->s.scan);
+  for (int q = 0; q < 6; ++q) dfree(c->s.work[q]);
+  dfree(c->s.scan_tmp);
+  dfree(c->s.counters);
+  dfree(c->s.key);
+  dfree(c->s.perm);
+  dfree(c->s.hist);
+  dfree(c->s.cursor);
+  dfree(c->s.stage);
+  dfree(c->s.stage_sp);
+  dfree(c->s.partials);
+  for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
+  c->tev.clear();
+  if (c->js_host) (void)hipHostFree(c->js_host);
+  if (c->host_counters) (void)hipHostFree(c->host_counters);
+  c->js_host = nullptr;
+  c->host_counters = nullptr;
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  c->stream = nullptr;
+}
+
+int create_impl(const picard_geometry *geo, const picard_species *sp, picard_ctx *c) {
+  c->geo = *geo;
+  c->spc = *sp;
+  c->device = geo->device;
+  PICARD_HIP(hipSetDevice(c->device));
+  PICARD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  GridDims &g = c->gd;
+  g.nx = geo->nx;
+  g.ny = geo->ny;
+  g.nz = geo->nz;
+  g.X = g.nx + 2;
+  g.Y = g.ny + 2;
+  g.Z = g.nz + 2;
+  g.G = (int64_t)g.X * g.Y * g.Z;
+  g.S = geo->species;
+  for (int q = 0; q < 3; ++q) {
+    g.flow[q] = geo->flow[q] ? 1 : 0;
+    g.lmin[q] = geo->lmin[q];
+    g.lmax[q] = geo->lmax[q];
+    g.gmin[q] = geo->gmin[q];
+    g.gmax[q] = geo->gmax[q];
+    g.dxyz[q] = geo->dxyz[q];
+  }
+  g.dV = g.dxyz[0] * g.dxyz[1] * g.dxyz[2];
+  c->cap = geo->max_particles;
+  c->K = (int64_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  const int64_t cap = c->cap;
+  for (int q = 0; q < 8; ++q) PICARD_TRY(dalloc(&c->p.r[q], cap));
+  PICARD_TRY(dalloc(&c->p.sp, cap));
+  PICARD_TRY(dalloc(&c->U, g.G));
+  PICARD_TRY(dalloc(&c->E, 3 * g.G));
+  PICARD_TRY(dalloc(&c->N, g.S * g.G));
+  PICARD_TRY(dalloc(&c->Nraw, g.S * g.G));
+  PICARD_TRY(dalloc(&c->C, g.G));
+  const int64_t DN = (2 * (int64_t)g.nx + 1) * (2 * (int64_t)g.ny + 1) * (2 * (int64_t)g.nz + 1);
+  PICARD_TRY(dalloc(&c->D, DN));
+  for (int q = 0; q < 3; ++q) PICARD_TRY(dalloc(&c->jb[q], g.G));
+  PICARD_TRY(dalloc(&c->b, g.G));
+  PICARD_TRY(dalloc(&c->consts, 16));
+  PICARD_TRY(dalloc(&c->js, 1));
+  PICARD_TRY(dalloc(&c->s.flags, cap));
+  PICARD_TRY(dalloc(&c->s.scan, cap + 1));
+  for (int q = 0; q < 6; ++q) PICARD_TRY(dalloc(&c->s.work[q], cap + 1));
+  const int64_t nb = (int64_t)g.S * c->K;
+  const int64_t scan_n = cap > nb ? cap : nb;
+  c->s.scan_tmp_len = scan_tmp_required(scan_n);
+  PICARD_TRY(dalloc(&c->s.scan_tmp, c->s.scan_tmp_len));
+  PICARD_TRY(dalloc(&c->s.counters, 16));
+  PICARD_TRY(dalloc(&c->s.key, cap));
+  PICARD_TRY(dalloc(&c->s.perm, cap));
+  PICARD_TRY(dalloc(&c->s.hist, nb + 1));
+  PICARD_TRY(dalloc(&c->s.cursor, nb));
+  const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
+  const int64_t pb = grid_blocks(g.G > ncell ? g.G : ncell, 256);
+  c->s.partials_len = pb;
+  PICARD_TRY(dalloc(&c->s.partials, pb));
+  PICARD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->js_host), sizeof(JacobiState)));
+  PICARD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->host_counters), 16 * sizeof(int32_t)));
+  // the reference zeroes U, N, E and the particle arrays at start (picard.f90:434-439)
+  PICARD_HIP(hipMemsetAsync(c->U, 0, sizeof(double) * g.G, c->stream));
+  PICARD_HIP(hipMemsetAsync(c->E, 0, sizeof(double) * 3 * g.G, c->stream));
+  PICARD_HIP(hipMemsetAsync(c->N, 0, sizeof(double) * g.S * g.G, c->stream));
+  PICARD_HIP(hipMemsetAsync(c->C, 0, sizeof(double) * g.G, c->stream));
+  PICARD_HIP(hipMemsetAsync(c->D, 0, sizeof(double) * DN, c->stream));
+  PICARD_TRY(upload_species(c));
+  return PICARD_OK;
+}
+
+int64_t field_len(const picard_ctx *c, int which) {
+  const GridDims &g = c->gd;
+  switch (which) {
+    case PICARD_FIELD_U: return g.G;
+    case PICARD_FIELD_E: return 3 * g.G;
+    case PICARD_FIELD_N: return (int64_t)g.S * g.G;
+    case PICARD_FIELD_C: return g.G;
+    case PICARD_FIELD_D:
+      return (2 * (int64_t)g.nx + 1) * (2 * (int64_t)g.ny + 1) * (2 * (int64_t)g.nz + 1);
+    default: return -1;
+  }
+}
+
+double *field_ptr(picard_ctx *c, int which) {
+  switch (which) {
+    case PICARD_FIELD_U: return c->U;
+    case PICARD_FIELD_E: return c->E;
+    case PICARD_FIELD_N: return c->N;
+    case PICARD_FIELD_C: return c->C;
+    case PICARD_FIELD_D: return c->D;
+    default: return nullptr;
+  }
+}
+
+#define CTX_CHECK(c)                          \
+  do {                                        \
+    if (!(c)) {                               \
+      ::picard::set_error("context is NULL"); \
+      return PICARD_ERR_ARG;                  \
+    }                                         \
+    PICARD_HIP(hipSetDevice((c)->device));    \
+  } while (0)
+
+}  // namespace
+}  // namespace picard
+
+using namespace picard;
+
+extern "C" {
+
+const char *picard_last_error(void) { return g_last_error.c_str(); }
+
+int picard_create(const picard_geometry *geo, const picard_species *sp, picard_ctx **out) {
+  if (!out || !sp) {
+    set_error("picard_create: NULL argument");
+    return PICARD_ERR_ARG;
+  }
+  *out = nullptr;
+  PICARD_TRY(check_geometry(geo));
+  picard_ctx *c = new picard_ctx();
+  const int rc = create_impl(geo, sp, c);
+  if (rc != PICARD_OK) {
+    release(c);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return PICARD_OK;
+}
+
+int picard_destroy(picard_ctx *c) {
+  if (!c) return PICARD_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  release(c);
+  delete c;
+  return PICARD_OK;
+}
+
+int picard_set_species(picard_ctx *c, const picard_species *sp) {
+  CTX_CHECK(c);
+  if (!sp) {
+    set_error("species is NULL");
+    return PICARD_ERR_ARG;
+  }
+  c->spc = *sp;
+  return upload_species(c);
+}
+
+int picard_synchronize(picard_ctx *c) {
+  CTX_CHECK(c);
+  PICARD_HIP(hipStreamSynchronize(c->stream));
+  return PICARD_OK;
+}
+
+int picard_particles_append(picard_ctx *c, const double *rp, const int32_t *sp, int64_t count) {
+  CTX_CHECK(c);
+  if (count < 0 || (count > 0 && (!rp || !sp))) {
+    set_error("particles_append: bad arguments");
+    return PICARD_ERR_ARG;
+  }
+  for (int64_t i = 0; i < count; ++i) {
+    if (sp[i] < 1 || sp[i] > c->gd.S) {
+      set_error("particles_append: species index outside 1..species");
+      return PICARD_ERR_ARG;
+    }
+  }
+  return launch_append(c, rp, sp, count);
+}
+
+int picard_particles_read(picard_ctx *c, int64_t first, int64_t count, double *rp, int32_t *sp) {
+  CTX_CHECK(c);
+  if (count < 0 || (count > 0 && !rp)) {
+    set_error("particles_read: bad arguments");
+    return PICARD_ERR_ARG;
+  }
+  return launch_read(c, first, count, rp, sp);
+}
+
+int picard_particles_count(picard_ctx *c, int64_t *n) {
+  CTX_CHECK(c);
+  if (!n) {
+    set_error("particles_count: NULL");
+    return PICARD_ERR_ARG;
+  }
+  *n = c->n;
+  return PICARD_OK;
+}
+
+int picard_advance_position(picard_ctx *c, double dt) {
+  CTX_CHECK(c);
+  return launch_push_x(c, 0, c->n, dt);
+}
+
+int picard_bc_outflow(picard_ctx *c, int64_t *num_local, int64_t *removed) {
+  CTX_CHECK(c);
+  int64_t nn = 0, rem = 0;
+  PICARD_TRY(launch_outflow(c, &nn, &rem));
+  if (num_local) *num_local = nn;
+  if (removed) *removed = rem;
+  return PICARD_OK;
+}
+
+int picard_particles_transfer(picard_ctx *c) {
+  CTX_CHECK(c);
+  return launch_wrap(c);
+}
+
+int picard_particles_gridify(picard_ctx *c) {
+  CTX_CHECK(c);
+  return launch_gridify(c);
+}
+
+int picard_calc_distance(picard_ctx *c) {
+  CTX_CHECK(c);
+  return launch_calc_distance(c);
+}
+
+int picard_bc_potentials(picard_ctx *c) {
+  CTX_CHECK(c);
+  return launch_bc_potentials(c);
+}
+
+int picard_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) {
+  CTX_CHECK(c);
+  if (maxit < 0 || !(maxerr >= 0.0)) {
+    set_error("calc_epotential: maxit must be >= 0 and maxerr >= 0");
+    return PICARD_ERR_ARG;
+  }
+  int64_t sw = 0;
+  const int rc = launch_calc_epotential(c, maxerr, maxit, &sw);
+  if (sweeps) *sweeps = sw;
+  return rc;
+}
+
+int picard_calc_efield(picard_ctx *c) {
+  CTX_CHECK(c);
+  return launch_calc_efield(c);
+}
+
+int picard_bc_fields(picard_ctx *c) {
+  CTX_CHECK(c);
+  return launch_bc_fields(c);
+}
+
+int picard_advance_velocity(picard_ctx *c, double dt) {
+  CTX_CHECK(c);
+  return launch_boris(c, dt);
+}
+
+int picard_field_read(picard_ctx *c, int32_t which, double *host, int64_t count) {
+  CTX_CHECK(c);
+  const int64_t len = field_len(c, which);
+  if (len < 0 || !host || count != len) {
+    set_error("field_read: unknown field or wrong element count");
+    return PICARD_ERR_ARG;
+  }
+  PICARD_HIP(hipMemcpyAsync(host, field_ptr(c, which), sizeof(double) * len,
+                            hipMemcpyDeviceToHost, c->stream));
+  PICARD_HIP(hipStreamSynchronize(c->stream));
+  return PICARD_OK;
+}
+
+int picard_field_write(picard_ctx *c, int32_t which, const double *host, int64_t count) {
+  CTX_CHECK(c);
+  const int64_t len = field_len(c, which);
+  if (len < 0 || !host || count != len) {
+    set_error("field_write: unknown field or wrong element count");
+    return PICARD_ERR_ARG;
+  }
+  PICARD_HIP(hipMemcpyAsync(field_ptr(c, which), host, sizeof(double) * len,
+                            hipMemcpyHostToDevice, c->stream));
+  PICARD_HIP(hipStreamSynchronize(c->stream));
+  if (which == PICARD_FIELD_D) c->have_D = true;
+  return PICARD_OK;
+}
+
+int picard_timing_enable(picard_ctx *c, int32_t enable) {
+  CTX_CHECK(c);
+  c->timing = enable != 0;
+  c->t_ms = 0.0;
+  c->t_launches = 0;
+  return PICARD_OK;
+}
+
+int picard_timing_read(picard_ctx *c, double *jacobi_ms, int64_t *jacobi_launches) {
+  CTX_CHECK(c);
+  if (jacobi_ms) *jacobi_ms = c->t_ms;
+  if (jacobi_launches) *jacobi_launches = c->t_launches;
+  return PICARD_OK;
+}
+
+int picard_step(picard_ctx *c, double dt, double maxerr, int32_t maxit, const double *produced,
+                const int32_t *produced_species, int64_t nproduced, picard_step_stats *stats) {
+  CTX_CHECK(c);
+  picard_step_stats st{};
+  PICARD_TRY(launch_push_x(c, 0, c->n, dt));                              // picard.f90:564
+  if (nproduced > 0) PICARD_TRY(picard_particles_append(c, produced, produced_species, nproduced));
+  PICARD_TRY(launch_outflow(c, &st.num_local, &st.removed));             // :569
+  PICARD_TRY(launch_wrap(c));                                            // :572
+  PICARD_TRY(launch_gridify(c));                                         // :577
+  PICARD_TRY(launch_bc_potentials(c));                                   // :581
+  const int rc = launch_calc_epotential(c, maxerr, maxit, &st.sweeps);   // :584
+  if (rc < 0) return rc;
+  PICARD_TRY(launch_calc_efield(c));                                     // :587
+  PICARD_TRY(launch_bc_fields(c));                                       // :589
+  st.pushed = c->n;
+  PICARD_TRY(launch_boris(c, dt));                                       // :630
+  if (stats) *stats = st;
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,707 @@
+// grid.hip -- FP64 grid kernels: ParticlesGridify (reference-order CIC
+// deposit), CalcDistance, BCpotentials, CalcEpotential (Jacobi with a
+// device-side stop), CalcEfield and BCfields, for one domain.
+//
+// One domain means every MPI exchange of the reference is a self-copy; those
+// copies become index arithmetic here (guard folds, periodic wraps), with the
+// same z -> y -> x composition order.
+#include <hip/hip_ext.h>
+
+#include "cic.h"
+#include "picard_internal.h"
+
+namespace picard {
+namespace {
+
+constexpr int kT = 256;
+
+// ---------------------------------------------------------------------------
+// ParticlesGridify (ParticlesGridify.f90:56-293)
+//
+// The reference accumulates N(s, node) sequentially over pp = n..1.  To get the
+// same sums bit for bit in parallel, particles are bucketed by (species, lower
+// node) and every (species, node) output merges its 8 contributing buckets in
+// descending particle index, adding in exactly the reference's order.
+
+__device__ __forceinline__ int64_t cell_key(const GridDims &g, int s0, const CicLoc &L) {
+  const int64_t K = (int64_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  return (int64_t)s0 * K + (L.idx[0] - 1) +
+         (int64_t)(g.nx + 1) * ((L.idx[1] - 1) + (int64_t)(g.ny + 1) * (L.idx[2] - 1));
+}
+
+__global__ void __launch_bounds__(kT) k_dep_key(ParticlesF64 p, int64_t n, GridDims g,
+                                                 int32_t *__restrict__ key,
+                                                 int32_t *__restrict__ hist,
+                                                 int32_t *__restrict__ errors) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const CicLoc L = cic_locate(g, p.r[0][i], p.r[1][i], p.r[2][i]);
+  const int s0 = p.sp[i] - 1;
+  if (!cic_in_bounds(g, L) || s0 < 0 || s0 >= g.S) {
+    key[i] = -1;
+    atomicAdd(errors, 1);
+    return;
+  }
+  const int32_t k = (int32_t)cell_key(g, s0, L);
+  key[i] = k;
+  atomicAdd(&hist[k], 1);
+}
+
+__global__ void __launch_bounds__(kT) k_dep_scatter(int64_t n, const int32_t *__restrict__ key,
+                                                     const int32_t *__restrict__ off,
+                                                     int32_t *__restrict__ cursor,
+                                                     int32_t *__restrict__ perm) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const int32_t k = key[i];
+  if (k < 0) return;
+  const int32_t pos = off[k] + atomicAdd(&cursor[k], 1);
+  perm[pos] = (int32_t)i;
+}
+
+// Each bucket sorted by descending particle index (insertion sort; buckets
+// hold a few dozen particles).
+__global__ void __launch_bounds__(kT) k_dep_bucket_sort(int64_t nbuckets,
+                                                         const int32_t *__restrict__ off,
+                                                         int32_t *__restrict__ perm) {
+  const int64_t b = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (b >= nbuckets) return;
+  const int32_t lo = off[b], hi = off[b + 1];
+  for (int32_t a = lo + 1; a < hi; ++a) {
+    const int32_t v = perm[a];
+    int32_t q = a - 1;
+    while (q >= lo && perm[q] < v) {
+      perm[q + 1] = perm[q];
+      --q;
+    }
+    perm[q + 1] = v;
+  }
+}
+
+// One thread per (node, species): 8-way merge by descending particle index.
+__global__ void __launch_bounds__(kT) k_dep_gather(ParticlesF64 p, GridDims g, const double *n2p,
+                                                    const int32_t *__restrict__ off,
+                                                    const int32_t *__restrict__ perm,
+                                                    double *__restrict__ Nraw) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= g.G * g.S) return;
+  const int s0 = (int)(t % g.S);
+  const int64_t node = t / g.S;
+  const int i = (int)(node % g.X) + 1;
+  const int j = (int)((node / g.X) % g.Y) + 1;
+  const int k = (int)(node / ((int64_t)g.X * g.Y)) + 1;
+  const int64_t K = (int64_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  int32_t head[8], end[8];
+#pragma unroll
+  for (int o = 0; o < 8; ++o) {
+    const int ox = o & 1, oy = (o >> 1) & 1, oz = (o >> 2) & 1;
+    const int ci = i - ox, cj = j - oy, ck = k - oz;
+    if (ci >= 1 && cj >= 1 && ck >= 1 && ci <= g.nx + 1 && cj <= g.ny + 1 && ck <= g.nz + 1) {
+      const int64_t b = (int64_t)s0 * K + (ci - 1) +
+                        (int64_t)(g.nx + 1) * ((cj - 1) + (int64_t)(g.ny + 1) * (ck - 1));
+      head[o] = off[b];
+      end[o] = off[b + 1];
+    } else {
+      head[o] = 0;
+      end[o] = 0;
+    }
+  }
+  const double w = n2p[s0];
+  double acc = 0.0;                      // N = 0.0d0 (ParticlesGridify.f90:59)
+  for (;;) {
+    int best = -1;
+    int32_t bestp = -1;
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      if (head[o] < end[o]) {
+        const int32_t pp = perm[head[o]];
+        if (pp > bestp) {
+          bestp = pp;
+          best = o;
+        }
+      }
+    }
+    if (best < 0) break;
+    ++head[best];
+    const CicLoc L = cic_locate(g, p.r[0][bestp], p.r[1][bestp], p.r[2][bestp]);
+    // node = idx + (ox, oy, oz): factor is rest where the offset is 1, drest where 0,
+    // which reproduces F1..F8 of ParticlesGridify.f90:81-88
+    const double fx = (best & 1) ? L.rest[0] : L.drest[0];
+    const double fy = (best & 2) ? L.rest[1] : L.drest[1];
+    const double fz = (best & 4) ? L.rest[2] : L.drest[2];
+    const double F = fx * fy * fz / g.dV;
+    acc = acc + F * w;
+  }
+  Nraw[node * g.S + s0] = acc;
+}
+
+// Guard handling of one dimension on one rank (ParticlesGridify.f90:129-285):
+// up to two source planes per node; *zero = the reference assigns 0.0d0.
+__device__ __forceinline__ int fold_src(int m, int n, int flow, int src[2], bool *zero) {
+  *zero = false;
+  if (flow) {
+    if (m == 1 || m == n + 2) { *zero = true; return 0; }
+    if (m == 2) { src[0] = 2; src[1] = 1; return 2; }
+    if (m == n + 1) { src[0] = n + 1; src[1] = n + 2; return 2; }
+  } else {
+    if (m == 1) { src[0] = 1; src[1] = n + 1; return 2; }
+    if (m == 2) { src[0] = 2; src[1] = n + 2; return 2; }
+    if (m == n + 1) { src[0] = n + 1; src[1] = 1; return 2; }
+    if (m == n + 2) { src[0] = n + 2; src[1] = 2; return 2; }
+  }
+  src[0] = m;
+  return 1;
+}
+
+// Folds z, then y, then x (the reference's order), then C = sum_s N(s)*charge(s).
+__global__ void __launch_bounds__(kT) k_fold_charge(GridDims g, const double *__restrict__ Nraw,
+                                                     double *__restrict__ N,
+                                                     double *__restrict__ C,
+                                                     const double *charge) {
+  const int64_t node = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (node >= g.G) return;
+  const int i = (int)(node % g.X) + 1;
+  const int j = (int)((node / g.X) % g.Y) + 1;
+  const int k = (int)(node / ((int64_t)g.X * g.Y)) + 1;
+  int sx[2], sy[2], sz[2];
+  bool zx, zy, zz;
+  const int nx_ = fold_src(i, g.nx, g.flow[0], sx, &zx);
+  const int ny_ = fold_src(j, g.ny, g.flow[1], sy, &zy);
+  const int nz_ = fold_src(k, g.nz, g.flow[2], sz, &zz);
+  double cacc = 0.0;                       // C = 0.0d0 (ParticlesGridify.f90:60)
+  for (int s = 0; s < g.S; ++s) {
+    double vx = 0.0;
+    if (!zx) {
+      for (int a = 0; a < nx_; ++a) {
+        double vy = 0.0;
+        if (!zy) {
+          for (int bb = 0; bb < ny_; ++bb) {
+            double vz = 0.0;
+            if (!zz) {
+              for (int cc = 0; cc < nz_; ++cc) {
+                const double v = Nraw[node_index(g, sx[a], sy[bb], sz[cc]) * g.S + s];
+                vz = cc == 0 ? v : vz + v;
+              }
+            }
+            vy = bb == 0 ? vz : vy + vz;
+          }
+        }
+        vx = a == 0 ? vy : vx + vy;
+      }
+    }
+    N[node * g.S + s] = vx;
+    cacc = cacc + vx * charge[s];          // ParticlesGridify.f90:288-293
+  }
+  C[node] = cacc;
+}
+
+// ---------------------------------------------------------------------------
+// CalcDistance (CalcDistance.f90:25-91), one thread per (ii, jj, kk).
+
+__device__ __forceinline__ double inv_len(double a, double b, double c) {
+  return 1.0 / sqrt(a * a + b * b + c * c);
+}
+
+__global__ void __launch_bounds__(kT) k_calc_distance(int Nx, int Ny, int Nz, double hx, double hy,
+                                                       double hz, int xf, int yf, int zf,
+                                                       double *__restrict__ D) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  const int64_t total = (int64_t)(Nx + 1) * (Ny + 1) * (Nz + 1);
+  if (t >= total) return;
+  const int ii = (int)(t % (Nx + 1)) + 1;
+  const int jj = (int)((t / (Nx + 1)) % (Ny + 1)) + 1;
+  const int kk = (int)(t / ((int64_t)(Nx + 1) * (Ny + 1))) + 1;
+  const double di = (double)(ii - 1), dj = (double)(jj - 1), dk = (double)(kk - 1);
+  double v = (ii == 1 && jj == 1 && kk == 1) ? 0.0 : inv_len(di * hx, dj * hy, dk * hz);
+  const double mx = ((double)Nx - di) * hx, my = ((double)Ny - dj) * hy, mz = ((double)Nz - dk) * hz;
+  if (xf) {
+    if (!yf && !zf) {
+      v = v + inv_len(di * hx, my, dk * hz);
+      v = v + inv_len(di * hx, dj * hy, mz);
+      v = v + inv_len(di * hx, my, mz);
+    } else if (!yf) {
+      v = v + inv_len(di * hx, my, dk * hz);
+    } else if (!zf) {
+      v = v + inv_len(di * hx, dj * hy, mz);
+    }
+  } else if (yf) {
+    if (!zf) {
+      v = v + inv_len(mx, dj * hy, dk * hz);
+      v = v + inv_len(di * hx, dj * hy, mz);
+      v = v + inv_len(mx, dj * hy, mz);
+    } else {
+      v = v + inv_len(mx, dj * hy, dk * hz);
+    }
+  } else if (zf) {
+    v = v + inv_len(mx, dj * hy, dk * hz);
+    v = v + inv_len(di * hx, my, dk * hz);
+    v = v + inv_len(mx, my, dk * hz);
+  } else {
+    v = v + inv_len(mx, dj * hy, dk * hz);
+    v = v + inv_len(di * hx, my, dk * hz);
+    v = v + inv_len(di * hx, dj * hy, mz);
+    v = v + inv_len(mx, my, dk * hz);
+    v = v + inv_len(di * hx, my, mz);
+    v = v + inv_len(mx, dj * hy, mz);
+    v = v + inv_len(mx, my, mz);
+  }
+  v = 1.0 * v;
+  const int64_t DX = 2 * (int64_t)Nx + 1, DY = 2 * (int64_t)Ny + 1;
+  const int pi = Nx + 1 + (ii - 1), pj = Ny + 1 + (jj - 1), pk = Nz + 1 + (kk - 1);
+  const int ni = Nx + 1 - (ii - 1), nj = Ny + 1 - (jj - 1), nk = Nz + 1 - (kk - 1);
+#define DAT(a, b, c) D[(int64_t)((a) - 1) + DX * ((int64_t)((b) - 1) + DY * (int64_t)((c) - 1))]
+  DAT(pi, pj, pk) = v; DAT(ni, pj, pk) = v; DAT(pi, nj, pk) = v; DAT(pi, pj, nk) = v;
+  DAT(ni, nj, pk) = v; DAT(pi, nj, nk) = v; DAT(ni, pj, nk) = v; DAT(ni, nj, nk) = v;
+#undef DAT
+}
+
+// ---------------------------------------------------------------------------
+// BCpotentials (BCpotentials.f90:51-141, one rank).  One lane per face node
+// sums C(interior) * D(window) sequentially in column-major order, which is
+// how amdflang evaluates the SUM of the array expression.
+
+constexpr int kFaceT = 64;
+
+__global__ void __launch_bounds__(kFaceT) k_bcpot_face(GridDims g, int dim,
+                                                        const double *__restrict__ C,
+                                                        const double *__restrict__ D,
+                                                        double *__restrict__ U, double kfac) {
+  const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
+  // transverse extents of this face pair
+  const int na = dim == 0 ? Ny : Nx;
+  const int nb = dim == 2 ? Ny : Nz;
+  const int64_t t = (int64_t)blockIdx.x * kFaceT + threadIdx.x;
+  if (t >= 2 * (int64_t)na * nb) return;
+  const int face = (int)(t / ((int64_t)na * nb));       // 0: low face, 1: high face
+  const int64_t rem = t % ((int64_t)na * nb);
+  const int aa = (int)(rem % na) + 1;                     // jj (x) or ii (y, z)
+  const int bb = (int)(rem / na) + 1;                     // kk (x, y) or jj (z)
+  int di, dj, dk;
+  if (dim == 0) {
+    dj = Ny + 1 + (1 - aa);
+    dk = Nz + 1 + (1 - bb);
+    di = face == 0 ? Nx + 1 + 1 : Nx + 1 - Nx;
+  } else if (dim == 1) {
+    di = Nx + 1 + (1 - aa);
+    dk = Nz + 1 + (1 - bb);
+    dj = face == 0 ? Ny + 1 + 1 : Ny + 1 - Ny;
+  } else {
+    di = Nx + 1 + (1 - aa);
+    dj = Ny + 1 + (1 - bb);
+    dk = face == 0 ? Nz + 1 + 1 : Nz + 1 - Nz;
+  }
+  const int64_t DX = 2 * (int64_t)Nx + 1, DY = 2 * (int64_t)Ny + 1;
+  double s = 0.0;
+  for (int k = 1; k <= Nz; ++k) {
+    for (int j = 1; j <= Ny; ++j) {
+      const double *crow = C + node_index(g, 2, j + 1, k + 1);
+      const double *drow = D + (int64_t)(di - 1) + DX * ((int64_t)(dj + j - 2) + DY * (int64_t)(dk + k - 2));
+      for (int i = 0; i < Nx; ++i) s = s + crow[i] * drow[i];
+    }
+  }
+  int64_t at;
+  if (dim == 0) at = node_index(g, face == 0 ? 1 : Nx + 2, aa + 1, bb + 1);
+  else if (dim == 1) at = node_index(g, aa + 1, face == 0 ? 1 : Ny + 2, bb + 1);
+  else at = node_index(g, aa + 1, bb + 1, face == 0 ? 1 : Nz + 2);
+  U[at] = s * kfac;
+}
+
+// ---------------------------------------------------------------------------
+// CalcEpotential (CalcEpotential.f90:28-242)
+//
+// Jacobi iterates U_q live in three rotating buffers jb[q % 3]; all three
+// start as copies of the input U (interior, flow faces, guards).  On one rank
+// the reference's periodic halo of iterate q >= 1 is a copy of iterate q's
+// interior at the wrapped position, and iterate 0 keeps whatever halo U came
+// in with; flow faces keep their input values throughout.
+//
+// Pass p computes U_p = J(U_{p-1}) and, for p >= 2, the residual r_{p-1} of
+// U_{p-1}, which the reference evaluates with the halo of U_{p-2} ("stale"
+// halo: CalcEpotential.f90:83-92 runs before the exchange of :100-239).  The
+// last block to finish a pass reduces the block sums in a fixed order and
+// applies the stop test of loop iteration ss = p (:60-62), so the decision is
+// taken one speculative sweep late and U_p is simply discarded on exit.
+
+struct JacobiCoef {
+  double ax, ay, az, af;
+  double dx2, dy2, dz2;
+};
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  // wave64 tree then across waves; fixed order, deterministic
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) s = s + sh[w];
+  }
+  return s;
+}
+
+__global__ void __launch_bounds__(kT) k_jacobi_bnorm(GridDims g, const double *__restrict__ C,
+                                                      double *__restrict__ b, double binv,
+                                                      double *__restrict__ partials,
+                                                      JacobiState *st, double maxerr) {
+  __shared__ double sh[kT / 64];
+  __shared__ bool amLast;
+  const int64_t node = (int64_t)blockIdx.x * kT + threadIdx.x;
+  double v2 = 0.0;
+  if (node < g.G) {
+    const double bv = -C[node] * binv;               // CalcEpotential.f90:50
+    b[node] = bv;
+    const int i = (int)(node % g.X) + 1;
+    const int j = (int)((node / g.X) % g.Y) + 1;
+    const int k = (int)(node / ((int64_t)g.X * g.Y)) + 1;
+    if (i >= 2 && i <= g.nx + 1 && j >= 2 && j <= g.ny + 1 && k >= 2 && k <= g.nz + 1)
+      v2 = bv * bv;
+  }
+  const double s = block_sum(v2, sh);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = s;
+    __threadfence();
+    const unsigned prev = atomicAdd(&st->arrive, 1u);
+    amLast = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (amLast && threadIdx.x == 0) {
+    __threadfence();
+    double tot = 0.0;
+    for (unsigned q = 0; q < gridDim.x; ++q) tot = tot + partials[q];
+    st->bsum = tot;
+    st->rsum = tot;                                  // r_global = b_global (:55)
+    st->thresh = maxerr * sqrt(tot);
+    st->arrive = 0;
+    // first stop test, ss = 1 (:60-62)
+    if (st->maxit >= 1 && sqrt(tot) <= st->thresh) {
+      st->done = 1;
+      st->result = 0;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kT) k_jacobi_pass(GridDims g, JacobiCoef jc,
+                                                     double *b0, double *b1, double *b2,
+                                                     const double *__restrict__ b,
+                                                     double *__restrict__ partials,
+                                                     JacobiState *st) {
+  __shared__ double sh[kT / 64];
+  __shared__ bool amLast;
+  if (st->done) return;
+  const int64_t p = st->pass;
+  double *bufs[3] = {b0, b1, b2};
+  const double *cur = bufs[(p - 1) % 3];           // U_{p-1}
+  double *out = bufs[p % 3];                        // U_p
+  const double *old = p >= 2 ? bufs[(p - 2) % 3] : nullptr;   // U_{p-2}
+  const bool wrap_cur = (p - 1) >= 1;               // halo of U_{p-1} = wrapped interior
+  const bool wrap_old = (p - 2) >= 1;
+
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
+  double r2 = 0.0;
+  if (t < ncell) {
+    const int i = (int)(t % g.nx) + 2;
+    const int j = (int)((t / g.nx) % g.ny) + 2;
+    const int k = (int)(t / ((int64_t)g.nx * g.ny)) + 2;
+    const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
+    const int64_t c = (int64_t)(i - 1) + X * ((int64_t)(j - 1) + (int64_t)g.Y * (k - 1));
+    const bool px = !g.flow[0], py = !g.flow[1], pz = !g.flow[2];
+    // neighbour indices for the update (halo of U_{p-1})
+    const int64_t xe = (px && wrap_cur && i == g.nx + 1) ? c - (int64_t)(g.nx - 1) : c + 1;
+    const int64_t xw = (px && wrap_cur && i == 2) ? c + (int64_t)(g.nx - 1) : c - 1;
+    const int64_t yn = (py && wrap_cur && j == g.ny + 1) ? c - (int64_t)(g.ny - 1) * X : c + X;
+    const int64_t ys = (py && wrap_cur && j == 2) ? c + (int64_t)(g.ny - 1) * X : c - X;
+    const int64_t zt = (pz && wrap_cur && k == g.nz + 1) ? c - (int64_t)(g.nz - 1) * XY : c + XY;
+    const int64_t zb = (pz && wrap_cur && k == 2) ? c + (int64_t)(g.nz - 1) * XY : c - XY;
+    const double ue = cur[xe], uw = cur[xw], un = cur[yn], us = cur[ys], ut = cur[zt], ub = cur[zb];
+    const double bc = b[c];
+    // CalcEpotential.f90:74-80
+    out[c] = jc.ax * (ue + uw) + jc.ay * (un + us) + jc.az * (ut + ub) - jc.af * bc;
+    if (p >= 2) {
+      // residual of U_{p-1} with the halo of U_{p-2}: periodic guard neighbours
+      // come from `old` (wrapped, or the stored input halo when p-2 == 0)
+      double re = ue, rw = uw, rn = un, rs = us, rt = ut, rb = ub;
+      if (px && i == g.nx + 1) re = wrap_old ? old[c - (int64_t)(g.nx - 1)] : old[c + 1];
+      if (px && i == 2) rw = wrap_old ? old[c + (int64_t)(g.nx - 1)] : old[c - 1];
+      if (py && j == g.ny + 1) rn = wrap_old ? old[c - (int64_t)(g.ny - 1) * X] : old[c + X];
+      if (py && j == 2) rs = wrap_old ? old[c + (int64_t)(g.ny - 1) * X] : old[c - X];
+      if (pz && k == g.nz + 1) rt = wrap_old ? old[c - (int64_t)(g.nz - 1) * XY] : old[c + XY];
+      if (pz && k == 2) rb = wrap_old ? old[c + (int64_t)(g.nz - 1) * XY] : old[c - XY];
+      const double u2 = 2.0 * cur[c];
+      // CalcEpotential.f90:83-92
+      const double res = bc - ((re + rw - u2) / jc.dx2 + (rn + rs - u2) / jc.dy2 +
+                               (rt + rb - u2) / jc.dz2);
+      r2 = res * res;
+    }
+  }
+  const double s = block_sum(r2, sh);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = s;
+    __threadfence();
+    const unsigned prev = atomicAdd(&st->arrive, 1u);
+    amLast = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (amLast && threadIdx.x == 0) {
+    __threadfence();
+    if (p >= 2) {
+      double tot = 0.0;
+      for (unsigned q = 0; q < gridDim.x; ++q) tot = tot + partials[q];
+      st->rsum = tot;
+      // stop test of loop iteration ss = p: exit with U_{p-1}
+      if (p <= st->maxit && sqrt(tot) <= st->thresh) {
+        st->done = 1;
+        st->result = p - 1;
+      }
+    }
+    if (!st->done && p == st->last) {
+      st->done = 1;
+      st->result = p;
+    }
+    st->pass = p + 1;
+    st->arrive = 0;
+  }
+}
+
+// After q >= 1 sweeps: U(node) = U_q at the composite periodic image of the
+// node (z, then y, then x copies of whole planes, CalcEpotential.f90:100-239);
+// nodes whose image lies on a flow face keep the input value found there.
+__global__ void __launch_bounds__(kT) k_jacobi_finish(GridDims g, double *b0, double *b1,
+                                                       double *b2, double *U, JacobiState *st) {
+  const int64_t node = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (node >= g.G) return;
+  const int64_t q = st->result;
+  if (q == 0) return;
+  const double *src_buf = q % 3 == 0 ? b0 : (q % 3 == 1 ? b1 : b2);
+  const int i = (int)(node % g.X) + 1;
+  const int j = (int)((node / g.X) % g.Y) + 1;
+  const int k = (int)(node / ((int64_t)g.X * g.Y)) + 1;
+  int si = i, sj = j, sk = k;
+  if (!g.flow[0]) si = i == 1 ? g.nx + 1 : (i == g.nx + 2 ? 2 : i);
+  if (!g.flow[1]) sj = j == 1 ? g.ny + 1 : (j == g.ny + 2 ? 2 : j);
+  if (!g.flow[2]) sk = k == 1 ? g.nz + 1 : (k == g.nz + 2 ? 2 : k);
+  const bool interior = si >= 2 && si <= g.nx + 1 && sj >= 2 && sj <= g.ny + 1 && sk >= 2 &&
+                        sk <= g.nz + 1;
+  const int64_t src = node_index(g, si, sj, sk);
+  if (interior) U[node] = src_buf[src];
+  else if (src != node) U[node] = src_buf[src];    // flow-face values are the same in all buffers
+}
+
+// ---------------------------------------------------------------------------
+// CalcEfield (CalcEfield.f90:27-228) + the on-rank halo copies: every node
+// takes the central difference at its composite periodic image, or zero when
+// that image lies in a flow guard (never filled on one rank).
+
+__global__ void __launch_bounds__(kT) k_efield(GridDims g, const double *__restrict__ U,
+                                                double *__restrict__ E) {
+  const int64_t node = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (node >= g.G) return;
+  const int i = (int)(node % g.X) + 1;
+  const int j = (int)((node / g.X) % g.Y) + 1;
+  const int k = (int)(node / ((int64_t)g.X * g.Y)) + 1;
+  int si = i, sj = j, sk = k;
+  if (!g.flow[0]) si = i == 1 ? g.nx + 1 : (i == g.nx + 2 ? 2 : i);
+  if (!g.flow[1]) sj = j == 1 ? g.ny + 1 : (j == g.ny + 2 ? 2 : j);
+  if (!g.flow[2]) sk = k == 1 ? g.nz + 1 : (k == g.nz + 2 ? 2 : k);
+  double ex = 0.0, ey = 0.0, ez = 0.0;
+  if (si >= 2 && si <= g.nx + 1 && sj >= 2 && sj <= g.ny + 1 && sk >= 2 && sk <= g.nz + 1) {
+    const int64_t c = node_index(g, si, sj, sk);
+    const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
+    ez = -(U[c + XY] - U[c - XY]) * 0.5 / g.dxyz[2];
+    ey = -(U[c + X] - U[c - X]) * 0.5 / g.dxyz[1];
+    ex = -(U[c + 1] - U[c - 1]) * 0.5 / g.dxyz[0];
+  }
+  E[3 * node + 0] = ex;
+  E[3 * node + 1] = ey;
+  E[3 * node + 2] = ez;
+}
+
+// BCfields.f90:31-90 on one rank: zero the guard planes of flow faces.
+__global__ void __launch_bounds__(kT) k_bc_fields(GridDims g, double *__restrict__ E) {
+  const int64_t node = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (node >= g.G) return;
+  const int i = (int)(node % g.X) + 1;
+  const int j = (int)((node / g.X) % g.Y) + 1;
+  const int k = (int)(node / ((int64_t)g.X * g.Y)) + 1;
+  const bool z = (g.flow[2] && (k == 1 || k == g.Z)) || (g.flow[1] && (j == 1 || j == g.Y)) ||
+                 (g.flow[0] && (i == 1 || i == g.X));
+  if (z) {
+    E[3 * node + 0] = 0.0;
+    E[3 * node + 1] = 0.0;
+    E[3 * node + 2] = 0.0;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+int launch_gridify(picard_ctx *c) {
+  const GridDims &g = c->gd;
+  const int64_t n = c->n;
+  const int64_t nb = (int64_t)g.S * c->K;
+  PICARD_HIP(hipMemsetAsync(c->s.hist, 0, sizeof(int32_t) * (nb + 1), c->stream));
+  PICARD_HIP(hipMemsetAsync(c->s.cursor, 0, sizeof(int32_t) * nb, c->stream));
+  PICARD_HIP(hipMemsetAsync(c->s.counters, 0, sizeof(int32_t), c->stream));
+  if (n > 0) {
+    k_dep_key<<<grid_blocks(n, kT), kT, 0, c->stream>>>(c->p, n, g, c->s.key, c->s.hist,
+                                                        c->s.counters);
+    PICARD_HIP(hipGetLastError());
+  }
+  PICARD_TRY(scan_exclusive_i32(c, c->s.hist, c->s.hist, nb));
+  if (n > 0) {
+    k_dep_scatter<<<grid_blocks(n, kT), kT, 0, c->stream>>>(n, c->s.key, c->s.hist, c->s.cursor,
+                                                            c->s.perm);
+    PICARD_HIP(hipGetLastError());
+    k_dep_bucket_sort<<<grid_blocks(nb, kT), kT, 0, c->stream>>>(nb, c->s.hist, c->s.perm);
+    PICARD_HIP(hipGetLastError());
+  }
+  k_dep_gather<<<grid_blocks(g.G * g.S, kT), kT, 0, c->stream>>>(c->p, g, c->consts, c->s.hist,
+                                                                   c->s.perm, c->Nraw);
+  PICARD_HIP(hipGetLastError());
+  k_fold_charge<<<grid_blocks(g.G, kT), kT, 0, c->stream>>>(g, c->Nraw, c->N, c->C,
+                                                            c->consts + 8);
+  PICARD_HIP(hipGetLastError());
+  PICARD_HIP(hipMemcpyAsync(c->host_counters, c->s.counters, sizeof(int32_t),
+                            hipMemcpyDeviceToHost, c->stream));
+  PICARD_HIP(hipStreamSynchronize(c->stream));
+  if (c->host_counters[0] != 0) {
+    set_error("ParticlesGridify: " + std::to_string(c->host_counters[0]) +
+              " particle(s) outside the grid arrays were not deposited");
+    return PICARD_ERR_RANGE;
+  }
+  return PICARD_OK;
+}
+
+int launch_calc_distance(picard_ctx *c) {
+  const GridDims &g = c->gd;
+  const int64_t total = (int64_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  const int64_t DN = (2 * (int64_t)g.nx + 1) * (2 * (int64_t)g.ny + 1) * (2 * (int64_t)g.nz + 1);
+  PICARD_HIP(hipMemsetAsync(c->D, 0, sizeof(double) * DN, c->stream));
+  k_calc_distance<<<grid_blocks(total, kT), kT, 0, c->stream>>>(
+      g.nx, g.ny, g.nz, g.dxyz[0], g.dxyz[1], g.dxyz[2], g.flow[0], g.flow[1], g.flow[2], c->D);
+  PICARD_HIP(hipGetLastError());
+  c->have_D = true;
+  return PICARD_OK;
+}
+
+int launch_bc_potentials(picard_ctx *c) {
+  const GridDims &g = c->gd;
+  if (!c->have_D) {
+    set_error("BCpotentials: the distance table is not set (call CalcDistance first)");
+    return PICARD_ERR_STATE;
+  }
+  PICARD_HIP(hipMemsetAsync(c->U, 0, sizeof(double) * g.G, c->stream));   // :36
+  const double tau = 6.2831853071795864769252867665590;
+  const double eps0 = 8.854187817e-12;
+  const double kfac = 1.0 / (2.0 * tau * eps0);
+  for (int dim = 0; dim < 3; ++dim) {
+    if (!g.flow[dim]) continue;
+    const int na = dim == 0 ? g.ny : g.nx;
+    const int nb = dim == 2 ? g.ny : g.nz;
+    const int64_t lanes = 2 * (int64_t)na * nb;
+    k_bcpot_face<<<grid_blocks(lanes, kFaceT), kFaceT, 0, c->stream>>>(g, dim, c->C, c->D, c->U,
+                                                                       kfac);
+    PICARD_HIP(hipGetLastError());
+  }
+  return PICARD_OK;
+}
+
+int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) {
+  const GridDims &g = c->gd;
+  const double dx = g.dxyz[0], dy = g.dxyz[1], dz = g.dxyz[2];
+  JacobiCoef jc;
+  const double den = 1.0 / ((dx * dy) * (dx * dy) + (dx * dz) * (dx * dz) + (dy * dz) * (dy * dz));
+  jc.ax = 0.5 * den * ((dy * dz) * (dy * dz));            // CalcEpotential.f90:44-47
+  jc.ay = 0.5 * den * ((dx * dz) * (dx * dz));
+  jc.az = 0.5 * den * ((dx * dy) * (dx * dy));
+  jc.af = 0.5 * den * ((dx * dy * dz) * (dx * dy * dz));
+  jc.dx2 = dx * dx;
+  jc.dy2 = dy * dy;
+  jc.dz2 = dz * dz;
+  const double eps0 = 8.854187817e-12;
+  const double binv = 1.0 / (g.dV * eps0);
+
+  JacobiState init{};
+  init.pass = 1;
+  init.maxit = maxit;
+  init.last = 2 * (int64_t)maxit + 1;
+  PICARD_HIP(hipMemcpyAsync(c->js, &init, sizeof(JacobiState), hipMemcpyHostToDevice, c->stream));
+  for (int q = 0; q < 3; ++q)
+    PICARD_HIP(hipMemcpyAsync(c->jb[q], c->U, sizeof(double) * g.G, hipMemcpyDeviceToDevice,
+                              c->stream));
+  const int nbn = grid_blocks(g.G, kT);
+  k_jacobi_bnorm<<<nbn, kT, 0, c->stream>>>(g, c->C, c->b, binv, c->s.partials, c->js, maxerr);
+  PICARD_HIP(hipGetLastError());
+
+  const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
+  const int nbp = grid_blocks(ncell, kT);
+  const int64_t last = init.last;
+  int64_t launched = 0;
+  // first batch from the previous solve's count (cold starts are alike from
+  // step to step), then short batches: few no-op passes after convergence
+  int64_t batch = c->last_sweeps > 64 ? (c->last_sweeps * 15) / 16 : 32;
+  const int64_t kMaxBatch = int64_t(1) << 16;
+  if (batch > kMaxBatch) batch = kMaxBatch;
+  for (;;) {
+    const int64_t todo = (last - launched) < batch ? (last - launched) : batch;
+    if (c->timing) {
+      const int64_t need = 2 * todo;
+      while ((int64_t)c->tev.size() < need) {
+        hipEvent_t e;
+        PICARD_HIP(hipEventCreate(&e));
+        c->tev.push_back(e);
+      }
+      for (int64_t q = 0; q < todo; ++q)
+        hipExtLaunchKernelGGL(k_jacobi_pass, dim3(nbp), dim3(kT), 0, c->stream, c->tev[2 * q],
+                              c->tev[2 * q + 1], 0, g, jc, c->jb[0], c->jb[1], c->jb[2],
+                              (const double *)c->b, c->s.partials, c->js);
+    } else {
+      for (int64_t q = 0; q < todo; ++q)
+        k_jacobi_pass<<<nbp, kT, 0, c->stream>>>(g, jc, c->jb[0], c->jb[1], c->jb[2], c->b,
+                                                 c->s.partials, c->js);
+    }
+    PICARD_HIP(hipGetLastError());
+    launched += todo;
+    PICARD_HIP(hipMemcpyAsync(c->js_host, c->js, sizeof(JacobiState), hipMemcpyDeviceToHost,
+                              c->stream));
+    PICARD_HIP(hipStreamSynchronize(c->stream));
+    if (c->timing) {
+      for (int64_t q = 0; q < todo; ++q) {
+        float ms = 0.0f;
+        PICARD_HIP(hipEventElapsedTime(&ms, c->tev[2 * q], c->tev[2 * q + 1]));
+        c->t_ms += ms;
+      }
+      c->t_launches += todo;
+    }
+    if (c->js_host->done) break;
+    if (launched >= last) {
+      set_error("CalcEpotential: pass accounting out of step");
+      return PICARD_ERR_STATE;
+    }
+    batch = 64;
+  }
+  c->last_sweeps = c->js_host->result;
+  k_jacobi_finish<<<nbn, kT, 0, c->stream>>>(g, c->jb[0], c->jb[1], c->jb[2], c->U, c->js);
+  PICARD_HIP(hipGetLastError());
+  *sweeps = c->js_host->result;
+  // Running all 2*maxit+1 sweeps means every stop test failed; the reference
+  // prints 'Maximum iteration reached!' on each sweep past maxit (:67-69).
+  return c->js_host->result == last ? PICARD_WARN_NOCONV : PICARD_OK;
+}
+
+int launch_calc_efield(picard_ctx *c) {
+  k_efield<<<grid_blocks(c->gd.G, kT), kT, 0, c->stream>>>(c->gd, c->U, c->E);
+  PICARD_HIP(hipGetLastError());
+  return PICARD_OK;
+}
+
+int launch_bc_fields(picard_ctx *c) {
+  k_bc_fields<<<grid_blocks(c->gd.G, kT), kT, 0, c->stream>>>(c->gd, c->E);
+  PICARD_HIP(hipGetLastError());
+  return PICARD_OK;
+}
+
+}  // namespace picard

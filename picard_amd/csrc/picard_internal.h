@@ -1,0 +1,153 @@
+// picard_internal.h -- device context and kernel launchers shared by the
+// translation units of libpicard_hip.so.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/picard_hip.h"
+
+namespace picard {
+
+void set_error(const std::string &msg);
+
+#define PICARD_HIP(call)                                                            \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      ::picard::set_error(std::string(#call) + ": " + hipGetErrorString(e_));       \
+      return PICARD_ERR_HIP;                                                        \
+    }                                                                               \
+  } while (0)
+
+#define PICARD_TRY(expr)            \
+  do {                              \
+    int rc_ = (expr);               \
+    if (rc_ < 0) return rc_;        \
+  } while (0)
+
+// Geometry constants passed by value to every grid kernel.
+struct GridDims {
+  int nx, ny, nz;            // interior cells
+  int X, Y, Z;               // with guards: n + 2
+  int64_t G;                 // X*Y*Z nodes
+  int S;                     // species
+  int flow[3];
+  double lmin[3], lmax[3], gmin[3], gmax[3], dxyz[3];
+  double dV;
+};
+
+// Particle arrays, structure of arrays.  r[0..2] position, r[3..5] velocity,
+// r[6], r[7] the reference's slots 7 and 8 (carried by every move).
+struct ParticlesF64 {
+  double *r[8];
+  int32_t *sp;
+};
+
+// Device-side scalars of the Jacobi solve (one per context).
+struct JacobiState {
+  double bsum;               // sum of b^2 over the interior
+  double thresh;             // maxerr * sqrt(bsum)
+  double rsum;               // last residual sum
+  int64_t pass;              // next pass p (1-based): computes iterate p
+  int64_t result;            // sweeps performed, valid when done
+  int64_t last;              // 2*maxit + 1
+  int32_t maxit;
+  int32_t done;
+  uint32_t arrive;           // block arrival counter for the last-block reduction
+  uint32_t pad;
+};
+
+struct DeviceScratch {
+  // generic
+  int32_t *flags = nullptr;        // per particle
+  int32_t *scan = nullptr;         // per particle (+1)
+  int32_t *work[6] = {};           // per particle each: outflow rotation/stream buffers
+  int32_t *scan_tmp = nullptr;     // block sums of the recursive scan
+  int64_t scan_tmp_len = 0;
+  int32_t *counters = nullptr;     // small device counters (errors, totals)
+  // deposit (reference order)
+  int32_t *key = nullptr;          // per particle
+  int32_t *perm = nullptr;         // per particle
+  int32_t *hist = nullptr;         // S*K + 1
+  int32_t *cursor = nullptr;       // S*K
+  // staging for uploads/downloads (AoS)
+  double *stage = nullptr;
+  int32_t *stage_sp = nullptr;
+  int64_t stage_cap = 0;
+  // reductions
+  double *partials = nullptr;
+  int64_t partials_len = 0;
+};
+
+}  // namespace picard
+
+struct picard_ctx {
+  picard_geometry geo;
+  picard_species spc;
+  picard::GridDims gd;
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  int64_t cap = 0;           // max particles
+  int64_t n = 0;             // num_local (authoritative on the host)
+  picard::ParticlesF64 p{};
+
+  // grids
+  double *U = nullptr, *E = nullptr, *N = nullptr, *Nraw = nullptr, *C = nullptr, *D = nullptr;
+  double *jb[3] = {nullptr, nullptr, nullptr};   // Jacobi iterate buffers
+  double *b = nullptr;
+  double *consts = nullptr;  // device: n2p[8], charge[8]
+  int64_t K = 0;             // deposit cells per species: (nx+1)(ny+1)(nz+1)
+  bool have_D = false;
+
+  picard::JacobiState *js = nullptr;   // device
+  picard::JacobiState *js_host = nullptr;   // pinned mirror
+  int32_t *host_counters = nullptr;    // pinned mirror of scratch.counters
+
+  picard::DeviceScratch s;
+
+  // solve-to-solve predictor of the Jacobi sweep count (sizes the first batch)
+  int64_t last_sweeps = 0;
+
+  // kernel timing of the Jacobi sweep (picard_timing_enable): event pairs
+  // stamped by the dispatch packets themselves
+  bool timing = false;
+  std::vector<hipEvent_t> tev;
+  double t_ms = 0.0;
+  int64_t t_launches = 0;
+};
+
+namespace picard {
+
+// ---- particles.hip ----
+int launch_push_x(picard_ctx *c, int64_t first, int64_t count, double dt);
+int launch_outflow(picard_ctx *c, int64_t *n_new, int64_t *removed);
+int launch_wrap(picard_ctx *c);
+int launch_boris(picard_ctx *c, double dt);
+int launch_append(picard_ctx *c, const double *host_rp, const int32_t *host_sp, int64_t count);
+int launch_read(picard_ctx *c, int64_t first, int64_t count, double *host_rp, int32_t *host_sp);
+
+// ---- grid.hip ----
+int launch_gridify(picard_ctx *c);
+int launch_calc_distance(picard_ctx *c);
+int launch_bc_potentials(picard_ctx *c);
+int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps);
+int launch_calc_efield(picard_ctx *c);
+int launch_bc_fields(picard_ctx *c);
+
+// ---- scan.hip ----
+// Exclusive prefix sum of n int32 values (in may alias out); out[n] = total.
+int scan_exclusive_i32(picard_ctx *c, const int32_t *in, int32_t *out, int64_t n);
+int64_t scan_tmp_required(int64_t n);
+
+inline int grid_blocks(int64_t n, int threads) {
+  int64_t b = (n + threads - 1) / threads;
+  return (int)(b < 1 ? 1 : b);
+}
+
+}  // namespace picard

@@ -1,0 +1,287 @@
+! picard_dropin.f90 -- the reference's time-step subroutines, with their exact
+! names and argument lists, running on the MI355X through libpicard_hip.so.
+!
+! A host written against the reference's call surface (picard.f90:564-631)
+! links these in place of AdvancePosition.f90, BCoutflow.f90,
+! ParticlesTransfer.f90, ParticlesGridify.f90, CalcDistance.f90,
+! BCpotentials.f90, CalcEpotential.f90, CalcEfield.f90, BCfields.f90 and
+! AdvanceVelocity.f90.
+!
+! Residency: the host calls picard_dropin_init once with its particle and
+! field arrays (static in the main program, as in picard.f90:91-113).  The
+! device keeps particles and fields between calls.  Particles the host appends
+! after num_local (BCinflow, ParticlesProduce) are uploaded by the next call;
+! host copies of device state are refreshed only by picard_dropin_download,
+! or after every call when eager_download is set.
+module picard_dropin_state
+  use iso_c_binding
+  use picard_hip_iface
+  implicit none
+
+  type(c_ptr), save :: ctx = c_null_ptr
+  integer(c_int64_t), save :: dev_count = 0        ! particles resident on the device
+  type(c_ptr), save :: host_rp = c_null_ptr        ! registered real_particles(8, max_per_proc)
+  type(c_ptr), save :: host_sp = c_null_ptr        ! registered int_particles(max_per_proc)
+  integer, save :: host_max = 0
+  logical, save :: eager_download = .false.
+  integer(c_int64_t), save :: last_sweeps = 0
+  integer(c_int64_t), save :: last_removed = 0
+
+contains
+
+  subroutine picard_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    if (rc < 0) then
+      write(*,'(a,a,a,i0,a,a)') 'picard: ', where, ' failed (status ', rc, '): ', picard_error_message()
+      error stop 1
+    end if
+  end subroutine picard_check
+
+  ! Create the device domain and register the host particle arrays.
+  subroutine picard_dropin_init(geo, spc, real_particles, int_particles, max_per_proc)
+    type(picard_geometry), intent(in) :: geo
+    type(picard_species), intent(in) :: spc
+    integer, intent(in) :: max_per_proc
+    real*8, intent(in), target :: real_particles(8, max_per_proc)
+    integer, intent(in), target :: int_particles(max_per_proc)
+    if (c_associated(ctx)) call picard_check(picard_destroy(ctx), 'picard_destroy')
+    call picard_check(picard_create(geo, spc, ctx), 'picard_create')
+    host_rp = c_loc(real_particles)
+    host_sp = c_loc(int_particles)
+    host_max = max_per_proc
+    dev_count = 0
+  end subroutine picard_dropin_init
+
+  subroutine picard_dropin_finalize()
+    if (c_associated(ctx)) call picard_check(picard_destroy(ctx), 'picard_destroy')
+    ctx = c_null_ptr
+  end subroutine picard_dropin_finalize
+
+  ! Upload what the host appended after the device's count.
+  subroutine picard_dropin_sync(num_local)
+    integer, intent(in) :: num_local
+    real(c_double), pointer :: rp(:,:)
+    integer(c_int32_t), pointer :: sp(:)
+    integer(c_int64_t) :: first, count
+    if (.not. c_associated(ctx)) then
+      write(*,*) 'picard: call picard_dropin_init before the time-step subroutines'
+      error stop 1
+    end if
+    if (num_local > host_max) then
+      write(*,*) 'picard: num_local exceeds max_per_proc'
+      error stop 1
+    end if
+    if (num_local < dev_count) then
+      write(*,*) 'picard: num_local shrank on the host; the device holds', dev_count
+      error stop 1
+    end if
+    count = int(num_local, c_int64_t) - dev_count
+    if (count == 0) return
+    call c_f_pointer(host_rp, rp, [8, host_max])
+    call c_f_pointer(host_sp, sp, [host_max])
+    first = dev_count + 1
+    call picard_check(picard_particles_append(ctx, c_loc(rp(1, first)), c_loc(sp(first)), count), &
+                      'particles_append')
+    dev_count = int(num_local, c_int64_t)
+  end subroutine picard_dropin_sync
+
+  ! Copy device particles back into the registered host arrays.
+  subroutine picard_dropin_download_particles()
+    real(c_double), pointer :: rp(:,:)
+    integer(c_int32_t), pointer :: sp(:)
+    if (dev_count == 0) return
+    call c_f_pointer(host_rp, rp, [8, host_max])
+    call c_f_pointer(host_sp, sp, [host_max])
+    call picard_check(picard_particles_read(ctx, 0_c_int64_t, dev_count, c_loc(rp), c_loc(sp)), &
+                      'particles_read')
+  end subroutine picard_dropin_download_particles
+
+  subroutine picard_dropin_download_field(which, a, n)
+    integer(c_int32_t), intent(in) :: which
+    integer(c_int64_t), intent(in) :: n
+    real(c_double), intent(inout), target :: a(n)
+    call picard_check(picard_field_read(ctx, which, c_loc(a), n), 'field_read')
+  end subroutine picard_dropin_download_field
+
+end module picard_dropin_state
+
+
+! AdvancePosition.f90:4
+subroutine AdvancePosition(real_particles, dt, max_per_proc, num_local)
+  use picard_dropin_state
+  implicit none
+  integer, intent(in) :: max_per_proc, num_local
+  real*8, intent(inout) :: real_particles(8, max_per_proc)
+  real*8, intent(in) :: dt
+  call picard_dropin_sync(num_local)
+  call picard_check(picard_advance_position(ctx, dt), 'AdvancePosition')
+  if (eager_download) call picard_dropin_download_particles()
+end subroutine AdvancePosition
+
+
+! BCoutflow.f90:5
+subroutine BCoutflow(real_particles, int_particles, Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, Lz_max, &
+                     xmin, xmax, ymin, ymax, zmin, zmax, max_per_proc, num_local, xflow, yflow, zflow)
+  use picard_dropin_state
+  implicit none
+  logical, intent(in) :: xflow, yflow, zflow
+  real*8, intent(in) :: Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, Lz_max, xmin, xmax, ymin, ymax, zmin, zmax
+  integer, intent(in) :: max_per_proc
+  real*8, intent(inout) :: real_particles(8, max_per_proc)
+  integer, intent(inout) :: int_particles(max_per_proc), num_local
+  integer(c_int64_t) :: n_new, removed
+  call picard_dropin_sync(num_local)
+  call picard_check(picard_bc_outflow(ctx, n_new, removed), 'BCoutflow')
+  num_local = int(n_new)
+  dev_count = n_new
+  last_removed = removed
+  if (eager_download) call picard_dropin_download_particles()
+end subroutine BCoutflow
+
+
+! ParticlesTransfer.f90:2 (one domain)
+subroutine ParticlesTransfer(real_particles, int_particles, Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, Lz_max, &
+                             xmin, xmax, ymin, ymax, zmin, zmax, max_per_proc, num_local, iprocs, jprocs, &
+                             kprocs, myid)
+  use picard_dropin_state
+  implicit none
+  integer, intent(in) :: max_per_proc, iprocs, jprocs, kprocs, myid
+  real*8, intent(in) :: Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, Lz_max, xmin, xmax, ymin, ymax, zmin, zmax
+  integer, intent(inout) :: int_particles(max_per_proc), num_local
+  real*8, intent(inout) :: real_particles(8, max_per_proc)
+  if (iprocs * jprocs * kprocs /= 1) then
+    write(*,*) 'picard: this build runs one domain per GPU (iprocs = jprocs = kprocs = 1)'
+    error stop 1
+  end if
+  call picard_dropin_sync(num_local)
+  call picard_check(picard_particles_transfer(ctx), 'ParticlesTransfer')
+  if (eager_download) call picard_dropin_download_particles()
+end subroutine ParticlesTransfer
+
+
+! ParticlesGridify.f90:1
+subroutine ParticlesGridify(N, C, real_particles, int_particles, Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, &
+                            Lz_max, dxyz, xmin, xmax, ymin, ymax, zmin, zmax, species, Nx_local, Ny_local, &
+                            Nz_local, max_per_proc, num_local, xflow, yflow, zflow, iprocs, jprocs, kprocs, &
+                            charge, n2p, myid)
+  use picard_dropin_state
+  implicit none
+  integer, intent(in) :: species, Nx_local, Ny_local, Nz_local, max_per_proc, iprocs, jprocs, kprocs, myid, &
+                         num_local
+  logical, intent(in) :: xflow, yflow, zflow
+  real*8, intent(in) :: Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, Lz_max, dxyz(3), xmin, xmax, ymin, ymax, &
+                        zmin, zmax, charge(8), n2p(8)
+  integer, intent(in) :: int_particles(max_per_proc)
+  real*8, intent(in) :: real_particles(8, max_per_proc)
+  real*8, intent(out) :: N(species, Nx_local+2, Ny_local+2, Nz_local+2)
+  real*8, intent(out) :: C(Nx_local+2, Ny_local+2, Nz_local+2)
+  integer(c_int64_t) :: G
+  call picard_dropin_sync(num_local)
+  call picard_check(picard_particles_gridify(ctx), 'ParticlesGridify')
+  if (eager_download) then
+    G = int(Nx_local+2, c_int64_t) * (Ny_local+2) * (Nz_local+2)
+    call picard_dropin_download_field(PICARD_FIELD_N, N, species * G)
+    call picard_dropin_download_field(PICARD_FIELD_C, C, G)
+  end if
+end subroutine ParticlesGridify
+
+
+! CalcDistance.f90:1 (global Nx = Nx_local on one domain)
+subroutine CalcDistance(D, Nx, Ny, Nz, dxyz, xflow, yflow, zflow)
+  use picard_dropin_state
+  implicit none
+  logical, intent(in) :: xflow, yflow, zflow
+  integer, intent(in) :: Nx, Ny, Nz
+  real*8, intent(in) :: dxyz(3)
+  real*8, intent(out) :: D(2*Nx+1, 2*Ny+1, 2*Nz+1)
+  call picard_check(picard_calc_distance(ctx), 'CalcDistance')
+  if (eager_download) &
+    call picard_dropin_download_field(PICARD_FIELD_D, D, int(2*Nx+1, c_int64_t) * (2*Ny+1) * (2*Nz+1))
+end subroutine CalcDistance
+
+
+! BCpotentials.f90:1
+subroutine BCpotentials(U, C, D, xmin, xmax, ymin, ymax, zmin, zmax, Nx_local, Ny_local, Nz_local, Nx, Ny, Nz, &
+                        xflow, yflow, zflow, species, iprocs, jprocs, kprocs, myid)
+  use picard_dropin_state
+  implicit none
+  logical, intent(in) :: xflow, yflow, zflow
+  integer, intent(in) :: Nx_local, Ny_local, Nz_local, Nx, Ny, Nz, species, iprocs, jprocs, kprocs, myid
+  real*8, intent(in) :: xmin, xmax, ymin, ymax, zmin, zmax
+  real*8, intent(in) :: D(2*Nx+1, 2*Ny+1, 2*Nz+1)
+  real*8, intent(in) :: C(Nx_local+2, Ny_local+2, Nz_local+2)
+  real*8, intent(inout) :: U(Nx_local+2, Ny_local+2, Nz_local+2)
+  call picard_check(picard_bc_potentials(ctx), 'BCpotentials')
+  if (eager_download) &
+    call picard_dropin_download_field(PICARD_FIELD_U, U, int(Nx_local+2, c_int64_t) * (Ny_local+2) * (Nz_local+2))
+end subroutine BCpotentials
+
+
+! CalcEpotential.f90:4
+subroutine CalcEpotential(U, C, Nx_local, Ny_local, Nz_local, Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, Lz_max, &
+                          xflow, yflow, zflow, xmin, xmax, ymin, ymax, zmin, zmax, dxyz, maxerr, maxit, &
+                          iprocs, jprocs, kprocs, species, myid)
+  use picard_dropin_state
+  implicit none
+  logical, intent(in) :: xflow, yflow, zflow
+  integer, intent(in) :: Nx_local, Ny_local, Nz_local, maxit, iprocs, jprocs, kprocs, species, myid
+  real*8, intent(in) :: Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, Lz_max, xmin, xmax, ymin, ymax, zmin, zmax, &
+                        maxerr, dxyz(3)
+  real*8, intent(in) :: C(Nx_local+2, Ny_local+2, Nz_local+2)
+  real*8, intent(inout) :: U(Nx_local+2, Ny_local+2, Nz_local+2)
+  integer(c_int) :: rc
+  rc = picard_calc_epotential(ctx, maxerr, int(maxit, c_int32_t), last_sweeps)
+  call picard_check(rc, 'CalcEpotential')
+  if (rc == PICARD_WARN_NOCONV) write(*,*) 'Maximum iteration reached!'
+  if (eager_download) &
+    call picard_dropin_download_field(PICARD_FIELD_U, U, int(Nx_local+2, c_int64_t) * (Ny_local+2) * (Nz_local+2))
+end subroutine CalcEpotential
+
+
+! CalcEfield.f90:3
+subroutine CalcEfield(E, U, Nx_local, Ny_local, Nz_local, dxyz, xflow, yflow, zflow, iprocs, jprocs, kprocs, myid)
+  use picard_dropin_state
+  implicit none
+  logical, intent(in) :: xflow, yflow, zflow
+  integer, intent(in) :: Nx_local, Ny_local, Nz_local, iprocs, jprocs, kprocs, myid
+  real*8, intent(in) :: U(Nx_local+2, Ny_local+2, Nz_local+2), dxyz(3)
+  real*8, intent(out) :: E(3, Nx_local+2, Ny_local+2, Nz_local+2)
+  call picard_check(picard_calc_efield(ctx), 'CalcEfield')
+  if (eager_download) &
+    call picard_dropin_download_field(PICARD_FIELD_E, E, 3_c_int64_t * (Nx_local+2) * (Ny_local+2) * (Nz_local+2))
+end subroutine CalcEfield
+
+
+! BCfields.f90:3
+subroutine BCfields(E, U, B0, E0, Nx_local, Ny_local, Nz_local, xflow, yflow, zflow, iprocs, jprocs, kprocs, myid)
+  use picard_dropin_state
+  implicit none
+  logical, intent(in) :: xflow, yflow, zflow
+  integer, intent(in) :: Nx_local, Ny_local, Nz_local, iprocs, jprocs, kprocs, myid
+  real*8 :: B0(4), E0(4)
+  real*8, intent(in) :: U(Nx_local+2, Ny_local+2, Nz_local+2)
+  real*8, intent(inout) :: E(3, Nx_local+2, Ny_local+2, Nz_local+2)
+  call picard_check(picard_bc_fields(ctx), 'BCfields')
+  if (eager_download) &
+    call picard_dropin_download_field(PICARD_FIELD_E, E, 3_c_int64_t * (Nx_local+2) * (Ny_local+2) * (Nz_local+2))
+end subroutine BCfields
+
+
+! AdvanceVelocity.f90:6
+subroutine AdvanceVelocity(real_particles, int_particles, E, B0, Lx_min, Ly_min, Lz_min, dxyz, dt, v0, ve, eta, &
+                           tB, sB, num_local, max_per_proc, species, Nx_local, Ny_local, Nz_local, fluid, gyration)
+  use picard_dropin_state
+  implicit none
+  logical, intent(in) :: fluid(8), gyration(8)
+  integer, intent(in) :: num_local, max_per_proc, species, Nx_local, Ny_local, Nz_local
+  real*8, intent(in) :: Lx_min, Ly_min, Lz_min, dxyz(3), dt, v0(4,8), ve(4), eta(8)
+  real*8, intent(in) :: tB(4,8), sB(4,8)
+  real*8, intent(inout) :: real_particles(8, max_per_proc)
+  integer, intent(in) :: int_particles(max_per_proc)
+  real*8, intent(in) :: B0(4)
+  real*8, intent(in) :: E(3, Nx_local+2, Ny_local+2, Nz_local+2)
+  call picard_dropin_sync(num_local)
+  call picard_check(picard_advance_velocity(ctx, dt), 'AdvanceVelocity')
+  if (eager_download) call picard_dropin_download_particles()
+end subroutine AdvanceVelocity

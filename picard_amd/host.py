@@ -1,0 +1,465 @@
+"""Host side of the PIC time step: run configuration, the derived constants of
+picard.f90, the host particle sources, and a device-resident Domain that calls
+the C ABI in the reference's main-loop order.
+
+Layouts match the reference (see include/picard_hip.h): particles are
+``float64[n, 8]`` + ``int32[n]`` (real_particles(8,n), int_particles(n)); grids
+are C-order ``[Nz+2, Ny+2, Nx+2(, comp)]`` views of the Fortran column-major
+arrays."""
+from __future__ import annotations
+
+import ctypes as ct
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+# physical constants, picard.f90:56-66
+TAU = 6.2831853071795864769252867665590
+EC = 1.60217646e-19
+MP = 1.6726216e-27
+ME = 9.1093819e-31
+EPS0 = 8.854187817e-12
+KB = 1.380650e-23
+MU0 = 1.25663706143591729538505735331180e-6
+
+
+@dataclass
+class Config:
+    """Run parameters: picard.f90 sections 1.1, 2.1.0 and 2.1.1 as data."""
+    nx: int = 32
+    ny: int = 32
+    nz: int = 32
+    species: int = 2
+    flow: tuple = (True, False, False)
+    gmin: tuple = (-160.0, -160.0, -160.0)
+    gmax: tuple = (160.0, 160.0, 160.0)
+    dt: float | None = None          # None: dt_wpe / omega_pe
+    dt_wpe: float = 0.1
+    maxerr: float = 1e-6
+    maxit: int = 2 ** 20
+    mass: tuple = (ME, MP, 18 * MP, 4 * MP, 44 * MP, MP, MP, MP)
+    charge3: tuple = (EC,) * 6       # charge(3:8), picard.f90:202-207
+    density: tuple = (0.0, 5.0e6, 0, 0, 0, 0, 0, 0)   # density(2:8) used
+    ppc: tuple = (0, 8, 0, 0, 0, 0, 0, 0)             # ppc(2:8) used
+    kelvin: tuple = (116045.0, 116045.0, 116045.0, 0, 0, 0, 0, 0)
+    v0: tuple = (4.0e5, 0.0, 0.0)    # bulk velocity, every species (picard.f90:238-248)
+    B0: tuple = (0.0, 0.0, 5.0e-9)
+    gyration: tuple = (True,) * 8
+    fluid: tuple = (False,) * 8
+    max_particles: int = 0           # 0: sized from the plasma
+    seed: int = 20261203
+
+
+def config1(**kw) -> Config:
+    """32^3, dx = 10 m, H+ and e- at 8 per cell, 400 km/s drift, 10 eV, 5 nT, x in/outflow."""
+    return Config(**kw)
+
+
+@dataclass
+class Derived:
+    dxyz: np.ndarray
+    dV: float
+    charge: np.ndarray
+    density: np.ndarray
+    ppc: np.ndarray
+    ipp: np.ndarray
+    epp: np.ndarray
+    hpp: np.ndarray
+    n2p: np.ndarray
+    eta: np.ndarray
+    ve: np.ndarray
+    v0: np.ndarray           # [8, 4]
+    B0: np.ndarray           # [4]
+    E0: np.ndarray           # [4]
+    tB: np.ndarray           # [8, 4]
+    sB: np.ndarray           # [8, 4]
+    mass: np.ndarray
+    kelvin: np.ndarray
+    dt: float
+    omega_pe: float
+
+
+def _fsum(vals):
+    """Fortran SUM: sequential from 0."""
+    s = 0.0
+    for v in vals:
+        s = s + float(v)
+    return s
+
+
+def derive(cfg: Config) -> Derived:
+    """picard.f90:333-466 for one rank."""
+    n = (cfg.nx, cfg.ny, cfg.nz)
+    L = [cfg.gmax[c] - cfg.gmin[c] for c in range(3)]
+    dxyz = np.array([L[c] / float(n[c]) for c in range(3)])       # :344-360 (iprocs = 1)
+    dV = dxyz[0] * dxyz[1] * dxyz[2]
+    mass = np.array(cfg.mass, np.float64)
+    charge = np.zeros(8)
+    charge[2:] = cfg.charge3
+    charge[0] = -EC                                              # :371
+    charge[1] = EC
+    ppc = np.array(cfg.ppc, np.int64)
+    density = np.array(cfg.density, np.float64)
+    for s in range(1, 8):                                        # :373
+        density[s] = density[s] * float(ppc[s]) / float(max(ppc[s], 1))
+    density[0] = _fsum(-density[s] * charge[s] for s in range(1, 8)) * (1.0 / charge[0])
+    for s in range(8):                                           # :375-380
+        if density[s] <= 0.0 or cfg.species < s + 1:
+            ppc[s] = 0
+            density[s] = 0.0
+    density[0] = _fsum(-density[s] * charge[s] / charge[0] for s in range(1, 8))   # :381
+    epp = np.array([max(int(0.5 - charge[s] / charge[0]), 0) for s in range(8)])   # :384
+    hpp = np.array([max(int(0.5 - charge[s] / charge[1]), 0) for s in range(8)])   # :385
+    hpp[0] = 0
+    ipp = np.zeros(8, np.int64)
+    ipp[2:] = ppc[2:]                                            # :388
+    ipp[1] = ppc[1] - int(np.sum(ipp[2:] * hpp[2:]))
+    ipp[0] = 0
+    ppc[0] = int(np.sum(ipp[1:] * epp[1:]))                      # :392
+    n2p = np.zeros(8)
+    for s in range(8):                                           # :394-401
+        if ppc[s] > 0:
+            n2p[s] = dV * density[s] * (1.0 / float(ppc[s]))
+        else:
+            n2p[s] = n2p[0]
+    eta = charge / mass                                          # :405
+    w = [float(ipp[s]) * (n2p[s] * mass[s] + float(epp[s]) * n2p[0] * mass[0] +
+                          float(hpp[s]) * n2p[1] * mass[1]) for s in range(8)]
+    v0 = np.zeros((8, 4))
+    v0[:, :3] = np.asarray(cfg.v0, np.float64)[None, :]
+    ve = np.zeros(4)
+    den = _fsum(w)
+    for c in range(3):                                           # :408-413
+        ve[c] = _fsum(w[s] * v0[s, c] for s in range(8)) / den if den != 0.0 else 0.0
+    for s in range(8):
+        v0[s, 3] = math.sqrt(_fsum(v0[s, c] ** 2 for c in range(3)))
+    ve[3] = math.sqrt(_fsum(ve[c] ** 2 for c in range(3)))
+    B0 = np.zeros(4)
+    B0[:3] = cfg.B0
+    B0[3] = math.sqrt(_fsum(B0[c] ** 2 for c in range(3)))
+    E0 = np.zeros(4)                                             # :424-427
+    E0[0] = ve[2] * B0[1] - ve[1] * B0[2]
+    E0[1] = ve[0] * B0[2] - ve[2] * B0[0]
+    E0[2] = ve[1] * B0[0] - ve[0] * B0[1]
+    E0[3] = math.sqrt(_fsum(E0[c] ** 2 for c in range(3)))
+    omega_pe = math.sqrt(density[0] * EC * EC / (EPS0 * mass[0])) if density[0] > 0 else 0.0
+    dt = cfg.dt if cfg.dt is not None else cfg.dt_wpe / omega_pe
+    tB = np.zeros((8, 4))
+    for s in range(8):                                           # :452-461
+        if cfg.gyration[s]:
+            for c in range(4):
+                tB[s, c] = 0.5 * dt * B0[c] * eta[s]
+    sB = np.zeros((8, 4))
+    for s in range(8):                                           # :463-466
+        for c in range(4):
+            sB[s, c] = 2.0 * tB[s, c] / (1.0 + tB[s, 3] ** 2)
+    return Derived(dxyz=dxyz, dV=dV, charge=charge, density=density, ppc=ppc, ipp=ipp, epp=epp,
+                   hpp=hpp, n2p=n2p, eta=eta, ve=ve, v0=v0, B0=B0, E0=E0, tB=tB, sB=sB,
+                   mass=mass, kelvin=np.array(cfg.kelvin, np.float64), dt=dt, omega_pe=omega_pe)
+
+
+# ---------------------------------------------------------------------------
+# host particle sources (ParticlesInitialize.f90, BCinflow.f90) with an
+# explicitly seeded generator in place of the Fortran runtime's random_number
+
+def _normal(rng, m):
+    """Box-Muller as Tools.f90:30-48: sqrt(-2 ln r2) sin(tau r1), r2 > 0."""
+    r1 = rng.random(m)
+    r2 = rng.random(m)
+    bad = r2 <= 0.0
+    while np.any(bad):
+        r2[bad] = rng.random(int(bad.sum()))
+        bad = r2 <= 0.0
+    return np.sqrt(-2.0 * np.log(r2)) * np.sin(TAU * r1)
+
+
+def _maxwellian(rng, m, v0, T, mass):
+    """Gaussian(u, v, t, m), Tools.f90:2-28."""
+    if T > 0.0 and mass > 0.0:
+        a = mass / (2.0 * KB * T)
+        pw = math.sqrt(1.0 / a)
+        u = np.stack([_normal(rng, m), _normal(rng, m), _normal(rng, m)], axis=1)
+        u = u * pw / math.sqrt(2.0)
+        return u + np.asarray(v0[:3])[None, :]
+    return np.repeat(np.asarray(v0[:3], np.float64)[None, :], m, axis=0)
+
+
+def _half_rotate(v, ve, tB):
+    """The dt/2 rotation with norm restore used by the sources
+    (ParticlesInitialize.f90:73-80)."""
+    vmin = v - ve[None, :3]
+    t = tB
+    vmid = np.empty_like(vmin)
+    vmid[:, 0] = vmin[:, 0] + vmin[:, 1] * t[2] - vmin[:, 2] * t[1]
+    vmid[:, 1] = vmin[:, 1] + vmin[:, 2] * t[0] - vmin[:, 0] * t[2]
+    vmid[:, 2] = vmin[:, 2] + vmin[:, 0] * t[1] - vmin[:, 1] * t[0]
+    s2 = np.sum(vmid * vmid, axis=1)
+    ok = s2 > 0.0
+    scale = np.ones_like(s2)
+    scale[ok] = np.sqrt(np.sum(vmin[ok] * vmin[ok], axis=1) / s2[ok])
+    return vmid * scale[:, None] + ve[None, :3]
+
+
+def _emit_cells(rng, d: Derived, cfg: Config, corners, species_from):
+    """Particles for a list of cell lower corners, in the reference's loop
+    order: cell, species ss, ion pp, then its epp electrons and hpp protons."""
+    out_r, out_s = [], []
+    ncell = corners.shape[0]
+    if ncell == 0:
+        return np.zeros((0, 8)), np.zeros(0, np.int32)
+    for ss in range(species_from, cfg.species + 1):
+        s = ss - 1
+        nion = int(d.ipp[s])
+        if nion == 0:
+            continue
+        group = 1 + int(d.epp[s]) + int(d.hpp[s])
+        m = ncell * nion
+        pos = (np.repeat(corners, nion, axis=0) +
+               d.dxyz[None, :] * rng.random((m, 3)))
+        blk = np.zeros((m, group, 8))
+        blk[:, :, 0:3] = pos[:, None, :]
+        blk[:, 0, 3:6] = _half_rotate(_maxwellian(rng, m, d.v0[s], d.kelvin[s], d.mass[s]), d.ve,
+                                      d.tB[s])
+        sp = np.zeros((m, group), np.int32)
+        sp[:, 0] = ss
+        for e in range(int(d.epp[s])):
+            blk[:, 1 + e, 3:6] = _half_rotate(_maxwellian(rng, m, d.v0[s], d.kelvin[0], d.mass[0]),
+                                              d.ve, d.tB[0])
+            sp[:, 1 + e] = 1
+        for h in range(int(d.hpp[s])):
+            q = 1 + int(d.epp[s]) + h
+            blk[:, q, 3:6] = _half_rotate(_maxwellian(rng, m, d.v0[s], d.kelvin[1], d.mass[1]),
+                                          d.ve, d.tB[1])
+            sp[:, q] = 2
+        out_r.append(blk.reshape(ncell, nion * group, 8))
+        out_s.append(sp.reshape(ncell, nion * group))
+    if not out_r:
+        return np.zeros((0, 8)), np.zeros(0, np.int32)
+    # order: cell-major, then species block
+    r = np.concatenate(out_r, axis=1).reshape(-1, 8)
+    s = np.concatenate(out_s, axis=1).reshape(-1).astype(np.int32)
+    return np.ascontiguousarray(r), np.ascontiguousarray(s)
+
+
+def _cell_corners(cfg: Config, d: Derived, ii, jj, kk):
+    """Lower corners x0 = Lmin + dx*(i-2) for 1-based cell indices (arrays)."""
+    K, J, I = np.meshgrid(kk, jj, ii, indexing="ij")
+    x0 = cfg.gmin[0] + d.dxyz[0] * (I.ravel() - 2)
+    y0 = cfg.gmin[1] + d.dxyz[1] * (J.ravel() - 2)
+    z0 = cfg.gmin[2] + d.dxyz[2] * (K.ravel() - 2)
+    return np.stack([x0, y0, z0], axis=1)
+
+
+def particles_initialize(cfg: Config, d: Derived, rng):
+    """ParticlesInitialize.f90:54-119: every interior cell, co-located ions and electrons."""
+    c = _cell_corners(cfg, d, np.arange(2, cfg.nx + 2), np.arange(2, cfg.ny + 2),
+                      np.arange(2, cfg.nz + 2))
+    return _emit_cells(rng, d, cfg, c, 1)
+
+
+def bc_inflow(cfg: Config, d: Derived, rng):
+    """BCinflow.f90: fresh plasma in the guard cells of every flow face."""
+    xf, yf, zf = cfg.flow
+    yneg = ypos = 1 if yf else 0
+    zneg = zpos = 1 if zf else 0
+    parts_r, parts_s = [], []
+
+    def add(corners, first_species):
+        r, s = _emit_cells(rng, d, cfg, corners, first_species)
+        parts_r.append(r)
+        parts_s.append(s)
+
+    jj = np.arange(2 - yneg, cfg.ny + 2 + ypos)
+    kk = np.arange(2 - zneg, cfg.nz + 2 + zpos)
+    if xf:
+        c = _cell_corners(cfg, d, np.array([cfg.nx + 2]), jj, kk)    # x0 = Lx_max (:59)
+        add(c, 1)
+        c = _cell_corners(cfg, d, np.array([1]), jj, kk)             # x0 = Lx_min - dx (:130)
+        add(c, 1)
+    ii = np.arange(2, cfg.nx + 2)
+    if yf:
+        add(_cell_corners(cfg, d, ii, np.array([cfg.ny + 2]), np.arange(2, cfg.nz + 2 + zpos)), 2)
+        add(_cell_corners(cfg, d, ii, np.array([1]), np.arange(2 - zneg, cfg.nz + 2)), 2)
+    if zf:
+        add(_cell_corners(cfg, d, ii, np.arange(2, cfg.ny + 2), np.array([cfg.nz + 2])), 2)
+        add(_cell_corners(cfg, d, ii, np.arange(2, cfg.ny + 2), np.array([1])), 2)
+    if not parts_r:
+        return np.zeros((0, 8)), np.zeros(0, np.int32)
+    return np.concatenate(parts_r), np.concatenate(parts_s).astype(np.int32)
+
+
+# ---------------------------------------------------------------------------
+
+def geometry(cfg: Config, d: Derived, max_particles: int, device: int = 0,
+             precision: int = _lib.PICARD_FP64) -> _lib.Geometry:
+    g = _lib.Geometry()
+    g.nx, g.ny, g.nz, g.species = cfg.nx, cfg.ny, cfg.nz, cfg.species
+    for c in range(3):
+        g.flow[c] = 1 if cfg.flow[c] else 0
+        g.lmin[c] = g.gmin[c] = cfg.gmin[c]
+        g.lmax[c] = g.gmax[c] = cfg.gmax[c]
+        g.dxyz[c] = d.dxyz[c]
+    g.precision = precision
+    g.deposit = _lib.PICARD_DEPOSIT_REFERENCE_ORDER
+    g.device = device
+    g.max_particles = max_particles
+    return g
+
+
+def species_struct(cfg: Config, d: Derived) -> _lib.Species:
+    s = _lib.Species()
+    for q in range(8):
+        s.charge[q] = d.charge[q]
+        s.n2p[q] = d.n2p[q]
+        s.eta[q] = d.eta[q]
+        s.fluid[q] = 1 if cfg.fluid[q] else 0
+        s.gyration[q] = 1 if cfg.gyration[q] else 0
+        for c in range(4):
+            s.tB[4 * q + c] = d.tB[q, c]
+            s.sB[4 * q + c] = d.sB[q, c]
+    for c in range(4):
+        s.ve[c] = d.ve[c]
+        s.B0[c] = d.B0[c]
+        s.E0[c] = d.E0[c]
+    return s
+
+
+def _dp(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_int32))
+
+
+class Domain:
+    """One device-resident domain; methods follow the reference's subroutines."""
+
+    def __init__(self, cfg: Config, max_particles: int | None = None, device: int = 0,
+                 d: Derived | None = None):
+        self.cfg = cfg
+        self.d = derive(cfg) if d is None else d
+        if max_particles is None:
+            per_cell = int(self.d.ppc[: cfg.species].sum())
+            max_particles = int(1.5 * per_cell * (cfg.nx + 2) * (cfg.ny + 2) * (cfg.nz + 2)) + 1024
+        self.geo = geometry(cfg, self.d, max_particles, device)
+        self.spc = species_struct(cfg, self.d)
+        h = ct.c_void_p()
+        check("picard_create", lib().picard_create(ct.byref(self.geo), ct.byref(self.spc),
+                                                   ct.byref(h)))
+        self.h = h
+        self.shape = (cfg.nz + 2, cfg.ny + 2, cfg.nx + 2)
+
+    def close(self):
+        if self.h:
+            lib().picard_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # particles
+    def append(self, rp, sp):
+        rp = np.ascontiguousarray(rp, np.float64)
+        sp = np.ascontiguousarray(sp, np.int32)
+        assert rp.ndim == 2 and rp.shape[1] == 8 and sp.shape == (rp.shape[0],)
+        check("picard_particles_append",
+              lib().picard_particles_append(self.h, _dp(rp), _ip(sp), rp.shape[0]))
+
+    @property
+    def num_local(self) -> int:
+        n = ct.c_int64()
+        check("picard_particles_count", lib().picard_particles_count(self.h, ct.byref(n)))
+        return n.value
+
+    def particles(self, first=0, count=None):
+        n = self.num_local
+        count = n - first if count is None else count
+        rp = np.zeros((count, 8), np.float64)
+        sp = np.zeros(count, np.int32)
+        check("picard_particles_read",
+              lib().picard_particles_read(self.h, first, count, _dp(rp), _ip(sp)))
+        return rp, sp
+
+    # fields
+    def field(self, which):
+        shp = {_lib.FIELD_U: self.shape, _lib.FIELD_C: self.shape,
+               _lib.FIELD_E: self.shape + (3,), _lib.FIELD_N: self.shape + (self.cfg.species,),
+               _lib.FIELD_D: (2 * self.cfg.nz + 1, 2 * self.cfg.ny + 1, 2 * self.cfg.nx + 1)}[which]
+        a = np.zeros(shp, np.float64)
+        check("picard_field_read", lib().picard_field_read(self.h, which, _dp(a), a.size))
+        return a
+
+    def set_field(self, which, a):
+        a = np.ascontiguousarray(a, np.float64)
+        check("picard_field_write", lib().picard_field_write(self.h, which, _dp(a), a.size))
+
+    # the call surface, picard.f90:564-631
+    def advance_position(self, dt=None):
+        check("picard_advance_position",
+              lib().picard_advance_position(self.h, self.d.dt if dt is None else dt))
+
+    def bc_outflow(self):
+        n, r = ct.c_int64(), ct.c_int64()
+        check("picard_bc_outflow", lib().picard_bc_outflow(self.h, ct.byref(n), ct.byref(r)))
+        return n.value, r.value
+
+    def particles_transfer(self):
+        check("picard_particles_transfer", lib().picard_particles_transfer(self.h))
+
+    def particles_gridify(self):
+        check("picard_particles_gridify", lib().picard_particles_gridify(self.h))
+
+    def calc_distance(self):
+        check("picard_calc_distance", lib().picard_calc_distance(self.h))
+
+    def bc_potentials(self):
+        check("picard_bc_potentials", lib().picard_bc_potentials(self.h))
+
+    def calc_epotential(self, maxerr=None, maxit=None):
+        sw = ct.c_int64()
+        rc = lib().picard_calc_epotential(self.h, self.cfg.maxerr if maxerr is None else maxerr,
+                                          self.cfg.maxit if maxit is None else maxit,
+                                          ct.byref(sw))
+        check("picard_calc_epotential", rc)
+        return sw.value
+
+    def calc_efield(self):
+        check("picard_calc_efield", lib().picard_calc_efield(self.h))
+
+    def bc_fields(self):
+        check("picard_bc_fields", lib().picard_bc_fields(self.h))
+
+    def advance_velocity(self, dt=None):
+        check("picard_advance_velocity",
+              lib().picard_advance_velocity(self.h, self.d.dt if dt is None else dt))
+
+    def step(self, produced=None):
+        st = _lib.StepStats()
+        if produced is None:
+            rp = np.zeros((0, 8))
+            sp = np.zeros(0, np.int32)
+        else:
+            rp, sp = (np.ascontiguousarray(produced[0], np.float64),
+                      np.ascontiguousarray(produced[1], np.int32))
+        check("picard_step", lib().picard_step(self.h, self.d.dt, self.cfg.maxerr, self.cfg.maxit,
+                                               _dp(rp), _ip(sp), rp.shape[0], ct.byref(st)))
+        return st
+
+    def timing(self, enable: bool):
+        check("picard_timing_enable", lib().picard_timing_enable(self.h, 1 if enable else 0))
+
+    def timing_read(self):
+        ms, n = ct.c_double(), ct.c_int64()
+        check("picard_timing_read", lib().picard_timing_read(self.h, ct.byref(ms), ct.byref(n)))
+        return ms.value, n.value
+
+    def synchronize(self):
+        check("picard_synchronize", lib().picard_synchronize(self.h))

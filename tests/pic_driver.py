@@ -1,0 +1,65 @@
+"""Drives the same PIC main loop (picard.f90:554-633, no output steps) on the
+oracle and on the GPU Domain from identical host-generated particles, so
+tests can compare them step by step."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle
+from picard_amd import host
+
+
+def oracle_grid(cfg: host.Config) -> oracle.Grid:
+    return oracle.Grid(cfg.nx, cfg.ny, cfg.nz, cfg.species, flow=tuple(cfg.flow),
+                       gmin=tuple(cfg.gmin), gmax=tuple(cfg.gmax))
+
+
+class OracleSim:
+    """The oracle's arrays for one domain (grows particle arrays as needed)."""
+
+    def __init__(self, cfg: host.Config, d: host.Derived, rp, sp):
+        self.cfg, self.d = cfg, d
+        self.g = oracle_grid(cfg)
+        self.rp = np.ascontiguousarray(rp.copy())
+        self.sp = np.ascontiguousarray(sp.copy())
+        self.n = rp.shape[0]
+        self.D = oracle.calc_distance(cfg.nx, cfg.ny, cfg.nz, self.g.dxyz, cfg.flow)
+        self.U = np.zeros(self.g.shape)
+        self.E = np.zeros(self.g.shape + (3,))
+        self.N = None
+        self.C = None
+
+    def append(self, rp, sp):
+        self.rp = np.ascontiguousarray(np.concatenate([self.rp[: self.n], rp]))
+        self.sp = np.ascontiguousarray(np.concatenate([self.sp[: self.n], sp]).astype(np.int32))
+        self.n = self.rp.shape[0]
+
+    def step(self, produced=None):
+        d, cfg = self.d, self.cfg
+        oracle.advance_position(self.rp, d.dt, self.n)                      # :564
+        if produced is not None and produced[0].shape[0]:
+            self.append(*produced)                                          # :566
+        n_new, removed = oracle.bc_outflow(self.g, self.rp, self.sp, self.n)  # :569
+        self.n = n_new
+        self.rp = np.ascontiguousarray(self.rp[: self.n])
+        self.sp = np.ascontiguousarray(self.sp[: self.n])
+        oracle.particles_transfer(self.g, self.rp, self.n)                  # :572
+        self.N, self.C, skipped = oracle.particles_gridify(self.g, self.rp, self.sp, d.charge,
+                                                           d.n2p, self.n)   # :577
+        assert skipped == 0
+        self.U = oracle.bc_potentials(self.g, self.C, self.D)               # :581
+        sweeps, _ = oracle.calc_epotential(self.g, self.U, self.C, cfg.maxerr, cfg.maxit)  # :584
+        self.E = oracle.calc_efield(self.g, self.U)                         # :587
+        oracle.bc_fields(self.g, self.E)                                    # :589
+        oracle.advance_velocity(self.g, self.rp, self.sp, self.E, d.B0, d.dt, d.ve, d.eta,
+                                d.tB, d.sB, [int(f) for f in cfg.fluid],
+                                [int(f) for f in cfg.gyration], self.n)     # :630
+        return dict(num_local=self.n, removed=len(removed), sweeps=sweeps)
+
+
+def small_config(n=8, ppc=4, flow=(True, False, False), steps_seed=5, **kw) -> host.Config:
+    half = 5.0 * n
+    if isinstance(ppc, int):
+        ppc = (0, ppc, 0, 0, 0, 0, 0, 0)
+    return host.Config(nx=n, ny=n, nz=n, flow=flow, gmin=(-half,) * 3, gmax=(half,) * 3,
+                       ppc=tuple(ppc), seed=steps_seed, **kw)
