@@ -68,6 +68,13 @@ typedef struct {
   int32_t fluid[8], gyration[8];
 } picard_species;
 
+/* Plasma composition used by the device-side sources (picard.f90:371-401). */
+typedef struct {
+  int32_t ipp[8], epp[8], hpp[8];   /* ions per cell; electrons, protons per ion */
+  double mass[8], kelvin[8];
+  double v0[32];                    /* v0(4,8): element (c,s) at [4*(s-1) + (c-1)] */
+} picard_plasma;
+
 typedef struct picard_ctx picard_ctx;
 
 const char *picard_last_error(void);
@@ -90,6 +97,15 @@ int picard_particles_append(picard_ctx *ctx, const double *real_particles,
 int picard_particles_read(picard_ctx *ctx, int64_t first, int64_t count, double *real_particles,
                           int32_t *int_particles);
 int picard_particles_count(picard_ctx *ctx, int64_t *num_local);
+
+/* Device-side particle sources, appended after num_local.  They replace the
+ * host loops of ParticlesInitialize (ParticlesInitialize.f90:2) and BCinflow
+ * (BCinflow.f90:2) with the same cell ranges, loop order and distributions,
+ * drawing from a counter-based generator (Philox4x32-10) keyed by `seed`
+ * instead of the Fortran runtime's random_number. */
+int picard_generate_initial(picard_ctx *ctx, const picard_plasma *plasma, uint64_t seed);
+int picard_generate_inflow(picard_ctx *ctx, const picard_plasma *plasma, uint64_t seed,
+                           int64_t step);
 
 /* ---- the time-step call surface (picard.f90:564-631) ---- */
 

@@ -34,7 +34,8 @@ EXPORTS = (
     "picard_particles_transfer", "picard_particles_gridify", "picard_calc_distance",
     "picard_bc_potentials", "picard_calc_epotential", "picard_calc_efield", "picard_bc_fields",
     "picard_advance_velocity", "picard_field_read", "picard_field_write", "picard_step",
-    "picard_timing_enable", "picard_timing_read",
+    "picard_timing_enable", "picard_timing_read", "picard_generate_initial",
+    "picard_generate_inflow",
 )
 
 
@@ -60,6 +61,11 @@ class Species(ct.Structure):
         ("ve", ct.c_double * 4), ("B0", ct.c_double * 4), ("E0", ct.c_double * 4),
         ("fluid", ct.c_int32 * 8), ("gyration", ct.c_int32 * 8),
     ]
+
+
+class Plasma(ct.Structure):
+    _fields_ = [("ipp", ct.c_int32 * 8), ("epp", ct.c_int32 * 8), ("hpp", ct.c_int32 * 8),
+                ("mass", ct.c_double * 8), ("kelvin", ct.c_double * 8), ("v0", ct.c_double * 32)]
 
 
 class StepStats(ct.Structure):
@@ -112,6 +118,8 @@ def lib():
                          ct.POINTER(StepStats)], ct.c_int),
         "picard_timing_enable": ([vp, ct.c_int32], ct.c_int),
         "picard_timing_read": ([vp, dp, lp], ct.c_int),
+        "picard_generate_initial": ([vp, ct.POINTER(Plasma), ct.c_uint64], ct.c_int),
+        "picard_generate_inflow": ([vp, ct.POINTER(Plasma), ct.c_uint64, ct.c_int64], ct.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "picard_internal.h"
+#include "sched.h"
 
 namespace picard {
 
@@ -54,12 +55,16 @@ int check_geometry(const picard_geometry *g) {
       return PICARD_ERR_ARG;
     }
   }
-  if (g->precision != PICARD_FP64) {
-    set_error("this context type supports precision PICARD_FP64");
+  if (g->precision != PICARD_FP64 && g->precision != PICARD_FP32_SPLIT) {
+    set_error("precision must be PICARD_FP64 or PICARD_FP32_SPLIT");
     return PICARD_ERR_ARG;
   }
-  if (g->deposit != PICARD_DEPOSIT_REFERENCE_ORDER) {
+  if (g->precision == PICARD_FP64 && g->deposit != PICARD_DEPOSIT_REFERENCE_ORDER) {
     set_error("the FP64 form deposits in reference order (PICARD_DEPOSIT_REFERENCE_ORDER)");
+    return PICARD_ERR_ARG;
+  }
+  if (g->precision == PICARD_FP32_SPLIT && (g->nx > 16000 || g->ny > 16000 || g->nz > 16000)) {
+    set_error("the FP32-split form keeps 16-bit cell coordinates (N <= 16000)");
     return PICARD_ERR_ARG;
   }
   const int64_t K = (int64_t)(g->nx + 1) * (g->ny + 1) * (g->nz + 1);
@@ -81,7 +86,34 @@ int upload_species(picard_ctx *c) {
   return PICARD_OK;
 }
 
+void free_split(ParticlesSplit &p) {
+  for (int d = 0; d < 3; ++d) {
+    dfree(p.c[d]);
+    dfree(p.f[d]);
+    dfree(p.v[d]);
+  }
+  dfree(p.sp);
+  dfree(p.id);
+}
+
+int alloc_split(ParticlesSplit &p, int64_t cap) {
+  for (int d = 0; d < 3; ++d) {
+    PICARD_TRY(dalloc(&p.c[d], cap));
+    PICARD_TRY(dalloc(&p.f[d], cap));
+    PICARD_TRY(dalloc(&p.v[d], cap));
+  }
+  PICARD_TRY(dalloc(&p.sp, cap));
+  PICARD_TRY(dalloc(&p.id, cap));
+  return PICARD_OK;
+}
+
 void release(picard_ctx *c) {
+  free_split(c->ps);
+  free_split(c->ps2);
+  dfree(c->Nfix);
+  dfree(c->E32);
+  dfree(c->r32);
+  for (int q = 0; q < 3; ++q) dfree(c->e32[q]);
   for (int q = 0; q < 8; ++q) dfree(c->p.r[q]);
   dfree(c->p.sp);
   dfree(c->U);
@@ -143,35 +175,52 @@ int create_impl(const picard_geometry *geo, const picard_species *sp, picard_ctx
   c->cap = geo->max_particles;
   c->K = (int64_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
   const int64_t cap = c->cap;
-  for (int q = 0; q < 8; ++q) PICARD_TRY(dalloc(&c->p.r[q], cap));
-  PICARD_TRY(dalloc(&c->p.sp, cap));
+  c->split = geo->precision == PICARD_FP32_SPLIT;
+  if (c->split) {
+    PICARD_TRY(alloc_split(c->ps, cap));
+    PICARD_TRY(alloc_split(c->ps2, cap));
+    PICARD_TRY(dalloc(&c->Nfix, g.S * g.G));
+    PICARD_TRY(dalloc(&c->E32, 4 * g.G));
+    PICARD_TRY(dalloc(&c->r32, g.G));
+    for (int q = 0; q < 3; ++q) PICARD_TRY(dalloc(&c->e32[q], g.G));
+    PICARD_HIP(hipMemsetAsync(c->E32, 0, sizeof(float) * 4 * g.G, c->stream));
+  } else {
+    for (int q = 0; q < 8; ++q) PICARD_TRY(dalloc(&c->p.r[q], cap));
+    PICARD_TRY(dalloc(&c->p.sp, cap));
+  }
   PICARD_TRY(dalloc(&c->U, g.G));
   PICARD_TRY(dalloc(&c->E, 3 * g.G));
   PICARD_TRY(dalloc(&c->N, g.S * g.G));
-  PICARD_TRY(dalloc(&c->Nraw, g.S * g.G));
+  if (!c->split) PICARD_TRY(dalloc(&c->Nraw, g.S * g.G));
   PICARD_TRY(dalloc(&c->C, g.G));
   const int64_t DN = (2 * (int64_t)g.nx + 1) * (2 * (int64_t)g.ny + 1) * (2 * (int64_t)g.nz + 1);
   PICARD_TRY(dalloc(&c->D, DN));
-  for (int q = 0; q < 3; ++q) PICARD_TRY(dalloc(&c->jb[q], g.G));
+  if (!c->split)
+    for (int q = 0; q < 3; ++q) PICARD_TRY(dalloc(&c->jb[q], g.G));
   PICARD_TRY(dalloc(&c->b, g.G));
   PICARD_TRY(dalloc(&c->consts, 16));
   PICARD_TRY(dalloc(&c->js, 1));
-  PICARD_TRY(dalloc(&c->s.flags, cap));
-  PICARD_TRY(dalloc(&c->s.scan, cap + 1));
-  for (int q = 0; q < 6; ++q) PICARD_TRY(dalloc(&c->s.work[q], cap + 1));
-  const int64_t nb = (int64_t)g.S * c->K;
+  if (!c->split) {
+    PICARD_TRY(dalloc(&c->s.flags, cap));
+    PICARD_TRY(dalloc(&c->s.scan, cap + 1));
+    for (int q = 0; q < 6; ++q) PICARD_TRY(dalloc(&c->s.work[q], cap + 1));
+  }
+  const int64_t nb = ((int64_t)g.S * c->K > c->K + 1 ? (int64_t)g.S * c->K : c->K + 1);
   const int64_t scan_n = cap > nb ? cap : nb;
   c->s.scan_tmp_len = scan_tmp_required(scan_n);
   PICARD_TRY(dalloc(&c->s.scan_tmp, c->s.scan_tmp_len));
   PICARD_TRY(dalloc(&c->s.counters, 16));
   PICARD_TRY(dalloc(&c->s.key, cap));
-  PICARD_TRY(dalloc(&c->s.perm, cap));
+  if (!c->split) PICARD_TRY(dalloc(&c->s.perm, cap));
   PICARD_TRY(dalloc(&c->s.hist, nb + 1));
   PICARD_TRY(dalloc(&c->s.cursor, nb));
   const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
-  const int64_t pb = grid_blocks(g.G > ncell ? g.G : ncell, 256);
+  int64_t pb = grid_blocks(g.G > ncell ? g.G : ncell, 256);
+  const int64_t pb3 = interior_launch(g.nx, g.ny, g.nz).blocks();
+  if (pb3 > pb) pb = pb3;
+  if (make_sched(g.nx, g.ny, g.nz).B > pb) pb = make_sched(g.nx, g.ny, g.nz).B;
   c->s.partials_len = pb;
-  PICARD_TRY(dalloc(&c->s.partials, pb));
+  PICARD_TRY(dalloc(&c->s.partials, 2 * pb));     // two launches' worth (read + write)
   PICARD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->js_host), sizeof(JacobiState)));
   PICARD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->host_counters), 16 * sizeof(int32_t)));
   // the reference zeroes U, N, E and the particle arrays at start (picard.f90:434-439)
@@ -281,7 +330,7 @@ int picard_particles_append(picard_ctx *c, const double *rp, const int32_t *sp, 
       return PICARD_ERR_ARG;
     }
   }
-  return launch_append(c, rp, sp, count);
+  return c->split ? sp_append(c, rp, sp, count) : launch_append(c, rp, sp, count);
 }
 
 int picard_particles_read(picard_ctx *c, int64_t first, int64_t count, double *rp, int32_t *sp) {
@@ -290,7 +339,7 @@ int picard_particles_read(picard_ctx *c, int64_t first, int64_t count, double *r
     set_error("particles_read: bad arguments");
     return PICARD_ERR_ARG;
   }
-  return launch_read(c, first, count, rp, sp);
+  return c->split ? sp_read(c, first, count, rp, sp) : launch_read(c, first, count, rp, sp);
 }
 
 int picard_particles_count(picard_ctx *c, int64_t *n) {
@@ -303,15 +352,52 @@ int picard_particles_count(picard_ctx *c, int64_t *n) {
   return PICARD_OK;
 }
 
+static int check_plasma(const picard_ctx *c, const picard_plasma *pl) {
+  if (!pl) {
+    set_error("plasma is NULL");
+    return PICARD_ERR_ARG;
+  }
+  for (int s = 0; s < 8; ++s) {
+    if (pl->ipp[s] < 0 || pl->epp[s] < 0 || pl->hpp[s] < 0 || pl->ipp[s] > 4096 ||
+        pl->epp[s] > 64 || pl->hpp[s] > 64) {
+      set_error("plasma: per-cell counts out of range");
+      return PICARD_ERR_ARG;
+    }
+  }
+  (void)c;
+  return PICARD_OK;
+}
+
+int picard_generate_initial(picard_ctx *c, const picard_plasma *pl, uint64_t seed) {
+  CTX_CHECK(c);
+  PICARD_TRY(check_plasma(c, pl));
+  return generate_initial(c, pl, seed);
+}
+
+int picard_generate_inflow(picard_ctx *c, const picard_plasma *pl, uint64_t seed, int64_t step) {
+  CTX_CHECK(c);
+  PICARD_TRY(check_plasma(c, pl));
+  return generate_inflow(c, pl, seed, step);
+}
+
 int picard_advance_position(picard_ctx *c, double dt) {
   CTX_CHECK(c);
+  if (c->split) return sp_push(c, dt);
   return launch_push_x(c, 0, c->n, dt);
 }
 
 int picard_bc_outflow(picard_ctx *c, int64_t *num_local, int64_t *removed) {
   CTX_CHECK(c);
   int64_t nn = 0, rem = 0;
-  PICARD_TRY(launch_outflow(c, &nn, &rem));
+  if (c->split) {
+    // the split form compacts by re-sorting on the cell (particle order changes)
+    const int64_t before = c->n;
+    PICARD_TRY(sp_sort(c, 0));
+    nn = c->n;
+    rem = before - c->n;
+  } else {
+    PICARD_TRY(launch_outflow(c, &nn, &rem));
+  }
   if (num_local) *num_local = nn;
   if (removed) *removed = rem;
   return PICARD_OK;
@@ -319,12 +405,13 @@ int picard_bc_outflow(picard_ctx *c, int64_t *num_local, int64_t *removed) {
 
 int picard_particles_transfer(picard_ctx *c) {
   CTX_CHECK(c);
+  if (c->split) return sp_sort(c, 1);
   return launch_wrap(c);
 }
 
 int picard_particles_gridify(picard_ctx *c) {
   CTX_CHECK(c);
-  return launch_gridify(c);
+  return c->split ? sp_gridify(c) : launch_gridify(c);
 }
 
 int picard_calc_distance(picard_ctx *c) {
@@ -344,24 +431,26 @@ int picard_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
     return PICARD_ERR_ARG;
   }
   int64_t sw = 0;
-  const int rc = launch_calc_epotential(c, maxerr, maxit, &sw);
+  const int rc = c->split ? sp_epotential(c, maxerr, maxit, &sw)
+                          : launch_calc_epotential(c, maxerr, maxit, &sw);
   if (sweeps) *sweeps = sw;
   return rc;
 }
 
 int picard_calc_efield(picard_ctx *c) {
   CTX_CHECK(c);
-  return launch_calc_efield(c);
+  return c->split ? sp_efield(c) : launch_calc_efield(c);
 }
 
 int picard_bc_fields(picard_ctx *c) {
   CTX_CHECK(c);
+  if (c->split) return PICARD_OK;   // flow guards of E32 are written as zero by sp_efield
   return launch_bc_fields(c);
 }
 
 int picard_advance_velocity(picard_ctx *c, double dt) {
   CTX_CHECK(c);
-  return launch_boris(c, dt);
+  return c->split ? sp_boris(c, dt) : launch_boris(c, dt);
 }
 
 int picard_field_read(picard_ctx *c, int32_t which, double *host, int64_t count) {
@@ -371,6 +460,7 @@ int picard_field_read(picard_ctx *c, int32_t which, double *host, int64_t count)
     set_error("field_read: unknown field or wrong element count");
     return PICARD_ERR_ARG;
   }
+  if (c->split && which == PICARD_FIELD_E) PICARD_TRY(sp_widen_e(c));
   PICARD_HIP(hipMemcpyAsync(host, field_ptr(c, which), sizeof(double) * len,
                             hipMemcpyDeviceToHost, c->stream));
   PICARD_HIP(hipStreamSynchronize(c->stream));
@@ -410,18 +500,35 @@ int picard_step(picard_ctx *c, double dt, double maxerr, int32_t maxit, const do
                 const int32_t *produced_species, int64_t nproduced, picard_step_stats *stats) {
   CTX_CHECK(c);
   picard_step_stats st{};
-  PICARD_TRY(launch_push_x(c, 0, c->n, dt));                              // picard.f90:564
-  if (nproduced > 0) PICARD_TRY(picard_particles_append(c, produced, produced_species, nproduced));
-  PICARD_TRY(launch_outflow(c, &st.num_local, &st.removed));             // :569
-  PICARD_TRY(launch_wrap(c));                                            // :572
-  PICARD_TRY(launch_gridify(c));                                         // :577
-  PICARD_TRY(launch_bc_potentials(c));                                   // :581
-  const int rc = launch_calc_epotential(c, maxerr, maxit, &st.sweeps);   // :584
-  if (rc < 0) return rc;
-  PICARD_TRY(launch_calc_efield(c));                                     // :587
-  PICARD_TRY(launch_bc_fields(c));                                       // :589
-  st.pushed = c->n;
-  PICARD_TRY(launch_boris(c, dt));                                       // :630
+  int rc = PICARD_OK;
+  if (c->split) {
+    PICARD_TRY(sp_push(c, dt));                                          // picard.f90:564
+    if (nproduced > 0) PICARD_TRY(picard_particles_append(c, produced, produced_species, nproduced));
+    const int64_t before = c->n;
+    PICARD_TRY(sp_sort(c, 1));                                           // :569, :572
+    st.num_local = c->n;
+    st.removed = before - c->n;
+    PICARD_TRY(sp_gridify(c));                                           // :577
+    PICARD_TRY(launch_bc_potentials(c));                                 // :581
+    rc = sp_epotential(c, maxerr, maxit, &st.sweeps);                    // :584
+    if (rc < 0) return rc;
+    PICARD_TRY(sp_efield(c));                                            // :587-589
+    st.pushed = c->n;
+    PICARD_TRY(sp_boris(c, dt));                                         // :630
+  } else {
+    PICARD_TRY(launch_push_x(c, 0, c->n, dt));                           // picard.f90:564
+    if (nproduced > 0) PICARD_TRY(picard_particles_append(c, produced, produced_species, nproduced));
+    PICARD_TRY(launch_outflow(c, &st.num_local, &st.removed));          // :569
+    PICARD_TRY(launch_wrap(c));                                         // :572
+    PICARD_TRY(launch_gridify(c));                                      // :577
+    PICARD_TRY(launch_bc_potentials(c));                                // :581
+    rc = launch_calc_epotential(c, maxerr, maxit, &st.sweeps);          // :584
+    if (rc < 0) return rc;
+    PICARD_TRY(launch_calc_efield(c));                                  // :587
+    PICARD_TRY(launch_bc_fields(c));                                    // :589
+    st.pushed = c->n;
+    PICARD_TRY(launch_boris(c, dt));                                    // :630
+  }
   if (stats) *stats = st;
   return rc;
 }

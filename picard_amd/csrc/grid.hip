@@ -9,6 +9,8 @@
 
 #include "cic.h"
 #include "picard_internal.h"
+#include "reduce.h"
+#include "sched.h"
 
 namespace picard {
 namespace {
@@ -327,19 +329,8 @@ struct JacobiCoef {
   double dx2, dy2, dz2;
 };
 
-__device__ __forceinline__ double block_sum(double v, double *sh) {
-  // wave64 tree then across waves; fixed order, deterministic
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) sh[wave] = v;
-  __syncthreads();
-  double s = 0.0;
-  if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int w = 0; w < nw; ++w) s = s + sh[w];
-  }
-  return s;
-}
+
+
 
 __global__ void __launch_bounds__(kT) k_jacobi_bnorm(GridDims g, const double *__restrict__ C,
                                                       double *__restrict__ b, double binv,
@@ -366,62 +357,89 @@ __global__ void __launch_bounds__(kT) k_jacobi_bnorm(GridDims g, const double *_
     amLast = (prev == gridDim.x - 1);
   }
   __syncthreads();
-  if (amLast && threadIdx.x == 0) {
-    __threadfence();
-    double tot = 0.0;
-    for (unsigned q = 0; q < gridDim.x; ++q) tot = tot + partials[q];
+  if (!amLast) return;
+  const double tot = final_sum(partials, gridDim.x, sh);
+  if (threadIdx.x == 0) {
     st->bsum = tot;
     st->rsum = tot;                                  // r_global = b_global (:55)
     st->thresh = maxerr * sqrt(tot);
     st->arrive = 0;
-    // first stop test, ss = 1 (:60-62)
-    if (st->maxit >= 1 && sqrt(tot) <= st->thresh) {
+    // first stop test, ss = 1 (:60-62); the decision goes to slot 0 (launch 0)
+    const bool stop = st->maxit >= 1 && sqrt(tot) <= st->thresh;
+    st->slot[0].done = stop ? 1 : 0;
+    st->slot[0].result = 0;
+    if (stop) {
       st->done = 1;
       st->result = 0;
     }
   }
 }
 
-__global__ void __launch_bounds__(kT) k_jacobi_pass(GridDims g, JacobiCoef jc,
+// Sum of the previous launch's per-block partials, by every block in the same
+// fixed order (no atomics, no fences: the kernel boundary orders the data).
+__device__ __forceinline__ double sum_partials(const double *__restrict__ P, int n, double *sh) {
+  double v = 0.0;
+  for (int q = lin_tid(); q < n; q += 256) v = v + P[q];
+  return block_sum(v, sh);
+}
+
+__global__ void __launch_bounds__(256) k_jacobi_pass(GridDims g, JacobiCoef jc, TileSched ts,
                                                      double *b0, double *b1, double *b2,
                                                      const double *__restrict__ b,
-                                                     double *__restrict__ partials,
-                                                     JacobiState *st) {
-  __shared__ double sh[kT / 64];
-  __shared__ bool amLast;
-  if (st->done) return;
-  const int64_t p = st->pass;
-  double *bufs[3] = {b0, b1, b2};
-  const double *cur = bufs[(p - 1) % 3];           // U_{p-1}
-  double *out = bufs[p % 3];                        // U_p
-  const double *old = p >= 2 ? bufs[(p - 2) % 3] : nullptr;   // U_{p-2}
-  const bool wrap_cur = (p - 1) >= 1;               // halo of U_{p-1} = wrapped interior
-  const bool wrap_old = (p - 2) >= 1;
+                                                     const double *__restrict__ Pin,
+                                                     double *__restrict__ Pout, JacobiState *st,
+                                                     int64_t p) {
+  __shared__ double sh[4];
+  __shared__ int decide;
+  const StopSlot prev = st->slot[(p - 1) & 1];
+  const int t0 = lin_tid();
+  // decision of this launch: carried over, or the stop test of loop
+  // iteration ss = p-1 on the residual of U_{p-2} (partials of launch p-1)
+  StopSlot cur = prev;
+  if (!prev.done) {
+    if (p >= 3) {
+      const double S = sum_partials(Pin, ts.B, sh);
+      if (t0 == 0) decide = (p - 1 <= st->maxit && sqrt(S) <= st->thresh) ? 1 : 0;
+      __syncthreads();
+      if (decide) {
+        cur.done = 1;
+        cur.result = p - 2;
+      }
+    }
+    if (!cur.done && p > st->last) {      // all 2*maxit+1 sweeps done (:58)
+      cur.done = 1;
+      cur.result = st->last;
+    }
+  }
+  if (blockIdx.x == 0 && t0 == 0) st->slot[p & 1] = cur;
+  if (cur.done) return;
 
-  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
-  const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
+  double *bufs[3] = {b0, b1, b2};
+  const double *curb = bufs[(p - 1) % 3];          // U_{p-1}
+  double *out = bufs[p % 3];                       // U_p
+  const double *old = p >= 2 ? bufs[(p - 2) % 3] : nullptr;   // U_{p-2}
+  const bool wrap_cur = (p - 1) >= 1;              // halo of U_{p-1} = wrapped interior
+  const bool wrap_old = (p - 2) >= 1;
+  const bool px = !g.flow[0], py = !g.flow[1], pz = !g.flow[2];
+  const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
   double r2 = 0.0;
-  if (t < ncell) {
-    const int i = (int)(t % g.nx) + 2;
-    const int j = (int)((t / g.nx) % g.ny) + 2;
-    const int k = (int)(t / ((int64_t)g.nx * g.ny)) + 2;
-    const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
-    const int64_t c = (int64_t)(i - 1) + X * ((int64_t)(j - 1) + (int64_t)g.Y * (k - 1));
-    const bool px = !g.flow[0], py = !g.flow[1], pz = !g.flow[2];
-    // neighbour indices for the update (halo of U_{p-1})
+  for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int k, bool valid) {
+    if (!valid) return;
+    const int64_t c = (int64_t)(i - 1) + X * (int64_t)(j - 1) + XY * (int64_t)(k - 1);
     const int64_t xe = (px && wrap_cur && i == g.nx + 1) ? c - (int64_t)(g.nx - 1) : c + 1;
     const int64_t xw = (px && wrap_cur && i == 2) ? c + (int64_t)(g.nx - 1) : c - 1;
     const int64_t yn = (py && wrap_cur && j == g.ny + 1) ? c - (int64_t)(g.ny - 1) * X : c + X;
     const int64_t ys = (py && wrap_cur && j == 2) ? c + (int64_t)(g.ny - 1) * X : c - X;
     const int64_t zt = (pz && wrap_cur && k == g.nz + 1) ? c - (int64_t)(g.nz - 1) * XY : c + XY;
     const int64_t zb = (pz && wrap_cur && k == 2) ? c + (int64_t)(g.nz - 1) * XY : c - XY;
-    const double ue = cur[xe], uw = cur[xw], un = cur[yn], us = cur[ys], ut = cur[zt], ub = cur[zb];
+    const double ue = curb[xe], uw = curb[xw], un = curb[yn], us = curb[ys], ut = curb[zt],
+                 ub = curb[zb];
     const double bc = b[c];
     // CalcEpotential.f90:74-80
     out[c] = jc.ax * (ue + uw) + jc.ay * (un + us) + jc.az * (ut + ub) - jc.af * bc;
     if (p >= 2) {
-      // residual of U_{p-1} with the halo of U_{p-2}: periodic guard neighbours
-      // come from `old` (wrapped, or the stored input halo when p-2 == 0)
+      // residual of U_{p-1} with the halo of U_{p-2}: periodic guard
+      // neighbours come from `old` (wrapped, or the stored input halo at p-2 = 0)
       double re = ue, rw = uw, rn = un, rs = us, rt = ut, rb = ub;
       if (px && i == g.nx + 1) re = wrap_old ? old[c - (int64_t)(g.nx - 1)] : old[c + 1];
       if (px && i == 2) rw = wrap_old ? old[c + (int64_t)(g.nx - 1)] : old[c - 1];
@@ -429,50 +447,24 @@ __global__ void __launch_bounds__(kT) k_jacobi_pass(GridDims g, JacobiCoef jc,
       if (py && j == 2) rs = wrap_old ? old[c + (int64_t)(g.ny - 1) * X] : old[c - X];
       if (pz && k == g.nz + 1) rt = wrap_old ? old[c - (int64_t)(g.nz - 1) * XY] : old[c + XY];
       if (pz && k == 2) rb = wrap_old ? old[c + (int64_t)(g.nz - 1) * XY] : old[c - XY];
-      const double u2 = 2.0 * cur[c];
+      const double u2 = 2.0 * curb[c];
       // CalcEpotential.f90:83-92
       const double res = bc - ((re + rw - u2) / jc.dx2 + (rn + rs - u2) / jc.dy2 +
                                (rt + rb - u2) / jc.dz2);
-      r2 = res * res;
+      r2 = r2 + res * res;
     }
-  }
-  const double s = block_sum(r2, sh);
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = s;
-    __threadfence();
-    const unsigned prev = atomicAdd(&st->arrive, 1u);
-    amLast = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (amLast && threadIdx.x == 0) {
-    __threadfence();
-    if (p >= 2) {
-      double tot = 0.0;
-      for (unsigned q = 0; q < gridDim.x; ++q) tot = tot + partials[q];
-      st->rsum = tot;
-      // stop test of loop iteration ss = p: exit with U_{p-1}
-      if (p <= st->maxit && sqrt(tot) <= st->thresh) {
-        st->done = 1;
-        st->result = p - 1;
-      }
-    }
-    if (!st->done && p == st->last) {
-      st->done = 1;
-      st->result = p;
-    }
-    st->pass = p + 1;
-    st->arrive = 0;
-  }
+  });
+  const double sblk = block_sum(r2, sh);
+  if (t0 == 0) Pout[blockIdx.x] = sblk;
 }
 
 // After q >= 1 sweeps: U(node) = U_q at the composite periodic image of the
 // node (z, then y, then x copies of whole planes, CalcEpotential.f90:100-239);
 // nodes whose image lies on a flow face keep the input value found there.
 __global__ void __launch_bounds__(kT) k_jacobi_finish(GridDims g, double *b0, double *b1,
-                                                       double *b2, double *U, JacobiState *st) {
+                                                       double *b2, double *U, int64_t q) {
   const int64_t node = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (node >= g.G) return;
-  const int64_t q = st->result;
   if (q == 0) return;
   const double *src_buf = q % 3 == 0 ? b0 : (q % 3 == 1 ? b1 : b2);
   const int i = (int)(node % g.X) + 1;
@@ -609,6 +601,17 @@ int launch_bc_potentials(picard_ctx *c) {
   return PICARD_OK;
 }
 
+// b = -C/(dV eps0), its interior norm, the threshold and the stop test of
+// loop iteration ss = 1 (CalcEpotential.f90:50-62).
+int launch_bnorm(picard_ctx *c, double maxerr) {
+  const GridDims &g = c->gd;
+  const double binv = 1.0 / (g.dV * 8.854187817e-12);
+  k_jacobi_bnorm<<<grid_blocks(g.G, kT), kT, 0, c->stream>>>(g, c->C, c->b, binv, c->s.partials,
+                                                             c->js, maxerr);
+  PICARD_HIP(hipGetLastError());
+  return PICARD_OK;
+}
+
 int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) {
   const GridDims &g = c->gd;
   const double dx = g.dxyz[0], dy = g.dxyz[1], dz = g.dxyz[2];
@@ -621,9 +624,6 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
   jc.dx2 = dx * dx;
   jc.dy2 = dy * dy;
   jc.dz2 = dz * dz;
-  const double eps0 = 8.854187817e-12;
-  const double binv = 1.0 / (g.dV * eps0);
-
   JacobiState init{};
   init.pass = 1;
   init.maxit = maxit;
@@ -633,38 +633,38 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
     PICARD_HIP(hipMemcpyAsync(c->jb[q], c->U, sizeof(double) * g.G, hipMemcpyDeviceToDevice,
                               c->stream));
   const int nbn = grid_blocks(g.G, kT);
-  k_jacobi_bnorm<<<nbn, kT, 0, c->stream>>>(g, c->C, c->b, binv, c->s.partials, c->js, maxerr);
-  PICARD_HIP(hipGetLastError());
+  PICARD_TRY(launch_bnorm(c, maxerr));
 
-  const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
-  const int nbp = grid_blocks(ncell, kT);
+  const TileSched ts = make_sched(g.nx, g.ny, g.nz);
   const int64_t last = init.last;
-  int64_t launched = 0;
+  double *P[2] = {c->s.partials, c->s.partials + c->s.partials_len};
+  int64_t p = 1;                  // next pass (launch number)
   // first batch from the previous solve's count (cold starts are alike from
   // step to step), then short batches: few no-op passes after convergence
   int64_t batch = c->last_sweeps > 64 ? (c->last_sweeps * 15) / 16 : 32;
   const int64_t kMaxBatch = int64_t(1) << 16;
   if (batch > kMaxBatch) batch = kMaxBatch;
+  StopSlot fin{};
   for (;;) {
-    const int64_t todo = (last - launched) < batch ? (last - launched) : batch;
+    const int64_t todo = (last + 1 - (p - 1)) < batch ? (last + 1 - (p - 1)) : batch;
     if (c->timing) {
-      const int64_t need = 2 * todo;
-      while ((int64_t)c->tev.size() < need) {
+      while ((int64_t)c->tev.size() < 2 * todo) {
         hipEvent_t e;
         PICARD_HIP(hipEventCreate(&e));
         c->tev.push_back(e);
       }
-      for (int64_t q = 0; q < todo; ++q)
-        hipExtLaunchKernelGGL(k_jacobi_pass, dim3(nbp), dim3(kT), 0, c->stream, c->tev[2 * q],
-                              c->tev[2 * q + 1], 0, g, jc, c->jb[0], c->jb[1], c->jb[2],
-                              (const double *)c->b, c->s.partials, c->js);
-    } else {
-      for (int64_t q = 0; q < todo; ++q)
-        k_jacobi_pass<<<nbp, kT, 0, c->stream>>>(g, jc, c->jb[0], c->jb[1], c->jb[2], c->b,
-                                                 c->s.partials, c->js);
+    }
+    for (int64_t q = 0; q < todo; ++q, ++p) {
+      if (c->timing)
+        hipExtLaunchKernelGGL(k_jacobi_pass, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
+                              c->tev[2 * q], c->tev[2 * q + 1], 0, g, jc, ts, c->jb[0], c->jb[1],
+                              c->jb[2], (const double *)c->b, (const double *)P[(p - 1) & 1],
+                              P[p & 1], c->js, p);
+      else
+        k_jacobi_pass<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(
+            g, jc, ts, c->jb[0], c->jb[1], c->jb[2], c->b, P[(p - 1) & 1], P[p & 1], c->js, p);
     }
     PICARD_HIP(hipGetLastError());
-    launched += todo;
     PICARD_HIP(hipMemcpyAsync(c->js_host, c->js, sizeof(JacobiState), hipMemcpyDeviceToHost,
                               c->stream));
     PICARD_HIP(hipStreamSynchronize(c->stream));
@@ -676,20 +676,21 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
       }
       c->t_launches += todo;
     }
-    if (c->js_host->done) break;
-    if (launched >= last) {
+    fin = c->js_host->slot[(p - 1) & 1];
+    if (fin.done) break;
+    if (p > last + 1) {
       set_error("CalcEpotential: pass accounting out of step");
       return PICARD_ERR_STATE;
     }
     batch = 64;
   }
-  c->last_sweeps = c->js_host->result;
-  k_jacobi_finish<<<nbn, kT, 0, c->stream>>>(g, c->jb[0], c->jb[1], c->jb[2], c->U, c->js);
+  c->last_sweeps = fin.result;
+  k_jacobi_finish<<<nbn, kT, 0, c->stream>>>(g, c->jb[0], c->jb[1], c->jb[2], c->U, fin.result);
   PICARD_HIP(hipGetLastError());
-  *sweeps = c->js_host->result;
+  *sweeps = fin.result;
   // Running all 2*maxit+1 sweeps means every stop test failed; the reference
   // prints 'Maximum iteration reached!' on each sweep past maxit (:67-69).
-  return c->js_host->result == last ? PICARD_WARN_NOCONV : PICARD_OK;
+  return fin.result == last ? PICARD_WARN_NOCONV : PICARD_OK;
 }
 
 int launch_calc_efield(picard_ctx *c) {

@@ -253,6 +253,10 @@ __global__ void __launch_bounds__(kT) k_soa_to_aos(ParticlesF64 p, int64_t first
   stage_sp[i] = p.sp[first + i];
 }
 
+constexpr int64_t kStageChunk = int64_t(1) << 21;
+
+}  // namespace
+
 int ensure_stage(picard_ctx *c, int64_t want) {
   if (c->s.stage_cap >= want) return PICARD_OK;
   if (c->s.stage) PICARD_HIP(hipFree(c->s.stage));
@@ -264,10 +268,6 @@ int ensure_stage(picard_ctx *c, int64_t want) {
   c->s.stage_cap = want;
   return PICARD_OK;
 }
-
-constexpr int64_t kStageChunk = int64_t(1) << 21;
-
-}  // namespace
 
 int launch_push_x(picard_ctx *c, int64_t first, int64_t count, double dt) {
   if (count <= 0) return PICARD_OK;

@@ -48,18 +48,38 @@ struct ParticlesF64 {
   int32_t *sp;
 };
 
+// FP32-split particles: lower-node coordinates, in-cell offsets, velocities.
+struct ParticlesSplit {
+  int16_t *c[3];
+  float *f[3];
+  float *v[3];
+  uint8_t *sp;
+  uint32_t *id;
+};
+
+// Stop state of a Jacobi solve as seen after one launch: kernel number s reads
+// slot (s-1)&1 and block 0 writes slot s&1, so no kernel reads a slot that
+// one of its own blocks writes.
+struct StopSlot {
+  int32_t done;
+  int32_t pad;
+  int64_t result;            // sweeps of the final iterate, valid when done
+};
+
 // Device-side scalars of the Jacobi solve (one per context).
 struct JacobiState {
+  StopSlot slot[2];
   double bsum;               // sum of b^2 over the interior
   double thresh;             // maxerr * sqrt(bsum)
   double rsum;               // last residual sum
-  int64_t pass;              // next pass p (1-based): computes iterate p
+  int64_t pass;              // FP64 form: next pass p; split form: sweeps folded into u
   int64_t result;            // sweeps performed, valid when done
   int64_t last;              // 2*maxit + 1
+  int64_t inner;             // split form: inner FP32 sweep count of the current outer step
   int32_t maxit;
   int32_t done;
+  int32_t stop_inner;
   uint32_t arrive;           // block arrival counter for the last-block reduction
-  uint32_t pad;
 };
 
 struct DeviceScratch {
@@ -96,6 +116,13 @@ struct picard_ctx {
   int64_t cap = 0;           // max particles
   int64_t n = 0;             // num_local (authoritative on the host)
   picard::ParticlesF64 p{};
+  bool split = false;        // PICARD_FP32_SPLIT
+  picard::ParticlesSplit ps{}, ps2{};
+  unsigned long long *Nfix = nullptr;   // split: fixed-point densities
+  float *E32 = nullptr;      // split: E as float4 per node
+  float *r32 = nullptr, *e32[3] = {nullptr, nullptr, nullptr};
+  int mx_inner = 64;         // split: FP32 sweeps per FP64 refinement
+  int64_t next_id = 0;
 
   // grids
   double *U = nullptr, *E = nullptr, *N = nullptr, *Nraw = nullptr, *C = nullptr, *D = nullptr;
@@ -132,7 +159,10 @@ int launch_boris(picard_ctx *c, double dt);
 int launch_append(picard_ctx *c, const double *host_rp, const int32_t *host_sp, int64_t count);
 int launch_read(picard_ctx *c, int64_t first, int64_t count, double *host_rp, int32_t *host_sp);
 
+int ensure_stage(picard_ctx *c, int64_t want);
+
 // ---- grid.hip ----
+int launch_bnorm(picard_ctx *c, double maxerr);
 int launch_gridify(picard_ctx *c);
 int launch_calc_distance(picard_ctx *c);
 int launch_bc_potentials(picard_ctx *c);
@@ -140,10 +170,41 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
 int launch_calc_efield(picard_ctx *c);
 int launch_bc_fields(picard_ctx *c);
 
+// ---- sources.hip ----
+int generate_initial(picard_ctx *c, const picard_plasma *pl, uint64_t seed);
+int generate_inflow(picard_ctx *c, const picard_plasma *pl, uint64_t seed, int64_t step);
+
+// ---- split.hip ----
+int sp_push(picard_ctx *c, double dt);
+int sp_sort(picard_ctx *c, int wrap);
+int sp_gridify(picard_ctx *c);
+int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps);
+int sp_efield(picard_ctx *c);
+int sp_widen_e(picard_ctx *c);
+int sp_boris(picard_ctx *c, double dt);
+int sp_append(picard_ctx *c, const double *host_rp, const int32_t *host_sp, int64_t count);
+int sp_read(picard_ctx *c, int64_t first, int64_t count, double *host_rp, int32_t *host_sp);
+
 // ---- scan.hip ----
 // Exclusive prefix sum of n int32 values (in may alias out); out[n] = total.
 int scan_exclusive_i32(picard_ctx *c, const int32_t *in, int32_t *out, int64_t n);
 int64_t scan_tmp_required(int64_t n);
+
+// 3D launch shape over the interior cells: x fastest, 256 threads per block.
+struct Launch3 {
+  dim3 grid, block;
+  int64_t blocks() const { return (int64_t)grid.x * grid.y * grid.z; }
+};
+
+inline Launch3 interior_launch(int nx, int ny, int nz) {
+  int bx = 64;
+  while (bx > 8 && bx / 2 >= nx) bx /= 2;
+  const int by = 256 / bx;
+  Launch3 L;
+  L.block = dim3(bx, by, 1);
+  L.grid = dim3((nx + bx - 1) / bx, (ny + by - 1) / by, nz);
+  return L;
+}
 
 inline int grid_blocks(int64_t n, int threads) {
   int64_t b = (n + threads - 1) / threads;

@@ -1,0 +1,750 @@
+// split.hip -- the FP32-split form of the time step.
+//
+// Particles: int16 lower-node coordinates (i, j, k, 0-based) plus FP32
+// in-cell offsets f = xi - floor(xi), xi = (x - Lmin)/dx + 1/2 (so the CIC
+// weights of ParticlesGridify.f90:70-88 are f and 1 - f), FP32 velocities,
+// a uint8 species and a uint32 id.  Every step re-sorts the particles by cell
+// (counting sort), which also drops the ones BCoutflow deletes, and replaces
+// ParticlesTransfer's single-domain wrap by index arithmetic.
+//
+// Deposit: int64 fixed point (2^-40 units) with integer guard folds, so the
+// densities do not depend on particle order.  Field solve: iterative
+// refinement -- FP32 Jacobi sweeps on the correction e of A e = r with an
+// FP64 outer residual r = b - A u and FP64 update u += e; the residual norm of
+// every sweep's iterate u + e_j (= r - A e_j) is accumulated in FP64, so the
+// stop rule of CalcEpotential.f90:60-62 is applied sweep by sweep.  E is
+// produced in FP32 for the gather; BCpotentials runs in FP64 (grid.hip).
+#include <hip/hip_ext.h>
+
+#include "cic.h"
+#include "picard_internal.h"
+#include "reduce.h"
+#include "sched.h"
+
+namespace picard {
+namespace {
+
+constexpr int kT = 256;
+constexpr double kFix = 1099511627776.0;       // 2^40
+constexpr double kFixInv = 1.0 / 1099511627776.0;
+
+__device__ __forceinline__ int64_t lin_key(const GridDims &g, int i, int j, int k) {
+  return (int64_t)i + (int64_t)(g.nx + 1) * ((int64_t)j + (int64_t)(g.ny + 1) * k);
+}
+
+// Carry a coordinate offset into [0, 1): returns the integer carry.
+__device__ __forceinline__ int carry(float &f) {
+  const float fl = floorf(f);
+  f = f - fl;
+  int c = (int)fl;
+  if (f >= 1.0f) {           // f - floor(f) rounded up to 1
+    f = 0.0f;
+    c += 1;
+  }
+  return c;
+}
+
+// AdvancePosition in the split form: xi += v dt/dx.
+__global__ void __launch_bounds__(kT) k_sp_push(ParticlesSplit p, int64_t n, float3 dtdx) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= n) return;
+  float fx = p.f[0][q] + p.v[0][q] * dtdx.x;
+  float fy = p.f[1][q] + p.v[1][q] * dtdx.y;
+  float fz = p.f[2][q] + p.v[2][q] * dtdx.z;
+  p.c[0][q] = (int16_t)(p.c[0][q] + carry(fx));
+  p.c[1][q] = (int16_t)(p.c[1][q] + carry(fy));
+  p.c[2][q] = (int16_t)(p.c[2][q] + carry(fz));
+  p.f[0][q] = fx;
+  p.f[1][q] = fy;
+  p.f[2][q] = fz;
+}
+
+// xi < 1/2 is x < Lmin; xi > n + 1/2 is x > Lmax.
+__device__ __forceinline__ bool below(int c, float f) { return c < 0 || (c == 0 && f < 0.5f); }
+__device__ __forceinline__ bool above(int c, float f, int n) {
+  return c > n || (c == n && f > 0.5f);
+}
+
+// BCoutflow.f90:28-66 on the split coordinates: flow faces, and the band one
+// domain width beyond the box in every dimension.
+__device__ __forceinline__ bool sp_delete(const GridDims &g, const int c[3], const float f[3]) {
+  const int n[3] = {g.nx, g.ny, g.nz};
+  bool del = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (g.flow[d] && (below(c[d], f[d]) || above(c[d], f[d], n[d]))) del = true;
+    if (below(c[d] + n[d], f[d]) || above(c[d] - n[d], f[d], n[d])) del = true;
+  }
+  return del;
+}
+
+// ParticlesTransfer.f90:494-519: periodic wrap by +-n cells.
+__device__ __forceinline__ void sp_wrap(const GridDims &g, int c[3], const float f[3]) {
+  const int n[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+  for (int d = 2; d >= 0; --d) {
+    if (below(c[d], f[d])) c[d] += n[d];
+    if (above(c[d], f[d], n[d])) c[d] -= n[d];
+  }
+}
+
+// Sort key of a kept particle (its lower node), -1 for a deleted one.
+__global__ void __launch_bounds__(kT) k_sp_flag(ParticlesSplit p, int64_t n, GridDims g,
+                                                 int wrap, int32_t *__restrict__ key,
+                                                 int32_t *__restrict__ hist) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= n) return;
+  int c[3] = {p.c[0][q], p.c[1][q], p.c[2][q]};
+  const float f[3] = {p.f[0][q], p.f[1][q], p.f[2][q]};
+  if (sp_delete(g, c, f)) {
+    key[q] = -1;
+    return;
+  }
+  if (wrap) {
+    sp_wrap(g, c, f);
+    p.c[0][q] = (int16_t)c[0];
+    p.c[1][q] = (int16_t)c[1];
+    p.c[2][q] = (int16_t)c[2];
+  }
+  const bool in = c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] <= g.nx && c[1] <= g.ny && c[2] <= g.nz;
+  const int32_t k = in ? (int32_t)lin_key(g, c[0], c[1], c[2]) : (int32_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  key[q] = k;
+  atomicAdd(&hist[k], 1);
+}
+
+__global__ void __launch_bounds__(kT) k_sp_scatter(ParticlesSplit src, ParticlesSplit dst, int64_t n,
+                                                    const int32_t *__restrict__ key,
+                                                    const int32_t *__restrict__ off,
+                                                    int32_t *__restrict__ cursor) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= n) return;
+  const int32_t k = key[q];
+  if (k < 0) return;
+  const int64_t at = off[k] + atomicAdd(&cursor[k], 1);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    dst.c[d][at] = src.c[d][q];
+    dst.f[d][at] = src.f[d][q];
+    dst.v[d][at] = src.v[d][q];
+  }
+  dst.sp[at] = src.sp[q];
+  dst.id[at] = src.id[q];
+}
+
+// Fixed-point CIC deposit (int64 atomics, order independent).
+__global__ void __launch_bounds__(kT) k_sp_deposit(ParticlesSplit p, int64_t n, GridDims g,
+                                                    unsigned long long *__restrict__ Nfix,
+                                                    int32_t *__restrict__ errors) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= n) return;
+  const int i = p.c[0][q], j = p.c[1][q], k = p.c[2][q];
+  if (i < 0 || j < 0 || k < 0 || i > g.nx || j > g.ny || k > g.nz) {
+    atomicAdd(errors, 1);
+    return;
+  }
+  const float fx = p.f[0][q], fy = p.f[1][q], fz = p.f[2][q];
+  const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+  const int s = p.sp[q] - 1;
+  const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
+  const int64_t base = (int64_t)i + X * j + XY * k;
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const float w = wx[a] * wy[b] * wz[c];
+        const unsigned long long W = (unsigned long long)(double(w) * kFix);
+        atomicAdd(&Nfix[(base + a + X * b + XY * c) * g.S + s], W);
+      }
+}
+
+// Integer guard folds (ParticlesGridify.f90:129-285, z, y, x) and
+// N = W * 2^-40 * n2p, C = sum_s N * charge.
+__device__ __forceinline__ int sp_fold_src(int m, int n, int flow, int src[2], bool *zero) {
+  *zero = false;
+  if (flow) {
+    if (m == 1 || m == n + 2) { *zero = true; return 0; }
+    if (m == 2) { src[0] = 2; src[1] = 1; return 2; }
+    if (m == n + 1) { src[0] = n + 1; src[1] = n + 2; return 2; }
+  } else {
+    if (m == 1) { src[0] = 1; src[1] = n + 1; return 2; }
+    if (m == 2) { src[0] = 2; src[1] = n + 2; return 2; }
+    if (m == n + 1) { src[0] = n + 1; src[1] = 1; return 2; }
+    if (m == n + 2) { src[0] = n + 2; src[1] = 2; return 2; }
+  }
+  src[0] = m;
+  return 1;
+}
+
+__global__ void __launch_bounds__(kT) k_sp_fold(GridDims g, const unsigned long long *__restrict__ Nfix,
+                                                 double *__restrict__ N, double *__restrict__ C,
+                                                 const double *consts) {
+  const int64_t node = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (node >= g.G) return;
+  const int i = (int)(node % g.X) + 1;
+  const int j = (int)((node / g.X) % g.Y) + 1;
+  const int k = (int)(node / ((int64_t)g.X * g.Y)) + 1;
+  int sx[2], sy[2], sz[2];
+  bool zx, zy, zz;
+  const int nxs = sp_fold_src(i, g.nx, g.flow[0], sx, &zx);
+  const int nys = sp_fold_src(j, g.ny, g.flow[1], sy, &zy);
+  const int nzs = sp_fold_src(k, g.nz, g.flow[2], sz, &zz);
+  double cacc = 0.0;
+  for (int s = 0; s < g.S; ++s) {
+    unsigned long long w = 0;
+    if (!zx && !zy && !zz)
+      for (int a = 0; a < nxs; ++a)
+        for (int b = 0; b < nys; ++b)
+          for (int c = 0; c < nzs; ++c) w += Nfix[node_index(g, sx[a], sy[b], sz[c]) * g.S + s];
+    const double v = (double)w * kFixInv * consts[s];
+    N[node * g.S + s] = v;
+    cacc = cacc + v * consts[8 + s];
+  }
+  C[node] = cacc;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision Jacobi
+
+struct MixCoef {
+  double ax, ay, az, af, dx2, dy2, dz2;
+  float axf, ayf, azf, aff, idx2f, idy2f, idz2f;
+};
+
+
+
+// neighbours of interior cell c with periodic wrap (flow guards read as is)
+struct Nb {
+  int64_t e, w, n, s, t, b;
+};
+
+__device__ __forceinline__ Nb neighbours(const GridDims &g, int i, int j, int k, int64_t c) {
+  const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
+  Nb r;
+  r.e = (!g.flow[0] && i == g.nx + 1) ? c - (int64_t)(g.nx - 1) : c + 1;
+  r.w = (!g.flow[0] && i == 2) ? c + (int64_t)(g.nx - 1) : c - 1;
+  r.n = (!g.flow[1] && j == g.ny + 1) ? c - (int64_t)(g.ny - 1) * X : c + X;
+  r.s = (!g.flow[1] && j == 2) ? c + (int64_t)(g.ny - 1) * X : c - X;
+  r.t = (!g.flow[2] && k == g.nz + 1) ? c - (int64_t)(g.nz - 1) * XY : c + XY;
+  r.b = (!g.flow[2] && k == 2) ? c + (int64_t)(g.nz - 1) * XY : c - XY;
+  return r;
+}
+
+__device__ __forceinline__ double sum_partials(const double *__restrict__ P, int n, double *sh) {
+  double v = 0.0;
+  for (int q = lin_tid(); q < n; q += 256) v = v + P[q];
+  return block_sum(v, sh);
+}
+
+// One launch of the mixed-precision solve.  Launch number `seq` reads the
+// stop state of launch seq-1 and writes its own (block 0).  If `check` >= 1,
+// the partials `Pin` of the previous launch hold the residual of iterate
+// `check`, and the stop test of loop iteration check+1 is applied to it
+// (CalcEpotential.f90:60-62); every block evaluates the same fixed-order sum,
+// so all reach the same decision without atomics.
+struct MxLaunch {
+  int64_t seq;
+  int64_t base;          // iterate held by u
+  int64_t j;             // inner sweep: e_j -> e_{j+1}
+  int64_t check;         // iterate whose residual Pin holds, or -1
+  int32_t m;             // inner sweeps of this outer step (update launch)
+};
+
+__device__ __forceinline__ StopSlot mx_decide(JacobiState *st, const MxLaunch &L,
+                                              const double *__restrict__ Pin, int nblk, double *sh,
+                                              int *decide) {
+  const StopSlot prev = st->slot[(L.seq - 1) & 1];
+  StopSlot cur = prev;
+  if (!prev.done && L.check >= 1) {
+    const double S = sum_partials(Pin, nblk, sh);
+    if (lin_tid() == 0) *decide = (L.check + 1 <= st->maxit && sqrt(S) <= st->thresh) ? 1 : 0;
+    __syncthreads();
+    if (*decide) {
+      cur.done = 1;
+      cur.result = L.check;
+    }
+  }
+  return cur;
+}
+
+// Outer step: r = b - A u in FP64 (fresh halos), r32 = r; partials of |r|^2.
+__global__ void __launch_bounds__(256) k_mx_residual(GridDims g, MixCoef mc, TileSched ts,
+                                                      const double *__restrict__ u,
+                                                      const double *__restrict__ b,
+                                                      float *__restrict__ r32,
+                                                      const double *__restrict__ Pin,
+                                                      double *__restrict__ Pout, JacobiState *st,
+                                                      MxLaunch L) {
+  __shared__ double sh[4];
+  __shared__ int decide;
+  const StopSlot cur = mx_decide(st, L, Pin, ts.B, sh, &decide);
+  if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
+  if (cur.done) return;
+  double r2 = 0.0;
+  for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int k, bool valid) {
+    if (!valid) return;
+    const int64_t c = node_index(g, i, j, k);
+    const Nb nb = neighbours(g, i, j, k, c);
+    const double u2 = 2.0 * u[c];
+    const double r = b[c] - ((u[nb.e] + u[nb.w] - u2) / mc.dx2 + (u[nb.n] + u[nb.s] - u2) / mc.dy2 +
+                             (u[nb.t] + u[nb.b] - u2) / mc.dz2);
+    r32[c] = (float)r;
+    r2 = r2 + r * r;
+  });
+  const double sblk = block_sum(r2, sh);
+  if (lin_tid() == 0) Pout[blockIdx.x] = sblk;
+}
+
+// Inner FP32 sweep e_j -> e_{j+1} (e_0 = 0; e_k lives in E[k % 3]) and, for
+// j >= 1, partials of the residual of iterate base+j, r - A e_j, accumulated
+// in FP64.  Flow guards of e are zero: the correction keeps the boundary.
+__global__ void __launch_bounds__(256) k_mx_sweep(GridDims g, MixCoef mc, TileSched ts,
+                                                   const float *__restrict__ r32, float *E0, float *E1,
+                                                   float *E2, const double *__restrict__ Pin,
+                                                   double *__restrict__ Pout, JacobiState *st,
+                                                   MxLaunch L) {
+  __shared__ double sh[4];
+  __shared__ int decide;
+  StopSlot cur = mx_decide(st, L, Pin, ts.B, sh, &decide);
+  if (!cur.done && L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
+    cur.done = 1;
+    cur.result = st->last;
+  }
+  if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
+  if (cur.done) return;
+  float *E[3] = {E0, E1, E2};
+  const float *e = L.j >= 1 ? E[L.j % 3] : nullptr;
+  float *out = E[(L.j + 1) % 3];
+  const bool res = L.j >= 1;
+  double r2 = 0.0;
+  for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int k, bool valid) {
+    if (!valid) return;
+    const int64_t c = node_index(g, i, j, k);
+    const float rc = r32[c];
+    if (!e) {
+      out[c] = -mc.aff * rc;
+      return;
+    }
+    const Nb nb = neighbours(g, i, j, k, c);
+    const float ee = (g.flow[0] && i == g.nx + 1) ? 0.0f : e[nb.e];
+    const float ew = (g.flow[0] && i == 2) ? 0.0f : e[nb.w];
+    const float en = (g.flow[1] && j == g.ny + 1) ? 0.0f : e[nb.n];
+    const float es = (g.flow[1] && j == 2) ? 0.0f : e[nb.s];
+    const float et = (g.flow[2] && k == g.nz + 1) ? 0.0f : e[nb.t];
+    const float eb = (g.flow[2] && k == 2) ? 0.0f : e[nb.b];
+    out[c] = mc.axf * (ee + ew) + mc.ayf * (en + es) + mc.azf * (et + eb) - mc.aff * rc;
+    if (res) {
+      const float e2 = 2.0f * e[c];
+      const float rr = rc - ((ee + ew - e2) * mc.idx2f + (en + es - e2) * mc.idy2f +
+                             (et + eb - e2) * mc.idz2f);
+      r2 = r2 + (double)rr * (double)rr;
+    }
+  });
+  const double sblk = block_sum(r2, sh);
+  if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
+}
+
+// u += e_k: k = m after a full outer step, or the converged (or last)
+// iterate's k when the solve ended inside this step.
+__global__ void __launch_bounds__(256) k_mx_update(GridDims g, TileSched ts, double *__restrict__ u,
+                                                    const float *E0, const float *E1,
+                                                    const float *E2, const double *__restrict__ Pin,
+                                                    JacobiState *st, MxLaunch L) {
+  __shared__ double sh[4];
+  __shared__ int decide;
+  const StopSlot prev = st->slot[(L.seq - 1) & 1];
+  StopSlot cur = mx_decide(st, L, Pin, ts.B, sh, &decide);
+  if (!cur.done && L.base + L.m >= st->last) {
+    cur.done = 1;
+    cur.result = st->last;
+  }
+  if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
+  // solves that ended before this outer step have nothing left to apply
+  if (prev.done && prev.result < L.base) return;
+  const int64_t k = cur.done ? cur.result - L.base : (int64_t)L.m;
+  if (k <= 0) return;
+  const float *E[3] = {E0, E1, E2};
+  const float *e = E[k % 3];
+  for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int kk, bool valid) {
+    if (!valid) return;
+    const int64_t c = node_index(g, i, j, kk);
+    u[c] = u[c] + (double)e[c];
+  });
+}
+
+// Periodic halo fill of U in place: every guard node of a periodic dimension
+// takes the value at its composite image (z, then y, then x plane copies).
+__global__ void __launch_bounds__(kT) k_mx_halo(GridDims g, double *__restrict__ U) {
+  const int64_t node = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (node >= g.G) return;
+  const int i = (int)(node % g.X) + 1;
+  const int j = (int)((node / g.X) % g.Y) + 1;
+  const int k = (int)(node / ((int64_t)g.X * g.Y)) + 1;
+  int si = i, sj = j, sk = k;
+  if (!g.flow[0]) si = i == 1 ? g.nx + 1 : (i == g.nx + 2 ? 2 : i);
+  if (!g.flow[1]) sj = j == 1 ? g.ny + 1 : (j == g.ny + 2 ? 2 : j);
+  if (!g.flow[2]) sk = k == 1 ? g.nz + 1 : (k == g.nz + 2 ? 2 : k);
+  const int64_t src = node_index(g, si, sj, sk);
+  if (src != node) U[node] = U[src];
+}
+
+// E = -grad U as FP32, with the same halo rules as the FP64 form.
+__global__ void __launch_bounds__(kT) k_sp_efield(GridDims g, const double *__restrict__ U,
+                                                   float *__restrict__ E) {
+  const int64_t node = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (node >= g.G) return;
+  const int i = (int)(node % g.X) + 1;
+  const int j = (int)((node / g.X) % g.Y) + 1;
+  const int k = (int)(node / ((int64_t)g.X * g.Y)) + 1;
+  int si = i, sj = j, sk = k;
+  if (!g.flow[0]) si = i == 1 ? g.nx + 1 : (i == g.nx + 2 ? 2 : i);
+  if (!g.flow[1]) sj = j == 1 ? g.ny + 1 : (j == g.ny + 2 ? 2 : j);
+  if (!g.flow[2]) sk = k == 1 ? g.nz + 1 : (k == g.nz + 2 ? 2 : k);
+  double ex = 0.0, ey = 0.0, ez = 0.0;
+  if (si >= 2 && si <= g.nx + 1 && sj >= 2 && sj <= g.ny + 1 && sk >= 2 && sk <= g.nz + 1) {
+    const int64_t c = node_index(g, si, sj, sk);
+    const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
+    ez = -(U[c + XY] - U[c - XY]) * 0.5 / g.dxyz[2];
+    ey = -(U[c + X] - U[c - X]) * 0.5 / g.dxyz[1];
+    ex = -(U[c + 1] - U[c - 1]) * 0.5 / g.dxyz[0];
+  }
+  E[4 * node + 0] = (float)ex;
+  E[4 * node + 1] = (float)ey;
+  E[4 * node + 2] = (float)ez;
+  E[4 * node + 3] = 0.0f;
+}
+
+struct BorisF32 {
+  float etadt2[8];
+  float tB[8][3], sB[8][3];
+  float ve[3];
+};
+
+// AdvanceVelocity.f90:39-121 in FP32: CIC gather of E (float4 per node) and
+// the Boris rotation in the ve frame.
+__global__ void __launch_bounds__(kT) k_sp_boris(ParticlesSplit p, int64_t n, GridDims g,
+                                                  const float4 *__restrict__ E, BorisF32 bc) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= n) return;
+  const int i = p.c[0][q], j = p.c[1][q], k = p.c[2][q];
+  if (i < 0 || j < 0 || k < 0 || i > g.nx || j > g.ny || k > g.nz) return;
+  const float fx = p.f[0][q], fy = p.f[1][q], fz = p.f[2][q];
+  const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+  const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
+  const int64_t base = (int64_t)i + X * j + XY * k;
+  float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const float w = wx[a] * wy[b] * wz[c];
+        const float4 e = E[base + a + X * b + XY * c];
+        ex += e.x * w;
+        ey += e.y * w;
+        ez += e.z * w;
+      }
+  const int s = p.sp[q] - 1;
+  const float h = bc.etadt2[s];
+  const float *t = bc.tB[s], *sb = bc.sB[s];
+  const float vm1 = p.v[0][q] - bc.ve[0] + h * ex;
+  const float vm2 = p.v[1][q] - bc.ve[1] + h * ey;
+  const float vm3 = p.v[2][q] - bc.ve[2] + h * ez;
+  const float vd1 = vm1 + vm2 * t[2] - vm3 * t[1];
+  const float vd2 = vm2 + vm3 * t[0] - vm1 * t[2];
+  const float vd3 = vm3 + vm1 * t[1] - vm2 * t[0];
+  const float vp1 = vm1 + vd2 * sb[2] - vd3 * sb[1];
+  const float vp2 = vm2 + vd3 * sb[0] - vd1 * sb[2];
+  const float vp3 = vm3 + vd1 * sb[1] - vd2 * sb[0];
+  p.v[0][q] = vp1 + bc.ve[0] + h * ex;
+  p.v[1][q] = vp2 + bc.ve[1] + h * ey;
+  p.v[2][q] = vp3 + bc.ve[2] + h * ez;
+}
+
+// Host FP64 AoS <-> split.
+__global__ void __launch_bounds__(kT) k_sp_from_aos(const double *__restrict__ stage,
+                                                     const int32_t *__restrict__ stage_sp,
+                                                     int64_t count, ParticlesSplit p, int64_t at,
+                                                     GridDims g, uint32_t id0) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= count) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double xi = (stage[8 * q + d] - g.lmin[d]) / g.dxyz[d] + 0.5;
+    double fl = floor(xi);
+    float f = (float)(xi - fl);
+    if (f >= 1.0f) {
+      f = 0.0f;
+      fl += 1.0;
+    }
+    p.c[d][at + q] = (int16_t)fl;
+    p.f[d][at + q] = f;
+    p.v[d][at + q] = (float)stage[8 * q + 3 + d];
+  }
+  p.sp[at + q] = (uint8_t)stage_sp[q];
+  p.id[at + q] = id0 + (uint32_t)q;
+}
+
+__global__ void __launch_bounds__(kT) k_sp_to_aos(ParticlesSplit p, int64_t first, int64_t count,
+                                                   GridDims g, double *__restrict__ stage,
+                                                   int32_t *__restrict__ stage_sp) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= count) return;
+  const int64_t s = first + q;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double xi = (double)p.c[d][s] + (double)p.f[d][s];
+    stage[8 * q + d] = g.lmin[d] + (xi - 0.5) * g.dxyz[d];
+    stage[8 * q + 3 + d] = (double)p.v[d][s];
+  }
+  stage[8 * q + 6] = 0.0;
+  stage[8 * q + 7] = (double)p.id[s];
+  stage_sp[q] = p.sp[s];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+int sp_sort(picard_ctx *c, int wrap) {
+  const GridDims &g = c->gd;
+  const int64_t n = c->n;
+  const int64_t K = c->K;                 // cells; bucket K collects out-of-grid keepers
+  PICARD_HIP(hipMemsetAsync(c->s.hist, 0, sizeof(int32_t) * (K + 2), c->stream));
+  PICARD_HIP(hipMemsetAsync(c->s.cursor, 0, sizeof(int32_t) * (K + 1), c->stream));
+  if (n > 0) {
+    k_sp_flag<<<grid_blocks(n, kT), kT, 0, c->stream>>>(c->ps, n, g, wrap, c->s.key, c->s.hist);
+    PICARD_HIP(hipGetLastError());
+  }
+  PICARD_TRY(scan_exclusive_i32(c, c->s.hist, c->s.hist, K + 1));
+  if (n > 0) {
+    k_sp_scatter<<<grid_blocks(n, kT), kT, 0, c->stream>>>(c->ps, c->ps2, n, c->s.key, c->s.hist,
+                                                           c->s.cursor);
+    PICARD_HIP(hipGetLastError());
+  }
+  PICARD_HIP(hipMemcpyAsync(c->host_counters, c->s.hist + K + 1, sizeof(int32_t),
+                            hipMemcpyDeviceToHost, c->stream));
+  PICARD_HIP(hipStreamSynchronize(c->stream));
+  const ParticlesSplit tmp = c->ps;
+  c->ps = c->ps2;
+  c->ps2 = tmp;
+  c->n = c->host_counters[0];
+  return PICARD_OK;
+}
+
+int sp_push(picard_ctx *c, double dt) {
+  if (c->n <= 0) return PICARD_OK;
+  const float3 dtdx = make_float3((float)(dt / c->gd.dxyz[0]), (float)(dt / c->gd.dxyz[1]),
+                                  (float)(dt / c->gd.dxyz[2]));
+  k_sp_push<<<grid_blocks(c->n, kT), kT, 0, c->stream>>>(c->ps, c->n, dtdx);
+  PICARD_HIP(hipGetLastError());
+  return PICARD_OK;
+}
+
+int sp_gridify(picard_ctx *c) {
+  const GridDims &g = c->gd;
+  PICARD_HIP(hipMemsetAsync(c->Nfix, 0, sizeof(unsigned long long) * g.S * g.G, c->stream));
+  PICARD_HIP(hipMemsetAsync(c->s.counters, 0, sizeof(int32_t), c->stream));
+  if (c->n > 0) {
+    k_sp_deposit<<<grid_blocks(c->n, kT), kT, 0, c->stream>>>(c->ps, c->n, g, c->Nfix,
+                                                              c->s.counters);
+    PICARD_HIP(hipGetLastError());
+  }
+  k_sp_fold<<<grid_blocks(g.G, kT), kT, 0, c->stream>>>(g, c->Nfix, c->N, c->C, c->consts);
+  PICARD_HIP(hipGetLastError());
+  PICARD_HIP(hipMemcpyAsync(c->host_counters, c->s.counters, sizeof(int32_t),
+                            hipMemcpyDeviceToHost, c->stream));
+  PICARD_HIP(hipStreamSynchronize(c->stream));
+  if (c->host_counters[0] != 0) {
+    set_error("ParticlesGridify (split): " + std::to_string(c->host_counters[0]) +
+              " particle(s) outside the grid were not deposited");
+    return PICARD_ERR_RANGE;
+  }
+  return PICARD_OK;
+}
+
+int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) {
+  const GridDims &g = c->gd;
+  const double dx = g.dxyz[0], dy = g.dxyz[1], dz = g.dxyz[2];
+  const double den = 1.0 / ((dx * dy) * (dx * dy) + (dx * dz) * (dx * dz) + (dy * dz) * (dy * dz));
+  MixCoef mc;
+  mc.ax = 0.5 * den * ((dy * dz) * (dy * dz));
+  mc.ay = 0.5 * den * ((dx * dz) * (dx * dz));
+  mc.az = 0.5 * den * ((dx * dy) * (dx * dy));
+  mc.af = 0.5 * den * ((dx * dy * dz) * (dx * dy * dz));
+  mc.dx2 = dx * dx;
+  mc.dy2 = dy * dy;
+  mc.dz2 = dz * dz;
+  mc.axf = (float)mc.ax; mc.ayf = (float)mc.ay; mc.azf = (float)mc.az; mc.aff = (float)mc.af;
+  mc.idx2f = (float)(1.0 / mc.dx2); mc.idy2f = (float)(1.0 / mc.dy2); mc.idz2f = (float)(1.0 / mc.dz2);
+  JacobiState init{};
+  init.maxit = maxit;
+  init.last = 2 * (int64_t)maxit + 1;
+  PICARD_HIP(hipMemcpyAsync(c->js, &init, sizeof(JacobiState), hipMemcpyHostToDevice, c->stream));
+  PICARD_TRY(launch_bnorm(c, maxerr));      // launch 0: b, |b|, the test of ss = 1
+  const TileSched ts = make_sched(g.nx, g.ny, g.nz);
+  const int m = c->mx_inner;
+  double *P[2] = {c->s.partials, c->s.partials + c->s.partials_len};
+  float *E0 = c->e32[0], *E1 = c->e32[1], *E2 = c->e32[2];
+  int64_t seq = 1, base = 0;
+  int pw = 0;                               // partials buffer the next writer uses
+  int64_t check = -1;                       // iterate whose residual the last writer stored
+  StopSlot fin{};
+  while (true) {
+    if (c->timing) {
+      while ((int64_t)c->tev.size() < 2 * m) {
+        hipEvent_t e;
+        PICARD_HIP(hipEventCreate(&e));
+        c->tev.push_back(e);
+      }
+    }
+    // outer residual of iterate `base` (its stop test happens in the next launch)
+    {
+      MxLaunch L{seq, base, 0, -1, m};
+      k_mx_residual<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, mc, ts, c->U, c->b, c->r32,
+                                                                P[pw ^ 1], P[pw], c->js, L);
+      ++seq;
+      check = base;
+      pw ^= 1;
+    }
+    for (int j = 0; j < m; ++j) {
+      // the iterate-0 test used b itself (launch 0); later ones use residuals
+      const int64_t chk = (check >= 1) ? check : -1;
+      MxLaunch L{seq, base, j, chk, m};
+      const double *pin = P[pw ^ 1];
+      double *pout = P[pw];
+      if (c->timing)
+        hipExtLaunchKernelGGL(k_mx_sweep, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
+                              c->tev[2 * j], c->tev[2 * j + 1], 0, g, mc, ts,
+                              (const float *)c->r32, E0, E1, E2, pin, pout, c->js, L);
+      else
+        k_mx_sweep<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, mc, ts, c->r32, E0, E1, E2, pin,
+                                                               pout, c->js, L);
+      ++seq;
+      if (j >= 1) {               // wrote the residual of iterate base + j
+        check = base + j;
+        pw ^= 1;
+      } else {
+        check = -1;
+      }
+    }
+    {
+      const int64_t chk = (check >= 1) ? check : -1;
+      MxLaunch L{seq, base, 0, chk, m};
+      k_mx_update<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, ts, c->U, E0, E1, E2, P[pw ^ 1],
+                                                              c->js, L);
+      ++seq;
+      check = -1;
+    }
+    PICARD_HIP(hipGetLastError());
+    PICARD_HIP(hipMemcpyAsync(c->js_host, c->js, sizeof(JacobiState), hipMemcpyDeviceToHost,
+                              c->stream));
+    PICARD_HIP(hipStreamSynchronize(c->stream));
+    if (c->timing) {
+      for (int q = 0; q < m; ++q) {
+        float ms = 0.0f;
+        PICARD_HIP(hipEventElapsedTime(&ms, c->tev[2 * q], c->tev[2 * q + 1]));
+        c->t_ms += ms;
+      }
+      c->t_launches += m;
+    }
+    fin = c->js_host->slot[(seq - 1) & 1];
+    if (fin.done) break;
+    base += m;
+  }
+  k_mx_halo<<<grid_blocks(g.G, kT), kT, 0, c->stream>>>(g, c->U);
+  PICARD_HIP(hipGetLastError());
+  *sweeps = fin.result;
+  c->last_sweeps = fin.result;
+  return fin.result == init.last ? PICARD_WARN_NOCONV : PICARD_OK;
+}
+
+int sp_efield(picard_ctx *c) {
+  k_sp_efield<<<grid_blocks(c->gd.G, kT), kT, 0, c->stream>>>(c->gd, c->U, c->E32);
+  PICARD_HIP(hipGetLastError());
+  return PICARD_OK;
+}
+
+int sp_boris(picard_ctx *c, double dt) {
+  if (c->n <= 0) return PICARD_OK;
+  BorisF32 bc;
+  for (int s = 0; s < 8; ++s) {
+    bc.etadt2[s] = (float)(0.5 * dt * c->spc.eta[s]);
+    for (int q = 0; q < 3; ++q) {
+      bc.tB[s][q] = (float)c->spc.tB[4 * s + q];
+      bc.sB[s][q] = (float)c->spc.sB[4 * s + q];
+    }
+  }
+  for (int q = 0; q < 3; ++q) bc.ve[q] = (float)c->spc.ve[q];
+  k_sp_boris<<<grid_blocks(c->n, kT), kT, 0, c->stream>>>(c->ps, c->n, c->gd,
+                                                          reinterpret_cast<const float4 *>(c->E32), bc);
+  PICARD_HIP(hipGetLastError());
+  return PICARD_OK;
+}
+
+int sp_append(picard_ctx *c, const double *host_rp, const int32_t *host_sp, int64_t count) {
+  if (count <= 0) return PICARD_OK;
+  if (c->n + count > c->cap) {
+    set_error("particles_append: capacity max_particles exceeded");
+    return PICARD_ERR_CAPACITY;
+  }
+  const int64_t chunk = int64_t(1) << 21;
+  PICARD_TRY(ensure_stage(c, count < chunk ? count : chunk));
+  for (int64_t done = 0; done < count; done += chunk) {
+    const int64_t m = (count - done) < chunk ? (count - done) : chunk;
+    PICARD_HIP(hipMemcpyAsync(c->s.stage, host_rp + 8 * done, sizeof(double) * 8 * m,
+                              hipMemcpyHostToDevice, c->stream));
+    PICARD_HIP(hipMemcpyAsync(c->s.stage_sp, host_sp + done, sizeof(int32_t) * m,
+                              hipMemcpyHostToDevice, c->stream));
+    k_sp_from_aos<<<grid_blocks(m, kT), kT, 0, c->stream>>>(c->s.stage, c->s.stage_sp, m, c->ps,
+                                                            c->n + done, c->gd,
+                                                            (uint32_t)(c->next_id + done));
+    PICARD_HIP(hipGetLastError());
+    PICARD_HIP(hipStreamSynchronize(c->stream));
+  }
+  c->n += count;
+  c->next_id += count;
+  return PICARD_OK;
+}
+
+int sp_read(picard_ctx *c, int64_t first, int64_t count, double *host_rp, int32_t *host_sp) {
+  if (count <= 0) return PICARD_OK;
+  if (first < 0 || first + count > c->n) {
+    set_error("particles_read: range outside [0, num_local)");
+    return PICARD_ERR_ARG;
+  }
+  const int64_t chunk = int64_t(1) << 21;
+  PICARD_TRY(ensure_stage(c, count < chunk ? count : chunk));
+  for (int64_t done = 0; done < count; done += chunk) {
+    const int64_t m = (count - done) < chunk ? (count - done) : chunk;
+    k_sp_to_aos<<<grid_blocks(m, kT), kT, 0, c->stream>>>(c->ps, first + done, m, c->gd, c->s.stage,
+                                                          c->s.stage_sp);
+    PICARD_HIP(hipGetLastError());
+    PICARD_HIP(hipMemcpyAsync(host_rp + 8 * done, c->s.stage, sizeof(double) * 8 * m,
+                              hipMemcpyDeviceToHost, c->stream));
+    if (host_sp)
+      PICARD_HIP(hipMemcpyAsync(host_sp + done, c->s.stage_sp, sizeof(int32_t) * m,
+                                hipMemcpyDeviceToHost, c->stream));
+    PICARD_HIP(hipStreamSynchronize(c->stream));
+  }
+  return PICARD_OK;
+}
+
+// E as FP64 (3 per node) for field_read in the split form.
+namespace {
+__global__ void k_sp_e_widen(int64_t G, const float *__restrict__ E32, double *__restrict__ E) {
+  const int64_t node = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (node >= G) return;
+  for (int d = 0; d < 3; ++d) E[3 * node + d] = (double)E32[4 * node + d];
+}
+}  // namespace
+
+int sp_widen_e(picard_ctx *c) {
+  k_sp_e_widen<<<grid_blocks(c->gd.G, kT), kT, 0, c->stream>>>(c->gd.G, c->E32, c->E);
+  PICARD_HIP(hipGetLastError());
+  return PICARD_OK;
+}
+
+}  // namespace picard

@@ -328,6 +328,19 @@ def species_struct(cfg: Config, d: Derived) -> _lib.Species:
     return s
 
 
+def plasma_struct(d: Derived) -> _lib.Plasma:
+    pl = _lib.Plasma()
+    for q in range(8):
+        pl.ipp[q] = int(d.ipp[q])
+        pl.epp[q] = int(d.epp[q])
+        pl.hpp[q] = int(d.hpp[q])
+        pl.mass[q] = d.mass[q]
+        pl.kelvin[q] = d.kelvin[q]
+        for c in range(4):
+            pl.v0[4 * q + c] = d.v0[q, c]
+    return pl
+
+
 def _dp(a):
     return a.ctypes.data_as(ct.POINTER(ct.c_double))
 
@@ -340,13 +353,14 @@ class Domain:
     """One device-resident domain; methods follow the reference's subroutines."""
 
     def __init__(self, cfg: Config, max_particles: int | None = None, device: int = 0,
-                 d: Derived | None = None):
+                 d: Derived | None = None, precision: int = _lib.PICARD_FP64):
         self.cfg = cfg
         self.d = derive(cfg) if d is None else d
         if max_particles is None:
             per_cell = int(self.d.ppc[: cfg.species].sum())
             max_particles = int(1.5 * per_cell * (cfg.nx + 2) * (cfg.ny + 2) * (cfg.nz + 2)) + 1024
-        self.geo = geometry(cfg, self.d, max_particles, device)
+        self.geo = geometry(cfg, self.d, max_particles, device, precision)
+        self.precision = precision
         self.spc = species_struct(cfg, self.d)
         h = ct.c_void_p()
         check("picard_create", lib().picard_create(ct.byref(self.geo), ct.byref(self.spc),
@@ -452,6 +466,17 @@ class Domain:
         check("picard_step", lib().picard_step(self.h, self.d.dt, self.cfg.maxerr, self.cfg.maxit,
                                                _dp(rp), _ip(sp), rp.shape[0], ct.byref(st)))
         return st
+
+    def generate_initial(self, seed: int):
+        """ParticlesInitialize on the device (Philox-keyed by seed)."""
+        pl = plasma_struct(self.d)
+        check("picard_generate_initial", lib().picard_generate_initial(self.h, ct.byref(pl), seed))
+
+    def generate_inflow(self, seed: int, step: int):
+        """BCinflow on the device for one step."""
+        pl = plasma_struct(self.d)
+        check("picard_generate_inflow",
+              lib().picard_generate_inflow(self.h, ct.byref(pl), seed, step))
 
     def timing(self, enable: bool):
         check("picard_timing_enable", lib().picard_timing_enable(self.h, 1 if enable else 0))
