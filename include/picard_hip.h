@@ -107,6 +107,28 @@ int picard_generate_initial(picard_ctx *ctx, const picard_plasma *plasma, uint64
 int picard_generate_inflow(picard_ctx *ctx, const picard_plasma *plasma, uint64_t seed,
                            int64_t step);
 
+/* Haser-model production parameters (picard.f90:258-302 set them; species
+ * index s-1 holds species s).  rate_scale multiplies the expected ion count
+ * per cell of ParticlesProduce.f90:70: 1.0 evaluates the line as written (a
+ * rate per second taken as a count per step), dt makes it a per-step count. */
+typedef struct {
+  double Qn[8], vn[8], nu_i[8], nu_d[8], Ek[8];
+  double rate_scale;
+} picard_haser;
+
+/* ParticlesProduce (ParticlesProduce.f90:1-113) on the device: the same
+ * cells, loop order, counts and companion electrons/protons, drawn from the
+ * counter-based generator keyed by `seed` and counted by `step`.  Appends
+ * after num_local; *produced receives the number of particles appended. */
+int picard_particles_produce(picard_ctx *ctx, const picard_plasma *plasma,
+                             const picard_haser *haser, uint64_t seed, int64_t step, double dt,
+                             int64_t *produced);
+/* Make picard_step run picard_particles_produce after AdvancePosition, where
+ * the reference calls ParticlesProduce (picard.f90:566), with step = 1, 2, ...
+ * counted by picard_step.  haser == NULL switches the source off. */
+int picard_set_production(picard_ctx *ctx, const picard_plasma *plasma,
+                          const picard_haser *haser, uint64_t seed);
+
 /* ---- the time-step call surface (picard.f90:564-631) ---- */
 
 /* AdvancePosition(real_particles, dt, max_per_proc, num_local)   AdvancePosition.f90:4 */

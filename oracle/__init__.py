@@ -293,3 +293,34 @@ def ref_advance_velocity(g: Grid, rp, sp, E, B0, dt, v0, ve, eta, tB, sB, n, flu
                        _d(_f8(ve, 4)), _d(_f8(eta)), _d(tB), _d(sB), _byref_i(n),
                        _byref_i(rp.shape[0]), _byref_i(g.species), _byref_i(g.nx),
                        _byref_i(g.ny), _byref_i(g.nz), _i(fluid), _i(gyration))
+
+
+def ref_particles_produce(g: Grid, rp, sp, n, seed, *, kelvin, mass, nu_d, nu_i, Ek, Qn, vn, v0,
+                          ve, ppc, epp, hpp, ipp, dt, n2p):
+    """Seeds the runtime generator with `seed`, then calls the reference's
+    ParticlesProduce(real_particles, int_particles, Lx_min, Ly_min, Lz_min,
+    dxyz, kelvin, mass, nu_d, nu_i, Ek, Qn, vn, v0, ve, max_per_proc,
+    num_local, ppc, epp, hpp, ipp, species, Nx_local, Ny_local, Nz_local, dt,
+    n2p) (ParticlesProduce.f90:1-3).  Returns the new num_local."""
+    R = ref()
+    R.picard_ref_seed(ct.c_int32(seed))
+    nl = ct.c_int32(n)
+    v0 = np.ascontiguousarray(np.asarray(v0, np.float64).reshape(8, 4))
+    R.particlesproduce_(_d(rp), _i(sp), _byref_d(g.gmin[0]), _byref_d(g.gmin[1]),
+                        _byref_d(g.gmin[2]), _d(np.asarray(g.dxyz, np.float64)), _d(_f8(kelvin)),
+                        _d(_f8(mass)), _d(_f8(nu_d)), _d(_f8(nu_i)), _d(_f8(Ek)), _d(_f8(Qn)),
+                        _d(_f8(vn)), _d(v0), _d(_f8(ve, 4)), _byref_i(rp.shape[0]), ct.byref(nl),
+                        _i(_i8(ppc)), _i(_i8(epp)), _i(_i8(hpp)), _i(_i8(ipp)),
+                        _byref_i(g.species), _byref_i(g.nx), _byref_i(g.ny), _byref_i(g.nz),
+                        _byref_d(dt), _d(_f8(n2p)))
+    return nl.value
+
+
+def ref_uniforms(seed, count):
+    """The first `count` draws of the runtime's random_number after seeding
+    with `seed` (the stream ref_particles_produce consumes)."""
+    R = ref()
+    R.picard_ref_seed(ct.c_int32(seed))
+    out = np.zeros(count, np.float64)
+    R.picard_ref_uniforms(_d(out), ct.c_int64(count))
+    return out

@@ -35,7 +35,7 @@ EXPORTS = (
     "picard_bc_potentials", "picard_calc_epotential", "picard_calc_efield", "picard_bc_fields",
     "picard_advance_velocity", "picard_field_read", "picard_field_write", "picard_step",
     "picard_timing_enable", "picard_timing_read", "picard_generate_initial",
-    "picard_generate_inflow",
+    "picard_generate_inflow", "picard_particles_produce", "picard_set_production",
 )
 
 
@@ -66,6 +66,11 @@ class Species(ct.Structure):
 class Plasma(ct.Structure):
     _fields_ = [("ipp", ct.c_int32 * 8), ("epp", ct.c_int32 * 8), ("hpp", ct.c_int32 * 8),
                 ("mass", ct.c_double * 8), ("kelvin", ct.c_double * 8), ("v0", ct.c_double * 32)]
+
+
+class Haser(ct.Structure):
+    _fields_ = [("Qn", ct.c_double * 8), ("vn", ct.c_double * 8), ("nu_i", ct.c_double * 8),
+                ("nu_d", ct.c_double * 8), ("Ek", ct.c_double * 8), ("rate_scale", ct.c_double)]
 
 
 class StepStats(ct.Structure):
@@ -120,6 +125,10 @@ def lib():
         "picard_timing_read": ([vp, dp, lp], ct.c_int),
         "picard_generate_initial": ([vp, ct.POINTER(Plasma), ct.c_uint64], ct.c_int),
         "picard_generate_inflow": ([vp, ct.POINTER(Plasma), ct.c_uint64, ct.c_int64], ct.c_int),
+        "picard_particles_produce": ([vp, ct.POINTER(Plasma), ct.POINTER(Haser), ct.c_uint64,
+                                      ct.c_int64, ct.c_double, lp], ct.c_int),
+        "picard_set_production": ([vp, ct.POINTER(Plasma), ct.POINTER(Haser), ct.c_uint64],
+                                  ct.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

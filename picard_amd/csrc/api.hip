@@ -122,6 +122,8 @@ void release(picard_ctx *c) {
   dfree(c->Nraw);
   dfree(c->C);
   dfree(c->D);
+  bcpot_fft_release(c);
+  dfree(c->haser_off);
   for (int q = 0; q < 3; ++q) dfree(c->jb[q]);
   dfree(c->b);
   dfree(c->consts);
@@ -380,6 +382,36 @@ int picard_generate_inflow(picard_ctx *c, const picard_plasma *pl, uint64_t seed
   return generate_inflow(c, pl, seed, step);
 }
 
+int picard_particles_produce(picard_ctx *c, const picard_plasma *pl, const picard_haser *hs,
+                             uint64_t seed, int64_t step, double dt, int64_t *produced) {
+  CTX_CHECK(c);
+  PICARD_TRY(check_plasma(c, pl));
+  if (!hs || !(hs->rate_scale >= 0.0) || !(dt >= 0.0)) {
+    set_error("particles_produce: haser parameters missing, or rate_scale/dt negative");
+    return PICARD_ERR_ARG;
+  }
+  return produce_haser(c, pl, hs, seed, step, dt, produced);
+}
+
+int picard_set_production(picard_ctx *c, const picard_plasma *pl, const picard_haser *hs,
+                          uint64_t seed) {
+  CTX_CHECK(c);
+  if (!hs) {
+    c->prod_on = false;
+    return PICARD_OK;
+  }
+  PICARD_TRY(check_plasma(c, pl));
+  if (!(hs->rate_scale >= 0.0)) {
+    set_error("set_production: rate_scale must be >= 0");
+    return PICARD_ERR_ARG;
+  }
+  c->prod_plasma = *pl;
+  c->prod_haser = *hs;
+  c->prod_seed = seed;
+  c->prod_on = true;
+  return PICARD_OK;
+}
+
 int picard_advance_position(picard_ctx *c, double dt) {
   CTX_CHECK(c);
   if (c->split) return sp_push(c, dt);
@@ -421,7 +453,7 @@ int picard_calc_distance(picard_ctx *c) {
 
 int picard_bc_potentials(picard_ctx *c) {
   CTX_CHECK(c);
-  return launch_bc_potentials(c);
+  return c->split ? launch_bc_potentials_fft(c) : launch_bc_potentials(c);
 }
 
 int picard_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) {
@@ -477,7 +509,10 @@ int picard_field_write(picard_ctx *c, int32_t which, const double *host, int64_t
   PICARD_HIP(hipMemcpyAsync(field_ptr(c, which), host, sizeof(double) * len,
                             hipMemcpyHostToDevice, c->stream));
   PICARD_HIP(hipStreamSynchronize(c->stream));
-  if (which == PICARD_FIELD_D) c->have_D = true;
+  if (which == PICARD_FIELD_D) {
+    c->have_D = true;
+    c->fft_ready = false;
+  }
   return PICARD_OK;
 }
 
@@ -504,12 +539,15 @@ int picard_step(picard_ctx *c, double dt, double maxerr, int32_t maxit, const do
   if (c->split) {
     PICARD_TRY(sp_push(c, dt));                                          // picard.f90:564
     if (nproduced > 0) PICARD_TRY(picard_particles_append(c, produced, produced_species, nproduced));
+    if (c->prod_on)                                                      // :566
+      PICARD_TRY(produce_haser(c, &c->prod_plasma, &c->prod_haser, c->prod_seed, c->steps_done + 1,
+                               dt, nullptr));
     const int64_t before = c->n;
     PICARD_TRY(sp_sort(c, 1));                                           // :569, :572
     st.num_local = c->n;
     st.removed = before - c->n;
     PICARD_TRY(sp_gridify(c));                                           // :577
-    PICARD_TRY(launch_bc_potentials(c));                                 // :581
+    PICARD_TRY(launch_bc_potentials_fft(c));                             // :581
     rc = sp_epotential(c, maxerr, maxit, &st.sweeps);                    // :584
     if (rc < 0) return rc;
     PICARD_TRY(sp_efield(c));                                            // :587-589
@@ -518,6 +556,9 @@ int picard_step(picard_ctx *c, double dt, double maxerr, int32_t maxit, const do
   } else {
     PICARD_TRY(launch_push_x(c, 0, c->n, dt));                           // picard.f90:564
     if (nproduced > 0) PICARD_TRY(picard_particles_append(c, produced, produced_species, nproduced));
+    if (c->prod_on)                                                      // :566
+      PICARD_TRY(produce_haser(c, &c->prod_plasma, &c->prod_haser, c->prod_seed, c->steps_done + 1,
+                               dt, nullptr));
     PICARD_TRY(launch_outflow(c, &st.num_local, &st.removed));          // :569
     PICARD_TRY(launch_wrap(c));                                         // :572
     PICARD_TRY(launch_gridify(c));                                      // :577
@@ -529,6 +570,7 @@ int picard_step(picard_ctx *c, double dt, double maxerr, int32_t maxit, const do
     st.pushed = c->n;
     PICARD_TRY(launch_boris(c, dt));                                    // :630
   }
+  ++c->steps_done;
   if (stats) *stats = st;
   return rc;
 }

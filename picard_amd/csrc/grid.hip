@@ -422,38 +422,63 @@ __global__ void __launch_bounds__(256) k_jacobi_pass(GridDims g, JacobiCoef jc, 
   const bool wrap_old = (p - 2) >= 1;
   const bool px = !g.flow[0], py = !g.flow[1], pz = !g.flow[2];
   const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
-  double r2 = 0.0;
-  for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int k, bool valid) {
-    if (!valid) return;
-    const int64_t c = (int64_t)(i - 1) + X * (int64_t)(j - 1) + XY * (int64_t)(k - 1);
-    const int64_t xe = (px && wrap_cur && i == g.nx + 1) ? c - (int64_t)(g.nx - 1) : c + 1;
-    const int64_t xw = (px && wrap_cur && i == 2) ? c + (int64_t)(g.nx - 1) : c - 1;
-    const int64_t yn = (py && wrap_cur && j == g.ny + 1) ? c - (int64_t)(g.ny - 1) * X : c + X;
-    const int64_t ys = (py && wrap_cur && j == 2) ? c + (int64_t)(g.ny - 1) * X : c - X;
-    const int64_t zt = (pz && wrap_cur && k == g.nz + 1) ? c - (int64_t)(g.nz - 1) * XY : c + XY;
-    const int64_t zb = (pz && wrap_cur && k == 2) ? c + (int64_t)(g.nz - 1) * XY : c - XY;
-    const double ue = curb[xe], uw = curb[xw], un = curb[yn], us = curb[ys], ut = curb[zt],
-                 ub = curb[zb];
-    const double bc = b[c];
-    // CalcEpotential.f90:74-80
-    out[c] = jc.ax * (ue + uw) + jc.ay * (un + us) + jc.az * (ut + ub) - jc.af * bc;
-    if (p >= 2) {
-      // residual of U_{p-1} with the halo of U_{p-2}: periodic guard
-      // neighbours come from `old` (wrapped, or the stored input halo at p-2 = 0)
-      double re = ue, rw = uw, rn = un, rs = us, rt = ut, rb = ub;
-      if (px && i == g.nx + 1) re = wrap_old ? old[c - (int64_t)(g.nx - 1)] : old[c + 1];
-      if (px && i == 2) rw = wrap_old ? old[c + (int64_t)(g.nx - 1)] : old[c - 1];
-      if (py && j == g.ny + 1) rn = wrap_old ? old[c - (int64_t)(g.ny - 1) * X] : old[c + X];
-      if (py && j == 2) rs = wrap_old ? old[c + (int64_t)(g.ny - 1) * X] : old[c - X];
-      if (pz && k == g.nz + 1) rt = wrap_old ? old[c - (int64_t)(g.nz - 1) * XY] : old[c + XY];
-      if (pz && k == 2) rb = wrap_old ? old[c + (int64_t)(g.nz - 1) * XY] : old[c - XY];
-      const double u2 = 2.0 * curb[c];
-      // CalcEpotential.f90:83-92
-      const double res = bc - ((re + rw - u2) / jc.dx2 + (rn + rs - u2) / jc.dy2 +
-                               (rt + rb - u2) / jc.dz2);
-      r2 = r2 + res * res;
+  const int nz = g.nz;
+  // plane index (0-based) of 1-based plane kk of the iterate, honouring the halo rule
+  auto zplane = [&](int kk, bool wrap) -> int64_t {
+    if (pz && wrap) {
+      if (kk == 1) return nz;
+      if (kk == nz + 2) return 1;
     }
-  });
+    return kk - 1;
+  };
+  double r2 = 0.0;
+  // 2.5D march: each thread owns an (i, j) column of its XCD's z-slab and
+  // keeps the z-neighbours of U_{p-1} in registers
+  const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
+  const int z0 = (xcd * nz) >> 3, z1 = ((xcd + 1) * nz) >> 3;
+  for (int t = w; z1 > z0 && t < ts.TP; t += ts.W) {
+    const int tx = t % ts.ntx, ty = t / ts.ntx;
+    const int i = tx * ts.bx + (int)threadIdx.x + 2;
+    const int j = ty * ts.by + (int)threadIdx.y + 2;
+    if (i > g.nx + 1 || j > g.ny + 1) continue;
+    const int64_t col = (int64_t)(i - 1) + X * (int64_t)(j - 1);
+    const int64_t oe = (px && wrap_cur && i == g.nx + 1) ? -(int64_t)(g.nx - 1) : 1;
+    const int64_t ow = (px && wrap_cur && i == 2) ? (int64_t)(g.nx - 1) : -1;
+    const int64_t on = (py && wrap_cur && j == g.ny + 1) ? -(int64_t)(g.ny - 1) * X : X;
+    const int64_t os = (py && wrap_cur && j == 2) ? (int64_t)(g.ny - 1) * X : -X;
+    int k = z0 + 2;
+    double zm = curb[col + XY * zplane(k - 1, wrap_cur)];
+    double zc = curb[col + XY * (k - 1)];
+    double zp = curb[col + XY * zplane(k + 1, wrap_cur)];
+#pragma unroll 2
+    for (; k <= z1 + 1; ++k) {
+      const int64_t c = col + XY * (int64_t)(k - 1);
+      const double ue = curb[c + oe], uw = curb[c + ow], un = curb[c + on], us = curb[c + os];
+      const double ut = zp, ub = zm;
+      const double bc = b[c];
+      // CalcEpotential.f90:74-80
+      out[c] = jc.ax * (ue + uw) + jc.ay * (un + us) + jc.az * (ut + ub) - jc.af * bc;
+      if (p >= 2) {
+        // residual of U_{p-1} with the halo of U_{p-2}: periodic guard
+        // neighbours come from `old` (wrapped, or the stored input halo at p-2 = 0)
+        double re = ue, rw = uw, rn = un, rs = us, rt = ut, rb = ub;
+        if (px && i == g.nx + 1) re = wrap_old ? old[c - (int64_t)(g.nx - 1)] : old[c + 1];
+        if (px && i == 2) rw = wrap_old ? old[c + (int64_t)(g.nx - 1)] : old[c - 1];
+        if (py && j == g.ny + 1) rn = wrap_old ? old[c - (int64_t)(g.ny - 1) * X] : old[c + X];
+        if (py && j == 2) rs = wrap_old ? old[c + (int64_t)(g.ny - 1) * X] : old[c - X];
+        if (pz && k == nz + 1) rt = wrap_old ? old[c - (int64_t)(nz - 1) * XY] : old[c + XY];
+        if (pz && k == 2) rb = wrap_old ? old[c + (int64_t)(nz - 1) * XY] : old[c - XY];
+        const double u2 = 2.0 * zc;
+        // CalcEpotential.f90:83-92
+        const double res = bc - ((re + rw - u2) / jc.dx2 + (rn + rs - u2) / jc.dy2 +
+                                 (rt + rb - u2) / jc.dz2);
+        r2 = r2 + res * res;
+      }
+      zm = zc;
+      zc = zp;
+      if (k + 1 <= z1 + 1) zp = curb[col + XY * zplane(k + 2, wrap_cur)];
+    }
+  }
   const double sblk = block_sum(r2, sh);
   if (t0 == 0) Pout[blockIdx.x] = sblk;
 }
@@ -576,6 +601,7 @@ int launch_calc_distance(picard_ctx *c) {
       g.nx, g.ny, g.nz, g.dxyz[0], g.dxyz[1], g.dxyz[2], g.flow[0], g.flow[1], g.flow[2], c->D);
   PICARD_HIP(hipGetLastError());
   c->have_D = true;
+  c->fft_ready = false;
   return PICARD_OK;
 }
 

@@ -132,6 +132,22 @@ struct picard_ctx {
   int64_t K = 0;             // deposit cells per species: (nx+1)(ny+1)(nz+1)
   bool have_D = false;
 
+  // split form: FFT evaluation of the open-boundary face sums
+  double2 *fft_K[3][2] = {};     // per flow dim and face: transformed face kernels, nd planes
+  double2 *fft_tw[3][2] = {};    // per flow dim: twiddles of the two transverse lengths
+  double2 *fft_work = nullptr;   // transformed charge planes + two face accumulators
+  int64_t fft_work_len = 0;
+  bool fft_ready = false;
+
+  // device Haser production (ParticlesProduce) run by picard_step
+  int32_t *haser_off = nullptr;  // per (cell, producing species): slot counts, then offsets
+  int64_t haser_len = 0;
+  bool prod_on = false;
+  picard_plasma prod_plasma{};
+  picard_haser prod_haser{};
+  uint64_t prod_seed = 0;
+  int64_t steps_done = 0;        // picard_step calls completed
+
   picard::JacobiState *js = nullptr;   // device
   picard::JacobiState *js_host = nullptr;   // pinned mirror
   int32_t *host_counters = nullptr;    // pinned mirror of scratch.counters
@@ -170,9 +186,16 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
 int launch_calc_efield(picard_ctx *c);
 int launch_bc_fields(picard_ctx *c);
 
+// ---- bcpot_fft.hip ----
+int bcpot_fft_prepare(picard_ctx *c);
+int launch_bc_potentials_fft(picard_ctx *c);
+void bcpot_fft_release(picard_ctx *c);
+
 // ---- sources.hip ----
 int generate_initial(picard_ctx *c, const picard_plasma *pl, uint64_t seed);
 int generate_inflow(picard_ctx *c, const picard_plasma *pl, uint64_t seed, int64_t step);
+int produce_haser(picard_ctx *c, const picard_plasma *pl, const picard_haser *hs, uint64_t seed,
+                  int64_t step, double dt, int64_t *produced);
 
 // ---- split.hip ----
 int sp_push(picard_ctx *c, double dt);

@@ -269,7 +269,196 @@ int emit_box(picard_ctx *c, const picard_plasma *pl, const CellBox &box, int fir
   return PICARD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Haser-model production (ParticlesProduce.f90:1-113).  Per interior cell and
+// species with a neutral outflow (vn > 0): neutral density nden at the cell
+// centre (:41-59, with the 8-point average distance when (4r)^3 < dV),
+// num_create = floor(nu_i * nden * dV / n2p * rate_scale + rnd) ions (:68-71),
+// each at a uniform point of the cell with velocity vn * r/|r| + ui * d/|d|,
+// d uniform in [0,1)^3 as the reference draws it, moved by v*dt*rnd7
+// (:73-81), followed by its epp electrons and hpp protons (:82-98).  Slots
+// follow the reference's loop order: cells (x fastest), species, ions.
+
+struct HaserConst {
+  int32_t Sp;                // producing species
+  int32_t sidx[8];           // their 0-based species indices
+  int32_t epp[8], hpp[8], group[8];
+  double Qn[8], vn[8], nu_i[8], nu_d[8], n2p[8];
+  double ui[8], ue[8], uh[8];
+  double dt, rate_scale;
+};
+
+__device__ __forceinline__ double sq(double a) { return a * a; }
+__device__ __forceinline__ double nu_i_nden_dv(double nu_i, double nden, double dV) {
+  return nu_i * nden * dV;
+}
+
+// Expected ion count of (cell, producing species k), ParticlesProduce.f90:41-70.
+__device__ double haser_lambda(const GridDims &g, const HaserConst &hc, int ci, int cj, int ck, int k) {
+  const double dx = g.dxyz[0], dy = g.dxyz[1], dz = g.dxyz[2];
+  const double dV = dx * dy * dz;
+  const double x0 = g.lmin[0] + dx * (ci - 2);
+  const double y0 = g.lmin[1] + dy * (cj - 2);
+  const double z0 = g.lmin[2] + dz * (ck - 2);
+  double r2 = sq(x0 + 0.5 * dx) + sq(y0 + 0.5 * dy) + sq(z0 + 0.5 * dz);
+  double r1 = sqrt(r2);
+  if ((4.0 * r1) * (4.0 * r1) * (4.0 * r1) < dV) {
+    double acc = 0.0;
+    for (int q = 0; q < 8; ++q) {
+      const double fx = (q & 4) ? 0.75 : 0.25, fy = (q & 2) ? 0.75 : 0.25, fz = (q & 1) ? 0.75 : 0.25;
+      acc += sqrt(sq(x0 + fx * dx) + sq(y0 + fy * dy) + sq(z0 + fz * dz));
+    }
+    r1 = acc * 0.125;
+    r2 = r1 * r1;
+  }
+  const int s = hc.sidx[k];
+  const double nden = hc.Qn[s] / (2.0 * 6.2831853071795864769252867665590 * r2 * hc.vn[s]) *
+                      exp(-hc.nu_d[s] * r1 / hc.vn[s]);
+  return nu_i_nden_dv(hc.nu_i[s], nden, dV) / hc.n2p[s] * hc.rate_scale;
+}
+
+// slots[cs] = num_create * group for cs = cell * Sp + k
+__global__ void __launch_bounds__(kT) k_haser_count(GridDims g, HaserConst hc, uint64_t key, uint32_t tag,
+                                                    int32_t *__restrict__ slots) {
+  const int64_t cs = (int64_t)blockIdx.x * kT + threadIdx.x;
+  const int64_t total = (int64_t)g.nx * g.ny * g.nz * hc.Sp;
+  if (cs >= total) return;
+  const int k = (int)(cs % hc.Sp);
+  const int64_t cell = cs / hc.Sp;
+  const int ci = 2 + (int)(cell % g.nx), cj = 2 + (int)((cell / g.nx) % g.ny);
+  const int ck = 2 + (int)(cell / ((int64_t)g.nx * g.ny));
+  Stream st{key, (uint32_t)cs, (uint32_t)((uint64_t)cs >> 32), tag};
+  const double rnd = st.uniform();
+  const double v = haser_lambda(g, hc, ci, cj, ck, k) + rnd;
+  const int32_t n = v < 2147483647.0 ? (int32_t)v : 2147483647;
+  slots[cs] = n * hc.group[hc.sidx[k]];
+}
+
+// One thread per appended slot; the thread of an ion's first slot writes the
+// ion and its electrons and protons.  off = exclusive scan of slots.
+template <typename Sink>
+__global__ void __launch_bounds__(kT) k_haser_emit(Sink sink, int64_t base, uint32_t id_base, GridDims g,
+                                                   HaserConst hc, uint64_t key, uint32_t tag,
+                                                   const int32_t *__restrict__ off, int64_t ncs) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= (int64_t)off[ncs]) return;
+  // cs = last entry with off[cs] <= t
+  int64_t lo = 0, hi = ncs - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if ((int64_t)off[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const int64_t cs = lo;
+  const int k = (int)(cs % hc.Sp);
+  const int s = hc.sidx[k];
+  const int64_t local = t - off[cs];
+  if (local % hc.group[s] != 0) return;
+  const int64_t pp = local / hc.group[s];
+  const int64_t cell = cs / hc.Sp;
+  const int ci = 2 + (int)(cell % g.nx), cj = 2 + (int)((cell / g.nx) % g.ny);
+  const int ck = 2 + (int)(cell / ((int64_t)g.nx * g.ny));
+  Stream st{key, (uint32_t)cs, (uint32_t)pp, tag};
+  double rp[7];
+  for (int q = 0; q < 7; ++q) rp[q] = st.uniform();
+  const double x0 = g.lmin[0] + g.dxyz[0] * (ci - 2);
+  const double y0 = g.lmin[1] + g.dxyz[1] * (cj - 2);
+  const double z0 = g.lmin[2] + g.dxyz[2] * (ck - 2);
+  double x[3] = {x0 + g.dxyz[0] * rp[0], y0 + g.dxyz[1] * rp[1], z0 + g.dxyz[2] * rp[2]};
+  const double rn = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  const double dn = sqrt(rp[3] * rp[3] + rp[4] * rp[4] + rp[5] * rp[5]);
+  double v[3];
+  for (int c = 0; c < 3; ++c) v[c] = hc.vn[s] * x[c] / rn + hc.ui[s] * rp[3 + c] / dn;
+  for (int c = 0; c < 3; ++c) x[c] = x[c] + v[c] * hc.dt * rp[6];
+  const int64_t q0 = base + t;
+  sink.put(g, q0, id_base + (uint32_t)t, s + 1, x[0], x[1], x[2], v);
+  auto companion = [&](int64_t q, int sp_, double u) {
+    double w[3], y[3];
+    for (int c = 0; c < 3; ++c) w[c] = v[c] - u * rp[3 + c] / dn;
+    for (int c = 0; c < 3; ++c) y[c] = x[c] - v[c] * hc.dt * rp[6] + w[c] * hc.dt * rp[6];
+    sink.put(g, q, id_base + (uint32_t)(q - base), sp_, y[0], y[1], y[2], w);
+  };
+  for (int e = 0; e < hc.epp[s]; ++e) companion(q0 + 1 + e, 1, hc.ue[s] + hc.ui[s]);
+  for (int h = 0; h < hc.hpp[s]; ++h) companion(q0 + 1 + hc.epp[s] + h, 2, hc.uh[s] + hc.ui[s]);
+}
+
+// Per-species speeds of ParticlesProduce.f90:26-36.
+HaserConst make_hc(const picard_ctx *c, const picard_plasma *pl, const picard_haser *hs, double dt) {
+  HaserConst hc{};
+  const GridDims &g = c->gd;
+  const double *m = pl->mass;
+  for (int s = 0; s < g.S; ++s) {
+    hc.epp[s] = pl->epp[s];
+    hc.hpp[s] = pl->hpp[s];
+    hc.group[s] = 1 + pl->epp[s] + pl->hpp[s];
+    hc.Qn[s] = hs->Qn[s];
+    hc.vn[s] = hs->vn[s];
+    hc.nu_d[s] = hs->nu_d[s];
+    const double n2p = c->spc.n2p[s];
+    hc.nu_i[s] = hs->nu_i[s];
+    hc.n2p[s] = n2p;
+    const double Ek = hs->Ek[s];
+    hc.ue[s] = sqrt(2.0 * m[s] * Ek * (1.0 / (m[0] * (m[s] + m[0]))));
+    hc.uh[s] = sqrt(2.0 * m[s] * Ek * (1.0 / (m[1] * (m[s] + m[1]))));
+    if (pl->epp[s] > 0) hc.ui[s] = sqrt(2.0 * m[0] * Ek * (1.0 / (m[s] * (m[s] + m[0]))));
+    else if (pl->hpp[s] > 0) hc.ui[s] = sqrt(2.0 * m[1] * Ek * (1.0 / (m[s] * (m[s] + m[1]))));
+    else hc.ui[s] = 0.0;
+    if (hs->vn[s] > 0.0 && n2p > 0.0 && hs->nu_i[s] > 0.0 && hs->Qn[s] > 0.0)
+      hc.sidx[hc.Sp++] = s;
+  }
+  hc.dt = dt;
+  hc.rate_scale = hs->rate_scale;
+  return hc;
+}
+
 }  // namespace
+
+int produce_haser(picard_ctx *c, const picard_plasma *pl, const picard_haser *hs, uint64_t seed,
+                  int64_t step, double dt, int64_t *produced) {
+  const GridDims &g = c->gd;
+  if (produced) *produced = 0;
+  const HaserConst hc = make_hc(c, pl, hs, dt);
+  if (hc.Sp == 0) return PICARD_OK;
+  const int64_t ncs = (int64_t)g.nx * g.ny * g.nz * hc.Sp;
+  if (c->haser_len < ncs + 1) {
+    if (c->haser_off) PICARD_HIP(hipFree(c->haser_off));
+    c->haser_off = nullptr;
+    c->haser_len = 0;
+    PICARD_HIP(hipMalloc(reinterpret_cast<void **>(&c->haser_off), sizeof(int32_t) * (ncs + 1)));
+    c->haser_len = ncs + 1;
+  }
+  if (c->s.scan_tmp_len < scan_tmp_required(ncs)) {
+    set_error("ParticlesProduce: scan workspace too small for the cell count");
+    return PICARD_ERR_CAPACITY;
+  }
+  // an independent Philox key per source kind; counter tag by step
+  const uint64_t key = seed ^ 0x5A5E12D4C0FFEE11ull;
+  const uint32_t tag = 2u * (uint32_t)step;
+  k_haser_count<<<grid_blocks(ncs, kT), kT, 0, c->stream>>>(g, hc, key, tag, c->haser_off);
+  PICARD_HIP(hipGetLastError());
+  PICARD_TRY(scan_exclusive_i32(c, c->haser_off, c->haser_off, ncs));
+  int32_t total = 0;
+  PICARD_HIP(hipMemcpyAsync(&total, c->haser_off + ncs, sizeof(int32_t), hipMemcpyDeviceToHost,
+                            c->stream));
+  PICARD_HIP(hipStreamSynchronize(c->stream));
+  if (total < 0 || c->n + total > c->cap) {
+    set_error("ParticlesProduce: capacity max_particles exceeded");
+    return PICARD_ERR_CAPACITY;
+  }
+  if (total > 0) {
+    if (c->split)
+      k_haser_emit<<<grid_blocks(total, kT), kT, 0, c->stream>>>(
+          SinkSplit{c->ps}, c->n, (uint32_t)c->next_id, g, hc, key, tag + 1u, c->haser_off, ncs);
+    else
+      k_haser_emit<<<grid_blocks(total, kT), kT, 0, c->stream>>>(
+          SinkF64{c->p}, c->n, (uint32_t)c->next_id, g, hc, key, tag + 1u, c->haser_off, ncs);
+    PICARD_HIP(hipGetLastError());
+  }
+  c->n += total;
+  c->next_id += total;
+  if (produced) *produced = total;
+  return PICARD_OK;
+}
 
 int generate_initial(picard_ctx *c, const picard_plasma *pl, uint64_t seed) {
   const GridDims &g = c->gd;

@@ -317,30 +317,59 @@ __global__ void __launch_bounds__(256) k_mx_sweep(GridDims g, MixCoef mc, TileSc
   const float *e = L.j >= 1 ? E[L.j % 3] : nullptr;
   float *out = E[(L.j + 1) % 3];
   const bool res = L.j >= 1;
+  const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
+  const int nz = g.nz;
   double r2 = 0.0;
-  for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int k, bool valid) {
-    if (!valid) return;
-    const int64_t c = node_index(g, i, j, k);
-    const float rc = r32[c];
+  // 2.5D march over the XCD's z-slab with the z-neighbours of e_j in registers
+  const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
+  const int z0 = (xcd * nz) >> 3, z1 = ((xcd + 1) * nz) >> 3;
+  for (int t = w; z1 > z0 && t < ts.TP; t += ts.W) {
+    const int tx = t % ts.ntx, ty = t / ts.ntx;
+    const int i = tx * ts.bx + (int)threadIdx.x + 2;
+    const int j = ty * ts.by + (int)threadIdx.y + 2;
+    if (i > g.nx + 1 || j > g.ny + 1) continue;
+    const int64_t col = (int64_t)(i - 1) + X * (int64_t)(j - 1);
     if (!e) {
-      out[c] = -mc.aff * rc;
-      return;
+      for (int k = z0 + 2; k <= z1 + 1; ++k) {
+        const int64_t c = col + XY * (int64_t)(k - 1);
+        out[c] = -mc.aff * r32[c];
+      }
+      continue;
     }
-    const Nb nb = neighbours(g, i, j, k, c);
-    const float ee = (g.flow[0] && i == g.nx + 1) ? 0.0f : e[nb.e];
-    const float ew = (g.flow[0] && i == 2) ? 0.0f : e[nb.w];
-    const float en = (g.flow[1] && j == g.ny + 1) ? 0.0f : e[nb.n];
-    const float es = (g.flow[1] && j == 2) ? 0.0f : e[nb.s];
-    const float et = (g.flow[2] && k == g.nz + 1) ? 0.0f : e[nb.t];
-    const float eb = (g.flow[2] && k == 2) ? 0.0f : e[nb.b];
-    out[c] = mc.axf * (ee + ew) + mc.ayf * (en + es) + mc.azf * (et + eb) - mc.aff * rc;
-    if (res) {
-      const float e2 = 2.0f * e[c];
+    // x/y neighbours: periodic wrap, or the zero correction of a flow guard
+    const bool fe = g.flow[0] && i == g.nx + 1, fw = g.flow[0] && i == 2;
+    const bool fn = g.flow[1] && j == g.ny + 1, fs = g.flow[1] && j == 2;
+    const int64_t oe = (!g.flow[0] && i == g.nx + 1) ? -(int64_t)(g.nx - 1) : 1;
+    const int64_t ow = (!g.flow[0] && i == 2) ? (int64_t)(g.nx - 1) : -1;
+    const int64_t on = (!g.flow[1] && j == g.ny + 1) ? -(int64_t)(g.ny - 1) * X : X;
+    const int64_t os = (!g.flow[1] && j == 2) ? (int64_t)(g.ny - 1) * X : -X;
+    // value of e_j on 1-based plane kk (guards: wrap or zero)
+    auto zval = [&](int kk) -> float {
+      if (kk == 1) return g.flow[2] ? 0.0f : e[col + XY * (int64_t)nz];
+      if (kk == nz + 2) return g.flow[2] ? 0.0f : e[col + XY];
+      return e[col + XY * (int64_t)(kk - 1)];
+    };
+    int k = z0 + 2;
+    float zm = zval(k - 1), zc = zval(k), zp = zval(k + 1);
+#pragma unroll 2
+    for (; k <= z1 + 1; ++k) {
+      const int64_t c = col + XY * (int64_t)(k - 1);
+      const float ee = fe ? 0.0f : e[c + oe];
+      const float ew = fw ? 0.0f : e[c + ow];
+      const float en = fn ? 0.0f : e[c + on];
+      const float es = fs ? 0.0f : e[c + os];
+      const float et = zp, eb = zm;
+      const float rc = r32[c];
+      out[c] = mc.axf * (ee + ew) + mc.ayf * (en + es) + mc.azf * (et + eb) - mc.aff * rc;
+      const float e2 = 2.0f * zc;
       const float rr = rc - ((ee + ew - e2) * mc.idx2f + (en + es - e2) * mc.idy2f +
                              (et + eb - e2) * mc.idz2f);
       r2 = r2 + (double)rr * (double)rr;
+      zm = zc;
+      zc = zp;
+      if (k + 1 <= z1 + 1) zp = zval(k + 2);
     }
-  });
+  }
   const double sblk = block_sum(r2, sh);
   if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
 }

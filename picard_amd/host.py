@@ -52,6 +52,14 @@ class Config:
     fluid: tuple = (False,) * 8
     max_particles: int = 0           # 0: sized from the plasma
     seed: int = 20261203
+    # Haser production (picard.f90:258-302), per species s at index s-1;
+    # rate_scale multiplies ParticlesProduce.f90:70's expected count
+    Qn: tuple = (0.0,) * 8
+    vn: tuple = (0.0,) * 8
+    nu_i: tuple = (0.0,) * 8
+    nu_d: tuple = (0.0,) * 8
+    Ek: tuple = (0.0,) * 8
+    rate_scale: float = 1.0
 
 
 def config1(**kw) -> Config:
@@ -328,6 +336,18 @@ def species_struct(cfg: Config, d: Derived) -> _lib.Species:
     return s
 
 
+def haser_struct(cfg: Config, d: Derived | None = None) -> _lib.Haser:
+    hs = _lib.Haser()
+    for q in range(8):
+        hs.Qn[q] = cfg.Qn[q]
+        hs.vn[q] = cfg.vn[q]
+        hs.nu_i[q] = cfg.nu_i[q]
+        hs.nu_d[q] = cfg.nu_d[q]
+        hs.Ek[q] = cfg.Ek[q]
+    hs.rate_scale = cfg.rate_scale
+    return hs
+
+
 def plasma_struct(d: Derived) -> _lib.Plasma:
     pl = _lib.Plasma()
     for q in range(8):
@@ -477,6 +497,28 @@ class Domain:
         pl = plasma_struct(self.d)
         check("picard_generate_inflow",
               lib().picard_generate_inflow(self.h, ct.byref(pl), seed, step))
+
+    def produce(self, seed: int, step: int) -> int:
+        """ParticlesProduce (Haser source) on the device for one step; returns
+        the number of particles appended."""
+        pl = plasma_struct(self.d)
+        hs = haser_struct(self.cfg, self.d)
+        n = ct.c_int64()
+        check("picard_particles_produce",
+              lib().picard_particles_produce(self.h, ct.byref(pl), ct.byref(hs), seed, step,
+                                             self.d.dt, ct.byref(n)))
+        return n.value
+
+    def set_production(self, seed: int | None):
+        """Run the Haser source inside step() (after the position push, as
+        picard.f90:566); None switches it off."""
+        if seed is None:
+            check("picard_set_production", lib().picard_set_production(self.h, None, None, 0))
+            return
+        pl = plasma_struct(self.d)
+        hs = haser_struct(self.cfg, self.d)
+        check("picard_set_production",
+              lib().picard_set_production(self.h, ct.byref(pl), ct.byref(hs), seed))
 
     def timing(self, enable: bool):
         check("picard_timing_enable", lib().picard_timing_enable(self.h, 1 if enable else 0))

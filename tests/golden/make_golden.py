@@ -84,7 +84,79 @@ def main():
                         sp=sp, E=E, B0=B4, dt=np.float64(dt), ve=ve, eta=eta, tB=tB, sB=sB,
                         fluid=fluid, gyration=gyration, n=np.int64(500),
                         dims=np.array([5, 4, 6]), flow=np.array(flow, np.int32))
+    produce_cases()
     print("golden vectors written to", HERE)
+
+
+def produce_cases():
+    """ParticlesProduce: two Haser cases, the reference's output and the
+    runtime uniform stream it consumed."""
+    from oracle import haser
+    from picard_amd import host
+    d_out = {}
+    cases = [
+        # heavy ion (18 amu, +e: one electron per ion), origin at a cell corner
+        dict(dims=(8, 8, 8), gmin=(-40.0,) * 3, gmax=(40.0,) * 3, species=3,
+             charge3=(1.602e-19,) * 6, target=400.0),
+        # origin at a cell centre (the 8-point distance branch); a +2e ion
+        # (two electrons) and a -e ion (one proton)
+        dict(dims=(8, 6, 7), gmin=(-45.0, -35.0, -35.0), gmax=(35.0, 25.0, 35.0), species=4,
+             charge3=(2 * 1.60217646e-19, -1.60217646e-19) + (1.60217646e-19,) * 4, target=300.0),
+    ]
+    for t, cs in enumerate(cases):
+        cfg = host.Config(nx=cs["dims"][0], ny=cs["dims"][1], nz=cs["dims"][2],
+                          species=cs["species"], gmin=cs["gmin"], gmax=cs["gmax"],
+                          charge3=cs["charge3"], ppc=(0, 4, 0, 0, 0, 0, 0, 0))
+        d = host.derive(cfg)
+        vn = np.zeros(8)
+        nu = np.zeros(8)
+        Ek = np.zeros(8)
+        Qn = np.zeros(8)
+        for s in range(2, cs["species"]):
+            vn[s] = 629.0 + 100.0 * s
+            nu[s] = 1.26e-7 * (s + 1)
+            Ek[s] = 15.2 * 1.60217646e-19
+            Qn[s] = 1.0
+        g = oracle.Grid(*cs["dims"], cs["species"], gmin=cs["gmin"], gmax=cs["gmax"])
+        kw = dict(mass=d.mass, Ek=Ek, Qn=Qn, vn=vn, nu_d=nu, nu_i=nu, n2p=d.n2p, epp=d.epp,
+                  hpp=d.hpp, dt=d.dt)
+        # scale Qn so that about `target` ions are produced
+        lam = 0.0
+        for kk in range(cfg.nz):
+            for jj in range(cfg.ny):
+                for ii in range(cfg.nx):
+                    x0 = cfg.gmin[0] + d.dxyz[0] * ii
+                    y0 = cfg.gmin[1] + d.dxyz[1] * jj
+                    z0 = cfg.gmin[2] + d.dxyz[2] * kk
+                    r1, r2 = haser.cell_distance(x0, y0, z0, *d.dxyz)
+                    for s in range(cs["species"]):
+                        if d.n2p[s] > 0:
+                            lam += haser.expected_count(r1, r2, s, Qn, vn, nu, nu, d.n2p, d.dV)
+        Qn = Qn * (cs["target"] / lam)
+        kw["Qn"] = Qn
+        seed = 4711 + t
+        cap = 20000
+        rp = np.zeros((cap, 8))
+        sp = np.zeros(cap, np.int32)
+        n0 = 5
+        rp[:n0] = 1.0
+        sp[:n0] = 2
+        n = oracle.ref_particles_produce(g, rp, sp, n0, seed, kelvin=d.kelvin, v0=d.v0, ve=d.ve,
+                                         ppc=d.ppc, ipp=d.ipp, **kw)
+        u = oracle.ref_uniforms(seed, 200000)
+        draws = haser.SequentialDraws(u)
+        haser.produce(cfg.nx, cfg.ny, cfg.nz, cfg.gmin, tuple(d.dxyz), cs["species"],
+                      draws=draws, **kw)
+        d_out.update({
+            f"dims{t}": np.array(cs["dims"], np.int32), f"gmin{t}": np.array(cs["gmin"]),
+            f"gmax{t}": np.array(cs["gmax"]), f"species{t}": np.int32(cs["species"]),
+            f"mass{t}": d.mass, f"Ek{t}": Ek, f"Qn{t}": Qn, f"vn{t}": vn, f"nu{t}": nu,
+            f"n2p{t}": d.n2p, f"epp{t}": d.epp, f"hpp{t}": d.hpp, f"dt{t}": np.float64(d.dt),
+            f"u{t}": u[:draws.pos], f"n0{t}": np.int64(n0), f"rp_out{t}": rp[n0:n],
+            f"sp_out{t}": sp[n0:n]})
+        print(f"produce case {t}: {n - n0} particles, {draws.pos} uniforms")
+    d_out["ncase"] = np.int64(len(cases))
+    np.savez_compressed(os.path.join(HERE, "ref_particles_produce.npz"), **d_out)
 
 
 if __name__ == "__main__":

@@ -158,3 +158,69 @@ def test_golden_advance_velocity(orc):
     oracle.advance_velocity(g, rp, z["sp"], z["E"], z["B0"], float(z["dt"]), z["ve"], z["eta"],
                             z["tB"], z["sB"], z["fluid"], z["gyration"], int(z["n"]))
     assert np.array_equal(rp, z["rp_out"])
+
+
+def _produce_from_golden(z, t):
+    from oracle import haser
+    nx, ny, nz = [int(v) for v in z[f"dims{t}"]]
+    gmin, gmax = z[f"gmin{t}"], z[f"gmax{t}"]
+    dxyz = tuple((gmax[c] - gmin[c]) / float(n) for c, n in enumerate((nx, ny, nz)))
+    draws = haser.SequentialDraws(z[f"u{t}"])
+    rp, sp = haser.produce(nx, ny, nz, tuple(gmin), dxyz, int(z[f"species{t}"]),
+                           mass=z[f"mass{t}"], Ek=z[f"Ek{t}"], Qn=z[f"Qn{t}"], vn=z[f"vn{t}"],
+                           nu_d=z[f"nu{t}"], nu_i=z[f"nu{t}"], n2p=z[f"n2p{t}"],
+                           epp=z[f"epp{t}"], hpp=z[f"hpp{t}"], dt=float(z[f"dt{t}"]), draws=draws)
+    return rp, sp, draws
+
+
+def test_golden_particles_produce():
+    """The Haser restatement, fed the runtime uniform stream the reference
+    consumed, reproduces the reference's appended particles bit for bit and
+    consumes exactly that stream."""
+    z = _golden("ref_particles_produce.npz")
+    for t in range(int(z["ncase"])):
+        rp, sp, draws = _produce_from_golden(z, t)
+        assert draws.pos == z[f"u{t}"].size
+        assert np.array_equal(sp, z[f"sp_out{t}"])
+        assert np.array_equal(rp[:, :6], z[f"rp_out{t}"][:, :6])
+
+
+def test_golden_particles_produce_invariants():
+    """Closed-form checks of the same outputs: each ion's velocity minus the
+    radial outflow vn*r/|r| has norm ui and lies in the positive octant (the
+    reference draws the direction from uniforms in [0,1)); each companion
+    differs from its ion by (u + ui) along that direction; the charge of every
+    group cancels."""
+    from oracle import haser
+    z = _golden("ref_particles_produce.npz")
+    for t in range(int(z["ncase"])):
+        rp, sp = z[f"rp_out{t}"], z[f"sp_out{t}"]
+        S = int(z[f"species{t}"])
+        m, Ek, epp, hpp = z[f"mass{t}"], z[f"Ek{t}"], z[f"epp{t}"], z[f"hpp{t}"]
+        ue, uh, ui = haser.speeds(m, Ek, epp, hpp, S)
+        dt = float(z[f"dt{t}"])
+        q = 0
+        while q < len(sp):
+            s = int(sp[q]) - 1
+            assert s >= 2
+            x, v = rp[q, :3], rp[q, 3:6]
+            d = None
+            # the ion's direction draw d is recovered from its companions
+            group = 1 + int(epp[s]) + int(hpp[s])
+            assert group >= 2
+            for k in range(1, group):
+                u = (ue[s] if k <= epp[s] else uh[s]) + ui[s]
+                dv = v - rp[q + k, 3:6]
+                assert abs(np.linalg.norm(dv) - u) <= 1e-9 * u
+                assert np.all(dv >= -1e-9 * u)
+                assert int(sp[q + k]) == (1 if k <= epp[s] else 2)
+                d = dv / u
+                # companion displacement is parallel to the velocity change
+                dx = rp[q + k, :3] - x
+                cross = np.cross(dx, dv)
+                assert np.linalg.norm(cross) <= 1e-9 * (np.linalg.norm(dx) * u + 1e-30)
+                assert np.linalg.norm(dx) <= u * dt * (1 + 1e-12)
+            radial = v - ui[s] * d
+            rn = np.linalg.norm(radial)
+            assert abs(rn - z[f"vn{t}"][s]) <= 1e-9 * rn
+            q += group
