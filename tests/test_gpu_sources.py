@@ -5,7 +5,7 @@ Maxwellian, and a seed reproduces the same particles."""
 import numpy as np
 import pytest
 
-from picard_amd import host
+from picard_amd import _lib, host
 
 pytestmark = pytest.mark.gpu
 
@@ -66,3 +66,101 @@ def test_generate_inflow_faces():
     r2, _ = dom.particles(rp.shape[0])
     assert not np.array_equal(r2[:, :3], rp[:, :3])    # a new draw each step
     dom.close()
+
+
+HASER_KEY_XOR = 0x5A5E12D4C0FFEE11
+
+
+def _haser_cfg(species=3, gmin=(-40.0,) * 3, gmax=(40.0,) * 3, dims=(8, 8, 8), scale=1.0):
+    cfg = host.Config(nx=dims[0], ny=dims[1], nz=dims[2], species=species, gmin=gmin, gmax=gmax,
+                      ppc=(0, 4, 0, 0, 0, 0, 0, 0),
+                      charge3=(2 * host.EC, -host.EC) + (host.EC,) * 4 if species == 4
+                      else (host.EC,) * 6)
+    vn = [0.0] * 8
+    nu = [0.0] * 8
+    Ek = [0.0] * 8
+    Qn = [0.0] * 8
+    for s in range(2, species):
+        vn[s] = 629.0 + 100.0 * s
+        nu[s] = 1.26e-7 * (s + 1)
+        Ek[s] = 15.2 * host.EC
+        Qn[s] = 4.0e19 * scale
+    cfg.vn, cfg.nu_i, cfg.nu_d, cfg.Ek, cfg.Qn = tuple(vn), tuple(nu), tuple(nu), tuple(Ek), tuple(Qn)
+    cfg.rate_scale = 1.0
+    return cfg
+
+
+def _haser_expect(cfg, d, seed, step):
+    """The oracle's ParticlesProduce restatement driven by the device's
+    Philox streams (count: (cs, 0, 2*step); ion: (cs, pp, 2*step+1))."""
+    from oracle import haser
+    from tests.philox import uniforms
+    key = (seed ^ HASER_KEY_XOR) & 0xFFFFFFFFFFFFFFFF
+
+    def count_fn(cs, ss):
+        return float(uniforms(key, cs & 0xFFFFFFFF, cs >> 32, 2 * step, 1)[0])
+
+    def ion_fn(cs, ss, pp):
+        return uniforms(key, cs, pp, 2 * step + 1, 7)
+
+    draws = haser.CellStreamDraws(count_fn, ion_fn, None)
+    return haser.produce(cfg.nx, cfg.ny, cfg.nz, cfg.gmin, tuple(d.dxyz), cfg.species,
+                         mass=d.mass, Ek=cfg.Ek, Qn=cfg.Qn, vn=cfg.vn, nu_d=cfg.nu_d,
+                         nu_i=cfg.nu_i, n2p=d.n2p, epp=d.epp, hpp=d.hpp, dt=d.dt, draws=draws,
+                         rate_scale=cfg.rate_scale, producing_only=True)
+
+
+@pytest.mark.parametrize("case", ["corner", "centre"])
+def test_haser_produce_matches_oracle(case):
+    """Device ParticlesProduce against the oracle restatement on the same
+    Philox draws: same counts, order and species; positions and velocities
+    equal to rounding (device exp/sqrt are not the host's)."""
+    print(f"[gpu-sources] haser {case}", flush=True)
+    if case == "corner":
+        cfg = _haser_cfg()
+    else:
+        cfg = _haser_cfg(species=4, gmin=(-45.0, -35.0, -35.0), gmax=(35.0, 25.0, 35.0),
+                         dims=(8, 6, 7))
+    d = host.derive(cfg)
+    rp_ref, sp_ref = _haser_expect(cfg, d, 77, 3)
+    assert 100 < len(sp_ref) < 20000
+    dom = host.Domain(cfg, max_particles=50000, d=d)
+    n = dom.produce(77, 3)
+    rp, sp = dom.particles()
+    print(f"[gpu-sources] haser {case}: {n} particles", flush=True)
+    assert n == len(sp_ref) == len(sp)
+    assert np.array_equal(sp, sp_ref)
+    err = np.abs(rp[:, :6] - rp_ref[:, :6]) / np.maximum(np.abs(rp_ref[:, :6]), 1.0)
+    assert err.max() <= 1e-13, err.max()
+    assert np.array_equal(rp[:, 7], np.arange(n, dtype=np.float64))   # ids in slot order
+    # split form: same particles through the integer-cell + FP32-offset store
+    dom2 = host.Domain(cfg, max_particles=50000, d=d, precision=_lib.PICARD_FP32_SPLIT)
+    assert dom2.produce(77, 3) == n
+    r2, s2 = dom2.particles()
+    assert np.array_equal(s2, sp_ref)
+    assert np.abs(r2[:, :3] - rp_ref[:, :3]).max() <= 1e-5 * float(d.dxyz[0])
+    assert np.abs(r2[:, 3:6] - rp_ref[:, 3:6]).max() <= 1e-6 * np.abs(rp_ref[:, 3:6]).max()
+    dom.close()
+    dom2.close()
+
+
+def test_haser_in_step():
+    """picard_step runs the source after the position push (picard.f90:566):
+    a step with production on appends what produce() appends for that step,
+    counted 1, 2, ... by the steps."""
+    print("[gpu-sources] haser in step", flush=True)
+    cfg = _haser_cfg(scale=0.5)
+    d = host.derive(cfg)
+    a = host.Domain(cfg, max_particles=100000, d=d)
+    a.calc_distance()
+    a.set_production(99)
+    st1 = a.step()
+    st2 = a.step()
+    b = host.Domain(cfg, max_particles=100000, d=d)
+    n1 = b.produce(99, 1)
+    n2 = b.produce(99, 2)
+    assert n1 > 0 and n2 > 0
+    assert st1.num_local + st1.removed == n1
+    assert st2.num_local + st2.removed == st1.num_local + n2
+    a.close()
+    b.close()

@@ -173,3 +173,18 @@ def test_parallel_outflow_rule_patterns(orc, pattern):
     par = _parallel_outflow(flags)
     assert nn == par.size
     assert np.array_equal(rp[:nn, 7].astype(np.int64), par)
+
+
+def test_philox_known_answers():
+    """Philox4x32-10 known-answer vectors published with Random123 (the
+    generator behind the device sources' streams)."""
+    from tests.philox import philox4x32
+    cases = [
+        ((0, 0, 0, 0), 0x0000000000000000, (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+        ((0xffffffff,) * 4, 0xffffffffffffffff, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+        ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), 0x299f31d0a4093822,
+         (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+    ]
+    for ctr, key, want in cases:
+        got = tuple(int(w) for w in philox4x32(*ctr, key))
+        assert got == want, [hex(v) for v in got]
