@@ -1,19 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark of the PIC time step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 256]
+                    [--form both|split|fp64] [--no-cpu-baseline]
+
+Workload: 256^3 cells of 10 m, H+ and e- at 32 per cell (1.07e9 particles),
+drifting Maxwellian 400 km/s at 10 eV, B = (0, 0, 5 nT), x in/outflow, y/z
+periodic, dt = 0.1/w_pe, Jacobi tolerance 1e-6, and a Haser source of 18 amu
+ions at the centre.  Plasma, inflow and production are generated on the device
+(Philox streams, same seeds for both forms).
 
 A step is one pass of the main loop (picard.f90:556-631, no output step):
-BCinflow's fresh guard-cell plasma is appended, then AdvancePosition,
-BCoutflow, ParticlesTransfer, ParticlesGridify, BCpotentials, CalcEpotential,
-CalcEfield, BCfields and AdvanceVelocity run on the device.
+BCinflow, AdvancePosition, ParticlesProduce, BCoutflow, ParticlesTransfer,
+ParticlesGridify, BCpotentials, CalcEpotential, CalcEfield, BCfields,
+AdvanceVelocity.
 
-metric: particle-steps/s = sum over timed steps of num_local at the velocity
-push, divided by the wall time of the timed steps (inputs resident in HBM;
-the inflow particles of every step are generated on the host beforehand).
+metric: particle-steps/s = sum over the timed steps of num_local at the
+velocity push / wall time of the timed steps (state resident in HBM).  The
+FP32-split form gives `value`; the FP64 form (reference order, bit-exact
+against the oracle) runs the same steps after it, and the line reports both,
+their ratio, and the split form's error against the FP64 form after the first
+step (phi, E, densities: normwise relative).
 
-Prints one JSON line (rank 0, N = 1) with the roofline of the dominant kernel
-(the Jacobi sweep) and a CPU baseline of the oracle on a bounded sample."""
+Prints one JSON line with the roofline of the dominant kernel (the split
+form's FP32 Jacobi sweep, timed live with per-launch HIP events) and a CPU
+baseline of the oracle on a bounded sample."""
 from __future__ import annotations
 
 import argparse
@@ -29,94 +40,141 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s FP32-split vs FP64 at ≤1e-5 rel φ err; p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
+SEED = 20261204
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def gpu_run(cfg, steps, warmup):
-    from picard_amd import host
+def _rel(a, b):
+    nb = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+
+
+def run_form(cfg, precision, steps, warmup, capture_after=1):
+    """W warmup + K timed steps of one form; returns timings and the fields
+    after step `capture_after` (read outside the timed region)."""
+    from picard_amd import _lib, host
     d = host.derive(cfg)
-    rng = np.random.default_rng(cfg.seed)
-    rp, sp = host.particles_initialize(cfg, d, rng)
-    inflow = [host.bc_inflow(cfg, d, rng) for _ in range(warmup + steps)]
-    dom = host.Domain(cfg, d=d)
-    dom.append(rp, sp)
+    ncell = cfg.nx * cfg.ny * cfg.nz
+    per_cell = int(d.ppc[:cfg.species].sum())
+    cap = int(1.12 * per_cell * ncell) + 64 * per_cell * (cfg.ny + 2) * (cfg.nz + 2) + (1 << 20)
+    name = "split" if precision == _lib.PICARD_FP32_SPLIT else "fp64"
+    t0 = time.perf_counter()
+    dom = host.Domain(cfg, max_particles=cap, d=d, precision=precision)
+    dom.generate_initial(SEED)
     dom.calc_distance()
+    dom.set_production(SEED)
     dom.synchronize()
-    for it in range(warmup):
-        dom.append(*inflow[it])
+    log(f"{name}: setup {time.perf_counter() - t0:.1f} s, {dom.num_local} particles, cap {cap}")
+    fields = None
+    it = 0
+    for _ in range(warmup):
+        it += 1
+        dom.generate_inflow(SEED, it)
         st = dom.step()
-        log(f"warmup {it + 1}/{warmup}: n={st.num_local} sweeps={st.sweeps}")
+        dom.synchronize()
+        log(f"{name}: warmup step {it}: n={st.num_local} sweeps={st.sweeps}")
+        if it == capture_after:
+            fields = {k: dom.field(w) for k, w in (("U", _lib.FIELD_U), ("E", _lib.FIELD_E),
+                                                    ("N", _lib.FIELD_N))}
     dom.synchronize()
     dom.timing(True)
-    pushed = 0
-    sweeps = []
+    pushed, sweeps = 0, []
     t0 = time.perf_counter()
-    for it in range(steps):
-        dom.append(*inflow[warmup + it])
+    for _ in range(steps):
+        it += 1
+        dom.generate_inflow(SEED, it)
         st = dom.step()
         pushed += st.pushed
         sweeps.append(st.sweeps)
+        log(f"{name}: timed step {it}: n={st.num_local} sweeps={st.sweeps} "
+            f"({time.perf_counter() - t0:.1f} s)")
     dom.synchronize()
     elapsed = time.perf_counter() - t0
     jac_ms, jac_launches = dom.timing_read()
     dom.timing(False)
-    log(f"timed {steps} steps in {elapsed:.3f} s, sweeps/step {sweeps}")
+    if fields is None:
+        fields = {k: dom.field(w) for k, w in (("U", _lib.FIELD_U), ("E", _lib.FIELD_E),
+                                                ("N", _lib.FIELD_N))}
     dom.close()
+    log(f"{name}: {steps} steps in {elapsed:.2f} s, sweeps {sweeps}, "
+        f"sweep kernel {1e3 * jac_ms / max(jac_launches, 1):.2f} us avg over {jac_launches}")
     return dict(pushed=pushed, elapsed=elapsed, sweeps=sweeps, jac_ms=jac_ms,
-                jac_launches=jac_launches)
+                jac_launches=jac_launches, fields=fields, captured_at=min(capture_after, it))
 
 
-def cpu_baseline(cfg, budget_s=20.0):
-    """The oracle (C restatement, 1 core) on the same workload: a bounded
-    number of steps after one warmup step."""
-    import oracle
+def cpu_baseline(budget_s=25.0):
+    """The oracle (C restatement of the reference order, FP64, 1 thread) on a
+    bounded sample of the workload: whole steps of a 48^3 block of it (same
+    cell size, plasma and 32 particles per cell per species, no source)."""
     from picard_amd import host
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
     from tests.pic_driver import OracleSim
+    cfg = host.config2(48, species=2, Qn=(0.0,) * 8)
     d = host.derive(cfg)
-    rng = np.random.default_rng(cfg.seed)
+    rng = np.random.default_rng(SEED)
     rp, sp = host.particles_initialize(cfg, d, rng)
     sim = OracleSim(cfg, d, rp, sp)
-    sim.append(*host.bc_inflow(cfg, d, rng))
-    sim.step()
     pushed, steps = 0, 0
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s and steps < 50:
+    while steps < 1 or (time.perf_counter() - t0 < budget_s * 0.5 and steps < 20):
         sim.append(*host.bc_inflow(cfg, d, rng))
         sim.step()
         pushed += sim.n
         steps += 1
+        log(f"cpu baseline: step {steps} n={sim.n} ({time.perf_counter() - t0:.1f} s)")
     el = time.perf_counter() - t0
-    _ = oracle
     return dict(value=pushed / el, unit="particle-steps/s", cores=1, kind="port",
-                sample=f"{steps} steps of the {cfg.nx}x{cfg.ny}x{cfg.nz} workload after 1 warmup "
-                       f"step, oracle C restatement (reference order, FP64), 1 thread, "
-                       f"{el:.1f} s")
+                sample=f"{steps} step(s) of a 48^3 block of the workload (10 m cells, H+ and e- "
+                       f"at 32 per cell, {sim.n} particles, no Haser source); oracle C "
+                       f"restatement in reference order, FP64, 1 thread, {el:.1f} s")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--form", choices=("both", "split", "fp64"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the domain-decomposed design is described in DESIGN.md); run with --gpus 1",
+        print("bench.py measures one GPU: this build has no multi-GPU path (the domain "
+              "decomposition it would use is described with the design); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
-    from picard_amd import host
-    cfg = host.config1()
-    r = gpu_run(cfg, args.steps, args.warmup)
-    value = r["pushed"] / r["elapsed"]
+    from picard_amd import _lib, host
+    cfg = host.config2(args.grid)
+    runs = {}
+    if args.form in ("both", "split"):
+        runs["split"] = run_form(cfg, _lib.PICARD_FP32_SPLIT, args.steps, args.warmup)
+    if args.form in ("both", "fp64"):
+        runs["fp64"] = run_form(cfg, _lib.PICARD_FP64, args.steps, args.warmup)
+    head = runs.get("split") or runs["fp64"]
+    headname = "split" if "split" in runs else "fp64"
+    value = head["pushed"] / head["elapsed"]
     ncell = cfg.nx * cfg.ny * cfg.nz
-    bytes_per_launch = 24.0 * ncell           # read U, read b, write U per interior cell (FP64)
-    avg_s = (r["jac_ms"] / 1e3) / max(r["jac_launches"], 1)
+    # algorithmic bytes per launch of the dominant kernel: FP32 sweep reads the
+    # correction e and the FP32 residual and writes e (12 B per interior
+    # cell); the FP64 pass reads U and b and writes U (24 B per cell)
+    bpc = 12.0 if headname == "split" else 24.0
+    bytes_per_launch = bpc * ncell
+    avg_s = (head["jac_ms"] / 1e3) / max(head["jac_launches"], 1)
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    per_form = {}
+    for k, r in runs.items():
+        per_form[k] = dict(value=r["pushed"] / r["elapsed"], ms_per_step=1e3 * r["elapsed"] / args.steps,
+                           sweeps_per_step=r["sweeps"],
+                           sweep_kernel_us=1e3 * r["jac_ms"] / max(r["jac_launches"], 1))
+    errors = None
+    if "split" in runs and "fp64" in runs:
+        a, b = runs["split"]["fields"], runs["fp64"]["fields"]
+        errors = {"after_step": runs["split"]["captured_at"],
+                  "phi_rel": _rel(a["U"], b["U"]), "E_rel": _rel(a["E"], b["E"]),
+                  "N_rel": _rel(a["N"], b["N"]),
+                  "phi_rel_inf": float(np.abs(a["U"] - b["U"]).max() / np.abs(b["U"]).max())}
     out = {
         "metric": METRIC,
         "value": value,
@@ -124,23 +182,27 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * r["elapsed"] / args.steps,
+        "ms_per_step": 1e3 * head["elapsed"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
+        "dtype": "f32+f64" if headname == "split" else "f64",
+        "data": "synthetic (device Philox plasma, inflow and Haser production)",
         "config": {
-            "workload": f"FP64 form, {cfg.nx}x{cfg.ny}x{cfg.nz} cells dx=10 m, H+ and e- at 8 "
-                        "per cell (~524k particles), drifting Maxwellian 400 km/s, 10 eV, "
-                        "B=(0,0,5 nT), x in/outflow, y/z periodic, dt=0.1/w_pe, Jacobi tol 1e-6",
-            "form": "fp64-reference-order",
+            "workload": f"{cfg.nx}^3 cells dx=10 m, H+ and e- at 32 per cell, 400 km/s, 10 eV, "
+                        f"B=(0,0,5 nT), x in/outflow, y/z periodic, dt=0.1/w_pe, Jacobi tol 1e-6, "
+                        f"Haser 18 amu source Q={host.HASER_Q_USED:.2g}/s (stated "
+                        f"{host.HASER_Q_STATED:.0e}/s scaled to ~2^18 ions/step), "
+                        f"value = {headname} form",
             "grid": [cfg.nx, cfg.ny, cfg.nz],
-            "mean_jacobi_sweeps_per_step": float(np.mean(r["sweeps"])),
+            "forms": per_form,
+            "split_over_fp64": (per_form["split"]["value"] / per_form["fp64"]["value"]
+                                if len(per_form) == 2 else None),
+            "split_error_vs_fp64": errors,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_jacobi_pass",
+            "kernel": "k_mx_sweep" if headname == "split" else "k_jacobi_pass",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -148,12 +210,12 @@ def main():
             "traffic": None,
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": avg_s * 1e6,
-            "launches": r["jac_launches"],
+            "launches": head["jac_launches"],
         },
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg)
+        out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
 
 

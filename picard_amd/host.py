@@ -67,6 +67,34 @@ def config1(**kw) -> Config:
     return Config(**kw)
 
 
+# Haser source of the 256^3 workload: Q = 1e26 s^-1, vn = 1 km/s and an
+# ionisation/loss scale of 1e6 m (nu_i = nu_d = vn / 1e6 m) as stated, counted
+# per step (rate_scale = dt).  At the reference's macroparticle weight for a
+# source-only species (n2p(3) = n2p(1), picard.f90:394-401) that is 7.9e8 ions
+# per step on the 2560 m box, more than half the plasma load, so the rate is
+# scaled to 3.3e22 s^-1: about 2^18 heavy ions (and as many electrons) per step.
+HASER_Q_STATED = 1.0e26
+HASER_Q_USED = 3.3e22
+
+
+def config2(n: int = 256, **kw) -> Config:
+    """n^3 cells of 10 m, H+ and e- at 32 per cell (1.07e9 particles at 256^3), the
+    drifting Maxwellian and B of the 32^3 case, x in/outflow, and a Haser source
+    of 18 amu ions (+e, one electron each) at the domain centre."""
+    half = 5.0 * n
+    vn = 1.0e3
+    nu = vn / 1.0e6
+    base = dict(nx=n, ny=n, nz=n, species=3, gmin=(-half,) * 3, gmax=(half,) * 3,
+                ppc=(0, 32, 0, 0, 0, 0, 0, 0),
+                Qn=(0.0, 0.0, HASER_Q_USED, 0, 0, 0, 0, 0), vn=(0.0, 0.0, vn, 0, 0, 0, 0, 0),
+                nu_i=(0.0, 0.0, nu, 0, 0, 0, 0, 0), nu_d=(0.0, 0.0, nu, 0, 0, 0, 0, 0),
+                Ek=(0.0, 0.0, 15.2 * EC, 0, 0, 0, 0, 0))
+    base.update(kw)
+    cfg = Config(**base)
+    cfg.rate_scale = derive(cfg).dt
+    return cfg
+
+
 @dataclass
 class Derived:
     dxyz: np.ndarray
