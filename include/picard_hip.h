@@ -43,7 +43,8 @@ enum { PICARD_FP64 = 0, PICARD_FP32_SPLIT = 1 };
 enum { PICARD_DEPOSIT_REFERENCE_ORDER = 0, PICARD_DEPOSIT_FIXED_POINT = 1 };
 enum {
   PICARD_FIELD_U = 0, PICARD_FIELD_E = 1, PICARD_FIELD_N = 2, PICARD_FIELD_C = 3,
-  PICARD_FIELD_D = 4
+  PICARD_FIELD_D = 4,
+  PICARD_FIELD_F = 5   /* number fluxes F(species, 3, nodes), after particles_gridify_all */
 };
 
 /* One domain (one MPI rank of the reference, here one GPU). */
@@ -129,6 +130,18 @@ int picard_particles_produce(picard_ctx *ctx, const picard_plasma *plasma,
 int picard_set_production(picard_ctx *ctx, const picard_plasma *plasma,
                           const picard_haser *haser, uint64_t seed);
 
+/* The diagnostics line of the main loop (picard.f90:649-672), computed on
+ * the device: uP = 0.5 sum n2p(s) mass(s) |v|^2 over the particles (:658-661),
+ * uE = 0.5 eps0 dV sum |E + E0|^2 over interior nodes (:655-657), uB = 0.5/mu0
+ * dV sum |B0|^2 over interior nodes (B = B0 everywhere, :420-422, :653).  The
+ * reference prints them divided by its initial energy u0 (:476-480). */
+typedef struct {
+  int64_t num_local, num_global;
+  double particle_energy, e_energy, b_energy;
+} picard_diag;
+
+int picard_diagnostics(picard_ctx *ctx, const double *mass /* [8] */, picard_diag *out);
+
 /* ---- the time-step call surface (picard.f90:564-631) ---- */
 
 /* AdvancePosition(real_particles, dt, max_per_proc, num_local)   AdvancePosition.f90:4 */
@@ -144,6 +157,11 @@ int picard_particles_transfer(picard_ctx *ctx);
 
 /* ParticlesGridify(N, C, real_particles, int_particles, ...)   ParticlesGridify.f90:1 */
 int picard_particles_gridify(picard_ctx *ctx);
+/* ParticlesGridifyAll (ParticlesGridifyAll.f90:1-455): ParticlesGridify's N
+ * and C plus the number fluxes F(s, c, node) = sum n2p(s) W v_c with the same
+ * guard folds; used on field-output steps (picard.f90:610-612).  FP64 form:
+ * reference summation order; split form: fixed point. */
+int picard_particles_gridify_all(picard_ctx *ctx);
 
 /* CalcDistance(D, Nx, Ny, Nz, dxyz, xflow, yflow, zflow)        CalcDistance.f90:1 */
 int picard_calc_distance(picard_ctx *ctx);
@@ -165,6 +183,13 @@ int picard_bc_fields(picard_ctx *ctx);
 /* AdvanceVelocity(real_particles, int_particles, E, B0, Lx_min.., dxyz, dt, v0, ve,
  *                 eta, tB, sB, num_local, ...)                   AdvanceVelocity.f90:6 */
 int picard_advance_velocity(picard_ctx *ctx, double dt);
+/* The output-step pair that replaces AdvanceVelocity on steps that write
+ * particles or fields (picard.f90:602-621): AdvanceVelocityImp
+ * (AdvanceVelocity.f90:240-347) takes v to the half step before the dumps,
+ * AdvanceVelocityExp (:128-238) completes the step after them.  The main
+ * loop passes deltat = dt/2. */
+int picard_advance_velocity_imp(picard_ctx *ctx, double deltat, double dt);
+int picard_advance_velocity_exp(picard_ctx *ctx, double deltat, double dt);
 
 /* Field transfer between device and host arrays of the layouts above;
  * `count` is the number of doubles. */

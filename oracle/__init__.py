@@ -114,6 +114,11 @@ def lib():
         L.orc_bc_fields.argtypes = [G, _dp]
         L.orc_advance_velocity.argtypes = [G, _dp, _ip, _dp, _dp, ct.c_double, _dp, _dp, _dp,
                                            _dp, ct.c_int64, _ip, _ip]
+        L.orc_particles_gridify_all.argtypes = [G, _dp, _dp, _dp, _dp, _ip, ct.c_int64, _dp, _dp]
+        L.orc_particles_gridify_all.restype = ct.c_int64
+        L.orc_diagnostics.argtypes = [G, _dp, _ip, ct.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_advance_velocity_ie.argtypes = [ct.c_int, G, _dp, _ip, _dp, _dp, ct.c_double,
+                                              ct.c_double, _dp, _dp, _dp, ct.c_int64, _ip, _ip]
         _lib = L
     return _lib
 
@@ -220,6 +225,41 @@ def advance_velocity(g: Grid, rp, sp, E, B0, dt, ve, eta, tB, sB, fluid=None, gy
                                _i(gyration))
 
 
+def advance_velocity_ie(mode, g: Grid, rp, sp, E, B0, deltat, dt, ve, eta, tB, fluid=None,
+                        gyration=None, n=None):
+    """AdvanceVelocityImp (mode 1) or AdvanceVelocityExp (mode 2)."""
+    n = rp.shape[0] if n is None else n
+    fluid = _i8([0] * 8 if fluid is None else fluid)
+    gyration = _i8([1] * 8 if gyration is None else gyration)
+    tB = np.ascontiguousarray(np.asarray(tB, np.float64).reshape(8, 4))
+    lib().orc_advance_velocity_ie(mode, ct.byref(g.cstruct()), _d(rp), _i(sp), _d(E),
+                                  _d(_f8(B0, 4)), deltat, dt, _d(_f8(ve, 4)), _d(_f8(eta)),
+                                  _d(tB), n, _i(fluid), _i(gyration))
+
+
+def particles_gridify_all(g: Grid, rp, sp, charge, n2p, n=None):
+    """ParticlesGridifyAll: returns (F [Z,Y,X,3,S], N [Z,Y,X,S], C, skipped)."""
+    n = rp.shape[0] if n is None else n
+    F = np.zeros(g.shape + (3, g.species))
+    N = np.zeros(g.shape + (g.species,))
+    C = np.zeros(g.shape)
+    sk = lib().orc_particles_gridify_all(ct.byref(g.cstruct()), _d(F), _d(N), _d(C),
+                                         _d(np.ascontiguousarray(rp)),
+                                         _i(np.ascontiguousarray(sp, np.int32)), n,
+                                         _d(_f8(charge)), _d(_f8(n2p)))
+    return F, N, C, sk
+
+
+def diagnostics(g: Grid, rp, sp, E, n2p, mass, E0, B0, n=None):
+    """(uP, uE, uB) of the diagnostics line, picard.f90:649-672."""
+    n = rp.shape[0] if n is None else n
+    out = np.zeros(3)
+    lib().orc_diagnostics(ct.byref(g.cstruct()), _d(np.ascontiguousarray(rp)),
+                          _i(np.ascontiguousarray(sp, np.int32)), n, _d(np.ascontiguousarray(E)),
+                          _d(_f8(n2p)), _d(_f8(mass)), _d(_f8(E0, 4)), _d(_f8(B0, 4)), _d(out))
+    return tuple(float(v) for v in out)
+
+
 # ---------------------------------------------------------------------------
 # the reference's own MPI-free subroutines, compiled from /root/reference by
 # oracle/Makefile into oracle/_ref (absent on machines without the reference)
@@ -270,6 +310,25 @@ def ref_calc_distance(nx, ny, nz, dxyz, flow):
     R.calcdistance_(_d(D), _byref_i(nx), _byref_i(ny), _byref_i(nz),
                     _d(np.asarray(dxyz, np.float64)), *[_byref_i(1 if f else 0) for f in flow])
     return D
+
+
+def ref_advance_velocity_ie(mode, g: Grid, rp, sp, E, B0, deltat, dt, v0, ve, eta, tB, n,
+                            fluid=None, gyration=None):
+    """AdvanceVelocityImp (mode 1) / AdvanceVelocityExp (mode 2) of the
+    reference: (real_particles, int_particles, E, B0, Lx_min, Ly_min, Lz_min,
+    dxyz, deltat, dt, v0, ve, eta, tB, num_local, max_per_proc, species,
+    Nx_local, Ny_local, Nz_local, fluid, gyration) (AdvanceVelocity.f90:128-130)."""
+    R = ref()
+    fluid = _i8([0] * 8 if fluid is None else fluid)
+    gyration = _i8([1] * 8 if gyration is None else gyration)
+    tB = np.ascontiguousarray(np.asarray(tB, np.float64).reshape(8, 4))
+    v0 = np.ascontiguousarray(np.asarray(v0, np.float64).reshape(8, 4))
+    fn = R.advancevelocityimp_ if mode == 1 else R.advancevelocityexp_
+    fn(_d(rp), _i(sp), _d(E), _d(_f8(B0, 4)), _byref_d(g.gmin[0]), _byref_d(g.gmin[1]),
+       _byref_d(g.gmin[2]), _d(np.asarray(g.dxyz, np.float64)), _byref_d(deltat), _byref_d(dt),
+       _d(v0), _d(_f8(ve, 4)), _d(_f8(eta)), _d(tB), _byref_i(n), _byref_i(rp.shape[0]),
+       _byref_i(g.species), _byref_i(g.nx), _byref_i(g.ny), _byref_i(g.nz), _i(fluid),
+       _i(gyration))
 
 
 def ref_bc_fields(g: Grid, E, U, B0, E0):
@@ -323,4 +382,51 @@ def ref_uniforms(seed, count):
     R.picard_ref_seed(ct.c_int32(seed))
     out = np.zeros(count, np.float64)
     R.picard_ref_uniforms(_d(out), ct.c_int64(count))
+    return out
+
+
+def _in_dir(path, fn):
+    """Run fn with the working directory set to `path` (the reference's
+    writers open save/... relative to it), restoring it afterwards."""
+    old = os.getcwd()
+    os.makedirs(os.path.join(path, "save", "particles"), exist_ok=True)
+    os.makedirs(os.path.join(path, "save", "fields"), exist_ok=True)
+    os.chdir(path)
+    try:
+        return fn()
+    finally:
+        os.chdir(old)
+
+
+def ref_dump_particles(rp, sp, n, iteration, myid, path):
+    """DumpParticles(real_particles, int_particles, num_local, max_per_proc,
+    writeParticles_iteration, myid) (Write.f90:2-31) writing under `path`.
+    Returns the file's bytes and the incremented iteration."""
+    R = ref()
+    it = ct.c_int32(iteration)
+    buf = np.ascontiguousarray(rp, np.float64)   # (n, 8) C order == real_particles(8, n)
+    _in_dir(path, lambda: R.dumpparticles_(_d(buf), _i(np.ascontiguousarray(sp, np.int32)),
+                                           _byref_i(n), _byref_i(buf.shape[0]), ct.byref(it),
+                                           _byref_i(myid)))
+    name = os.path.join(path, "save", "particles", f"particles_{iteration:05d}_{myid:05d}")
+    with open(name, "rb") as f:
+        return f.read(), it.value
+
+
+def ref_dump_fields(U, N, F, E, nx, ny, nz, iteration, species, myid, path):
+    """DumpFields(U, N, F, E, Nx_local, Ny_local, Nz_local,
+    writeFields_iteration, species, myid) (Write.f90:34-523) writing under
+    `path`.  Arrays in this package's C-order views.  Returns {name: bytes}."""
+    R = ref()
+    it = ct.c_int32(iteration)
+    _in_dir(path, lambda: R.dumpfields_(_d(np.ascontiguousarray(U)), _d(np.ascontiguousarray(N)),
+                                        _d(np.ascontiguousarray(F)), _d(np.ascontiguousarray(E)),
+                                        _byref_i(nx), _byref_i(ny), _byref_i(nz), ct.byref(it),
+                                        _byref_i(species), _byref_i(myid)))
+    out = {}
+    d = os.path.join(path, "save", "fields")
+    for name in sorted(os.listdir(d)):
+        if name.endswith(f"_{iteration:05d}_{myid:05d}"):
+            with open(os.path.join(d, name), "rb") as f:
+                out[name] = f.read()
     return out

@@ -112,8 +112,7 @@ static inline int cic_in_bounds(const orc_grid *g, const int idx[3]) {
 
 /* Guard fold for one dimension, ParticlesGridify.f90:129-285 on one rank.
  * dim 0 = x, 1 = y, 2 = z.  Works on whole planes of N(S, ...). */
-static void gridify_fold(const orc_grid *g, double *N, int dim) {
-  const int S = g->species;
+static void gridify_fold_n(const orc_grid *g, double *N, int dim, int S) {
   const int X = g->nx + 2, Y = g->ny + 2, Z = g->nz + 2;
   const int n = dim == 0 ? g->nx : (dim == 1 ? g->ny : g->nz);
   const int e1 = dim == 0 ? Y : X;          /* extents of the plane */
@@ -148,6 +147,10 @@ static void gridify_fold(const orc_grid *g, double *N, int dim) {
       }
     }
   }
+}
+
+static void gridify_fold(const orc_grid *g, double *N, int dim) {
+  gridify_fold_n(g, N, dim, g->species);
 }
 
 /* ParticlesGridify.f90:56-293 */
@@ -573,4 +576,167 @@ void orc_advance_velocity(const orc_grid *g, double *rp, const int32_t *sp, cons
       r[5] = vp3 + ve[2] + h * Ep[2];
     }
   }
+}
+
+/* ---- diagnostics (picard.f90:649-672, one rank) -------------------------
+ * out[0] = uP = 0.5 * SUM_pp n2p(s) mass(s) (v1^2 + v2^2 + v3^2)   (:658-661)
+ * out[1] = uE = SUM_interior |E + E0|^2 * 0.5 * eps0 * dV          (:655-657)
+ * out[2] = uB = SUM_interior,c B(c)^2 * 0.5 / mu0 * dV, B = B0     (:653, :420-422)
+ * Each SUM runs sequentially in Fortran array order (first index fastest). */
+void orc_diagnostics(const orc_grid *g, const double *rp, const int32_t *sp, int64_t n,
+                     const double *E, const double *n2p, const double *mass, const double *E0,
+                     const double *B0, double out[3]) {
+  const double eps0 = 8.854187817e-12, mu0 = 1.25663706143591729538505735331180e-6;
+  const double dV = g->dxyz[0] * g->dxyz[1] * g->dxyz[2];
+  double up = 0.0;
+  for (int64_t pp = 0; pp < n; ++pp) {
+    const double *r = rp + 8 * pp;
+    const int s = sp[pp] - 1;
+    up = up + n2p[s] * mass[s] * (r[3] * r[3] + r[4] * r[4] + r[5] * r[5]);
+  }
+  out[0] = up * 0.5;
+  const int X = g->nx + 2, Y = g->ny + 2;
+  double ue = 0.0, ub = 0.0;
+  for (int k = 2; k <= g->nz + 1; ++k)
+    for (int j = 2; j <= g->ny + 1; ++j)
+      for (int i = 2; i <= g->nx + 1; ++i) {
+        const double *e = E + 3 * ((int64_t)(i - 1) + (int64_t)X * ((j - 1) + (int64_t)Y * (k - 1)));
+        const double a = e[0] + E0[0], b = e[1] + E0[1], c = e[2] + E0[2];
+        ue = ue + (a * a + b * b + c * c);
+        for (int q = 0; q < 3; ++q) ub = ub + B0[q] * B0[q];
+      }
+  out[1] = ue * 0.5 * eps0 * dV;
+  out[2] = ub * 0.5 / mu0 * dV;
+}
+
+/* ---- AdvanceVelocityImp (mode 1, AdvanceVelocity.f90:240-347) and
+ * AdvanceVelocityExp (mode 2, :128-238): same gather and fluid branch as
+ * AdvanceVelocity; the kick and a norm-preserving rotation with etadt2 and
+ * tB scaled by 2 deltat/dt (:156-157, :267-268). */
+void orc_advance_velocity_ie(int mode, const orc_grid *g, double *rp, const int32_t *sp,
+                             const double *E, const double B0[4], double deltat, double dt,
+                             const double ve[4], const double eta[8], const double tB[32],
+                             int64_t n, const int32_t fluid[8], const int32_t gyration[8]) {
+  const double dV = g->dxyz[0] * g->dxyz[1] * g->dxyz[2];
+  double etadt2[8], tBt[32];
+  for (int s = 0; s < 8; ++s) etadt2[s] = 0.5 * dt * eta[s] * 2.0 * deltat / dt;
+  for (int q = 0; q < 32; ++q) tBt[q] = tB[q] * 2.0 * deltat / dt;
+  for (int64_t pp = n; pp >= 1; --pp) {
+    double *r = rp + 8 * (pp - 1);
+    double rest[3], drest[3];
+    int idx[3];
+    cic_locate(g, r, rest, drest, idx);
+    if (!cic_in_bounds(g, idx)) continue;
+    const double F1 = drest[0] * drest[1] * drest[2];
+    const double F2 = rest[0] * drest[1] * drest[2];
+    const double F3 = drest[0] * rest[1] * drest[2];
+    const double F4 = drest[0] * drest[1] * rest[2];
+    const double F5 = rest[0] * rest[1] * drest[2];
+    const double F6 = drest[0] * rest[1] * rest[2];
+    const double F7 = rest[0] * drest[1] * rest[2];
+    const double F8 = rest[0] * rest[1] * rest[2];
+    const int i = idx[0], j = idx[1], k = idx[2];
+    double Ep[3];
+    for (int c = 0; c < 3; ++c) {
+      const double e111 = E[3 * gidx(g, i, j, k) + c], e211 = E[3 * gidx(g, i + 1, j, k) + c];
+      const double e121 = E[3 * gidx(g, i, j + 1, k) + c], e112 = E[3 * gidx(g, i, j, k + 1) + c];
+      const double e221 = E[3 * gidx(g, i + 1, j + 1, k) + c];
+      const double e122 = E[3 * gidx(g, i, j + 1, k + 1) + c];
+      const double e212 = E[3 * gidx(g, i + 1, j, k + 1) + c];
+      const double e222 = E[3 * gidx(g, i + 1, j + 1, k + 1) + c];
+      Ep[c] = e111 * F1 + e211 * F2 + e121 * F3 + e112 * F4 + e221 * F5 + e122 * F6 +
+              e212 * F7 + e222 * F8;
+      Ep[c] = Ep[c] / dV;
+    }
+    const int s = sp[pp - 1] - 1;
+    if (fluid[s]) {
+      if (gyration[s]) {
+        const double B1 = B0[0], B2 = B0[1], B3 = B0[2];
+        const double vdb = r[3] * B1 + r[4] * B2 + r[5] * B3;
+        const double vm1 = vdb * B1, vm2 = vdb * B2, vm3 = vdb * B3;
+        const double vd1 = vm1 + (Ep[1] * B3 - Ep[2] * B2);
+        const double vd2 = vm2 + (Ep[2] * B1 - Ep[0] * B3);
+        const double vd3 = vm3 + (Ep[0] * B2 - Ep[1] * B1);
+        const double inv = 1.0 / (B1 * B1 + B2 * B2 + B3 * B3);
+        r[3] = vd1 * inv; r[4] = vd2 * inv; r[5] = vd3 * inv;
+      }
+      continue;
+    }
+    const double h = etadt2[s];
+    const double *t = tBt + 4 * s;
+    double vm1 = r[3] - ve[0], vm2 = r[4] - ve[1], vm3 = r[5] - ve[2];
+    if (mode == 1) {                                                /* :297 */
+      vm1 = vm1 + h * Ep[0];
+      vm2 = vm2 + h * Ep[1];
+      vm3 = vm3 + h * Ep[2];
+    }
+    double vd1 = vm1 + vm2 * t[2] - vm3 * t[1];                     /* :300-302 */
+    double vd2 = vm2 + vm3 * t[0] - vm1 * t[2];
+    double vd3 = vm3 + vm1 * t[1] - vm2 * t[0];
+    const double sd = vd1 * vd1 + vd2 * vd2 + vd3 * vd3;
+    if (sd > 0.0) {                                                 /* :305-307 */
+      const double f = sqrt((vm1 * vm1 + vm2 * vm2 + vm3 * vm3) / sd);
+      vd1 = vd1 * f; vd2 = vd2 * f; vd3 = vd3 * f;
+    }
+    if (mode == 1) {                                                /* :310 */
+      r[3] = vd1 + ve[0]; r[4] = vd2 + ve[1]; r[5] = vd3 + ve[2];
+    } else {                                                        /* :222 */
+      r[3] = vd1 + ve[0] + h * Ep[0];
+      r[4] = vd2 + ve[1] + h * Ep[1];
+      r[5] = vd3 + ve[2] + h * Ep[2];
+    }
+  }
+}
+
+/* ---- ParticlesGridifyAll.f90:56-452 (one rank): ParticlesGridify's N and C
+ * plus F(s, c, node) = F(s, c, node) + Fk * n2p(s) * v_c in the same loop
+ * (:126-212), then the same z, y, x guard folds on N and F (:238-420). */
+int64_t orc_particles_gridify_all(const orc_grid *g, double *F, double *N, double *C,
+                                  const double *rp, const int32_t *sp, int64_t n,
+                                  const double charge[8], const double n2p[8]) {
+  const int S = g->species;
+  const int64_t G = (int64_t)(g->nx + 2) * (g->ny + 2) * (g->nz + 2);
+  const double dV = g->dxyz[0] * g->dxyz[1] * g->dxyz[2];
+  int64_t skipped = 0;
+  for (int64_t q = 0; q < 3 * (int64_t)S * G; ++q) F[q] = 0.0;   /* :61 */
+  for (int64_t q = 0; q < (int64_t)S * G; ++q) N[q] = 0.0;
+  for (int64_t q = 0; q < G; ++q) C[q] = 0.0;
+  for (int64_t pp = n; pp >= 1; --pp) {
+    const double *r = rp + 8 * (pp - 1);
+    double rest[3], drest[3];
+    int idx[3];
+    cic_locate(g, r, rest, drest, idx);
+    if (!cic_in_bounds(g, idx)) { ++skipped; continue; }
+    double Fk[8];
+    Fk[0] = drest[0] * drest[1] * drest[2] / dV;
+    Fk[1] = rest[0] * drest[1] * drest[2] / dV;
+    Fk[2] = drest[0] * rest[1] * drest[2] / dV;
+    Fk[3] = drest[0] * drest[1] * rest[2] / dV;
+    Fk[4] = rest[0] * rest[1] * drest[2] / dV;
+    Fk[5] = drest[0] * rest[1] * rest[2] / dV;
+    Fk[6] = rest[0] * drest[1] * rest[2] / dV;
+    Fk[7] = rest[0] * rest[1] * rest[2] / dV;
+    /* node offsets of F1..F8 */
+    static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                                  {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
+    const int s = sp[pp - 1] - 1;
+    const double w = n2p[s];
+    for (int q = 0; q < 8; ++q) {
+      const int64_t node = gidx(g, idx[0] + off[q][0], idx[1] + off[q][1], idx[2] + off[q][2]);
+      N[(int64_t)S * node + s] = N[(int64_t)S * node + s] + Fk[q] * w;
+    }
+    for (int c = 0; c < 3; ++c)
+      for (int q = 0; q < 8; ++q) {
+        const int64_t node = gidx(g, idx[0] + off[q][0], idx[1] + off[q][1], idx[2] + off[q][2]);
+        double *f = F + (int64_t)3 * S * node + (int64_t)S * c + s;
+        *f = *f + Fk[q] * w * r[3 + c];
+      }
+  }
+  for (int dim = 2; dim >= 0; --dim) {
+    gridify_fold_n(g, N, dim, S);
+    gridify_fold_n(g, F, dim, 3 * S);
+  }
+  for (int s = 0; s < S; ++s)
+    for (int64_t q = 0; q < G; ++q) C[q] = C[q] + N[(int64_t)S * q + s] * charge[s];
+  return skipped;
 }

@@ -83,7 +83,24 @@ void orc_advance_velocity(const orc_grid *g, double *rp, const int32_t *sp, cons
 int64_t orc_calc_epotential_pairwise(const orc_grid *g, double *U, const double *C,
                                      double maxerr, int32_t maxit);
 
+/* Energies of the diagnostics line (picard.f90:649-672): out = {uP, uE, uB}. */
+void orc_diagnostics(const orc_grid *g, const double *rp, const int32_t *sp, int64_t n,
+                     const double *E, const double *n2p, const double *mass, const double *E0,
+                     const double *B0, double out[3]);
+
+/* AdvanceVelocityImp (mode 1) / AdvanceVelocityExp (mode 2), AdvanceVelocity.f90:128-347. */
+void orc_advance_velocity_ie(int mode, const orc_grid *g, double *rp, const int32_t *sp,
+                             const double *E, const double B0[4], double deltat, double dt,
+                             const double ve[4], const double eta[8], const double tB[32],
+                             int64_t n, const int32_t fluid[8], const int32_t gyration[8]);
+
+/* ParticlesGridifyAll (ParticlesGridifyAll.f90:1-455): F(species, 3, nodes), N, C. */
+int64_t orc_particles_gridify_all(const orc_grid *g, double *F, double *N, double *C,
+                                  const double *rp, const int32_t *sp, int64_t n,
+                                  const double charge[8], const double n2p[8]);
+
 #ifdef __cplusplus
 }
 #endif
+
 #endif

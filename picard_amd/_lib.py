@@ -24,7 +24,7 @@ PICARD_FP32_SPLIT = 1
 PICARD_DEPOSIT_REFERENCE_ORDER = 0
 PICARD_DEPOSIT_FIXED_POINT = 1
 
-FIELD_U, FIELD_E, FIELD_N, FIELD_C, FIELD_D = 0, 1, 2, 3, 4
+FIELD_U, FIELD_E, FIELD_N, FIELD_C, FIELD_D, FIELD_F = 0, 1, 2, 3, 4, 5
 
 # every symbol include/picard_hip.h declares
 EXPORTS = (
@@ -36,6 +36,8 @@ EXPORTS = (
     "picard_advance_velocity", "picard_field_read", "picard_field_write", "picard_step",
     "picard_timing_enable", "picard_timing_read", "picard_generate_initial",
     "picard_generate_inflow", "picard_particles_produce", "picard_set_production",
+    "picard_diagnostics", "picard_advance_velocity_imp", "picard_advance_velocity_exp",
+    "picard_particles_gridify_all",
 )
 
 
@@ -71,6 +73,12 @@ class Plasma(ct.Structure):
 class Haser(ct.Structure):
     _fields_ = [("Qn", ct.c_double * 8), ("vn", ct.c_double * 8), ("nu_i", ct.c_double * 8),
                 ("nu_d", ct.c_double * 8), ("Ek", ct.c_double * 8), ("rate_scale", ct.c_double)]
+
+
+class Diag(ct.Structure):
+    _fields_ = [("num_local", ct.c_int64), ("num_global", ct.c_int64),
+                ("particle_energy", ct.c_double), ("e_energy", ct.c_double),
+                ("b_energy", ct.c_double)]
 
 
 class StepStats(ct.Structure):
@@ -129,6 +137,10 @@ def lib():
                                       ct.c_int64, ct.c_double, lp], ct.c_int),
         "picard_set_production": ([vp, ct.POINTER(Plasma), ct.POINTER(Haser), ct.c_uint64],
                                   ct.c_int),
+        "picard_diagnostics": ([vp, dp, ct.POINTER(Diag)], ct.c_int),
+        "picard_advance_velocity_imp": ([vp, ct.c_double, ct.c_double], ct.c_int),
+        "picard_particles_gridify_all": ([vp], ct.c_int),
+        "picard_advance_velocity_exp": ([vp, ct.c_double, ct.c_double], ct.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

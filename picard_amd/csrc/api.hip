@@ -113,7 +113,8 @@ void release(picard_ctx *c) {
   dfree(c->Nfix);
   dfree(c->E32);
   dfree(c->r32);
-  for (int q = 0; q < 3; ++q) dfree(c->e32[q]);
+  for (int q = 0; q < 2; ++q) dfree(c->e32[q]);
+  for (int q = 0; q < 2; ++q) dfree(c->eo32[q]);
   for (int q = 0; q < 8; ++q) dfree(c->p.r[q]);
   dfree(c->p.sp);
   dfree(c->U);
@@ -124,6 +125,9 @@ void release(picard_ctx *c) {
   dfree(c->D);
   bcpot_fft_release(c);
   dfree(c->haser_off);
+  dfree(c->F);
+  dfree(c->Fraw);
+  dfree(c->Ffix);
   for (int q = 0; q < 3; ++q) dfree(c->jb[q]);
   dfree(c->b);
   dfree(c->consts);
@@ -184,7 +188,8 @@ int create_impl(const picard_geometry *geo, const picard_species *sp, picard_ctx
     PICARD_TRY(dalloc(&c->Nfix, g.S * g.G));
     PICARD_TRY(dalloc(&c->E32, 4 * g.G));
     PICARD_TRY(dalloc(&c->r32, g.G));
-    for (int q = 0; q < 3; ++q) PICARD_TRY(dalloc(&c->e32[q], g.G));
+    for (int q = 0; q < 2; ++q) PICARD_TRY(dalloc(&c->e32[q], g.G));
+    for (int q = 0; q < 2; ++q) PICARD_TRY(dalloc(&c->eo32[q], g.G));
     PICARD_HIP(hipMemsetAsync(c->E32, 0, sizeof(float) * 4 * g.G, c->stream));
   } else {
     for (int q = 0; q < 8; ++q) PICARD_TRY(dalloc(&c->p.r[q], cap));
@@ -222,7 +227,7 @@ int create_impl(const picard_geometry *geo, const picard_species *sp, picard_ctx
   if (pb3 > pb) pb = pb3;
   if (make_sched(g.nx, g.ny, g.nz).B > pb) pb = make_sched(g.nx, g.ny, g.nz).B;
   c->s.partials_len = pb;
-  PICARD_TRY(dalloc(&c->s.partials, 2 * pb));     // two launches' worth (read + write)
+  PICARD_TRY(dalloc(&c->s.partials, 4 * pb));     // two launches' worth of two sums each
   PICARD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->js_host), sizeof(JacobiState)));
   PICARD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->host_counters), 16 * sizeof(int32_t)));
   // the reference zeroes U, N, E and the particle arrays at start (picard.f90:434-439)
@@ -244,6 +249,7 @@ int64_t field_len(const picard_ctx *c, int which) {
     case PICARD_FIELD_C: return g.G;
     case PICARD_FIELD_D:
       return (2 * (int64_t)g.nx + 1) * (2 * (int64_t)g.ny + 1) * (2 * (int64_t)g.nz + 1);
+    case PICARD_FIELD_F: return c->F ? 3 * (int64_t)g.S * g.G : -1;
     default: return -1;
   }
 }
@@ -255,6 +261,7 @@ double *field_ptr(picard_ctx *c, int which) {
     case PICARD_FIELD_N: return c->N;
     case PICARD_FIELD_C: return c->C;
     case PICARD_FIELD_D: return c->D;
+    case PICARD_FIELD_F: return c->F;
     default: return nullptr;
   }
 }
@@ -269,6 +276,19 @@ double *field_ptr(picard_ctx *c, int which) {
   } while (0)
 
 }  // namespace
+
+int ensure_flux(picard_ctx *c) {
+  const GridDims &g = c->gd;
+  const int64_t len = 3 * (int64_t)g.S * g.G;
+  if (!c->F) PICARD_TRY(dalloc(&c->F, len));
+  if (c->split) {
+    if (!c->Ffix) PICARD_TRY(dalloc(&c->Ffix, len));
+  } else {
+    if (!c->Fraw) PICARD_TRY(dalloc(&c->Fraw, len));
+  }
+  return PICARD_OK;
+}
+
 }  // namespace picard
 
 using namespace picard;
@@ -412,6 +432,33 @@ int picard_set_production(picard_ctx *c, const picard_plasma *pl, const picard_h
   return PICARD_OK;
 }
 
+int picard_advance_velocity_imp(picard_ctx *c, double deltat, double dt) {
+  CTX_CHECK(c);
+  if (!(dt != 0.0)) {
+    set_error("advance_velocity_imp: dt must be nonzero");
+    return PICARD_ERR_ARG;
+  }
+  return c->split ? sp_boris(c, dt, 1, deltat) : launch_boris(c, dt, 1, deltat);
+}
+
+int picard_advance_velocity_exp(picard_ctx *c, double deltat, double dt) {
+  CTX_CHECK(c);
+  if (!(dt != 0.0)) {
+    set_error("advance_velocity_exp: dt must be nonzero");
+    return PICARD_ERR_ARG;
+  }
+  return c->split ? sp_boris(c, dt, 2, deltat) : launch_boris(c, dt, 2, deltat);
+}
+
+int picard_diagnostics(picard_ctx *c, const double *mass, picard_diag *out) {
+  CTX_CHECK(c);
+  if (!mass || !out) {
+    set_error("diagnostics: mass and out must be non-null");
+    return PICARD_ERR_ARG;
+  }
+  return diagnostics(c, mass, out);
+}
+
 int picard_advance_position(picard_ctx *c, double dt) {
   CTX_CHECK(c);
   if (c->split) return sp_push(c, dt);
@@ -439,6 +486,11 @@ int picard_particles_transfer(picard_ctx *c) {
   CTX_CHECK(c);
   if (c->split) return sp_sort(c, 1);
   return launch_wrap(c);
+}
+
+int picard_particles_gridify_all(picard_ctx *c) {
+  CTX_CHECK(c);
+  return c->split ? sp_gridify(c, true) : launch_gridify(c, true);
 }
 
 int picard_particles_gridify(picard_ctx *c) {

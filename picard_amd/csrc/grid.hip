@@ -81,10 +81,15 @@ __global__ void __launch_bounds__(kT) k_dep_bucket_sort(int64_t nbuckets,
 }
 
 // One thread per (node, species): 8-way merge by descending particle index.
+// FLUX also accumulates the number fluxes of ParticlesGridifyAll
+// (ParticlesGridifyAll.f90:126-212), F(s, c, node) += F_k n2p(s) v_c, in the
+// same order; Fraw is laid out (s fastest, then c, then node).
+template <bool FLUX>
 __global__ void __launch_bounds__(kT) k_dep_gather(ParticlesF64 p, GridDims g, const double *n2p,
                                                     const int32_t *__restrict__ off,
                                                     const int32_t *__restrict__ perm,
-                                                    double *__restrict__ Nraw) {
+                                                    double *__restrict__ Nraw,
+                                                    double *__restrict__ Fraw) {
   const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (t >= g.G * g.S) return;
   const int s0 = (int)(t % g.S);
@@ -110,6 +115,7 @@ __global__ void __launch_bounds__(kT) k_dep_gather(ParticlesF64 p, GridDims g, c
   }
   const double w = n2p[s0];
   double acc = 0.0;                      // N = 0.0d0 (ParticlesGridify.f90:59)
+  double fl[3] = {0.0, 0.0, 0.0};        // F = 0.0d0 (ParticlesGridifyAll.f90:61)
   for (;;) {
     int best = -1;
     int32_t bestp = -1;
@@ -133,8 +139,16 @@ __global__ void __launch_bounds__(kT) k_dep_gather(ParticlesF64 p, GridDims g, c
     const double fz = (best & 4) ? L.rest[2] : L.drest[2];
     const double F = fx * fy * fz / g.dV;
     acc = acc + F * w;
+    if (FLUX) {
+      const double Fw = F * w;
+      fl[0] = fl[0] + Fw * p.r[3][bestp];
+      fl[1] = fl[1] + Fw * p.r[4][bestp];
+      fl[2] = fl[2] + Fw * p.r[5][bestp];
+    }
   }
   Nraw[node * g.S + s0] = acc;
+  if (FLUX)
+    for (int c = 0; c < 3; ++c) Fraw[(node * 3 + c) * g.S + s0] = fl[c];
 }
 
 // Guard handling of one dimension on one rank (ParticlesGridify.f90:129-285):
@@ -195,6 +209,45 @@ __global__ void __launch_bounds__(kT) k_fold_charge(GridDims g, const double *__
     cacc = cacc + vx * charge[s];          // ParticlesGridify.f90:288-293
   }
   C[node] = cacc;
+}
+
+// The same guard folds for the fluxes (ParticlesGridifyAll.f90:238-420):
+// nc = 3*S components per node.
+__global__ void __launch_bounds__(kT) k_fold_flux(GridDims g, const double *__restrict__ Fraw,
+                                                   double *__restrict__ F) {
+  const int64_t node = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (node >= g.G) return;
+  const int i = (int)(node % g.X) + 1;
+  const int j = (int)((node / g.X) % g.Y) + 1;
+  const int k = (int)(node / ((int64_t)g.X * g.Y)) + 1;
+  int sx[2], sy[2], sz[2];
+  bool zx, zy, zz;
+  const int nx_ = fold_src(i, g.nx, g.flow[0], sx, &zx);
+  const int ny_ = fold_src(j, g.ny, g.flow[1], sy, &zy);
+  const int nz_ = fold_src(k, g.nz, g.flow[2], sz, &zz);
+  const int nc = 3 * g.S;
+  for (int q = 0; q < nc; ++q) {
+    double vx = 0.0;
+    if (!zx) {
+      for (int a = 0; a < nx_; ++a) {
+        double vy = 0.0;
+        if (!zy) {
+          for (int bb = 0; bb < ny_; ++bb) {
+            double vz = 0.0;
+            if (!zz) {
+              for (int cc = 0; cc < nz_; ++cc) {
+                const double v = Fraw[node_index(g, sx[a], sy[bb], sz[cc]) * nc + q];
+                vz = cc == 0 ? v : vz + v;
+              }
+            }
+            vy = bb == 0 ? vz : vy + vz;
+          }
+        }
+        vx = a == 0 ? vy : vx + vy;
+      }
+    }
+    F[node * nc + q] = vx;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -258,54 +311,152 @@ __global__ void __launch_bounds__(kT) k_calc_distance(int Nx, int Ny, int Nz, do
 }
 
 // ---------------------------------------------------------------------------
-// BCpotentials (BCpotentials.f90:51-141, one rank).  One lane per face node
-// sums C(interior) * D(window) sequentially in column-major order, which is
-// how amdflang evaluates the SUM of the array expression.
+// BCpotentials (BCpotentials.f90:51-141, one rank).  Every face node's value
+// is the sequential sum, in Fortran column-major order (k, then j, then i
+// fastest), of C(interior) * D(window); that is how amdflang evaluates the
+// SUM of the array expression, and each lane below keeps exactly that order.
+//
+// Lanes of a wave take 64 consecutive face nodes along one transverse axis a
+// and are skewed along the loop over that same axis: at wave step t lane l
+// handles a-index t + l.  The D offset along a is (t + l) - (a0 + l), the same
+// for every lane, so D is read once per wave (uniform, scalar loads) and each
+// lane reads its own charge.  Each lane also accumulates R face nodes along
+// the other transverse axis b, which share the charge and differ only in D.
+// Steps at which a lane is outside [1, N_a] add nothing, so every lane adds
+// the same products in the same order as the reference.
 
-constexpr int kFaceT = 64;
+constexpr int kFaceR = 4;          // face nodes per lane along b
+constexpr int kFaceWaves = 4;      // waves per block, along b
 
-__global__ void __launch_bounds__(kFaceT) k_bcpot_face(GridDims g, int dim,
-                                                        const double *__restrict__ C,
-                                                        const double *__restrict__ D,
-                                                        double *__restrict__ U, double kfac) {
+// x faces (lanes along y, skew on j).  Ct holds the interior charge with y
+// fastest, Ct[(k-1)*Nx*Ny + (i-1)*Ny + (j-1)], so the lanes' reads coalesce.
+__global__ void __launch_bounds__(64 * kFaceWaves) k_bcpot_x(GridDims g, const double *__restrict__ Ct,
+                                                             const double *__restrict__ D,
+                                                             double *__restrict__ U, double kfac) {
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
-  // transverse extents of this face pair
-  const int na = dim == 0 ? Ny : Nx;
-  const int nb = dim == 2 ? Ny : Nz;
-  const int64_t t = (int64_t)blockIdx.x * kFaceT + threadIdx.x;
-  if (t >= 2 * (int64_t)na * nb) return;
-  const int face = (int)(t / ((int64_t)na * nb));       // 0: low face, 1: high face
-  const int64_t rem = t % ((int64_t)na * nb);
-  const int aa = (int)(rem % na) + 1;                     // jj (x) or ii (y, z)
-  const int bb = (int)(rem / na) + 1;                     // kk (x, y) or jj (z)
-  int di, dj, dk;
-  if (dim == 0) {
-    dj = Ny + 1 + (1 - aa);
-    dk = Nz + 1 + (1 - bb);
-    di = face == 0 ? Nx + 1 + 1 : Nx + 1 - Nx;
-  } else if (dim == 1) {
-    di = Nx + 1 + (1 - aa);
-    dk = Nz + 1 + (1 - bb);
-    dj = face == 0 ? Ny + 1 + 1 : Ny + 1 - Ny;
-  } else {
-    di = Nx + 1 + (1 - aa);
-    dj = Ny + 1 + (1 - bb);
-    dk = face == 0 ? Nz + 1 + 1 : Nz + 1 - Nz;
-  }
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform per wave
+  const int face = blockIdx.z;
+  const int a0 = blockIdx.x * 64 + 1;                               // jj of lane 0
+  const int b0 = (blockIdx.y * kFaceWaves + wave) * kFaceR + 1;     // kk of r = 0
+  if (b0 > Nz) return;
+  const int La = Ny - a0 + 1 < 64 ? Ny - a0 + 1 : 64;              // active lanes
+  const bool active = lane < La;
   const int64_t DX = 2 * (int64_t)Nx + 1, DY = 2 * (int64_t)Ny + 1;
-  double s = 0.0;
+  const int di = face == 0 ? Nx + 2 : 1;                           // BCpotentials.f90:60/63
+  double s[kFaceR];
+#pragma unroll
+  for (int r = 0; r < kFaceR; ++r) s[r] = 0.0;
   for (int k = 1; k <= Nz; ++k) {
-    for (int j = 1; j <= Ny; ++j) {
-      const double *crow = C + node_index(g, 2, j + 1, k + 1);
-      const double *drow = D + (int64_t)(di - 1) + DX * ((int64_t)(dj + j - 2) + DY * (int64_t)(dk + k - 2));
-      for (int i = 0; i < Nx; ++i) s = s + crow[i] * drow[i];
+    const double *ck = Ct + (int64_t)(k - 1) * Nx * Ny;
+    for (int t = 1 - (La - 1); t <= Ny; ++t) {
+      const int j = t + lane;
+      const bool on = active && j >= 1 && j <= Ny;
+      const int64_t ydz = (int64_t)(Ny + t - a0) * DX;              // y index Ny+1+j-jj, 0-based
+      const double *drow[kFaceR];
+#pragma unroll
+      for (int r = 0; r < kFaceR; ++r) {
+        const int bb = b0 + r <= Nz ? b0 + r : Nz;                 // clamp: unused lanes of r
+        drow[r] = D + (int64_t)(di - 1) + ydz + DX * DY * (int64_t)(Nz + k - bb);
+      }
+      if (on) {
+        const double *crow = ck + (j - 1);
+        for (int i = 0; i < Nx; ++i) {
+          const double c = crow[(int64_t)i * Ny];
+#pragma unroll
+          for (int r = 0; r < kFaceR; ++r) s[r] = s[r] + c * drow[r][i];
+        }
+      }
     }
   }
-  int64_t at;
-  if (dim == 0) at = node_index(g, face == 0 ? 1 : Nx + 2, aa + 1, bb + 1);
-  else if (dim == 1) at = node_index(g, aa + 1, face == 0 ? 1 : Ny + 2, bb + 1);
-  else at = node_index(g, aa + 1, bb + 1, face == 0 ? 1 : Nz + 2);
-  U[at] = s * kfac;
+  if (!active) return;
+#pragma unroll
+  for (int r = 0; r < kFaceR; ++r) {
+    const int bb = b0 + r;
+    if (bb > Nz) break;
+    U[node_index(g, face == 0 ? 1 : Nx + 2, a0 + lane + 1, bb + 1)] = s[r] * kfac;
+  }
+}
+
+// y faces (dim 1: lanes along x, b = z) and z faces (dim 2: lanes along x,
+// b = y); skew on the innermost loop i, so the charge reads coalesce as is.
+__global__ void __launch_bounds__(64 * kFaceWaves) k_bcpot_yz(GridDims g, int dim,
+                                                              const double *__restrict__ C,
+                                                              const double *__restrict__ D,
+                                                              double *__restrict__ U, double kfac) {
+  const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform per wave
+  const int face = blockIdx.z;
+  const int nb = dim == 1 ? Nz : Ny;
+  const int a0 = blockIdx.x * 64 + 1;                               // ii of lane 0
+  const int b0 = (blockIdx.y * kFaceWaves + wave) * kFaceR + 1;     // kk (dim 1) or jj (dim 2)
+  if (b0 > nb) return;
+  const int La = Nx - a0 + 1 < 64 ? Nx - a0 + 1 : 64;
+  const bool active = lane < La;
+  const int64_t DX = 2 * (int64_t)Nx + 1, DY = 2 * (int64_t)Ny + 1;
+  double s[kFaceR];
+#pragma unroll
+  for (int r = 0; r < kFaceR; ++r) s[r] = 0.0;
+  for (int k = 1; k <= Nz; ++k) {
+    for (int j = 1; j <= Ny; ++j) {
+      // D offset of (j, k) for each r: dim 1: y fixed by the face, z by kk;
+      // dim 2: y by jj, z fixed by the face (BCpotentials.f90:91/94, 122/125)
+      int64_t dyz[kFaceR];
+#pragma unroll
+      for (int r = 0; r < kFaceR; ++r) {
+        const int bb = b0 + r <= nb ? b0 + r : nb;
+        int64_t yi, zi;                                             // 0-based D indices
+        if (dim == 1) {
+          yi = face == 0 ? (int64_t)Ny + j : (int64_t)j - 1;
+          zi = (int64_t)Nz + k - bb;
+        } else {
+          yi = (int64_t)Ny + j - bb;
+          zi = face == 0 ? (int64_t)Nz + k : (int64_t)k - 1;
+        }
+        dyz[r] = DX * (yi + DY * zi);
+      }
+      const double *crow = C + node_index(g, 2, j + 1, k + 1);
+      for (int t = 1 - (La - 1); t <= Nx; ++t) {
+        const int i = t + lane;
+        const int64_t xi = (int64_t)Nx + t - a0;                    // x index Nx+1+i-ii, 0-based
+        if (active && i >= 1 && i <= Nx) {
+          const double c = crow[i - 1];
+#pragma unroll
+          for (int r = 0; r < kFaceR; ++r) s[r] = s[r] + c * D[xi + dyz[r]];
+        }
+      }
+    }
+  }
+  if (!active) return;
+#pragma unroll
+  for (int r = 0; r < kFaceR; ++r) {
+    const int bb = b0 + r;
+    if (bb > nb) break;
+    const int ii = a0 + lane;
+    const int64_t at = dim == 1 ? node_index(g, ii + 1, face == 0 ? 1 : Ny + 2, bb + 1)
+                                : node_index(g, ii + 1, bb + 1, face == 0 ? 1 : Nz + 2);
+    U[at] = s[r] * kfac;
+  }
+}
+
+// Interior charge with y fastest, for k_bcpot_x.
+__global__ void __launch_bounds__(256) k_charge_yfirst(GridDims g, const double *__restrict__ C,
+                                                       double *__restrict__ Ct) {
+  __shared__ double tile[32][33];
+  const int k = blockIdx.z + 1;
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;           // 32 x 8
+  for (int q = ty; q < 32; q += 8) {
+    const int i = i0 + tx + 1, j = j0 + q + 1;
+    if (i <= g.nx && j <= g.ny) tile[q][tx] = C[node_index(g, i + 1, j + 1, k + 1)];
+  }
+  __syncthreads();
+  for (int q = ty; q < 32; q += 8) {
+    const int j = j0 + tx + 1, i = i0 + q + 1;
+    if (i <= g.nx && j <= g.ny)
+      Ct[(int64_t)(k - 1) * g.nx * g.ny + (int64_t)(i - 1) * g.ny + (j - 1)] = tile[tx][q];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -555,7 +706,7 @@ __global__ void __launch_bounds__(kT) k_bc_fields(GridDims g, double *__restrict
 
 // ---------------------------------------------------------------------------
 
-int launch_gridify(picard_ctx *c) {
+int launch_gridify(picard_ctx *c, bool flux) {
   const GridDims &g = c->gd;
   const int64_t n = c->n;
   const int64_t nb = (int64_t)g.S * c->K;
@@ -575,8 +726,16 @@ int launch_gridify(picard_ctx *c) {
     k_dep_bucket_sort<<<grid_blocks(nb, kT), kT, 0, c->stream>>>(nb, c->s.hist, c->s.perm);
     PICARD_HIP(hipGetLastError());
   }
-  k_dep_gather<<<grid_blocks(g.G * g.S, kT), kT, 0, c->stream>>>(c->p, g, c->consts, c->s.hist,
-                                                                   c->s.perm, c->Nraw);
+  if (flux) {
+    PICARD_TRY(ensure_flux(c));
+    k_dep_gather<true><<<grid_blocks(g.G * g.S, kT), kT, 0, c->stream>>>(
+        c->p, g, c->consts, c->s.hist, c->s.perm, c->Nraw, c->Fraw);
+    PICARD_HIP(hipGetLastError());
+    k_fold_flux<<<grid_blocks(g.G, kT), kT, 0, c->stream>>>(g, c->Fraw, c->F);
+  } else {
+    k_dep_gather<false><<<grid_blocks(g.G * g.S, kT), kT, 0, c->stream>>>(
+        c->p, g, c->consts, c->s.hist, c->s.perm, c->Nraw, nullptr);
+  }
   PICARD_HIP(hipGetLastError());
   k_fold_charge<<<grid_blocks(g.G, kT), kT, 0, c->stream>>>(g, c->Nraw, c->N, c->C,
                                                             c->consts + 8);
@@ -615,13 +774,21 @@ int launch_bc_potentials(picard_ctx *c) {
   const double tau = 6.2831853071795864769252867665590;
   const double eps0 = 8.854187817e-12;
   const double kfac = 1.0 / (2.0 * tau * eps0);
-  for (int dim = 0; dim < 3; ++dim) {
+  const int per_block = kFaceR * kFaceWaves;
+  if (g.flow[0]) {
+    // the third Jacobi buffer is free until CalcEpotential copies U into it
+    double *Ct = c->jb[2];
+    k_charge_yfirst<<<dim3((g.nx + 31) / 32, (g.ny + 31) / 32, g.nz), 256, 0, c->stream>>>(g, c->C,
+                                                                                         Ct);
+    k_bcpot_x<<<dim3((g.ny + 63) / 64, (g.nz + per_block - 1) / per_block, 2), 64 * kFaceWaves, 0,
+                c->stream>>>(g, Ct, c->D, c->U, kfac);
+    PICARD_HIP(hipGetLastError());
+  }
+  for (int dim = 1; dim < 3; ++dim) {
     if (!g.flow[dim]) continue;
-    const int na = dim == 0 ? g.ny : g.nx;
-    const int nb = dim == 2 ? g.ny : g.nz;
-    const int64_t lanes = 2 * (int64_t)na * nb;
-    k_bcpot_face<<<grid_blocks(lanes, kFaceT), kFaceT, 0, c->stream>>>(g, dim, c->C, c->D, c->U,
-                                                                       kfac);
+    const int nb = dim == 1 ? g.nz : g.ny;
+    k_bcpot_yz<<<dim3((g.nx + 63) / 64, (nb + per_block - 1) / per_block, 2), 64 * kFaceWaves, 0,
+                 c->stream>>>(g, dim, c->C, c->D, c->U, kfac);
     PICARD_HIP(hipGetLastError());
   }
   return PICARD_OK;

@@ -160,7 +160,13 @@ struct BorisConst {
   int32_t fluid[8], gyration[8];
 };
 
-// AdvanceVelocity.f90:34-121, one thread per particle.
+// MODE 0: AdvanceVelocity (AdvanceVelocity.f90:34-121), the Boris push.
+// MODE 1: AdvanceVelocityImp (:240-347), the electric half kick then the
+//         norm-preserving rotation, leaving v at the half step of output steps.
+// MODE 2: AdvanceVelocityExp (:128-238), the rotation then the electric kick.
+// etadt2 and tB in `bc` are the mode's own (scaled by 2 deltat/dt for 1, 2).
+// One thread per particle.
+template <int MODE>
 __global__ void __launch_bounds__(kT) k_boris(ParticlesF64 p, int64_t n, GridDims g,
                                                const double *__restrict__ E, BorisConst bc,
                                                int32_t *__restrict__ errors) {
@@ -219,18 +225,47 @@ __global__ void __launch_bounds__(kT) k_boris(ParticlesF64 p, int64_t n, GridDim
   }
   const double h = bc.etadt2[s];
   const double *t = bc.tB[s], *q = bc.sB[s];
-  const double vm1 = v1 - bc.ve[0] + h * Ep[0];
-  const double vm2 = v2 - bc.ve[1] + h * Ep[1];
-  const double vm3 = v3 - bc.ve[2] + h * Ep[2];
-  const double vd1 = vm1 + vm2 * t[2] - vm3 * t[1];
-  const double vd2 = vm2 + vm3 * t[0] - vm1 * t[2];
-  const double vd3 = vm3 + vm1 * t[1] - vm2 * t[0];
-  const double vp1 = vm1 + vd2 * q[2] - vd3 * q[1];
-  const double vp2 = vm2 + vd3 * q[0] - vd1 * q[2];
-  const double vp3 = vm3 + vd1 * q[1] - vd2 * q[0];
-  p.r[3][i] = vp1 + bc.ve[0] + h * Ep[0];
-  p.r[4][i] = vp2 + bc.ve[1] + h * Ep[1];
-  p.r[5][i] = vp3 + bc.ve[2] + h * Ep[2];
+  if (MODE == 0) {
+    const double vm1 = v1 - bc.ve[0] + h * Ep[0];
+    const double vm2 = v2 - bc.ve[1] + h * Ep[1];
+    const double vm3 = v3 - bc.ve[2] + h * Ep[2];
+    const double vd1 = vm1 + vm2 * t[2] - vm3 * t[1];
+    const double vd2 = vm2 + vm3 * t[0] - vm1 * t[2];
+    const double vd3 = vm3 + vm1 * t[1] - vm2 * t[0];
+    const double vp1 = vm1 + vd2 * q[2] - vd3 * q[1];
+    const double vp2 = vm2 + vd3 * q[0] - vd1 * q[2];
+    const double vp3 = vm3 + vd1 * q[1] - vd2 * q[0];
+    p.r[3][i] = vp1 + bc.ve[0] + h * Ep[0];
+    p.r[4][i] = vp2 + bc.ve[1] + h * Ep[1];
+    p.r[5][i] = vp3 + bc.ve[2] + h * Ep[2];
+    return;
+  }
+  // Imp: kick in the ve frame before the rotation; Exp: after it
+  double vm1 = v1 - bc.ve[0], vm2 = v2 - bc.ve[1], vm3 = v3 - bc.ve[2];
+  if (MODE == 1) {
+    vm1 = vm1 + h * Ep[0];
+    vm2 = vm2 + h * Ep[1];
+    vm3 = vm3 + h * Ep[2];
+  }
+  double vd1 = vm1 + vm2 * t[2] - vm3 * t[1];
+  double vd2 = vm2 + vm3 * t[0] - vm1 * t[2];
+  double vd3 = vm3 + vm1 * t[1] - vm2 * t[0];
+  const double sd = vd1 * vd1 + vd2 * vd2 + vd3 * vd3;
+  if (sd > 0.0) {                                   // normalise (:207-209, :333-335)
+    const double f = sqrt((vm1 * vm1 + vm2 * vm2 + vm3 * vm3) / sd);
+    vd1 = vd1 * f;
+    vd2 = vd2 * f;
+    vd3 = vd3 * f;
+  }
+  if (MODE == 1) {
+    p.r[3][i] = vd1 + bc.ve[0];
+    p.r[4][i] = vd2 + bc.ve[1];
+    p.r[5][i] = vd3 + bc.ve[2];
+  } else {
+    p.r[3][i] = vd1 + bc.ve[0] + h * Ep[0];
+    p.r[4][i] = vd2 + bc.ve[1] + h * Ep[1];
+    p.r[5][i] = vd3 + bc.ve[2] + h * Ep[2];
+  }
 }
 
 __global__ void __launch_bounds__(kT) k_aos_to_soa(const double *__restrict__ stage,
@@ -336,15 +371,18 @@ int launch_wrap(picard_ctx *c) {
   return PICARD_OK;
 }
 
-int launch_boris(picard_ctx *c, double dt) {
+int launch_boris(picard_ctx *c, double dt, int mode, double deltat) {
   if (c->n <= 0) return PICARD_OK;
   BorisConst bc;
   for (int s = 0; s < 8; ++s) {
-    bc.etadt2[s] = 0.5 * dt * c->spc.eta[s];                 // AdvanceVelocity.f90:35
-    for (int q = 0; q < 3; ++q) {
-      bc.tB[s][q] = c->spc.tB[4 * s + q];
-      bc.sB[s][q] = c->spc.sB[4 * s + q];
+    if (mode == 0) {
+      bc.etadt2[s] = 0.5 * dt * c->spc.eta[s];               // AdvanceVelocity.f90:35
+      for (int q = 0; q < 3; ++q) bc.tB[s][q] = c->spc.tB[4 * s + q];
+    } else {                                                // :156-157, :267-268
+      bc.etadt2[s] = 0.5 * dt * c->spc.eta[s] * 2.0 * deltat / dt;
+      for (int q = 0; q < 3; ++q) bc.tB[s][q] = c->spc.tB[4 * s + q] * 2.0 * deltat / dt;
     }
+    for (int q = 0; q < 3; ++q) bc.sB[s][q] = c->spc.sB[4 * s + q];
     bc.fluid[s] = c->spc.fluid[s];
     bc.gyration[s] = c->spc.gyration[s];
   }
@@ -353,8 +391,13 @@ int launch_boris(picard_ctx *c, double dt) {
     bc.B0[q] = c->spc.B0[q];
   }
   PICARD_HIP(hipMemsetAsync(c->s.counters, 0, sizeof(int32_t), c->stream));
-  k_boris<<<grid_blocks(c->n, kT), kT, 0, c->stream>>>(c->p, c->n, c->gd, c->E, bc,
-                                                        c->s.counters);
+  const int nb = grid_blocks(c->n, kT);
+  if (mode == 0)
+    k_boris<0><<<nb, kT, 0, c->stream>>>(c->p, c->n, c->gd, c->E, bc, c->s.counters);
+  else if (mode == 1)
+    k_boris<1><<<nb, kT, 0, c->stream>>>(c->p, c->n, c->gd, c->E, bc, c->s.counters);
+  else
+    k_boris<2><<<nb, kT, 0, c->stream>>>(c->p, c->n, c->gd, c->E, bc, c->s.counters);
   PICARD_HIP(hipGetLastError());
   PICARD_HIP(hipMemcpyAsync(c->host_counters, c->s.counters, sizeof(int32_t),
                             hipMemcpyDeviceToHost, c->stream));

@@ -120,7 +120,8 @@ struct picard_ctx {
   picard::ParticlesSplit ps{}, ps2{};
   unsigned long long *Nfix = nullptr;   // split: fixed-point densities
   float *E32 = nullptr;      // split: E as float4 per node
-  float *r32 = nullptr, *e32[3] = {nullptr, nullptr, nullptr};
+  float *r32 = nullptr, *e32[2] = {nullptr, nullptr};
+  float *eo32[2] = {nullptr, nullptr};   // split: odd inner iterates on periodic boundary planes
   int mx_inner = 64;         // split: FP32 sweeps per FP64 refinement
   int64_t next_id = 0;
 
@@ -138,6 +139,11 @@ struct picard_ctx {
   double2 *fft_work = nullptr;   // transformed charge planes + two face accumulators
   int64_t fft_work_len = 0;
   bool fft_ready = false;
+
+  // number fluxes of ParticlesGridifyAll (allocated on first use)
+  double *F = nullptr;                 // F(species, 3, node)
+  double *Fraw = nullptr;              // FP64 form: before the guard folds
+  unsigned long long *Ffix = nullptr;  // split form: fixed-point accumulators
 
   // device Haser production (ParticlesProduce) run by picard_step
   int32_t *haser_off = nullptr;  // per (cell, producing species): slot counts, then offsets
@@ -171,7 +177,8 @@ namespace picard {
 int launch_push_x(picard_ctx *c, int64_t first, int64_t count, double dt);
 int launch_outflow(picard_ctx *c, int64_t *n_new, int64_t *removed);
 int launch_wrap(picard_ctx *c);
-int launch_boris(picard_ctx *c, double dt);
+// mode 0: AdvanceVelocity; 1: AdvanceVelocityImp; 2: AdvanceVelocityExp (deltat)
+int launch_boris(picard_ctx *c, double dt, int mode = 0, double deltat = 0.0);
 int launch_append(picard_ctx *c, const double *host_rp, const int32_t *host_sp, int64_t count);
 int launch_read(picard_ctx *c, int64_t first, int64_t count, double *host_rp, int32_t *host_sp);
 
@@ -179,12 +186,16 @@ int ensure_stage(picard_ctx *c, int64_t want);
 
 // ---- grid.hip ----
 int launch_bnorm(picard_ctx *c, double maxerr);
-int launch_gridify(picard_ctx *c);
+// flux: also the number fluxes F of ParticlesGridifyAll
+int launch_gridify(picard_ctx *c, bool flux = false);
 int launch_calc_distance(picard_ctx *c);
 int launch_bc_potentials(picard_ctx *c);
 int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps);
 int launch_calc_efield(picard_ctx *c);
 int launch_bc_fields(picard_ctx *c);
+
+// ---- diag.hip ----
+int diagnostics(picard_ctx *c, const double *mass, picard_diag *out);
 
 // ---- bcpot_fft.hip ----
 int bcpot_fft_prepare(picard_ctx *c);
@@ -200,11 +211,13 @@ int produce_haser(picard_ctx *c, const picard_plasma *pl, const picard_haser *hs
 // ---- split.hip ----
 int sp_push(picard_ctx *c, double dt);
 int sp_sort(picard_ctx *c, int wrap);
-int sp_gridify(picard_ctx *c);
+int sp_gridify(picard_ctx *c, bool flux = false);
+// allocates the flux arrays (F, and Fraw or Ffix) on first use
+int ensure_flux(picard_ctx *c);
 int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps);
 int sp_efield(picard_ctx *c);
 int sp_widen_e(picard_ctx *c);
-int sp_boris(picard_ctx *c, double dt);
+int sp_boris(picard_ctx *c, double dt, int mode = 0, double deltat = 0.0);
 int sp_append(picard_ctx *c, const double *host_rp, const int32_t *host_sp, int64_t count);
 int sp_read(picard_ctx *c, int64_t first, int64_t count, double *host_rp, int32_t *host_sp);
 

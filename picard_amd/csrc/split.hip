@@ -131,9 +131,16 @@ __global__ void __launch_bounds__(kT) k_sp_scatter(ParticlesSplit src, Particles
   dst.id[at] = src.id[q];
 }
 
-// Fixed-point CIC deposit (int64 atomics, order independent).
+// Fixed-point CIC deposit (int64 atomics, order independent).  FLUX adds
+// the number fluxes of ParticlesGridifyAll (ParticlesGridifyAll.f90:126-212)
+// as weight * v_c in units of 2^-24 m/s (Ffix: s fastest, then c, then node).
+constexpr double kFluxFix = 16777216.0;         // 2^24
+constexpr double kFluxFixInv = 1.0 / 16777216.0;
+
+template <bool FLUX>
 __global__ void __launch_bounds__(kT) k_sp_deposit(ParticlesSplit p, int64_t n, GridDims g,
                                                     unsigned long long *__restrict__ Nfix,
+                                                    unsigned long long *__restrict__ Ffix,
                                                     int32_t *__restrict__ errors) {
   const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (q >= n) return;
@@ -155,7 +162,15 @@ __global__ void __launch_bounds__(kT) k_sp_deposit(ParticlesSplit p, int64_t n, 
       for (int a = 0; a < 2; ++a) {
         const float w = wx[a] * wy[b] * wz[c];
         const unsigned long long W = (unsigned long long)(double(w) * kFix);
-        atomicAdd(&Nfix[(base + a + X * b + XY * c) * g.S + s], W);
+        const int64_t node = base + a + X * b + XY * c;
+        atomicAdd(&Nfix[node * g.S + s], W);
+        if (FLUX) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const long long Wf = llrint((double)w * (double)p.v[d][q] * kFluxFix);
+            atomicAdd(&Ffix[(node * 3 + d) * g.S + s], (unsigned long long)Wf);
+          }
+        }
       }
 }
 
@@ -202,6 +217,31 @@ __global__ void __launch_bounds__(kT) k_sp_fold(GridDims g, const unsigned long 
     cacc = cacc + v * consts[8 + s];
   }
   C[node] = cacc;
+}
+
+// Integer guard folds of the fluxes, F = Wf * 2^-24 * n2p (ParticlesGridifyAll.f90:238-420).
+__global__ void __launch_bounds__(kT) k_sp_fold_flux(GridDims g,
+                                                      const unsigned long long *__restrict__ Ffix,
+                                                      double *__restrict__ F, const double *consts) {
+  const int64_t node = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (node >= g.G) return;
+  const int i = (int)(node % g.X) + 1;
+  const int j = (int)((node / g.X) % g.Y) + 1;
+  const int k = (int)(node / ((int64_t)g.X * g.Y)) + 1;
+  int sx[2], sy[2], sz[2];
+  bool zx, zy, zz;
+  const int nxs = sp_fold_src(i, g.nx, g.flow[0], sx, &zx);
+  const int nys = sp_fold_src(j, g.ny, g.flow[1], sy, &zy);
+  const int nzs = sp_fold_src(k, g.nz, g.flow[2], sz, &zz);
+  const int nc = 3 * g.S;
+  for (int q = 0; q < nc; ++q) {
+    unsigned long long w = 0;
+    if (!zx && !zy && !zz)
+      for (int a = 0; a < nxs; ++a)
+        for (int b = 0; b < nys; ++b)
+          for (int c = 0; c < nzs; ++c) w += Ffix[node_index(g, sx[a], sy[b], sz[c]) * nc + q];
+    F[node * nc + q] = (double)(long long)w * kFluxFixInv * consts[q % g.S];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -268,11 +308,19 @@ __device__ __forceinline__ StopSlot mx_decide(JacobiState *st, const MxLaunch &L
   return cur;
 }
 
-// Outer step: r = b - A u in FP64 (fresh halos), r32 = r; partials of |r|^2.
+// Outer step: r = b - A u in FP64 with fresh periodic images, r32 = r (the
+// right-hand side of the inner sweeps); partials of |r_stale|^2 for the stop
+// test of iterate `base`.  The reference evaluates that residual with the
+// periodic halo of the previous iterate (CalcEpotential.f90:83-92), so a
+// neighbour across a periodic boundary takes U_{base-1} = u - (e_m - e_{m-1}):
+// em and eo hold e_m and e_{m-1} of the previous outer step (null when base
+// is 0, whose test is the |b| test of the first loop iteration).
 __global__ void __launch_bounds__(256) k_mx_residual(GridDims g, MixCoef mc, TileSched ts,
                                                       const double *__restrict__ u,
                                                       const double *__restrict__ b,
                                                       float *__restrict__ r32,
+                                                      const float *__restrict__ em,
+                                                      const float *__restrict__ eo,
                                                       const double *__restrict__ Pin,
                                                       double *__restrict__ Pout, JacobiState *st,
                                                       MxLaunch L) {
@@ -281,124 +329,311 @@ __global__ void __launch_bounds__(256) k_mx_residual(GridDims g, MixCoef mc, Til
   const StopSlot cur = mx_decide(st, L, Pin, ts.B, sh, &decide);
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   if (cur.done) return;
+  const bool stale = em != nullptr;
   double r2 = 0.0;
   for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int k, bool valid) {
     if (!valid) return;
     const int64_t c = node_index(g, i, j, k);
     const Nb nb = neighbours(g, i, j, k, c);
     const double u2 = 2.0 * u[c];
-    const double r = b[c] - ((u[nb.e] + u[nb.w] - u2) / mc.dx2 + (u[nb.n] + u[nb.s] - u2) / mc.dy2 +
-                             (u[nb.t] + u[nb.b] - u2) / mc.dz2);
+    double ue = u[nb.e], uw = u[nb.w], un = u[nb.n], us = u[nb.s], ut = u[nb.t], ub = u[nb.b];
+    const double r = b[c] - ((ue + uw - u2) / mc.dx2 + (un + us - u2) / mc.dy2 +
+                             (ut + ub - u2) / mc.dz2);
     r32[c] = (float)r;
-    r2 = r2 + r * r;
+    double rs = r;
+    if (stale) {
+      auto prev = [&](int64_t n) { return u[n] - ((double)em[n] - (double)eo[n]); };
+      bool any = false;
+      if (!g.flow[0] && i == g.nx + 1) { ue = prev(nb.e); any = true; }
+      if (!g.flow[0] && i == 2) { uw = prev(nb.w); any = true; }
+      if (!g.flow[1] && j == g.ny + 1) { un = prev(nb.n); any = true; }
+      if (!g.flow[1] && j == 2) { us = prev(nb.s); any = true; }
+      if (!g.flow[2] && k == g.nz + 1) { ut = prev(nb.t); any = true; }
+      if (!g.flow[2] && k == 2) { ub = prev(nb.b); any = true; }
+      if (any)
+        rs = b[c] - ((ue + uw - u2) / mc.dx2 + (un + us - u2) / mc.dy2 + (ut + ub - u2) / mc.dz2);
+    }
+    r2 = r2 + rs * rs;
   });
   const double sblk = block_sum(r2, sh);
   if (lin_tid() == 0) Pout[blockIdx.x] = sblk;
 }
 
-// Inner FP32 sweep e_j -> e_{j+1} (e_0 = 0; e_k lives in E[k % 3]) and, for
-// j >= 1, partials of the residual of iterate base+j, r - A e_j, accumulated
-// in FP64.  Flow guards of e are zero: the correction keeps the boundary.
-__global__ void __launch_bounds__(256) k_mx_sweep(GridDims g, MixCoef mc, TileSched ts,
-                                                   const float *__restrict__ r32, float *E0, float *E1,
-                                                   float *E2, const double *__restrict__ Pin,
-                                                   double *__restrict__ Pout, JacobiState *st,
-                                                   MxLaunch L) {
+// ---------------------------------------------------------------------------
+// Two inner sweeps per launch (temporal blocking).  Launch q takes e_{2q}
+// (buffer q % 2) to e_{2q+2} (buffer (q+1) % 2); e_{2q+1} lives only in LDS.
+// A block owns a BX x BY tile of the interior and marches it along its XCD's
+// z-slab.  For each virtual plane v it stages e_{2q} on the tile plus two
+// rings, computes e_{2q+1} on the tile plus one ring (the residual of e_{2q}
+// on the tile as a by-product), and then e_{2q+2} on plane v - 1 of the tile
+// (with the residual of e_{2q+1}).  Ring points outside the interior take
+// their periodic images, or the zero correction of a flow guard, so the
+// values equal those of two single sweeps.  Both residuals are summed per
+// block in FP64, in a fixed order.
+//
+// The residuals are those of the stop test, which the reference evaluates
+// with the periodic halo of the previous iterate (CalcEpotential.f90:83-92):
+// a neighbour across a periodic boundary takes e_{2q+1}'s predecessor e_{2q}
+// (from the staged planes) in the residual of e_{2q+1}, and e_{2q-1} in the
+// residual of e_{2q}.  e_{2q-1} exists only in the previous launch, which
+// leaves its values on the periodic boundary planes in `eo_in`; this launch
+// leaves those of e_{2q+1} in `eo_out`.
+
+
+struct MxLaunch2 {
+  int64_t seq;
+  int64_t base;          // iterate held by u
+  int64_t q;             // e_{2q} -> e_{2q+2}
+  int64_t check1;        // iterate whose residual Pin1 holds, or -1
+  int64_t check2;        // iterate whose residual Pin2 holds, or -1
+  int32_t m;             // inner sweeps of this outer step
+};
+
+// Stop tests of loop iterations check1 + 1 and check2 + 1, in that order
+// (CalcEpotential.f90:60-62).
+__device__ __forceinline__ StopSlot mx_decide2(JacobiState *st, const MxLaunch2 &L,
+                                               const double *__restrict__ Pin1,
+                                               const double *__restrict__ Pin2, int nblk,
+                                               double *sh, int *decide) {
+  const StopSlot prev = st->slot[(L.seq - 1) & 1];
+  StopSlot cur = prev;
+  if (prev.done) return cur;
+  const int64_t chk[2] = {L.check1, L.check2};
+  const double *pin[2] = {Pin1, Pin2};
+  for (int h = 0; h < 2; ++h) {
+    if (chk[h] < 1) continue;
+    const double S = sum_partials(pin[h], nblk, sh);
+    if (lin_tid() == 0) *decide = (chk[h] + 1 <= st->maxit && sqrt(S) <= st->thresh) ? 1 : 0;
+    __syncthreads();
+    const int d = *decide;
+    __syncthreads();
+    if (d) {
+      cur.done = 1;
+      cur.result = chk[h];
+      break;
+    }
+  }
+  return cur;
+}
+
+// Physical 1-based index of virtual index v along an axis of n interior
+// cells: periodic image, or 0 for a flow guard / beyond.
+__device__ __forceinline__ int vmap(int v, int n, int flow) {
+  if (v >= 2 && v <= n + 1) return v;
+  if (flow) return 0;
+  int r = (v - 2) % n;
+  if (r < 0) r += n;
+  return r + 2;
+}
+
+// Tile BX x (256 / BX): BX = 32 for nx >= 32, else 16 or 8 (the stencil
+// schedule's tile width, capped at 32).
+template <int BX>
+__global__ void __launch_bounds__(256) k_mx_sweep2(GridDims g, MixCoef mc, TileSched ts,
+                                                    const float *__restrict__ r32,
+                                                    const float *E0, const float *E1,
+                                                    float *F0, float *F1,
+                                                    const float *__restrict__ eo_in,
+                                                    float *__restrict__ eo_out,
+                                                    const double *__restrict__ Pin1,
+                                                    const double *__restrict__ Pin2,
+                                                    double *__restrict__ Pout1,
+                                                    double *__restrict__ Pout2, JacobiState *st,
+                                                    MxLaunch2 L) {
+  constexpr int kB2X = BX, kB2Y = 256 / BX;
+  constexpr int kAX = kB2X + 4, kAY = kB2Y + 4;  // e_{2q} staging: tile + 2 rings
+  constexpr int kRX = kB2X + 2, kRY = kB2Y + 2;  // e_{2q+1} and r32: tile + 1 ring
   __shared__ double sh[4];
   __shared__ int decide;
-  StopSlot cur = mx_decide(st, L, Pin, ts.B, sh, &decide);
-  if (!cur.done && L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
+  __shared__ float A[5][kAY][kAX];     // planes v-3 .. v+1 (sweep 2 reads v-2 .. v)
+  __shared__ float Bq[3][kRY][kRX];
+  __shared__ float R[3][kRY][kRX];
+  StopSlot cur = mx_decide2(st, L, Pin1, Pin2, ts.B, sh, &decide);
+  if (!cur.done && L.base + 2 * L.q >= st->last) {   // iterate 2*maxit+1 reached (:58)
     cur.done = 1;
     cur.result = st->last;
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   if (cur.done) return;
-  float *E[3] = {E0, E1, E2};
-  const float *e = L.j >= 1 ? E[L.j % 3] : nullptr;
-  float *out = E[(L.j + 1) % 3];
-  const bool res = L.j >= 1;
+  const float *e = (L.q & 1) ? E1 : E0;          // e_{2q}; unused when q == 0 (e_0 = 0)
+  float *out = (L.q & 1) ? F0 : F1;              // e_{2q+2}
+  const bool zero_in = L.q == 0;
   const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
-  const int nz = g.nz;
-  double r2 = 0.0;
-  // 2.5D march over the XCD's z-slab with the z-neighbours of e_j in registers
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int lt = lin_tid();
   const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
   const int z0 = (xcd * nz) >> 3, z1 = ((xcd + 1) * nz) >> 3;
-  for (int t = w; z1 > z0 && t < ts.TP; t += ts.W) {
-    const int tx = t % ts.ntx, ty = t / ts.ntx;
-    const int i = tx * ts.bx + (int)threadIdx.x + 2;
-    const int j = ty * ts.by + (int)threadIdx.y + 2;
-    if (i > g.nx + 1 || j > g.ny + 1) continue;
-    const int64_t col = (int64_t)(i - 1) + X * (int64_t)(j - 1);
-    if (!e) {
-      for (int k = z0 + 2; k <= z1 + 1; ++k) {
-        const int64_t c = col + XY * (int64_t)(k - 1);
-        out[c] = -mc.aff * r32[c];
+  const int P0 = z0 + 2, P1 = z1 + 1;            // output planes (1-based)
+  double ra = 0.0, rb = 0.0;
+  for (int t = w; P1 >= P0 && t < ts.TP; t += ts.W) {
+    const int i0 = (t % ts.ntx) * kB2X + 2, j0 = (t / ts.ntx) * kB2Y + 2;
+    // stage e_{2q} of virtual plane v (tile + 2 rings) into A[v % 5]
+    auto stageA = [&](int v) {
+      const int pk = vmap(v, nz, g.flow[2]);
+      float(*a)[kAX] = A[v % 5];
+      for (int idx = lt; idx < kAX * kAY; idx += 256) {
+        const int ax = idx % kAX, ay = idx / kAX;
+        float val = 0.0f;
+        if (!zero_in && pk) {
+          const int pi = vmap(i0 - 2 + ax, nx, g.flow[0]);
+          const int pj = vmap(j0 - 2 + ay, ny, g.flow[1]);
+          if (pi && pj) val = e[(int64_t)(pi - 1) + X * (int64_t)(pj - 1) + XY * (int64_t)(pk - 1)];
+        }
+        a[ay][ax] = val;
       }
-      continue;
-    }
-    // x/y neighbours: periodic wrap, or the zero correction of a flow guard
-    const bool fe = g.flow[0] && i == g.nx + 1, fw = g.flow[0] && i == 2;
-    const bool fn = g.flow[1] && j == g.ny + 1, fs = g.flow[1] && j == 2;
-    const int64_t oe = (!g.flow[0] && i == g.nx + 1) ? -(int64_t)(g.nx - 1) : 1;
-    const int64_t ow = (!g.flow[0] && i == 2) ? (int64_t)(g.nx - 1) : -1;
-    const int64_t on = (!g.flow[1] && j == g.ny + 1) ? -(int64_t)(g.ny - 1) * X : X;
-    const int64_t os = (!g.flow[1] && j == 2) ? (int64_t)(g.ny - 1) * X : -X;
-    // value of e_j on 1-based plane kk (guards: wrap or zero)
-    auto zval = [&](int kk) -> float {
-      if (kk == 1) return g.flow[2] ? 0.0f : e[col + XY * (int64_t)nz];
-      if (kk == nz + 2) return g.flow[2] ? 0.0f : e[col + XY];
-      return e[col + XY * (int64_t)(kk - 1)];
     };
-    int k = z0 + 2;
-    float zm = zval(k - 1), zc = zval(k), zp = zval(k + 1);
-#pragma unroll 2
-    for (; k <= z1 + 1; ++k) {
-      const int64_t c = col + XY * (int64_t)(k - 1);
-      const float ee = fe ? 0.0f : e[c + oe];
-      const float ew = fw ? 0.0f : e[c + ow];
-      const float en = fn ? 0.0f : e[c + on];
-      const float es = fs ? 0.0f : e[c + os];
-      const float et = zp, eb = zm;
-      const float rc = r32[c];
-      out[c] = mc.axf * (ee + ew) + mc.ayf * (en + es) + mc.azf * (et + eb) - mc.aff * rc;
-      const float e2 = 2.0f * zc;
-      const float rr = rc - ((ee + ew - e2) * mc.idx2f + (en + es - e2) * mc.idy2f +
-                             (et + eb - e2) * mc.idz2f);
-      r2 = r2 + (double)rr * (double)rr;
-      zm = zc;
-      zc = zp;
-      if (k + 1 <= z1 + 1) zp = zval(k + 2);
+    // r32 of virtual plane v on the tile + 1 ring into R[v % 3]
+    auto stageR = [&](int v) {
+      const int pk = vmap(v, nz, g.flow[2]);
+      float(*r)[kRX] = R[v % 3];
+      for (int idx = lt; idx < kRX * kRY; idx += 256) {
+        const int rx = idx % kRX, ry = idx / kRX;
+        float val = 0.0f;
+        if (pk) {
+          const int pi = vmap(i0 - 1 + rx, nx, g.flow[0]);
+          const int pj = vmap(j0 - 1 + ry, ny, g.flow[1]);
+          if (pi && pj) val = r32[(int64_t)(pi - 1) + X * (int64_t)(pj - 1) + XY * (int64_t)(pk - 1)];
+        }
+        r[ry][rx] = val;
+      }
+    };
+    stageA(P0 - 2);
+    stageA(P0 - 1);
+    for (int v = P0 - 1; v <= P1 + 1; ++v) {
+      stageA(v + 1);
+      stageR(v);
+      __syncthreads();
+      // e_{2q+1} on plane v, tile + 1 ring; residual of e_{2q} on the tile
+      {
+        const bool vin = vmap(v, nz, g.flow[2]) != 0;
+        const float(*am)[kAX] = A[(v + 4) % 5];   // plane v - 1
+        const float(*ac)[kAX] = A[v % 5];
+        const float(*ap)[kAX] = A[(v + 1) % 5];
+        float(*bo)[kRX] = Bq[v % 3];
+        const float(*rr)[kRX] = R[v % 3];
+        for (int idx = lt; idx < kRX * kRY; idx += 256) {
+          const int rx = idx % kRX, ry = idx / kRX;
+          const int ax = rx + 1, ay = ry + 1;
+          const int vi = i0 - 1 + rx, vj = j0 - 1 + ry;
+          const bool pin = vin && vmap(vi, nx, g.flow[0]) && vmap(vj, ny, g.flow[1]);
+          float val = 0.0f;
+          if (pin) {
+            const float ee = ac[ay][ax + 1], ew = ac[ay][ax - 1];
+            const float en = ac[ay + 1][ax], es = ac[ay - 1][ax];
+            const float et = ap[ay][ax], eb = am[ay][ax];
+            const float rc = rr[ry][rx];
+            val = mc.axf * (ee + ew) + mc.ayf * (en + es) + mc.azf * (et + eb) - mc.aff * rc;
+            const bool own = rx >= 1 && rx <= kB2X && ry >= 1 && ry <= kB2Y && v >= P0 &&
+                             v <= P1 && vi <= nx + 1 && vj <= ny + 1;
+            if (own) {
+              // periodic images across each face of this cell
+              const bool ie = !g.flow[0] && vi == nx + 1, iw = !g.flow[0] && vi == 2;
+              const bool in_ = !g.flow[1] && vj == ny + 1, is = !g.flow[1] && vj == 2;
+              const bool it = !g.flow[2] && v == nz + 1, ib = !g.flow[2] && v == 2;
+              const int64_t c = (int64_t)(vi - 1) + X * (int64_t)(vj - 1) + XY * (int64_t)(v - 1);
+              if (ie || iw || in_ || is || it || ib) eo_out[c] = val;
+              // residual of e_{2q} with e_{2q-1} across periodic boundaries
+              if (!zero_in) {
+                const float se = ie ? eo_in[c - (int64_t)(nx - 1)] : ee;
+                const float sw = iw ? eo_in[c + (int64_t)(nx - 1)] : ew;
+                const float sn = in_ ? eo_in[c - (int64_t)(ny - 1) * X] : en;
+                const float ss = is ? eo_in[c + (int64_t)(ny - 1) * X] : es;
+                const float st_ = it ? eo_in[c - (int64_t)(nz - 1) * XY] : et;
+                const float sb = ib ? eo_in[c + (int64_t)(nz - 1) * XY] : eb;
+                const float e2 = 2.0f * ac[ay][ax];
+                const float res = rc - ((se + sw - e2) * mc.idx2f + (sn + ss - e2) * mc.idy2f +
+                                        (st_ + sb - e2) * mc.idz2f);
+                ra = ra + (double)res * (double)res;
+              }
+            }
+          }
+          bo[ry][rx] = val;
+        }
+      }
+      __syncthreads();
+      // e_{2q+2} on plane v - 1 of the tile; residual of e_{2q+1}
+      const int p = v - 1;
+      if (p >= P0) {
+        const int tx = lt % kB2X, ty = lt / kB2X;
+        const int vi = i0 + tx, vj = j0 + ty;
+        if (vi <= nx + 1 && vj <= ny + 1) {
+          const int rx = tx + 1, ry = ty + 1;
+          const float(*bm)[kRX] = Bq[(p + 2) % 3];
+          const float(*bc)[kRX] = Bq[p % 3];
+          const float(*bp)[kRX] = Bq[(p + 1) % 3];
+          const float ee = bc[ry][rx + 1], ew = bc[ry][rx - 1];
+          const float en = bc[ry + 1][rx], es = bc[ry - 1][rx];
+          const float et = bp[ry][rx], eb = bm[ry][rx];
+          const float rc = R[p % 3][ry][rx];
+          out[(int64_t)(vi - 1) + X * (int64_t)(vj - 1) + XY * (int64_t)(p - 1)] =
+              mc.axf * (ee + ew) + mc.ayf * (en + es) + mc.azf * (et + eb) - mc.aff * rc;
+          // residual of e_{2q+1} with e_{2q} across periodic boundaries
+          const int ax = rx + 1, ay = ry + 1;
+          const float se = (!g.flow[0] && vi == nx + 1) ? A[p % 5][ay][ax + 1] : ee;
+          const float sw = (!g.flow[0] && vi == 2) ? A[p % 5][ay][ax - 1] : ew;
+          const float sn = (!g.flow[1] && vj == ny + 1) ? A[p % 5][ay + 1][ax] : en;
+          const float ss = (!g.flow[1] && vj == 2) ? A[p % 5][ay - 1][ax] : es;
+          const float st_ = (!g.flow[2] && p == nz + 1) ? A[(p + 1) % 5][ay][ax] : et;
+          const float sb = (!g.flow[2] && p == 2) ? A[(p + 4) % 5][ay][ax] : eb;
+          const float e2 = 2.0f * bc[ry][rx];
+          const float res = rc - ((se + sw - e2) * mc.idx2f + (sn + ss - e2) * mc.idy2f +
+                                  (st_ + sb - e2) * mc.idz2f);
+          rb = rb + (double)res * (double)res;
+        }
+      }
     }
+    __syncthreads();   // the next tile restages A, R
   }
-  const double sblk = block_sum(r2, sh);
-  if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
+  const double sa = block_sum(ra, sh);
+  if (lin_tid() == 0 && !zero_in) Pout1[blockIdx.x] = sa;
+  __syncthreads();   // thread 0 has read sh before it is reused
+  const double sb = block_sum(rb, sh);
+  if (lin_tid() == 0) Pout2[blockIdx.x] = sb;
 }
 
-// u += e_k: k = m after a full outer step, or the converged (or last)
-// iterate's k when the solve ended inside this step.
-__global__ void __launch_bounds__(256) k_mx_update(GridDims g, TileSched ts, double *__restrict__ u,
-                                                    const float *E0, const float *E1,
-                                                    const float *E2, const double *__restrict__ Pin,
-                                                    JacobiState *st, MxLaunch L) {
+// u += e_k after an outer step of launches of k_mx_sweep2: even k from its
+// buffer, odd k = 2q + 1 recomputed as one sweep of e_{2q} (e_0 = 0).
+__global__ void __launch_bounds__(256) k_mx_update2(GridDims g, MixCoef mc, TileSched ts,
+                                                     double *__restrict__ u,
+                                                     const float *__restrict__ r32,
+                                                     const float *E0, const float *E1,
+                                                     const double *__restrict__ Pin1,
+                                                     const double *__restrict__ Pin2,
+                                                     JacobiState *st, MxLaunch2 L) {
   __shared__ double sh[4];
   __shared__ int decide;
   const StopSlot prev = st->slot[(L.seq - 1) & 1];
-  StopSlot cur = mx_decide(st, L, Pin, ts.B, sh, &decide);
+  StopSlot cur = mx_decide2(st, L, Pin1, Pin2, ts.B, sh, &decide);
   if (!cur.done && L.base + L.m >= st->last) {
     cur.done = 1;
     cur.result = st->last;
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
-  // solves that ended before this outer step have nothing left to apply
   if (prev.done && prev.result < L.base) return;
   const int64_t k = cur.done ? cur.result - L.base : (int64_t)L.m;
   if (k <= 0) return;
-  const float *E[3] = {E0, E1, E2};
-  const float *e = E[k % 3];
+  const int64_t qk = k >> 1;
+  const float *e = (qk & 1) ? E1 : E0;           // e_{2 qk}
+  const bool odd = k & 1, zero_in = qk == 0;
+  const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
   for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int kk, bool valid) {
     if (!valid) return;
     const int64_t c = node_index(g, i, j, kk);
-    u[c] = u[c] + (double)e[c];
+    float ek;
+    if (!odd) {
+      ek = e[c];
+    } else {
+      auto at = [&](int vi, int vj, int vk) -> float {
+        if (zero_in) return 0.0f;
+        const int pi = vmap(vi, g.nx, g.flow[0]), pj = vmap(vj, g.ny, g.flow[1]);
+        const int pk = vmap(vk, g.nz, g.flow[2]);
+        if (!pi || !pj || !pk) return 0.0f;
+        return e[(int64_t)(pi - 1) + X * (int64_t)(pj - 1) + XY * (int64_t)(pk - 1)];
+      };
+      ek = mc.axf * (at(i + 1, j, kk) + at(i - 1, j, kk)) + mc.ayf * (at(i, j + 1, kk) + at(i, j - 1, kk)) +
+           mc.azf * (at(i, j, kk + 1) + at(i, j, kk - 1)) - mc.aff * r32[c];
+    }
+    u[c] = u[c] + (double)ek;
   });
 }
 
@@ -451,7 +686,10 @@ struct BorisF32 {
 };
 
 // AdvanceVelocity.f90:39-121 in FP32: CIC gather of E (float4 per node) and
-// the Boris rotation in the ve frame.
+// the Boris rotation in the ve frame (MODE 0); the output-step pair
+// AdvanceVelocityImp (MODE 1, :240-347) and AdvanceVelocityExp (MODE 2,
+// :128-238) with their norm-preserving rotation.
+template <int MODE>
 __global__ void __launch_bounds__(kT) k_sp_boris(ParticlesSplit p, int64_t n, GridDims g,
                                                   const float4 *__restrict__ E, BorisF32 bc) {
   const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
@@ -478,18 +716,46 @@ __global__ void __launch_bounds__(kT) k_sp_boris(ParticlesSplit p, int64_t n, Gr
   const int s = p.sp[q] - 1;
   const float h = bc.etadt2[s];
   const float *t = bc.tB[s], *sb = bc.sB[s];
-  const float vm1 = p.v[0][q] - bc.ve[0] + h * ex;
-  const float vm2 = p.v[1][q] - bc.ve[1] + h * ey;
-  const float vm3 = p.v[2][q] - bc.ve[2] + h * ez;
-  const float vd1 = vm1 + vm2 * t[2] - vm3 * t[1];
-  const float vd2 = vm2 + vm3 * t[0] - vm1 * t[2];
-  const float vd3 = vm3 + vm1 * t[1] - vm2 * t[0];
-  const float vp1 = vm1 + vd2 * sb[2] - vd3 * sb[1];
-  const float vp2 = vm2 + vd3 * sb[0] - vd1 * sb[2];
-  const float vp3 = vm3 + vd1 * sb[1] - vd2 * sb[0];
-  p.v[0][q] = vp1 + bc.ve[0] + h * ex;
-  p.v[1][q] = vp2 + bc.ve[1] + h * ey;
-  p.v[2][q] = vp3 + bc.ve[2] + h * ez;
+  if (MODE == 0) {
+    const float vm1 = p.v[0][q] - bc.ve[0] + h * ex;
+    const float vm2 = p.v[1][q] - bc.ve[1] + h * ey;
+    const float vm3 = p.v[2][q] - bc.ve[2] + h * ez;
+    const float vd1 = vm1 + vm2 * t[2] - vm3 * t[1];
+    const float vd2 = vm2 + vm3 * t[0] - vm1 * t[2];
+    const float vd3 = vm3 + vm1 * t[1] - vm2 * t[0];
+    const float vp1 = vm1 + vd2 * sb[2] - vd3 * sb[1];
+    const float vp2 = vm2 + vd3 * sb[0] - vd1 * sb[2];
+    const float vp3 = vm3 + vd1 * sb[1] - vd2 * sb[0];
+    p.v[0][q] = vp1 + bc.ve[0] + h * ex;
+    p.v[1][q] = vp2 + bc.ve[1] + h * ey;
+    p.v[2][q] = vp3 + bc.ve[2] + h * ez;
+    return;
+  }
+  float vm1 = p.v[0][q] - bc.ve[0], vm2 = p.v[1][q] - bc.ve[1], vm3 = p.v[2][q] - bc.ve[2];
+  if (MODE == 1) {
+    vm1 = vm1 + h * ex;
+    vm2 = vm2 + h * ey;
+    vm3 = vm3 + h * ez;
+  }
+  float vd1 = vm1 + vm2 * t[2] - vm3 * t[1];
+  float vd2 = vm2 + vm3 * t[0] - vm1 * t[2];
+  float vd3 = vm3 + vm1 * t[1] - vm2 * t[0];
+  const float sd = vd1 * vd1 + vd2 * vd2 + vd3 * vd3;
+  if (sd > 0.0f) {
+    const float f = sqrtf((vm1 * vm1 + vm2 * vm2 + vm3 * vm3) / sd);
+    vd1 *= f;
+    vd2 *= f;
+    vd3 *= f;
+  }
+  if (MODE == 1) {
+    p.v[0][q] = vd1 + bc.ve[0];
+    p.v[1][q] = vd2 + bc.ve[1];
+    p.v[2][q] = vd3 + bc.ve[2];
+  } else {
+    p.v[0][q] = vd1 + bc.ve[0] + h * ex;
+    p.v[1][q] = vd2 + bc.ve[1] + h * ey;
+    p.v[2][q] = vd3 + bc.ve[2] + h * ez;
+  }
 }
 
 // Host FP64 AoS <-> split.
@@ -572,16 +838,25 @@ int sp_push(picard_ctx *c, double dt) {
   return PICARD_OK;
 }
 
-int sp_gridify(picard_ctx *c) {
+int sp_gridify(picard_ctx *c, bool flux) {
   const GridDims &g = c->gd;
   PICARD_HIP(hipMemsetAsync(c->Nfix, 0, sizeof(unsigned long long) * g.S * g.G, c->stream));
   PICARD_HIP(hipMemsetAsync(c->s.counters, 0, sizeof(int32_t), c->stream));
+  if (flux) {
+    PICARD_TRY(ensure_flux(c));
+    PICARD_HIP(hipMemsetAsync(c->Ffix, 0, sizeof(unsigned long long) * 3 * g.S * g.G, c->stream));
+  }
   if (c->n > 0) {
-    k_sp_deposit<<<grid_blocks(c->n, kT), kT, 0, c->stream>>>(c->ps, c->n, g, c->Nfix,
-                                                              c->s.counters);
+    if (flux)
+      k_sp_deposit<true><<<grid_blocks(c->n, kT), kT, 0, c->stream>>>(c->ps, c->n, g, c->Nfix,
+                                                                      c->Ffix, c->s.counters);
+    else
+      k_sp_deposit<false><<<grid_blocks(c->n, kT), kT, 0, c->stream>>>(c->ps, c->n, g, c->Nfix,
+                                                                       nullptr, c->s.counters);
     PICARD_HIP(hipGetLastError());
   }
   k_sp_fold<<<grid_blocks(g.G, kT), kT, 0, c->stream>>>(g, c->Nfix, c->N, c->C, c->consts);
+  if (flux) k_sp_fold_flux<<<grid_blocks(g.G, kT), kT, 0, c->stream>>>(g, c->Ffix, c->F, c->consts);
   PICARD_HIP(hipGetLastError());
   PICARD_HIP(hipMemcpyAsync(c->host_counters, c->s.counters, sizeof(int32_t),
                             hipMemcpyDeviceToHost, c->stream));
@@ -592,6 +867,41 @@ int sp_gridify(picard_ctx *c) {
     return PICARD_ERR_RANGE;
   }
   return PICARD_OK;
+}
+
+// Tile schedule of the two-sweep kernels: the stencil schedule with the tile
+// width capped at 32.
+TileSched make_sched2(const GridDims &g) {
+  TileSched s = make_sched(g.nx, g.ny, g.nz);
+  if (s.bx > 32) {
+    s.bx = 32;
+    s.by = 256 / s.bx;
+    s.ntx = (g.nx + s.bx - 1) / s.bx;
+    s.nty = (g.ny + s.by - 1) / s.by;
+    s.TP = s.ntx * s.nty;
+    s.W = s.TP < 256 ? s.TP : 256;
+    if (s.W < 4) s.W = 4;
+    s.B = 8 * s.W;
+  }
+  return s;
+}
+
+template <int BX>
+void launch_sweep2(picard_ctx *c, const TileSched &ts, const MixCoef &mc, const double *const pin[2],
+                   double *const pout[2], const MxLaunch2 &L, hipEvent_t t0, hipEvent_t t1) {
+  const dim3 grid(ts.B), block(ts.bx, ts.by);
+  float *E0 = c->e32[0], *E1 = c->e32[1];
+  // launch q leaves e_{2q+1} on the periodic boundary planes in eo32[q & 1]
+  const float *eo_in = c->eo32[(L.q + 1) & 1];
+  float *eo_out = c->eo32[L.q & 1];
+  if (t0)
+    hipExtLaunchKernelGGL(k_mx_sweep2<BX>, grid, block, 0, c->stream, t0, t1, 0, c->gd, mc, ts,
+                          (const float *)c->r32, (const float *)E0, (const float *)E1, E0, E1,
+                          eo_in, eo_out, pin[0], pin[1], pout[0], pout[1], c->js, L);
+  else
+    k_mx_sweep2<BX><<<grid, block, 0, c->stream>>>(c->gd, mc, ts, c->r32, E0, E1, E0, E1, eo_in,
+                                                    eo_out, pin[0], pin[1], pout[0], pout[1], c->js,
+                                                    L);
 }
 
 int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) {
@@ -613,71 +923,72 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
   init.last = 2 * (int64_t)maxit + 1;
   PICARD_HIP(hipMemcpyAsync(c->js, &init, sizeof(JacobiState), hipMemcpyHostToDevice, c->stream));
   PICARD_TRY(launch_bnorm(c, maxerr));      // launch 0: b, |b|, the test of ss = 1
-  const TileSched ts = make_sched(g.nx, g.ny, g.nz);
-  const int m = c->mx_inner;
-  double *P[2] = {c->s.partials, c->s.partials + c->s.partials_len};
-  float *E0 = c->e32[0], *E1 = c->e32[1], *E2 = c->e32[2];
+  const TileSched ts = make_sched2(g);
+  if (ts.B > c->s.partials_len) {
+    set_error("CalcEpotential (split): partial-sum buffer smaller than the launch");
+    return PICARD_ERR_STATE;
+  }
+  const int m = c->mx_inner;                // even: two sweeps per launch
+  const int m2 = m / 2;
+  // partial sums: launch `seq` writes set seq & 1 (two arrays) and reads set (seq-1) & 1
+  double *P[2][2];
+  for (int q = 0; q < 4; ++q) P[q >> 1][q & 1] = c->s.partials + (int64_t)q * c->s.partials_len;
   int64_t seq = 1, base = 0;
-  int pw = 0;                               // partials buffer the next writer uses
-  int64_t check = -1;                       // iterate whose residual the last writer stored
   StopSlot fin{};
   while (true) {
     if (c->timing) {
-      while ((int64_t)c->tev.size() < 2 * m) {
+      while ((int64_t)c->tev.size() < 2 * m2) {
         hipEvent_t e;
         PICARD_HIP(hipEventCreate(&e));
         c->tev.push_back(e);
       }
     }
-    // outer residual of iterate `base` (its stop test happens in the next launch)
+    // outer residual of iterate `base` (tested by the first sweep launch);
+    // its stale-halo norm needs e_m and e_{m-1} of the previous outer step
     {
       MxLaunch L{seq, base, 0, -1, m};
-      k_mx_residual<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, mc, ts, c->U, c->b, c->r32,
-                                                                P[pw ^ 1], P[pw], c->js, L);
+      const float *em = base > 0 ? c->e32[m2 & 1] : nullptr;
+      const float *eo = base > 0 ? c->eo32[(m2 - 1) & 1] : nullptr;
+      k_mx_residual<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, mc, ts, c->U, c->b, c->r32, em,
+                                                                eo, P[(seq - 1) & 1][0],
+                                                                P[seq & 1][0], c->js, L);
       ++seq;
-      check = base;
-      pw ^= 1;
     }
-    for (int j = 0; j < m; ++j) {
-      // the iterate-0 test used b itself (launch 0); later ones use residuals
-      const int64_t chk = (check >= 1) ? check : -1;
-      MxLaunch L{seq, base, j, chk, m};
-      const double *pin = P[pw ^ 1];
-      double *pout = P[pw];
-      if (c->timing)
-        hipExtLaunchKernelGGL(k_mx_sweep, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
-                              c->tev[2 * j], c->tev[2 * j + 1], 0, g, mc, ts,
-                              (const float *)c->r32, E0, E1, E2, pin, pout, c->js, L);
-      else
-        k_mx_sweep<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, mc, ts, c->r32, E0, E1, E2, pin,
-                                                               pout, c->js, L);
-      ++seq;
-      if (j >= 1) {               // wrote the residual of iterate base + j
-        check = base + j;
-        pw ^= 1;
+    for (int q = 0; q < m2; ++q) {
+      MxLaunch2 L{seq, base, q, -1, -1, m};
+      if (q == 0) {
+        L.check1 = base;                      // the FP64 residual just computed
       } else {
-        check = -1;
+        L.check1 = q >= 2 ? base + 2 * q - 2 : -1;
+        L.check2 = base + 2 * q - 1;
       }
+      const double *pin[2] = {P[(seq - 1) & 1][0], P[(seq - 1) & 1][1]};
+      double *const pout[2] = {P[seq & 1][0], P[seq & 1][1]};
+      hipEvent_t t0 = c->timing ? c->tev[2 * q] : nullptr;
+      hipEvent_t t1 = c->timing ? c->tev[2 * q + 1] : nullptr;
+      if (ts.bx == 32) launch_sweep2<32>(c, ts, mc, pin, pout, L, t0, t1);
+      else if (ts.bx == 16) launch_sweep2<16>(c, ts, mc, pin, pout, L, t0, t1);
+      else launch_sweep2<8>(c, ts, mc, pin, pout, L, t0, t1);
+      ++seq;
     }
     {
-      const int64_t chk = (check >= 1) ? check : -1;
-      MxLaunch L{seq, base, 0, chk, m};
-      k_mx_update<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, ts, c->U, E0, E1, E2, P[pw ^ 1],
-                                                              c->js, L);
+      MxLaunch2 L{seq, base, m2, m2 >= 2 ? base + m - 2 : -1, base + m - 1, m};
+      k_mx_update2<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(
+          g, mc, ts, c->U, c->r32, c->e32[0], c->e32[1], P[(seq - 1) & 1][0], P[(seq - 1) & 1][1],
+          c->js, L);
       ++seq;
-      check = -1;
     }
     PICARD_HIP(hipGetLastError());
     PICARD_HIP(hipMemcpyAsync(c->js_host, c->js, sizeof(JacobiState), hipMemcpyDeviceToHost,
                               c->stream));
     PICARD_HIP(hipStreamSynchronize(c->stream));
     if (c->timing) {
-      for (int q = 0; q < m; ++q) {
+      for (int q = 0; q < m2; ++q) {
         float ms = 0.0f;
         PICARD_HIP(hipEventElapsedTime(&ms, c->tev[2 * q], c->tev[2 * q + 1]));
         c->t_ms += ms;
       }
-      c->t_launches += m;
+      c->t_launches += m2;
     }
     fin = c->js_host->slot[(seq - 1) & 1];
     if (fin.done) break;
@@ -696,19 +1007,23 @@ int sp_efield(picard_ctx *c) {
   return PICARD_OK;
 }
 
-int sp_boris(picard_ctx *c, double dt) {
+int sp_boris(picard_ctx *c, double dt, int mode, double deltat) {
   if (c->n <= 0) return PICARD_OK;
   BorisF32 bc;
+  const double scale = mode == 0 ? 1.0 : 2.0 * deltat / dt;   // AdvanceVelocity.f90:156-157
   for (int s = 0; s < 8; ++s) {
-    bc.etadt2[s] = (float)(0.5 * dt * c->spc.eta[s]);
+    bc.etadt2[s] = (float)(0.5 * dt * c->spc.eta[s] * scale);
     for (int q = 0; q < 3; ++q) {
-      bc.tB[s][q] = (float)c->spc.tB[4 * s + q];
+      bc.tB[s][q] = (float)(c->spc.tB[4 * s + q] * scale);
       bc.sB[s][q] = (float)c->spc.sB[4 * s + q];
     }
   }
   for (int q = 0; q < 3; ++q) bc.ve[q] = (float)c->spc.ve[q];
-  k_sp_boris<<<grid_blocks(c->n, kT), kT, 0, c->stream>>>(c->ps, c->n, c->gd,
-                                                          reinterpret_cast<const float4 *>(c->E32), bc);
+  const int nb = grid_blocks(c->n, kT);
+  const float4 *E4 = reinterpret_cast<const float4 *>(c->E32);
+  if (mode == 0) k_sp_boris<0><<<nb, kT, 0, c->stream>>>(c->ps, c->n, c->gd, E4, bc);
+  else if (mode == 1) k_sp_boris<1><<<nb, kT, 0, c->stream>>>(c->ps, c->n, c->gd, E4, bc);
+  else k_sp_boris<2><<<nb, kT, 0, c->stream>>>(c->ps, c->n, c->gd, E4, bc);
   PICARD_HIP(hipGetLastError());
   return PICARD_OK;
 }

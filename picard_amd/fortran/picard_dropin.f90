@@ -285,3 +285,88 @@ subroutine AdvanceVelocity(real_particles, int_particles, E, B0, Lx_min, Ly_min,
   call picard_check(picard_advance_velocity(ctx, dt), 'AdvanceVelocity')
   if (eager_download) call picard_dropin_download_particles()
 end subroutine AdvanceVelocity
+
+
+! AdvanceVelocity.f90:128 (output steps, before the dumps)
+subroutine AdvanceVelocityImp(real_particles, int_particles, E, B0, Lx_min, Ly_min, Lz_min, dxyz, deltat, dt, &
+                              v0, ve, eta, tB, num_local, max_per_proc, species, Nx_local, Ny_local, Nz_local, &
+                              fluid, gyration)
+  use picard_dropin_state
+  implicit none
+  logical, intent(in) :: fluid(8), gyration(8)
+  integer, intent(in) :: num_local, max_per_proc, species, Nx_local, Ny_local, Nz_local
+  real*8, intent(in) :: Lx_min, Ly_min, Lz_min, dxyz(3), deltat, dt, v0(4,8), ve(4), eta(8)
+  real*8, intent(in) :: tB(4,8)
+  real*8, intent(inout) :: real_particles(8, max_per_proc)
+  integer, intent(in) :: int_particles(max_per_proc)
+  real*8, intent(in) :: B0(4)
+  real*8, intent(in) :: E(3, Nx_local+2, Ny_local+2, Nz_local+2)
+  call picard_dropin_sync(num_local)
+  call picard_check(picard_advance_velocity_imp(ctx, deltat, dt), 'AdvanceVelocityImp')
+  if (eager_download) call picard_dropin_download_particles()
+end subroutine AdvanceVelocityImp
+
+
+! AdvanceVelocity.f90:240 (output steps, after the dumps)
+subroutine AdvanceVelocityExp(real_particles, int_particles, E, B0, Lx_min, Ly_min, Lz_min, dxyz, deltat, dt, &
+                              v0, ve, eta, tB, num_local, max_per_proc, species, Nx_local, Ny_local, Nz_local, &
+                              fluid, gyration)
+  use picard_dropin_state
+  implicit none
+  logical, intent(in) :: fluid(8), gyration(8)
+  integer, intent(in) :: num_local, max_per_proc, species, Nx_local, Ny_local, Nz_local
+  real*8, intent(in) :: Lx_min, Ly_min, Lz_min, dxyz(3), deltat, dt, v0(4,8), ve(4), eta(8)
+  real*8, intent(in) :: tB(4,8)
+  real*8, intent(inout) :: real_particles(8, max_per_proc)
+  integer, intent(in) :: int_particles(max_per_proc)
+  real*8, intent(in) :: B0(4)
+  real*8, intent(in) :: E(3, Nx_local+2, Ny_local+2, Nz_local+2)
+  call picard_dropin_sync(num_local)
+  call picard_check(picard_advance_velocity_exp(ctx, deltat, dt), 'AdvanceVelocityExp')
+  if (eager_download) call picard_dropin_download_particles()
+end subroutine AdvanceVelocityExp
+
+
+! ParticlesGridifyAll.f90:1 (field-output steps).  The dumps read F, N and C
+! on the host, so they are always copied back.
+subroutine ParticlesGridifyAll(F, N, C, real_particles, int_particles, Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, &
+                               Lz_max, dxyz, xmin, xmax, ymin, ymax, zmin, zmax, species, Nx_local, Ny_local, &
+                               Nz_local, max_per_proc, num_local, xflow, yflow, zflow, iprocs, jprocs, kprocs, &
+                               charge, n2p, myid)
+  use picard_dropin_state
+  implicit none
+  integer, intent(in) :: species, Nx_local, Ny_local, Nz_local, max_per_proc, iprocs, jprocs, kprocs, myid, &
+                         num_local
+  logical, intent(in) :: xflow, yflow, zflow
+  real*8, intent(in) :: Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, Lz_max, dxyz(3), xmin, xmax, ymin, ymax, &
+                        zmin, zmax, charge(8), n2p(8)
+  integer, intent(in) :: int_particles(max_per_proc)
+  real*8, intent(in) :: real_particles(8, max_per_proc)
+  real*8, intent(out) :: F(species, 3, Nx_local+2, Ny_local+2, Nz_local+2)
+  real*8, intent(out) :: N(species, Nx_local+2, Ny_local+2, Nz_local+2)
+  real*8, intent(out) :: C(Nx_local+2, Ny_local+2, Nz_local+2)
+  integer(c_int64_t) :: G
+  call picard_dropin_sync(num_local)
+  call picard_check(picard_particles_gridify_all(ctx), 'ParticlesGridifyAll')
+  G = int(Nx_local+2, c_int64_t) * (Ny_local+2) * (Nz_local+2)
+  call picard_dropin_download_field(PICARD_FIELD_F, F, 3 * species * G)
+  call picard_dropin_download_field(PICARD_FIELD_N, N, species * G)
+  call picard_dropin_download_field(PICARD_FIELD_C, C, G)
+end subroutine ParticlesGridifyAll
+
+
+! The diagnostics line of the main loop (picard.f90:649-672) from device
+! state: num_global and the particle, electric and magnetic energies.
+subroutine picard_dropin_diagnostics(mass, num_global, uP, uE, uB)
+  use picard_dropin_state
+  implicit none
+  real*8, intent(in) :: mass(8)
+  integer, intent(out) :: num_global
+  real*8, intent(out) :: uP, uE, uB
+  type(picard_diag) :: dg
+  call picard_check(picard_diagnostics(ctx, mass, dg), 'diagnostics')
+  num_global = int(dg%num_global)
+  uP = dg%particle_energy
+  uE = dg%e_energy
+  uB = dg%b_energy
+end subroutine picard_dropin_diagnostics

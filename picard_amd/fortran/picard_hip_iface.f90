@@ -10,7 +10,8 @@ module picard_hip_iface
   integer(c_int32_t), parameter :: PICARD_FP64 = 0
   integer(c_int32_t), parameter :: PICARD_DEPOSIT_REFERENCE_ORDER = 0
   integer(c_int32_t), parameter :: PICARD_FIELD_U = 0, PICARD_FIELD_E = 1, &
-                                   PICARD_FIELD_N = 2, PICARD_FIELD_C = 3, PICARD_FIELD_D = 4
+                                   PICARD_FIELD_N = 2, PICARD_FIELD_C = 3, PICARD_FIELD_D = 4, &
+                                   PICARD_FIELD_F = 5
 
   type, bind(C) :: picard_geometry
     integer(c_int32_t) :: nx, ny, nz, species
@@ -26,6 +27,11 @@ module picard_hip_iface
     real(c_double) :: ve(4), B0(4), E0(4)
     integer(c_int32_t) :: fluid(8), gyration(8)
   end type picard_species
+
+  type, bind(C) :: picard_diag
+    integer(c_int64_t) :: num_local, num_global
+    real(c_double) :: particle_energy, e_energy, b_energy
+  end type picard_diag
 
   interface
     function picard_last_error() bind(C, name='picard_last_error')
@@ -115,6 +121,32 @@ module picard_hip_iface
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), value :: dt
+    end function
+
+    integer(c_int) function picard_advance_velocity_imp(ctx, deltat, dt) &
+        bind(C, name='picard_advance_velocity_imp')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat, dt
+    end function
+
+    integer(c_int) function picard_advance_velocity_exp(ctx, deltat, dt) &
+        bind(C, name='picard_advance_velocity_exp')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat, dt
+    end function
+
+    integer(c_int) function picard_particles_gridify_all(ctx) bind(C, name='picard_particles_gridify_all')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function picard_diagnostics(ctx, mass, out) bind(C, name='picard_diagnostics')
+      import :: c_int, c_ptr, c_double, picard_diag
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: mass(8)
+      type(picard_diag), intent(out) :: out
     end function
 
     integer(c_int) function picard_field_read(ctx, which, host, count) bind(C, name='picard_field_read')

@@ -454,7 +454,8 @@ class Domain:
     def field(self, which):
         shp = {_lib.FIELD_U: self.shape, _lib.FIELD_C: self.shape,
                _lib.FIELD_E: self.shape + (3,), _lib.FIELD_N: self.shape + (self.cfg.species,),
-               _lib.FIELD_D: (2 * self.cfg.nz + 1, 2 * self.cfg.ny + 1, 2 * self.cfg.nx + 1)}[which]
+               _lib.FIELD_D: (2 * self.cfg.nz + 1, 2 * self.cfg.ny + 1, 2 * self.cfg.nx + 1),
+               _lib.FIELD_F: self.shape + (3, self.cfg.species)}[which]
         a = np.zeros(shp, np.float64)
         check("picard_field_read", lib().picard_field_read(self.h, which, _dp(a), a.size))
         return a
@@ -547,6 +548,32 @@ class Domain:
         hs = haser_struct(self.cfg, self.d)
         check("picard_set_production",
               lib().picard_set_production(self.h, ct.byref(pl), ct.byref(hs), seed))
+
+    def particles_gridify_all(self):
+        """ParticlesGridifyAll: N, C and the number fluxes (field FIELD_F)."""
+        check("picard_particles_gridify_all", lib().picard_particles_gridify_all(self.h))
+
+    def advance_velocity_imp(self, deltat=None):
+        """AdvanceVelocityImp (output steps; the main loop passes deltat = dt/2)."""
+        deltat = 0.5 * self.d.dt if deltat is None else deltat
+        check("picard_advance_velocity_imp",
+              lib().picard_advance_velocity_imp(self.h, deltat, self.d.dt))
+
+    def advance_velocity_exp(self, deltat=None):
+        """AdvanceVelocityExp (output steps; the main loop passes deltat = dt/2)."""
+        deltat = 0.5 * self.d.dt if deltat is None else deltat
+        check("picard_advance_velocity_exp",
+              lib().picard_advance_velocity_exp(self.h, deltat, self.d.dt))
+
+    def diagnostics(self) -> dict:
+        """The diagnostics line (picard.f90:649-672): particle count and the
+        particle, electric and magnetic energies, computed on the device."""
+        out = _lib.Diag()
+        mass = np.ascontiguousarray(self.d.mass, np.float64)
+        check("picard_diagnostics", lib().picard_diagnostics(self.h, _dp(mass), ct.byref(out)))
+        return dict(num_local=out.num_local, num_global=out.num_global,
+                    particle_energy=out.particle_energy, e_energy=out.e_energy,
+                    b_energy=out.b_energy)
 
     def timing(self, enable: bool):
         check("picard_timing_enable", lib().picard_timing_enable(self.h, 1 if enable else 0))

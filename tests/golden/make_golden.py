@@ -84,8 +84,56 @@ def main():
                         sp=sp, E=E, B0=B4, dt=np.float64(dt), ve=ve, eta=eta, tB=tB, sB=sB,
                         fluid=fluid, gyration=gyration, n=np.int64(500),
                         dims=np.array([5, 4, 6]), flow=np.array(flow, np.int32))
+    # AdvanceVelocityImp / AdvanceVelocityExp with deltat = dt/2
+    d = dict(rp_in=rp, sp=sp, E=E, B0=B4, dt=np.float64(dt), ve=ve, eta=eta, tB=tB,
+             fluid=fluid, gyration=gyration, n=np.int64(500), dims=np.array([5, 4, 6]),
+             flow=np.array(flow, np.int32))
+    for mode in (1, 2):
+        out = rp.copy()
+        oracle.ref_advance_velocity_ie(mode, g, out, sp, E, B4, 0.5 * dt, dt, np.zeros((8, 4)),
+                                       ve, eta, tB, 500, fluid, gyration)
+        d[f"rp_out{mode}"] = out
+    np.savez_compressed(os.path.join(HERE, "ref_advance_velocity_ie.npz"), **d)
     produce_cases()
+    dump_cases()
     print("golden vectors written to", HERE)
+
+
+def _spread(rng, shape):
+    """Values over many decades and both signs, with exact zeros and ties."""
+    a = rng.normal(size=shape) * 10.0 ** rng.integers(-120, 120, size=shape)
+    flat = a.reshape(-1)
+    flat[::17] = 0.0
+    flat[5::23] = -0.0
+    flat[3::29] = 0.015625
+    flat[7::31] = -9.99995e10
+    return a
+
+
+def dump_cases():
+    """DumpParticles / DumpFields of the reference (Write.f90) on spread
+    values: the inputs and the files' bytes."""
+    import tempfile
+    rng = np.random.default_rng(99)
+    n = 40
+    rp = _spread(rng, (n, 8))
+    sp = rng.integers(1, 4, n).astype(np.int32)
+    nx, ny, nz, S = 4, 3, 2, 3
+    shp = (nz + 2, ny + 2, nx + 2)
+    U = _spread(rng, shp)
+    E = _spread(rng, shp + (3,))
+    N = _spread(rng, shp + (S,))
+    F = _spread(rng, shp + (3, S))
+    d = dict(rp=rp, sp=sp, U=U, E=E, N=N, F=F, dims=np.array([nx, ny, nz, S]))
+    with tempfile.TemporaryDirectory() as t:
+        text, it = oracle.ref_dump_particles(rp, sp, n, 7, 0, t)
+        d["particles_txt"] = np.frombuffer(text, np.uint8)
+        files = oracle.ref_dump_fields(U, N, F, E, nx, ny, nz, 2, S, 0, t)
+    d["field_names"] = np.array(sorted(files))
+    for name, b in files.items():
+        d["txt_" + name] = np.frombuffer(b, np.uint8)
+    np.savez_compressed(os.path.join(HERE, "ref_dump.npz"), **d)
+    print(f"dump: {len(files)} field files")
 
 
 def produce_cases():

@@ -177,6 +177,68 @@ def test_advance_velocity(mode):
     dom.close()
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("kind", ["boris", "fluid"])
+def test_advance_velocity_imp_exp(mode, kind):
+    """AdvanceVelocityImp (1) / AdvanceVelocityExp (2) at deltat = dt/2,
+    bit-exact against the oracle (FP64 form)."""
+    _say(f"advance velocity imp/exp mode={mode} {kind}")
+    fluid = [1, 0, 1, 0, 0, 0, 0, 0] if kind == "fluid" else [0] * 8
+    gyration = [1] * 8
+    cfg = small_config(6, flow=(True, False, True), species=3,
+                       density=(0, 5e6, 1e6, 0, 0, 0, 0, 0), ppc=(0, 4, 2, 0, 0, 0, 0, 0),
+                       fluid=tuple(bool(f) for f in fluid), gyration=tuple(bool(f) for f in gyration),
+                       B0=(1.1e-9, -2e-9, 5e-9))
+    d = host.derive(cfg)
+    g = oracle_grid(cfg)
+    rng = np.random.default_rng(11 + mode)
+    rp, sp = random_particles(rng, g, 20000, species=3)
+    E = rng.normal(0.0, 1e-3, size=g.shape + (3,))
+    dom = _domain(cfg)
+    dom.append(rp, sp)
+    dom.set_field(_lib.FIELD_E, E)
+    if mode == 1:
+        dom.advance_velocity_imp()
+    else:
+        dom.advance_velocity_exp()
+    a = rp.copy()
+    oracle.advance_velocity_ie(mode, g, a, sp, E, d.B0, 0.5 * d.dt, d.dt, d.ve, d.eta, d.tB,
+                               fluid, gyration)
+    got, _ = dom.particles()
+    _eq(got, a, f"AdvanceVelocity{'Imp' if mode == 1 else 'Exp'}")
+    dom.close()
+
+
+@pytest.mark.parametrize("flow", [(True, False, False), (False, True, True), (True, True, True)])
+def test_gridify_all(flow):
+    """ParticlesGridifyAll: N, C and fluxes F bit-exact against the oracle
+    (FP64 form); the split form's fixed-point fluxes within 1e-6."""
+    _say(f"gridify_all flow={flow}")
+    cfg = small_config(7, flow=flow, species=3, density=(0, 5e6, 1e6, 0, 0, 0, 0, 0),
+                       ppc=(0, 4, 2, 0, 0, 0, 0, 0))
+    d = host.derive(cfg)
+    g = oracle_grid(cfg)
+    rng = np.random.default_rng(40)
+    rp, sp = random_particles(rng, g, 30000, species=3)
+    F0, N0, C0, sk = oracle.particles_gridify_all(g, rp, sp, d.charge, d.n2p)
+    assert sk == 0
+    dom = _domain(cfg)
+    dom.append(rp, sp)
+    dom.particles_gridify_all()
+    _eq(dom.field(_lib.FIELD_N), N0, "GridifyAll N")
+    _eq(dom.field(_lib.FIELD_C), C0, "GridifyAll C")
+    _eq(dom.field(_lib.FIELD_F), F0, "GridifyAll F")
+    dom.close()
+    sdom = host.Domain(cfg, max_particles=100000, d=d, precision=_lib.PICARD_FP32_SPLIT)
+    sdom.append(rp, sp)
+    sdom.particles_gridify_all()
+    Fs = sdom.field(_lib.FIELD_F)
+    err = np.linalg.norm(Fs - F0) / np.linalg.norm(F0)
+    _say(f"split flux rel err {err:.2e}")
+    assert err <= 1e-6
+    sdom.close()
+
+
 def _run_phases(cfg, steps, label):
     """Each subroutine of the main loop in turn on both sides, compared after
     every phase, so a divergence is reported at the phase that causes it."""

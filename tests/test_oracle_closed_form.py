@@ -352,3 +352,68 @@ def test_efield_quadratic_and_halos(orc, flow):
     E2 = E.copy()
     oracle.bc_fields(g, E2)
     assert np.array_equal(E2, E)        # on one rank BCfields re-zeroes zeros
+
+
+def test_diagnostics_closed_forms(orc):
+    """The diagnostics energies (picard.f90:649-672) on fields and particles
+    with closed-form sums: uniform E, constant B0, per-species speeds."""
+    g = oracle.Grid(5, 4, 3, 3)
+    dV = g.dxyz[0] * g.dxyz[1] * g.dxyz[2]
+    ncell = 5 * 4 * 3
+    E = np.zeros(g.shape + (3,))
+    E[1:-1, 1:-1, 1:-1] = (1.5, -2.0, 0.25)
+    E[0] = 100.0                          # guard planes do not count
+    E0 = (0.5, 0.0, -0.25, 0.0)
+    B0 = (1e-9, -2e-9, 5e-9, 0.0)
+    rng = np.random.default_rng(4)
+    n = 300
+    sp = rng.integers(1, 4, n).astype(np.int32)
+    speed = np.array([3.0e5, 1.0e4, 2.0e3])[sp - 1]
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    rp = np.zeros((n, 8))
+    rp[:, 3:6] = d * speed[:, None]
+    n2p = [2.0e8, 3.0e8, 5.0e7] + [0.0] * 5
+    mass = [9.1e-31, 1.67e-27, 3.0e-26] + [0.0] * 5
+    uP, uE, uB = oracle.diagnostics(g, rp, sp, E, n2p, mass, E0, B0)
+    eps0, mu0 = 8.854187817e-12, 1.25663706143591729538505735331180e-6
+    want_P = 0.5 * sum(n2p[s] * mass[s] * np.sum(speed[sp == s + 1] ** 2) for s in range(3))
+    want_E = 0.5 * eps0 * dV * ncell * ((1.5 + 0.5) ** 2 + 2.0 ** 2 + 0.0 ** 2)
+    want_B = 0.5 / mu0 * dV * ncell * (1e-18 + 4e-18 + 25e-18)
+    assert abs(uP - want_P) <= 1e-12 * want_P
+    assert abs(uE - want_E) <= 1e-12 * want_E
+    assert abs(uB - want_B) <= 1e-12 * want_B
+
+
+@pytest.mark.parametrize("flow", [(True, False, False), (False, True, True), (True, True, True)])
+def test_gridify_all_invariants(orc, flow):
+    """ParticlesGridifyAll: N and C equal ParticlesGridify's bit for bit; the
+    interior sum of F(s, c) is n2p(s) * sum of v_c over species s (guard folds
+    conserve, to rounding); particles with one common velocity give F = N v."""
+    g = oracle.Grid(6, 5, 4, 2, flow=flow)
+    rng = np.random.default_rng(8)
+    n = 3000
+    rp = np.zeros((n, 8))
+    for c in range(3):
+        rp[:, c] = rng.uniform(g.gmin[c], g.gmax[c], n)
+    rp[:, 3:6] = rng.normal(0, 1e5, (n, 3))
+    sp = rng.integers(1, 3, n).astype(np.int32)
+    charge = [-1.6e-19, 1.6e-19] + [0.0] * 6
+    n2p = [3.0e8, 2.0e8] + [0.0] * 6
+    F, N, C, sk = oracle.particles_gridify_all(g, rp, sp, charge, n2p)
+    N0, C0, sk0 = oracle.particles_gridify(g, rp, sp, charge, n2p)
+    assert sk == sk0 == 0
+    assert np.array_equal(N, N0) and np.array_equal(C, C0)
+    inner = (slice(1, -1),) * 3
+    for s in range(2):
+        for c in range(3):
+            # the eight CIC weights of a particle sum to one (F1..F8 are
+            # volume fractions), so every particle adds n2p * v_c in total
+            want = n2p[s] * rp[sp == s + 1, 3 + c].sum()
+            got = F[inner][..., c, s].sum()
+            assert abs(got - want) <= 1e-12 * n2p[s] * np.abs(rp[:, 3 + c]).sum()
+    rp2 = rp.copy()
+    rp2[:, 3:6] = (2.0e5, -3.0e4, 7.0e3)
+    F2, N2, _, _ = oracle.particles_gridify_all(g, rp2, sp, charge, n2p)
+    for c in range(3):
+        assert np.allclose(F2[..., c, :], N2 * rp2[0, 3 + c], rtol=1e-13, atol=0)

@@ -114,6 +114,56 @@ def test_advance_velocity_vs_ref(orc, flow, mode):
     assert not np.array_equal(a, rp)
 
 
+@needs_ref
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("kind", ["boris", "fluid"])
+def test_advance_velocity_imp_exp_vs_ref(orc, mode, kind):
+    """AdvanceVelocityImp (1) / AdvanceVelocityExp (2) with deltat = dt/2, as
+    the main loop calls them on output steps (picard.f90:603, :616)."""
+    fluid = [1, 0, 1, 0, 0, 0, 0, 0] if kind == "fluid" else [0] * 8
+    gyration = [1] * 8
+    g, rp, sp, E, B4, dt, v0, ve, eta, tB, sB, n = _velocity_case(33 + mode, (True, False, False),
+                                                                  fluid, gyration)
+    a, b = rp.copy(), rp.copy()
+    oracle.ref_advance_velocity_ie(mode, g, a, sp, E, B4, 0.5 * dt, dt, v0, ve, eta, tB, n,
+                                   fluid, gyration)
+    oracle.advance_velocity_ie(mode, g, b, sp, E, B4, 0.5 * dt, dt, ve, eta, tB, fluid,
+                               gyration, n)
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, rp)
+
+
+def test_imp_exp_closed_forms(orc):
+    """With E = 0 both halves only rotate: |v - ve| is kept exactly (to
+    rounding) and Imp followed by Exp rotates by 2*atan(|tB|) in total, the
+    angle of one Boris step; with B = 0 the pair adds dt*eta*E."""
+    g, rp, sp, E, B4, dt, v0, ve, eta, tB, sB, n = _velocity_case(5, (True, False, False),
+                                                                  [0] * 8, [1] * 8)
+    E0 = np.zeros_like(E)
+    a = rp.copy()
+    oracle.advance_velocity_ie(1, g, a, sp, E0, B4, 0.5 * dt, dt, ve, eta, tB, n=n)
+    oracle.advance_velocity_ie(2, g, a, sp, E0, B4, 0.5 * dt, dt, ve, eta, tB, n=n)
+    u0 = rp[:n, 3:6] - ve[None, :3]
+    u1 = a[:n, 3:6] - ve[None, :3]
+    assert np.allclose(np.linalg.norm(u1, axis=1), np.linalg.norm(u0, axis=1), rtol=1e-14)
+    bhat = B4[:3] / B4[3]
+    p0 = u0 - np.outer(u0 @ bhat, bhat)
+    p1 = u1 - np.outer(u1 @ bhat, bhat)
+    ang = np.arctan2(np.linalg.norm(np.cross(p0, p1), axis=1), np.sum(p0 * p1, axis=1))
+    tb = np.linalg.norm(tB.reshape(8, 4)[sp[:n] - 1, :3], axis=1)
+    # each half rotates by atan(|tB'|) with |tB'| = |tB| (deltat = dt/2): 2 atan in total
+    assert np.allclose(ang, 2 * np.arctan(tb), rtol=1e-6, atol=1e-15)
+    # B = 0: pure kicks, dt*eta*E in total
+    tz = np.zeros((8, 4))
+    b = rp.copy()
+    Eu = np.zeros_like(E)
+    Eu[..., 0] = 2e-3
+    oracle.advance_velocity_ie(1, g, b, sp, Eu, B4 * 0, 0.5 * dt, dt, ve, eta, tz, n=n)
+    oracle.advance_velocity_ie(2, g, b, sp, Eu, B4 * 0, 0.5 * dt, dt, ve, eta, tz, n=n)
+    dv = b[:n, 3] - rp[:n, 3]
+    assert np.allclose(dv, dt * eta[sp[:n] - 1] * 2e-3, rtol=1e-9)
+
+
 # ---------------------------------------------------------------------------
 # golden vectors written by tests/golden/make_golden.py from oracle/_ref
 
@@ -224,3 +274,14 @@ def test_golden_particles_produce_invariants():
             rn = np.linalg.norm(radial)
             assert abs(rn - z[f"vn{t}"][s]) <= 1e-9 * rn
             q += group
+
+
+def test_golden_advance_velocity_imp_exp(orc):
+    z = _golden("ref_advance_velocity_ie.npz")
+    g = oracle.Grid(*[int(v) for v in z["dims"]], 3, flow=tuple(bool(f) for f in z["flow"]))
+    dt = float(z["dt"])
+    for mode in (1, 2):
+        rp = z["rp_in"].copy()
+        oracle.advance_velocity_ie(mode, g, rp, z["sp"], z["E"], z["B0"], 0.5 * dt, dt, z["ve"],
+                                   z["eta"], z["tB"], z["fluid"], z["gyration"], int(z["n"]))
+        assert np.array_equal(rp, z[f"rp_out{mode}"])
