@@ -114,7 +114,7 @@ void release(picard_ctx *c) {
   dfree(c->E32);
   dfree(c->r32);
   for (int q = 0; q < 2; ++q) dfree(c->e32[q]);
-  for (int q = 0; q < 2; ++q) dfree(c->eo32[q]);
+  for (int q = 0; q < 3; ++q) dfree(c->eo32[q]);
   for (int q = 0; q < 8; ++q) dfree(c->p.r[q]);
   dfree(c->p.sp);
   dfree(c->U);
@@ -189,7 +189,7 @@ int create_impl(const picard_geometry *geo, const picard_species *sp, picard_ctx
     PICARD_TRY(dalloc(&c->E32, 4 * g.G));
     PICARD_TRY(dalloc(&c->r32, g.G));
     for (int q = 0; q < 2; ++q) PICARD_TRY(dalloc(&c->e32[q], g.G));
-    for (int q = 0; q < 2; ++q) PICARD_TRY(dalloc(&c->eo32[q], g.G));
+    for (int q = 0; q < 3; ++q) PICARD_TRY(dalloc(&c->eo32[q], g.G));
     PICARD_HIP(hipMemsetAsync(c->E32, 0, sizeof(float) * 4 * g.G, c->stream));
   } else {
     for (int q = 0; q < 8; ++q) PICARD_TRY(dalloc(&c->p.r[q], cap));
@@ -227,7 +227,7 @@ int create_impl(const picard_geometry *geo, const picard_species *sp, picard_ctx
   if (pb3 > pb) pb = pb3;
   if (make_sched(g.nx, g.ny, g.nz).B > pb) pb = make_sched(g.nx, g.ny, g.nz).B;
   c->s.partials_len = pb;
-  PICARD_TRY(dalloc(&c->s.partials, 4 * pb));     // two launches' worth of two sums each
+  PICARD_TRY(dalloc(&c->s.partials, 4 * pb));     // two launches' worth (read + write) and scratch
   PICARD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->js_host), sizeof(JacobiState)));
   PICARD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->host_counters), 16 * sizeof(int32_t)));
   // the reference zeroes U, N, E and the particle arrays at start (picard.f90:434-439)

@@ -121,7 +121,7 @@ struct picard_ctx {
   unsigned long long *Nfix = nullptr;   // split: fixed-point densities
   float *E32 = nullptr;      // split: E as float4 per node
   float *r32 = nullptr, *e32[2] = {nullptr, nullptr};
-  float *eo32[2] = {nullptr, nullptr};   // split: odd inner iterates on periodic boundary planes
+  float *eo32[3] = {nullptr, nullptr, nullptr};   // split: inner iterates j % 3 on periodic boundary planes
   int mx_inner = 64;         // split: FP32 sweeps per FP64 refinement
   int64_t next_id = 0;
 

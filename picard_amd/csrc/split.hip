@@ -359,281 +359,140 @@ __global__ void __launch_bounds__(256) k_mx_residual(GridDims g, MixCoef mc, Til
   if (lin_tid() == 0) Pout[blockIdx.x] = sblk;
 }
 
-// ---------------------------------------------------------------------------
-// Two inner sweeps per launch (temporal blocking).  Launch q takes e_{2q}
-// (buffer q % 2) to e_{2q+2} (buffer (q+1) % 2); e_{2q+1} lives only in LDS.
-// A block owns a BX x BY tile of the interior and marches it along its XCD's
-// z-slab.  For each virtual plane v it stages e_{2q} on the tile plus two
-// rings, computes e_{2q+1} on the tile plus one ring (the residual of e_{2q}
-// on the tile as a by-product), and then e_{2q+2} on plane v - 1 of the tile
-// (with the residual of e_{2q+1}).  Ring points outside the interior take
-// their periodic images, or the zero correction of a flow guard, so the
-// values equal those of two single sweeps.  Both residuals are summed per
-// block in FP64, in a fixed order.
+// Inner FP32 sweep e_j -> e_{j+1} (e_0 = 0; e_k lives in E[k % 2]) and, for
+// j >= 1, partials of the residual of iterate base+j, r - A e_j, accumulated
+// in FP64.  Flow guards of e are zero: the correction keeps the boundary.
+// Two buffers suffice: the stop test of iterate base+j runs at the start of
+// the launch that would overwrite e_j, before any block writes.
 //
-// The residuals are those of the stop test, which the reference evaluates
-// with the periodic halo of the previous iterate (CalcEpotential.f90:83-92):
-// a neighbour across a periodic boundary takes e_{2q+1}'s predecessor e_{2q}
-// (from the staged planes) in the residual of e_{2q+1}, and e_{2q-1} in the
-// residual of e_{2q}.  e_{2q-1} exists only in the previous launch, which
-// leaves its values on the periodic boundary planes in `eo_in`; this launch
-// leaves those of e_{2q+1} in `eo_out`.
-
-
-struct MxLaunch2 {
-  int64_t seq;
-  int64_t base;          // iterate held by u
-  int64_t q;             // e_{2q} -> e_{2q+2}
-  int64_t check1;        // iterate whose residual Pin1 holds, or -1
-  int64_t check2;        // iterate whose residual Pin2 holds, or -1
-  int32_t m;             // inner sweeps of this outer step
-};
-
-// Stop tests of loop iterations check1 + 1 and check2 + 1, in that order
-// (CalcEpotential.f90:60-62).
-__device__ __forceinline__ StopSlot mx_decide2(JacobiState *st, const MxLaunch2 &L,
-                                               const double *__restrict__ Pin1,
-                                               const double *__restrict__ Pin2, int nblk,
-                                               double *sh, int *decide) {
-  const StopSlot prev = st->slot[(L.seq - 1) & 1];
-  StopSlot cur = prev;
-  if (prev.done) return cur;
-  const int64_t chk[2] = {L.check1, L.check2};
-  const double *pin[2] = {Pin1, Pin2};
-  for (int h = 0; h < 2; ++h) {
-    if (chk[h] < 1) continue;
-    const double S = sum_partials(pin[h], nblk, sh);
-    if (lin_tid() == 0) *decide = (chk[h] + 1 <= st->maxit && sqrt(S) <= st->thresh) ? 1 : 0;
-    __syncthreads();
-    const int d = *decide;
-    __syncthreads();
-    if (d) {
-      cur.done = 1;
-      cur.result = chk[h];
-      break;
-    }
-  }
-  return cur;
-}
-
-// Physical 1-based index of virtual index v along an axis of n interior
-// cells: periodic image, or 0 for a flow guard / beyond.
-__device__ __forceinline__ int vmap(int v, int n, int flow) {
-  if (v >= 2 && v <= n + 1) return v;
-  if (flow) return 0;
-  int r = (v - 2) % n;
-  if (r < 0) r += n;
-  return r + 2;
-}
-
-// Tile BX x (256 / BX): BX = 32 for nx >= 32, else 16 or 8 (the stencil
-// schedule's tile width, capped at 32).
-template <int BX>
-__global__ void __launch_bounds__(256) k_mx_sweep2(GridDims g, MixCoef mc, TileSched ts,
-                                                    const float *__restrict__ r32,
-                                                    const float *E0, const float *E1,
-                                                    float *F0, float *F1,
-                                                    const float *__restrict__ eo_in,
-                                                    float *__restrict__ eo_out,
-                                                    const double *__restrict__ Pin1,
-                                                    const double *__restrict__ Pin2,
-                                                    double *__restrict__ Pout1,
-                                                    double *__restrict__ Pout2, JacobiState *st,
-                                                    MxLaunch2 L) {
-  constexpr int kB2X = BX, kB2Y = 256 / BX;
-  constexpr int kAX = kB2X + 4, kAY = kB2Y + 4;  // e_{2q} staging: tile + 2 rings
-  constexpr int kRX = kB2X + 2, kRY = kB2Y + 2;  // e_{2q+1} and r32: tile + 1 ring
+// The residual is that of the stop test, which the reference evaluates with
+// the periodic halo of the previous iterate (CalcEpotential.f90:83-92): a
+// neighbour across a periodic boundary takes e_{j-1}.  e_{j-1} is kept, for
+// the cells of the periodic boundary planes only, in slot (j-1) % 3 of `eo`;
+// this launch leaves e_{j+1} there in slot (j+1) % 3.
+__global__ void __launch_bounds__(256) k_mx_sweep(GridDims g, MixCoef mc, TileSched ts,
+                                                   const float *__restrict__ r32, float *E0, float *E1,
+                                                   float *eo0, float *eo1, float *eo2,
+                                                   const double *__restrict__ Pin,
+                                                   double *__restrict__ Pout, JacobiState *st,
+                                                   MxLaunch L) {
   __shared__ double sh[4];
   __shared__ int decide;
-  __shared__ float A[5][kAY][kAX];     // planes v-3 .. v+1 (sweep 2 reads v-2 .. v)
-  __shared__ float Bq[3][kRY][kRX];
-  __shared__ float R[3][kRY][kRX];
-  StopSlot cur = mx_decide2(st, L, Pin1, Pin2, ts.B, sh, &decide);
-  if (!cur.done && L.base + 2 * L.q >= st->last) {   // iterate 2*maxit+1 reached (:58)
+  StopSlot cur = mx_decide(st, L, Pin, ts.B, sh, &decide);
+  if (!cur.done && L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
     cur.done = 1;
     cur.result = st->last;
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   if (cur.done) return;
-  const float *e = (L.q & 1) ? E1 : E0;          // e_{2q}; unused when q == 0 (e_0 = 0)
-  float *out = (L.q & 1) ? F0 : F1;              // e_{2q+2}
-  const bool zero_in = L.q == 0;
+  float *E[2] = {E0, E1};
+  float *EO[3] = {eo0, eo1, eo2};
+  const float *e = L.j >= 1 ? E[L.j % 2] : nullptr;
+  float *out = E[(L.j + 1) % 2];
+  float *eo_out = EO[(L.j + 1) % 3];
+  const float *eo_prev = L.j >= 2 ? EO[(L.j - 1) % 3] : nullptr;   // e_0 = 0 has no slot
+  const bool res = L.j >= 1;
   const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const int lt = lin_tid();
+  const int nz = g.nz;
+  const bool pz = !g.flow[2];
+  double r2 = 0.0;
+  // 2.5D march over the XCD's z-slab with the z-neighbours of e_j in registers
   const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
   const int z0 = (xcd * nz) >> 3, z1 = ((xcd + 1) * nz) >> 3;
-  const int P0 = z0 + 2, P1 = z1 + 1;            // output planes (1-based)
-  double ra = 0.0, rb = 0.0;
-  for (int t = w; P1 >= P0 && t < ts.TP; t += ts.W) {
-    const int i0 = (t % ts.ntx) * kB2X + 2, j0 = (t / ts.ntx) * kB2Y + 2;
-    // stage e_{2q} of virtual plane v (tile + 2 rings) into A[v % 5]
-    auto stageA = [&](int v) {
-      const int pk = vmap(v, nz, g.flow[2]);
-      float(*a)[kAX] = A[v % 5];
-      for (int idx = lt; idx < kAX * kAY; idx += 256) {
-        const int ax = idx % kAX, ay = idx / kAX;
-        float val = 0.0f;
-        if (!zero_in && pk) {
-          const int pi = vmap(i0 - 2 + ax, nx, g.flow[0]);
-          const int pj = vmap(j0 - 2 + ay, ny, g.flow[1]);
-          if (pi && pj) val = e[(int64_t)(pi - 1) + X * (int64_t)(pj - 1) + XY * (int64_t)(pk - 1)];
-        }
-        a[ay][ax] = val;
+  for (int t = w; z1 > z0 && t < ts.TP; t += ts.W) {
+    const int tx = t % ts.ntx, ty = t / ts.ntx;
+    const int i = tx * ts.bx + (int)threadIdx.x + 2;
+    const int j = ty * ts.by + (int)threadIdx.y + 2;
+    if (i > g.nx + 1 || j > g.ny + 1) continue;
+    const int64_t col = (int64_t)(i - 1) + X * (int64_t)(j - 1);
+    // x/y neighbours: periodic wrap, or the zero correction of a flow guard
+    const bool fe = g.flow[0] && i == g.nx + 1, fw = g.flow[0] && i == 2;
+    const bool fn = g.flow[1] && j == g.ny + 1, fs = g.flow[1] && j == 2;
+    const bool we = !g.flow[0] && i == g.nx + 1, ww = !g.flow[0] && i == 2;   // wrapped
+    const bool wn = !g.flow[1] && j == g.ny + 1, ws = !g.flow[1] && j == 2;
+    const bool edge_xy = we || ww || wn || ws;
+    if (!e) {
+      for (int k = z0 + 2; k <= z1 + 1; ++k) {
+        const int64_t c = col + XY * (int64_t)(k - 1);
+        const float v = -mc.aff * r32[c];
+        out[c] = v;
+        if (edge_xy || (pz && (k == 2 || k == nz + 1))) eo_out[c] = v;
       }
-    };
-    // r32 of virtual plane v on the tile + 1 ring into R[v % 3]
-    auto stageR = [&](int v) {
-      const int pk = vmap(v, nz, g.flow[2]);
-      float(*r)[kRX] = R[v % 3];
-      for (int idx = lt; idx < kRX * kRY; idx += 256) {
-        const int rx = idx % kRX, ry = idx / kRX;
-        float val = 0.0f;
-        if (pk) {
-          const int pi = vmap(i0 - 1 + rx, nx, g.flow[0]);
-          const int pj = vmap(j0 - 1 + ry, ny, g.flow[1]);
-          if (pi && pj) val = r32[(int64_t)(pi - 1) + X * (int64_t)(pj - 1) + XY * (int64_t)(pk - 1)];
-        }
-        r[ry][rx] = val;
-      }
-    };
-    stageA(P0 - 2);
-    stageA(P0 - 1);
-    for (int v = P0 - 1; v <= P1 + 1; ++v) {
-      stageA(v + 1);
-      stageR(v);
-      __syncthreads();
-      // e_{2q+1} on plane v, tile + 1 ring; residual of e_{2q} on the tile
-      {
-        const bool vin = vmap(v, nz, g.flow[2]) != 0;
-        const float(*am)[kAX] = A[(v + 4) % 5];   // plane v - 1
-        const float(*ac)[kAX] = A[v % 5];
-        const float(*ap)[kAX] = A[(v + 1) % 5];
-        float(*bo)[kRX] = Bq[v % 3];
-        const float(*rr)[kRX] = R[v % 3];
-        for (int idx = lt; idx < kRX * kRY; idx += 256) {
-          const int rx = idx % kRX, ry = idx / kRX;
-          const int ax = rx + 1, ay = ry + 1;
-          const int vi = i0 - 1 + rx, vj = j0 - 1 + ry;
-          const bool pin = vin && vmap(vi, nx, g.flow[0]) && vmap(vj, ny, g.flow[1]);
-          float val = 0.0f;
-          if (pin) {
-            const float ee = ac[ay][ax + 1], ew = ac[ay][ax - 1];
-            const float en = ac[ay + 1][ax], es = ac[ay - 1][ax];
-            const float et = ap[ay][ax], eb = am[ay][ax];
-            const float rc = rr[ry][rx];
-            val = mc.axf * (ee + ew) + mc.ayf * (en + es) + mc.azf * (et + eb) - mc.aff * rc;
-            const bool own = rx >= 1 && rx <= kB2X && ry >= 1 && ry <= kB2Y && v >= P0 &&
-                             v <= P1 && vi <= nx + 1 && vj <= ny + 1;
-            if (own) {
-              // periodic images across each face of this cell
-              const bool ie = !g.flow[0] && vi == nx + 1, iw = !g.flow[0] && vi == 2;
-              const bool in_ = !g.flow[1] && vj == ny + 1, is = !g.flow[1] && vj == 2;
-              const bool it = !g.flow[2] && v == nz + 1, ib = !g.flow[2] && v == 2;
-              const int64_t c = (int64_t)(vi - 1) + X * (int64_t)(vj - 1) + XY * (int64_t)(v - 1);
-              if (ie || iw || in_ || is || it || ib) eo_out[c] = val;
-              // residual of e_{2q} with e_{2q-1} across periodic boundaries
-              if (!zero_in) {
-                const float se = ie ? eo_in[c - (int64_t)(nx - 1)] : ee;
-                const float sw = iw ? eo_in[c + (int64_t)(nx - 1)] : ew;
-                const float sn = in_ ? eo_in[c - (int64_t)(ny - 1) * X] : en;
-                const float ss = is ? eo_in[c + (int64_t)(ny - 1) * X] : es;
-                const float st_ = it ? eo_in[c - (int64_t)(nz - 1) * XY] : et;
-                const float sb = ib ? eo_in[c + (int64_t)(nz - 1) * XY] : eb;
-                const float e2 = 2.0f * ac[ay][ax];
-                const float res = rc - ((se + sw - e2) * mc.idx2f + (sn + ss - e2) * mc.idy2f +
-                                        (st_ + sb - e2) * mc.idz2f);
-                ra = ra + (double)res * (double)res;
-              }
-            }
-          }
-          bo[ry][rx] = val;
-        }
-      }
-      __syncthreads();
-      // e_{2q+2} on plane v - 1 of the tile; residual of e_{2q+1}
-      const int p = v - 1;
-      if (p >= P0) {
-        const int tx = lt % kB2X, ty = lt / kB2X;
-        const int vi = i0 + tx, vj = j0 + ty;
-        if (vi <= nx + 1 && vj <= ny + 1) {
-          const int rx = tx + 1, ry = ty + 1;
-          const float(*bm)[kRX] = Bq[(p + 2) % 3];
-          const float(*bc)[kRX] = Bq[p % 3];
-          const float(*bp)[kRX] = Bq[(p + 1) % 3];
-          const float ee = bc[ry][rx + 1], ew = bc[ry][rx - 1];
-          const float en = bc[ry + 1][rx], es = bc[ry - 1][rx];
-          const float et = bp[ry][rx], eb = bm[ry][rx];
-          const float rc = R[p % 3][ry][rx];
-          out[(int64_t)(vi - 1) + X * (int64_t)(vj - 1) + XY * (int64_t)(p - 1)] =
-              mc.axf * (ee + ew) + mc.ayf * (en + es) + mc.azf * (et + eb) - mc.aff * rc;
-          // residual of e_{2q+1} with e_{2q} across periodic boundaries
-          const int ax = rx + 1, ay = ry + 1;
-          const float se = (!g.flow[0] && vi == nx + 1) ? A[p % 5][ay][ax + 1] : ee;
-          const float sw = (!g.flow[0] && vi == 2) ? A[p % 5][ay][ax - 1] : ew;
-          const float sn = (!g.flow[1] && vj == ny + 1) ? A[p % 5][ay + 1][ax] : en;
-          const float ss = (!g.flow[1] && vj == 2) ? A[p % 5][ay - 1][ax] : es;
-          const float st_ = (!g.flow[2] && p == nz + 1) ? A[(p + 1) % 5][ay][ax] : et;
-          const float sb = (!g.flow[2] && p == 2) ? A[(p + 4) % 5][ay][ax] : eb;
-          const float e2 = 2.0f * bc[ry][rx];
-          const float res = rc - ((se + sw - e2) * mc.idx2f + (sn + ss - e2) * mc.idy2f +
-                                  (st_ + sb - e2) * mc.idz2f);
-          rb = rb + (double)res * (double)res;
-        }
-      }
+      continue;
     }
-    __syncthreads();   // the next tile restages A, R
+    const int64_t oe = we ? -(int64_t)(g.nx - 1) : 1;
+    const int64_t ow = ww ? (int64_t)(g.nx - 1) : -1;
+    const int64_t on = wn ? -(int64_t)(g.ny - 1) * X : X;
+    const int64_t os = ws ? (int64_t)(g.ny - 1) * X : -X;
+    // value of e_j on 1-based plane kk (guards: wrap or zero)
+    auto zval = [&](int kk) -> float {
+      if (kk == 1) return g.flow[2] ? 0.0f : e[col + XY * (int64_t)nz];
+      if (kk == nz + 2) return g.flow[2] ? 0.0f : e[col + XY];
+      return e[col + XY * (int64_t)(kk - 1)];
+    };
+    // e_{j-1} at a wrapped neighbour (0 for j = 1)
+    auto prev = [&](int64_t n) -> float { return eo_prev ? eo_prev[n] : 0.0f; };
+    int k = z0 + 2;
+    float zm = zval(k - 1), zc = zval(k), zp = zval(k + 1);
+#pragma unroll 2
+    for (; k <= z1 + 1; ++k) {
+      const int64_t c = col + XY * (int64_t)(k - 1);
+      const float ee = fe ? 0.0f : e[c + oe];
+      const float ew = fw ? 0.0f : e[c + ow];
+      const float en = fn ? 0.0f : e[c + on];
+      const float es = fs ? 0.0f : e[c + os];
+      const float et = zp, eb = zm;
+      const float rc = r32[c];
+      const float v = mc.axf * (ee + ew) + mc.ayf * (en + es) + mc.azf * (et + eb) - mc.aff * rc;
+      out[c] = v;
+      const bool wt = pz && k == nz + 1, wb = pz && k == 2;
+      float se = ee, sw = ew, sn = en, ss = es, st_ = et, sb = eb;
+      if (edge_xy || wt || wb) {
+        eo_out[c] = v;
+        if (we) se = prev(c + oe);
+        if (ww) sw = prev(c + ow);
+        if (wn) sn = prev(c + on);
+        if (ws) ss = prev(c + os);
+        if (wt) st_ = prev(c - XY * (int64_t)(nz - 1));
+        if (wb) sb = prev(c + XY * (int64_t)(nz - 1));
+      }
+      const float e2 = 2.0f * zc;
+      const float rr = rc - ((se + sw - e2) * mc.idx2f + (sn + ss - e2) * mc.idy2f +
+                             (st_ + sb - e2) * mc.idz2f);
+      r2 = r2 + (double)rr * (double)rr;
+      zm = zc;
+      zc = zp;
+      if (k + 1 <= z1 + 1) zp = zval(k + 2);
+    }
   }
-  const double sa = block_sum(ra, sh);
-  if (lin_tid() == 0 && !zero_in) Pout1[blockIdx.x] = sa;
-  __syncthreads();   // thread 0 has read sh before it is reused
-  const double sb = block_sum(rb, sh);
-  if (lin_tid() == 0) Pout2[blockIdx.x] = sb;
+  const double sblk = block_sum(r2, sh);
+  if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
 }
 
-// u += e_k after an outer step of launches of k_mx_sweep2: even k from its
-// buffer, odd k = 2q + 1 recomputed as one sweep of e_{2q} (e_0 = 0).
-__global__ void __launch_bounds__(256) k_mx_update2(GridDims g, MixCoef mc, TileSched ts,
-                                                     double *__restrict__ u,
-                                                     const float *__restrict__ r32,
-                                                     const float *E0, const float *E1,
-                                                     const double *__restrict__ Pin1,
-                                                     const double *__restrict__ Pin2,
-                                                     JacobiState *st, MxLaunch2 L) {
+// u += e_k: k = m after a full outer step, or the converged (or last)
+// iterate's k when the solve ended inside this step.
+__global__ void __launch_bounds__(256) k_mx_update(GridDims g, TileSched ts, double *__restrict__ u,
+                                                    const float *E0, const float *E1,
+                                                    const double *__restrict__ Pin,
+                                                    JacobiState *st, MxLaunch L) {
   __shared__ double sh[4];
   __shared__ int decide;
   const StopSlot prev = st->slot[(L.seq - 1) & 1];
-  StopSlot cur = mx_decide2(st, L, Pin1, Pin2, ts.B, sh, &decide);
+  StopSlot cur = mx_decide(st, L, Pin, ts.B, sh, &decide);
   if (!cur.done && L.base + L.m >= st->last) {
     cur.done = 1;
     cur.result = st->last;
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
+  // solves that ended before this outer step have nothing left to apply
   if (prev.done && prev.result < L.base) return;
   const int64_t k = cur.done ? cur.result - L.base : (int64_t)L.m;
   if (k <= 0) return;
-  const int64_t qk = k >> 1;
-  const float *e = (qk & 1) ? E1 : E0;           // e_{2 qk}
-  const bool odd = k & 1, zero_in = qk == 0;
-  const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
+  const float *e = (k & 1) ? E1 : E0;
   for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int kk, bool valid) {
     if (!valid) return;
     const int64_t c = node_index(g, i, j, kk);
-    float ek;
-    if (!odd) {
-      ek = e[c];
-    } else {
-      auto at = [&](int vi, int vj, int vk) -> float {
-        if (zero_in) return 0.0f;
-        const int pi = vmap(vi, g.nx, g.flow[0]), pj = vmap(vj, g.ny, g.flow[1]);
-        const int pk = vmap(vk, g.nz, g.flow[2]);
-        if (!pi || !pj || !pk) return 0.0f;
-        return e[(int64_t)(pi - 1) + X * (int64_t)(pj - 1) + XY * (int64_t)(pk - 1)];
-      };
-      ek = mc.axf * (at(i + 1, j, kk) + at(i - 1, j, kk)) + mc.ayf * (at(i, j + 1, kk) + at(i, j - 1, kk)) +
-           mc.azf * (at(i, j, kk + 1) + at(i, j, kk - 1)) - mc.aff * r32[c];
-    }
-    u[c] = u[c] + (double)ek;
+    u[c] = u[c] + (double)e[c];
   });
 }
 
@@ -869,41 +728,6 @@ int sp_gridify(picard_ctx *c, bool flux) {
   return PICARD_OK;
 }
 
-// Tile schedule of the two-sweep kernels: the stencil schedule with the tile
-// width capped at 32.
-TileSched make_sched2(const GridDims &g) {
-  TileSched s = make_sched(g.nx, g.ny, g.nz);
-  if (s.bx > 32) {
-    s.bx = 32;
-    s.by = 256 / s.bx;
-    s.ntx = (g.nx + s.bx - 1) / s.bx;
-    s.nty = (g.ny + s.by - 1) / s.by;
-    s.TP = s.ntx * s.nty;
-    s.W = s.TP < 256 ? s.TP : 256;
-    if (s.W < 4) s.W = 4;
-    s.B = 8 * s.W;
-  }
-  return s;
-}
-
-template <int BX>
-void launch_sweep2(picard_ctx *c, const TileSched &ts, const MixCoef &mc, const double *const pin[2],
-                   double *const pout[2], const MxLaunch2 &L, hipEvent_t t0, hipEvent_t t1) {
-  const dim3 grid(ts.B), block(ts.bx, ts.by);
-  float *E0 = c->e32[0], *E1 = c->e32[1];
-  // launch q leaves e_{2q+1} on the periodic boundary planes in eo32[q & 1]
-  const float *eo_in = c->eo32[(L.q + 1) & 1];
-  float *eo_out = c->eo32[L.q & 1];
-  if (t0)
-    hipExtLaunchKernelGGL(k_mx_sweep2<BX>, grid, block, 0, c->stream, t0, t1, 0, c->gd, mc, ts,
-                          (const float *)c->r32, (const float *)E0, (const float *)E1, E0, E1,
-                          eo_in, eo_out, pin[0], pin[1], pout[0], pout[1], c->js, L);
-  else
-    k_mx_sweep2<BX><<<grid, block, 0, c->stream>>>(c->gd, mc, ts, c->r32, E0, E1, E0, E1, eo_in,
-                                                    eo_out, pin[0], pin[1], pout[0], pout[1], c->js,
-                                                    L);
-}
-
 int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) {
   const GridDims &g = c->gd;
   const double dx = g.dxyz[0], dy = g.dxyz[1], dz = g.dxyz[2];
@@ -923,72 +747,76 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
   init.last = 2 * (int64_t)maxit + 1;
   PICARD_HIP(hipMemcpyAsync(c->js, &init, sizeof(JacobiState), hipMemcpyHostToDevice, c->stream));
   PICARD_TRY(launch_bnorm(c, maxerr));      // launch 0: b, |b|, the test of ss = 1
-  const TileSched ts = make_sched2(g);
-  if (ts.B > c->s.partials_len) {
-    set_error("CalcEpotential (split): partial-sum buffer smaller than the launch");
-    return PICARD_ERR_STATE;
-  }
-  const int m = c->mx_inner;                // even: two sweeps per launch
-  const int m2 = m / 2;
-  // partial sums: launch `seq` writes set seq & 1 (two arrays) and reads set (seq-1) & 1
-  double *P[2][2];
-  for (int q = 0; q < 4; ++q) P[q >> 1][q & 1] = c->s.partials + (int64_t)q * c->s.partials_len;
+  const TileSched ts = make_sched(g.nx, g.ny, g.nz);
+  const int m = c->mx_inner;
+  double *P[2] = {c->s.partials, c->s.partials + c->s.partials_len};
+  float *E0 = c->e32[0], *E1 = c->e32[1];
+  float *const *EO = c->eo32;
   int64_t seq = 1, base = 0;
+  int pw = 0;                               // partials buffer the next writer uses
+  int64_t check = -1;                       // iterate whose residual the last writer stored
   StopSlot fin{};
   while (true) {
     if (c->timing) {
-      while ((int64_t)c->tev.size() < 2 * m2) {
+      while ((int64_t)c->tev.size() < 2 * m) {
         hipEvent_t e;
         PICARD_HIP(hipEventCreate(&e));
         c->tev.push_back(e);
       }
     }
-    // outer residual of iterate `base` (tested by the first sweep launch);
-    // its stale-halo norm needs e_m and e_{m-1} of the previous outer step
+    // outer residual of iterate `base` (its stop test happens in the next
+    // launch); the stale-halo norm needs e_m and e_{m-1} of the previous outer
     {
       MxLaunch L{seq, base, 0, -1, m};
-      const float *em = base > 0 ? c->e32[m2 & 1] : nullptr;
-      const float *eo = base > 0 ? c->eo32[(m2 - 1) & 1] : nullptr;
+      const float *em = base > 0 ? c->e32[m & 1] : nullptr;
+      const float *eo = base > 0 ? EO[(m - 1) % 3] : nullptr;
       k_mx_residual<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, mc, ts, c->U, c->b, c->r32, em,
-                                                                eo, P[(seq - 1) & 1][0],
-                                                                P[seq & 1][0], c->js, L);
+                                                                eo, P[pw ^ 1], P[pw], c->js, L);
       ++seq;
+      check = base;
+      pw ^= 1;
     }
-    for (int q = 0; q < m2; ++q) {
-      MxLaunch2 L{seq, base, q, -1, -1, m};
-      if (q == 0) {
-        L.check1 = base;                      // the FP64 residual just computed
-      } else {
-        L.check1 = q >= 2 ? base + 2 * q - 2 : -1;
-        L.check2 = base + 2 * q - 1;
-      }
-      const double *pin[2] = {P[(seq - 1) & 1][0], P[(seq - 1) & 1][1]};
-      double *const pout[2] = {P[seq & 1][0], P[seq & 1][1]};
-      hipEvent_t t0 = c->timing ? c->tev[2 * q] : nullptr;
-      hipEvent_t t1 = c->timing ? c->tev[2 * q + 1] : nullptr;
-      if (ts.bx == 32) launch_sweep2<32>(c, ts, mc, pin, pout, L, t0, t1);
-      else if (ts.bx == 16) launch_sweep2<16>(c, ts, mc, pin, pout, L, t0, t1);
-      else launch_sweep2<8>(c, ts, mc, pin, pout, L, t0, t1);
+    for (int j = 0; j < m; ++j) {
+      // the iterate-0 test used b itself (launch 0); later ones use residuals
+      const int64_t chk = (check >= 1) ? check : -1;
+      MxLaunch L{seq, base, j, chk, m};
+      const double *pin = P[pw ^ 1];
+      double *pout = P[pw];
+      if (c->timing)
+        hipExtLaunchKernelGGL(k_mx_sweep, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
+                              c->tev[2 * j], c->tev[2 * j + 1], 0, g, mc, ts,
+                              (const float *)c->r32, E0, E1, EO[0], EO[1], EO[2], pin, pout, c->js,
+                              L);
+      else
+        k_mx_sweep<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, mc, ts, c->r32, E0, E1, EO[0],
+                                                               EO[1], EO[2], pin, pout, c->js, L);
       ++seq;
+      if (j >= 1) {               // wrote the residual of iterate base + j
+        check = base + j;
+        pw ^= 1;
+      } else {
+        check = -1;
+      }
     }
     {
-      MxLaunch2 L{seq, base, m2, m2 >= 2 ? base + m - 2 : -1, base + m - 1, m};
-      k_mx_update2<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(
-          g, mc, ts, c->U, c->r32, c->e32[0], c->e32[1], P[(seq - 1) & 1][0], P[(seq - 1) & 1][1],
-          c->js, L);
+      const int64_t chk = (check >= 1) ? check : -1;
+      MxLaunch L{seq, base, 0, chk, m};
+      k_mx_update<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, ts, c->U, E0, E1, P[pw ^ 1],
+                                                              c->js, L);
       ++seq;
+      check = -1;
     }
     PICARD_HIP(hipGetLastError());
     PICARD_HIP(hipMemcpyAsync(c->js_host, c->js, sizeof(JacobiState), hipMemcpyDeviceToHost,
                               c->stream));
     PICARD_HIP(hipStreamSynchronize(c->stream));
     if (c->timing) {
-      for (int q = 0; q < m2; ++q) {
+      for (int q = 0; q < m; ++q) {
         float ms = 0.0f;
         PICARD_HIP(hipEventElapsedTime(&ms, c->tev[2 * q], c->tev[2 * q + 1]));
         c->t_ms += ms;
       }
-      c->t_launches += m2;
+      c->t_launches += m;
     }
     fin = c->js_host->slot[(seq - 1) & 1];
     if (fin.done) break;

@@ -10,6 +10,7 @@ program picard_host
   use picard_hip_iface
   use picard_dropin_state
   use picard_setup
+  use picard_output
   implicit none
 
   type(run_config) :: cfg
@@ -17,6 +18,10 @@ program picard_host
   type(picard_geometry) :: geo
   type(picard_species) :: spc
   real*8, allocatable :: real_particles(:,:), U(:,:,:), E(:,:,:,:), N(:,:,:,:), C(:,:,:), D(:,:,:)
+  real*8, allocatable :: F(:,:,:,:,:)
+  real*8 :: time, uP, uE, uB, u0
+  integer :: num_global, fields_dump, particles_dump
+  logical :: out_fields, out_particles
   integer, allocatable :: int_particles(:)
   integer :: max_per_proc, num_local, iteration, nargs, per_cell
   integer(c_int64_t) :: pushed, sweeps_total
@@ -38,6 +43,8 @@ program picard_host
   allocate(U(cfg%nx+2, cfg%ny+2, cfg%nz+2), C(cfg%nx+2, cfg%ny+2, cfg%nz+2))
   allocate(E(3, cfg%nx+2, cfg%ny+2, cfg%nz+2), N(cfg%species, cfg%nx+2, cfg%ny+2, cfg%nz+2))
   allocate(D(2*cfg%nx+1, 2*cfg%ny+1, 2*cfg%nz+1))
+  allocate(F(cfg%species, 3, cfg%nx+2, cfg%ny+2, cfg%nz+2))
+  F = 0d0
   real_particles = 0d0; int_particles = 0; U = 0d0; E = 0d0; N = 0d0; C = 0d0; D = 0d0
   Lx_min = cfg%xmin; Lx_max = cfg%xmax; Ly_min = cfg%ymin; Ly_max = cfg%ymax
   Lz_min = cfg%zmin; Lz_max = cfg%zmax
@@ -51,10 +58,16 @@ program picard_host
   write(*,'(a,3i5,a,i0,a,es12.5)') 'picard_host: grid', cfg%nx, cfg%ny, cfg%nz, '  particles ', &
       num_local, '  dt ', dc%dt
 
+  ! the initial energy the diagnostics line is divided by (picard.f90:476-480)
+  u0 = initial_energy(cfg, dc)
+  time = 0d0
+  fields_dump = 0
+  particles_dump = 0
   pushed = 0
   sweeps_total = 0
   call system_clock(t0, rate)
   do iteration = 1, cfg%steps
+    time = time + dc%dt                                                                       ! :556
     call inflow_particles(cfg, dc, real_particles, int_particles, num_local, max_per_proc)   ! :560
     call AdvancePosition(real_particles, dc%dt, max_per_proc, num_local)                      ! :564
     call BCoutflow(real_particles, int_particles, Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, Lz_max, &
@@ -75,9 +88,43 @@ program picard_host
                         cfg%zmax, dc%dxyz, cfg%maxerr, cfg%maxit, 1, 1, 1, cfg%species, 0)    ! :584
     call CalcEfield(E, U, cfg%nx, cfg%ny, cfg%nz, dc%dxyz, cfg%xflow, cfg%yflow, cfg%zflow, 1, 1, 1, 0)  ! :587
     call BCfields(E, U, dc%B0, dc%E0, cfg%nx, cfg%ny, cfg%nz, cfg%xflow, cfg%yflow, cfg%zflow, 1, 1, 1, 0) ! :589
-    call AdvanceVelocity(real_particles, int_particles, E, dc%B0, Lx_min, Ly_min, Lz_min, dc%dxyz, dc%dt, &
-                         dc%v0, dc%ve, dc%eta, dc%tB, dc%sB, num_local, max_per_proc, cfg%species, cfg%nx, &
-                         cfg%ny, cfg%nz, cfg%fluid, cfg%gyration)                            ! :630
+    ! output steps split the velocity push around the dumps (:600-628)
+    out_fields = cfg%write_fields .and. dmod(time, cfg%write_fields_interval) .lt. dc%dt
+    out_particles = cfg%write_particles .and. dmod(time, cfg%write_particles_interval) .lt. dc%dt
+    if (out_fields .or. out_particles) then
+      call AdvanceVelocityImp(real_particles, int_particles, E, dc%B0, Lx_min, Ly_min, Lz_min, dc%dxyz, &
+                              0.5d0*dc%dt, dc%dt, dc%v0, dc%ve, dc%eta, dc%tB, num_local, max_per_proc, &
+                              cfg%species, cfg%nx, cfg%ny, cfg%nz, cfg%fluid, cfg%gyration)  ! :603
+      if (out_particles) then
+        call picard_dropin_download_particles()
+        call dump_particles(real_particles, int_particles, num_local, particles_dump, 0)      ! :607
+        particles_dump = particles_dump + 1
+      end if
+      if (out_fields) then
+        call ParticlesGridifyAll(F, N, C, real_particles, int_particles, Lx_min, Lx_max, Ly_min, Ly_max, &
+                                 Lz_min, Lz_max, dc%dxyz, cfg%xmin, cfg%xmax, cfg%ymin, cfg%ymax, cfg%zmin, &
+                                 cfg%zmax, cfg%species, cfg%nx, cfg%ny, cfg%nz, max_per_proc, num_local, &
+                                 cfg%xflow, cfg%yflow, cfg%zflow, 1, 1, 1, dc%charge, dc%n2p, 0)  ! :611
+        call picard_dropin_download_field(PICARD_FIELD_U, U, size(U, kind=c_int64_t))
+        call picard_dropin_download_field(PICARD_FIELD_E, E, size(E, kind=c_int64_t))
+        call dump_fields(U, N, F, E, cfg%species, fields_dump, 0)                            ! :614
+        fields_dump = fields_dump + 1
+      end if
+      call AdvanceVelocityExp(real_particles, int_particles, E, dc%B0, Lx_min, Ly_min, Lz_min, dc%dxyz, &
+                              0.5d0*dc%dt, dc%dt, dc%v0, dc%ve, dc%eta, dc%tB, num_local, max_per_proc, &
+                              cfg%species, cfg%nx, cfg%ny, cfg%nz, cfg%fluid, cfg%gyration)  ! :616
+    else
+      call AdvanceVelocity(real_particles, int_particles, E, dc%B0, Lx_min, Ly_min, Lz_min, dc%dxyz, dc%dt, &
+                           dc%v0, dc%ve, dc%eta, dc%tB, dc%sB, num_local, max_per_proc, cfg%species, cfg%nx, &
+                           cfg%ny, cfg%nz, cfg%fluid, cfg%gyration)                          ! :630
+    end if
+    if (cfg%write_output_interval > 0) then
+      if (mod(iteration, cfg%write_output_interval) == 0) then                               ! :647-672
+        call picard_dropin_diagnostics(dc%mass, num_global, uP, uE, uB)
+        write(*, fmt='(I8,a,E12.6,a,I8,a,I8,a,I8,a,E12.6,a,E12.6,a,E12.6)') iteration, '   ', time, &
+            '   ', 0, '   ', num_local, '   ', num_global, '   ', uP/u0, '   ', uE/u0, '   ', uB/u0
+      end if
+    end if
     pushed = pushed + num_local
     sweeps_total = sweeps_total + last_sweeps
     write(*,'(a,i6,a,i9,a,i7,a,i8)') 'step', iteration, '  num_local', num_local, '  removed', &

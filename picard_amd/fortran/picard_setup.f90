@@ -15,6 +15,7 @@ module picard_setup
   real*8, parameter :: TAU = 6.2831853071795864769252867665590d0
   real*8, parameter :: EC = 1.60217646d-19, MP = 1.6726216d-27, ME = 9.1093819d-31
   real*8, parameter :: EPS0 = 8.854187817d-12, KB = 1.380650d-23
+  real*8, parameter :: MU0 = 1.25663706143591729538505735331180d-6
 
   type run_config
     integer :: nx = 32, ny = 32, nz = 32, species = 2, steps = 10, maxit = 2**20, seed = 20261203
@@ -28,6 +29,11 @@ module picard_setup
     real*8 :: v0(3) = [4d5, 0d0, 0d0]
     real*8 :: B0(3) = [0d0, 0d0, 5d-9]
     logical :: gyration(8) = .true., fluid(8) = .false.
+    ! output (picard.f90 section 1.1): dumps when dmod(time, interval) < dt,
+    ! the diagnostics line every write_output_interval steps
+    logical :: write_fields = .false., write_particles = .false.
+    real*8 :: write_fields_interval = 1d30, write_particles_interval = 1d30
+    integer :: write_output_interval = 0
   end type run_config
 
   type derived
@@ -45,16 +51,23 @@ contains
     integer :: nx, ny, nz, species, steps, maxit, seed, ppc(8), u, ios
     logical :: xflow, yflow, zflow, gyration(8), fluid(8)
     real*8 :: xmin, xmax, ymin, ymax, zmin, zmax, dt_wpe, maxerr, mass(8), density(8), kelvin(8), &
-              v0(3), B0(3)
+              v0(3), B0(3), write_fields_interval, write_particles_interval
+    logical :: write_fields, write_particles
+    integer :: write_output_interval
     namelist /picard_run/ nx, ny, nz, species, steps, maxit, seed, ppc, xflow, yflow, zflow, gyration, &
                           fluid, xmin, xmax, ymin, ymax, zmin, zmax, dt_wpe, maxerr, mass, density, &
-                          kelvin, v0, B0
+                          kelvin, v0, B0, write_fields, write_particles, write_fields_interval, &
+                          write_particles_interval, write_output_interval
     nx = cfg%nx; ny = cfg%ny; nz = cfg%nz; species = cfg%species; steps = cfg%steps
     maxit = cfg%maxit; seed = cfg%seed; ppc = cfg%ppc; xflow = cfg%xflow; yflow = cfg%yflow
     zflow = cfg%zflow; gyration = cfg%gyration; fluid = cfg%fluid; xmin = cfg%xmin; xmax = cfg%xmax
     ymin = cfg%ymin; ymax = cfg%ymax; zmin = cfg%zmin; zmax = cfg%zmax; dt_wpe = cfg%dt_wpe
     maxerr = cfg%maxerr; mass = cfg%mass; density = cfg%density; kelvin = cfg%kelvin; v0 = cfg%v0
     B0 = cfg%B0
+    write_fields = cfg%write_fields; write_particles = cfg%write_particles
+    write_fields_interval = cfg%write_fields_interval
+    write_particles_interval = cfg%write_particles_interval
+    write_output_interval = cfg%write_output_interval
     open(newunit=u, file=path, status='old', action='read', iostat=ios)
     if (ios /= 0) then
       write(*,*) 'picard: cannot open ', trim(path)
@@ -68,9 +81,30 @@ contains
     cfg%ymin = ymin; cfg%ymax = ymax; cfg%zmin = zmin; cfg%zmax = zmax; cfg%dt_wpe = dt_wpe
     cfg%maxerr = maxerr; cfg%mass = mass; cfg%density = density; cfg%kelvin = kelvin; cfg%v0 = v0
     cfg%B0 = B0
+    cfg%write_fields = write_fields; cfg%write_particles = write_particles
+    cfg%write_fields_interval = write_fields_interval
+    cfg%write_particles_interval = write_particles_interval
+    cfg%write_output_interval = write_output_interval
   end subroutine read_config
 
   ! picard.f90:333-466 on one domain (iprocs = jprocs = kprocs = 1).
+  ! u0 = uE0 + uB0 + uP0 of picard.f90:476-480, the scale of the diagnostics line
+  real*8 function initial_energy(cfg, d) result(u0)
+    type(run_config), intent(in) :: cfg
+    type(derived), intent(in) :: d
+    real*8 :: vol, uE0, uB0, uP0
+    integer :: s
+    vol = (cfg%xmax - cfg%xmin) * (cfg%ymax - cfg%ymin) * (cfg%zmax - cfg%zmin)
+    uE0 = 0.5d0 * (d%E0(4)**2 * EPS0) * vol
+    uB0 = 0.5d0 * (d%B0(4)**2 / MU0) * vol
+    uP0 = 0d0
+    do s = 1, 8
+      uP0 = uP0 + d%density(s) * (d%mass(s) * d%v0(4, s)**2 + 3d0 * KB * d%kelvin(s))
+    end do
+    uP0 = 0.5d0 * uP0 * vol
+    u0 = uE0 + uB0 + uP0
+  end function initial_energy
+
   subroutine derive(cfg, d)
     type(run_config), intent(in) :: cfg
     type(derived), intent(out) :: d

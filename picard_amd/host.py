@@ -549,6 +549,36 @@ class Domain:
         check("picard_set_production",
               lib().picard_set_production(self.h, ct.byref(pl), ct.byref(hs), seed))
 
+    def output_step(self, path, write_fields: bool, write_particles: bool, fields_index: int = 0,
+                    particles_index: int = 0, produced=None):
+        """One main-loop iteration on an output step (picard.f90:556-628): the
+        phases of step() up to BCfields, then AdvanceVelocityImp, the particle
+        dump, ParticlesGridifyAll and the field dump, and AdvanceVelocityExp.
+        Writes under `path` in the reference's layout; returns the file names."""
+        from . import output
+        self.advance_position()                                   # :564
+        if produced is not None:
+            self.append(*produced)                                # :566
+        self.bc_outflow()                                         # :569
+        self.particles_transfer()                                 # :572
+        self.particles_gridify()                                  # :577
+        self.bc_potentials()                                      # :581
+        sweeps = self.calc_epotential()                           # :584
+        self.calc_efield()                                        # :587
+        self.bc_fields()                                          # :589
+        self.advance_velocity_imp()                               # :603
+        names = []
+        if write_particles:                                       # :606-608
+            rp, sp = self.particles()
+            names.append(output.write_particles(path, rp, sp, particles_index))
+        if write_fields:                                          # :610-615
+            self.particles_gridify_all()
+            names += output.write_fields(path, self.field(_lib.FIELD_U), self.field(_lib.FIELD_N),
+                                         self.field(_lib.FIELD_F), self.field(_lib.FIELD_E),
+                                         fields_index)
+        self.advance_velocity_exp()                               # :616
+        return names, sweeps
+
     def particles_gridify_all(self):
         """ParticlesGridifyAll: N, C and the number fluxes (field FIELD_F)."""
         check("picard_particles_gridify_all", lib().picard_particles_gridify_all(self.h))

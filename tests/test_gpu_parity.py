@@ -239,6 +239,61 @@ def test_gridify_all(flow):
     sdom.close()
 
 
+def test_output_step_matches_oracle(tmp_path):
+    """An output step (picard.f90:600-616): Imp push, particle dump,
+    ParticlesGridifyAll, field dump, Exp push; the FP64 form's particles and
+    every dumped file equal the oracle's, after one ordinary step."""
+    from picard_amd import output
+    _say("output step")
+    cfg = small_config(8, ppc=4)
+    d = host.derive(cfg)
+    rng = np.random.default_rng(71)
+    rp, sp = host.particles_initialize(cfg, d, rng)
+    inflow = [host.bc_inflow(cfg, d, rng) for _ in range(2)]
+    sim = OracleSim(cfg, d, rp, sp)
+    dom = _domain(cfg)
+    dom.append(rp, sp)
+    dom.calc_distance()
+    sim.append(*inflow[0])
+    sim.step()
+    dom.append(*inflow[0])
+    dom.step()
+    # oracle output step
+    sim.append(*inflow[1])
+    g = sim.g
+    oracle.advance_position(sim.rp, d.dt, sim.n)
+    n_new, _ = oracle.bc_outflow(g, sim.rp, sim.sp, sim.n)
+    sim.n = n_new
+    sim.rp, sim.sp = np.ascontiguousarray(sim.rp[:n_new]), np.ascontiguousarray(sim.sp[:n_new])
+    oracle.particles_transfer(g, sim.rp, sim.n)
+    N, C, _ = oracle.particles_gridify(g, sim.rp, sim.sp, d.charge, d.n2p)
+    U = oracle.bc_potentials(g, C, sim.D)
+    oracle.calc_epotential(g, U, C, cfg.maxerr, cfg.maxit)
+    E = oracle.calc_efield(g, U)
+    oracle.bc_fields(g, E)
+    fl, gy = [int(f) for f in cfg.fluid], [int(f) for f in cfg.gyration]
+    oracle.advance_velocity_ie(1, g, sim.rp, sim.sp, E, d.B0, 0.5 * d.dt, d.dt, d.ve, d.eta, d.tB,
+                               fl, gy)
+    ref_dir = tmp_path / "oracle"
+    output.write_particles(str(ref_dir), sim.rp, sim.sp, 0)
+    F, N2, C2, _ = oracle.particles_gridify_all(g, sim.rp, sim.sp, d.charge, d.n2p)
+    output.write_fields(str(ref_dir), U, N2, F, E, 0)
+    oracle.advance_velocity_ie(2, g, sim.rp, sim.sp, E, d.B0, 0.5 * d.dt, d.dt, d.ve, d.eta, d.tB,
+                               fl, gy)
+    # device output step
+    dom.append(*inflow[1])
+    names, _ = dom.output_step(str(tmp_path / "gpu"), True, True)
+    got, gsp = dom.particles()
+    _eq(got, sim.rp, "particles after the output step")
+    assert np.array_equal(gsp, sim.sp)
+    assert len(names) == 1 + 4 + 4 * cfg.species        # particles, UE, E, N, X, Y, Z
+    for name in names:
+        rel = os.path.relpath(name, str(tmp_path / "gpu"))
+        with open(name, "rb") as a, open(ref_dir / rel, "rb") as b:
+            assert a.read() == b.read(), rel
+    dom.close()
+
+
 def _run_phases(cfg, steps, label):
     """Each subroutine of the main loop in turn on both sides, compared after
     every phase, so a divergence is reported at the phase that causes it."""
