@@ -534,12 +534,20 @@ __device__ __forceinline__ double sum_partials(const double *__restrict__ P, int
   return block_sum(v, sh);
 }
 
-__global__ void __launch_bounds__(256) k_jacobi_pass(GridDims g, JacobiCoef jc, TileSched ts,
-                                                     double *b0, double *b1, double *b2,
-                                                     const double *__restrict__ b,
+// Uniform geometry of the pass kernel, kept small (32-bit, no box data) so
+// it stays in scalar registers.
+struct PassGeo {
+  int32_t nx, ny, nz;
+  uint32_t X, XY;        // node strides of j and k
+  int32_t px, py, pz;    // periodic flags
+  int32_t ntx, TP, W;    // tile schedule (tile width = blockDim.x)
+};
+
+__global__ void __launch_bounds__(256, 8) k_jacobi_pass(PassGeo pg, JacobiCoef jc, double *b0, double *b1,
+                                                     double *b2, const double *__restrict__ b,
                                                      const double *__restrict__ Pin,
                                                      double *__restrict__ Pout, JacobiState *st,
-                                                     int64_t p) {
+                                                     int32_t p) {
   __shared__ double sh[4];
   __shared__ int decide;
   const StopSlot prev = st->slot[(p - 1) & 1];
@@ -549,7 +557,7 @@ __global__ void __launch_bounds__(256) k_jacobi_pass(GridDims g, JacobiCoef jc, 
   StopSlot cur = prev;
   if (!prev.done) {
     if (p >= 3) {
-      const double S = sum_partials(Pin, ts.B, sh);
+      const double S = sum_partials(Pin, (int)gridDim.x, sh);
       if (t0 == 0) decide = (p - 1 <= st->maxit && sqrt(S) <= st->thresh) ? 1 : 0;
       __syncthreads();
       if (decide) {
@@ -565,45 +573,46 @@ __global__ void __launch_bounds__(256) k_jacobi_pass(GridDims g, JacobiCoef jc, 
   if (blockIdx.x == 0 && t0 == 0) st->slot[p & 1] = cur;
   if (cur.done) return;
 
-  double *bufs[3] = {b0, b1, b2};
-  const double *curb = bufs[(p - 1) % 3];          // U_{p-1}
-  double *out = bufs[p % 3];                       // U_p
-  const double *old = p >= 2 ? bufs[(p - 2) % 3] : nullptr;   // U_{p-2}
+  const int r = p % 3;
+  const double *curb = r == 1 ? b0 : (r == 2 ? b1 : b2);           // U_{p-1} in bufs[(p-1)%3]
+  double *out = r == 0 ? b0 : (r == 1 ? b1 : b2);                  // U_p in bufs[p%3]
+  const double *old = p >= 2 ? (r == 2 ? b0 : (r == 0 ? b1 : b2)) : nullptr;   // U_{p-2}
   const bool wrap_cur = (p - 1) >= 1;              // halo of U_{p-1} = wrapped interior
   const bool wrap_old = (p - 2) >= 1;
-  const bool px = !g.flow[0], py = !g.flow[1], pz = !g.flow[2];
-  const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
-  const int nz = g.nz;
+  const bool px = pg.px, py = pg.py, pz = pg.pz;
+  const uint32_t X = pg.X, XY = pg.XY;
+  const int nx = pg.nx, ny = pg.ny, nz = pg.nz;
   // plane index (0-based) of 1-based plane kk of the iterate, honouring the halo rule
-  auto zplane = [&](int kk, bool wrap) -> int64_t {
+  auto zplane = [&](int kk, bool wrap) -> uint32_t {
     if (pz && wrap) {
-      if (kk == 1) return nz;
-      if (kk == nz + 2) return 1;
+      if (kk == 1) return (uint32_t)nz;
+      if (kk == nz + 2) return 1u;
     }
-    return kk - 1;
+    return (uint32_t)(kk - 1);
   };
   double r2 = 0.0;
   // 2.5D march: each thread owns an (i, j) column of its XCD's z-slab and
   // keeps the z-neighbours of U_{p-1} in registers
   const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
   const int z0 = (xcd * nz) >> 3, z1 = ((xcd + 1) * nz) >> 3;
-  for (int t = w; z1 > z0 && t < ts.TP; t += ts.W) {
-    const int tx = t % ts.ntx, ty = t / ts.ntx;
-    const int i = tx * ts.bx + (int)threadIdx.x + 2;
-    const int j = ty * ts.by + (int)threadIdx.y + 2;
-    if (i > g.nx + 1 || j > g.ny + 1) continue;
-    const int64_t col = (int64_t)(i - 1) + X * (int64_t)(j - 1);
-    const int64_t oe = (px && wrap_cur && i == g.nx + 1) ? -(int64_t)(g.nx - 1) : 1;
-    const int64_t ow = (px && wrap_cur && i == 2) ? (int64_t)(g.nx - 1) : -1;
-    const int64_t on = (py && wrap_cur && j == g.ny + 1) ? -(int64_t)(g.ny - 1) * X : X;
-    const int64_t os = (py && wrap_cur && j == 2) ? (int64_t)(g.ny - 1) * X : -X;
+  for (int t = w; z1 > z0 && t < pg.TP; t += pg.W) {
+    const int tx = t % pg.ntx, ty = t / pg.ntx;
+    const int i = tx * (int)blockDim.x + (int)threadIdx.x + 2;
+    const int j = ty * (int)blockDim.y + (int)threadIdx.y + 2;
+    if (i > nx + 1 || j > ny + 1) continue;
+    const uint32_t col = (uint32_t)(i - 1) + X * (uint32_t)(j - 1);
+    const uint32_t oe = (px && wrap_cur && i == nx + 1) ? 0u - (uint32_t)(nx - 1) : 1u;
+    const uint32_t ow = (px && wrap_cur && i == 2) ? (uint32_t)(nx - 1) : 0u - 1u;
+    const uint32_t on = (py && wrap_cur && j == ny + 1) ? 0u - (uint32_t)(ny - 1) * X : X;
+    const uint32_t os = (py && wrap_cur && j == 2) ? (uint32_t)(ny - 1) * X : 0u - X;
+    const bool edge_xy = (px && (i == nx + 1 || i == 2)) || (py && (j == ny + 1 || j == 2));
     int k = z0 + 2;
     double zm = curb[col + XY * zplane(k - 1, wrap_cur)];
-    double zc = curb[col + XY * (k - 1)];
+    double zc = curb[col + XY * (uint32_t)(k - 1)];
     double zp = curb[col + XY * zplane(k + 1, wrap_cur)];
 #pragma unroll 2
     for (; k <= z1 + 1; ++k) {
-      const int64_t c = col + XY * (int64_t)(k - 1);
+      const uint32_t c = col + XY * (uint32_t)(k - 1);
       const double ue = curb[c + oe], uw = curb[c + ow], un = curb[c + on], us = curb[c + os];
       const double ut = zp, ub = zm;
       const double bc = b[c];
@@ -613,12 +622,14 @@ __global__ void __launch_bounds__(256) k_jacobi_pass(GridDims g, JacobiCoef jc, 
         // residual of U_{p-1} with the halo of U_{p-2}: periodic guard
         // neighbours come from `old` (wrapped, or the stored input halo at p-2 = 0)
         double re = ue, rw = uw, rn = un, rs = us, rt = ut, rb = ub;
-        if (px && i == g.nx + 1) re = wrap_old ? old[c - (int64_t)(g.nx - 1)] : old[c + 1];
-        if (px && i == 2) rw = wrap_old ? old[c + (int64_t)(g.nx - 1)] : old[c - 1];
-        if (py && j == g.ny + 1) rn = wrap_old ? old[c - (int64_t)(g.ny - 1) * X] : old[c + X];
-        if (py && j == 2) rs = wrap_old ? old[c + (int64_t)(g.ny - 1) * X] : old[c - X];
-        if (pz && k == nz + 1) rt = wrap_old ? old[c - (int64_t)(nz - 1) * XY] : old[c + XY];
-        if (pz && k == 2) rb = wrap_old ? old[c + (int64_t)(nz - 1) * XY] : old[c - XY];
+        if (edge_xy || (pz && (k == nz + 1 || k == 2))) {
+          if (px && i == nx + 1) re = wrap_old ? old[c - (uint32_t)(nx - 1)] : old[c + 1];
+          if (px && i == 2) rw = wrap_old ? old[c + (uint32_t)(nx - 1)] : old[c - 1];
+          if (py && j == ny + 1) rn = wrap_old ? old[c - (uint32_t)(ny - 1) * X] : old[c + X];
+          if (py && j == 2) rs = wrap_old ? old[c + (uint32_t)(ny - 1) * X] : old[c - X];
+          if (pz && k == nz + 1) rt = wrap_old ? old[c - (uint32_t)(nz - 1) * XY] : old[c + XY];
+          if (pz && k == 2) rb = wrap_old ? old[c + (uint32_t)(nz - 1) * XY] : old[c - XY];
+        }
         const double u2 = 2.0 * zc;
         // CalcEpotential.f90:83-92
         const double res = bc - ((re + rw - u2) / jc.dx2 + (rn + rs - u2) / jc.dy2 +
@@ -828,7 +839,27 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
   const int nbn = grid_blocks(g.G, kT);
   PICARD_TRY(launch_bnorm(c, maxerr));
 
-  const TileSched ts = make_sched(g.nx, g.ny, g.nz);
+  TileSched ts = make_sched(g.nx, g.ny, g.nz);
+  if (c->pass_wmax <= 0) {
+    // blocks per XCD resident at once: the XCD slab schedule in one round
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jacobi_pass, 256, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        per_cu <= 0 || cus < 8)
+      c->pass_wmax = 256;
+    else
+      c->pass_wmax = (cus / 8) * per_cu;
+  }
+  if (ts.W > c->pass_wmax) {
+    ts.W = c->pass_wmax < 4 ? 4 : c->pass_wmax;
+    ts.B = 8 * ts.W;
+  }
+  PassGeo pg;
+  pg.nx = g.nx; pg.ny = g.ny; pg.nz = g.nz;
+  pg.X = (uint32_t)g.X;
+  pg.XY = (uint32_t)g.X * (uint32_t)g.Y;
+  pg.px = !g.flow[0]; pg.py = !g.flow[1]; pg.pz = !g.flow[2];
+  pg.ntx = ts.ntx; pg.TP = ts.TP; pg.W = ts.W;
   const int64_t last = init.last;
   double *P[2] = {c->s.partials, c->s.partials + c->s.partials_len};
   int64_t p = 1;                  // next pass (launch number)
@@ -850,12 +881,12 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
     for (int64_t q = 0; q < todo; ++q, ++p) {
       if (c->timing)
         hipExtLaunchKernelGGL(k_jacobi_pass, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
-                              c->tev[2 * q], c->tev[2 * q + 1], 0, g, jc, ts, c->jb[0], c->jb[1],
+                              c->tev[2 * q], c->tev[2 * q + 1], 0, pg, jc, c->jb[0], c->jb[1],
                               c->jb[2], (const double *)c->b, (const double *)P[(p - 1) & 1],
-                              P[p & 1], c->js, p);
+                              P[p & 1], c->js, (int32_t)p);
       else
         k_jacobi_pass<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(
-            g, jc, ts, c->jb[0], c->jb[1], c->jb[2], c->b, P[(p - 1) & 1], P[p & 1], c->js, p);
+            pg, jc, c->jb[0], c->jb[1], c->jb[2], c->b, P[(p - 1) & 1], P[p & 1], c->js, (int32_t)p);
     }
     PICARD_HIP(hipGetLastError());
     PICARD_HIP(hipMemcpyAsync(c->js_host, c->js, sizeof(JacobiState), hipMemcpyDeviceToHost,

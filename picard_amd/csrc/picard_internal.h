@@ -123,6 +123,8 @@ struct picard_ctx {
   float *r32 = nullptr, *e32[2] = {nullptr, nullptr};
   float *eo32[3] = {nullptr, nullptr, nullptr};   // split: inner iterates j % 3 on periodic boundary planes
   int mx_inner = 64;         // split: FP32 sweeps per FP64 refinement
+  int sweep_wmax = 0;        // split: resident blocks per XCD of the sweep kernel (0: not yet known)
+  int pass_wmax = 0;         // FP64: the same for the Jacobi pass kernel
   int64_t next_id = 0;
 
   // grids

@@ -16,6 +16,8 @@
 // produced in FP32 for the gather; BCpotentials runs in FP64 (grid.hip).
 #include <hip/hip_ext.h>
 
+#include <cmath>
+
 #include "cic.h"
 #include "picard_internal.h"
 #include "reduce.h"
@@ -284,11 +286,24 @@ __device__ __forceinline__ double sum_partials(const double *__restrict__ P, int
 // (CalcEpotential.f90:60-62); every block evaluates the same fixed-order sum,
 // so all reach the same decision without atomics.
 struct MxLaunch {
-  int64_t seq;
-  int64_t base;          // iterate held by u
-  int64_t j;             // inner sweep: e_j -> e_{j+1}
-  int64_t check;         // iterate whose residual Pin holds, or -1
+  int32_t seq;
+  int32_t base;          // iterate held by u
+  int32_t j;             // inner sweep: e_j -> e_{j+1}
+  int32_t check;         // iterate whose residual Pin holds, or -1
   int32_t m;             // inner sweeps of this outer step (update launch)
+};
+
+// Uniform geometry of the sweep kernel, kept small (32-bit, no FP64 box
+// data) so it stays in scalar registers.
+struct SweepGeo {
+  int32_t nx, ny, nz;
+  uint32_t X, XY;        // node strides of j and k
+  int32_t fx, fy, fz;    // flow flags
+  int32_t ntx, TP, W;    // tile schedule (tile width = blockDim.x)
+};
+
+struct SweepCoef {
+  float wx, wy, wz, kappa, af, idx2, idy2, idz2;
 };
 
 __device__ __forceinline__ StopSlot mx_decide(JacobiState *st, const MxLaunch &L,
@@ -370,7 +385,7 @@ __global__ void __launch_bounds__(256) k_mx_residual(GridDims g, MixCoef mc, Til
 // neighbour across a periodic boundary takes e_{j-1}.  e_{j-1} is kept, for
 // the cells of the periodic boundary planes only, in slot (j-1) % 3 of `eo`;
 // this launch leaves e_{j+1} there in slot (j+1) % 3.
-__global__ void __launch_bounds__(256) k_mx_sweep(GridDims g, MixCoef mc, TileSched ts,
+__global__ void __launch_bounds__(256, 8) k_mx_sweep(SweepGeo sg, SweepCoef cf,
                                                    const float *__restrict__ r32, float *E0, float *E1,
                                                    float *eo0, float *eo1, float *eo2,
                                                    const double *__restrict__ Pin,
@@ -378,87 +393,92 @@ __global__ void __launch_bounds__(256) k_mx_sweep(GridDims g, MixCoef mc, TileSc
                                                    MxLaunch L) {
   __shared__ double sh[4];
   __shared__ int decide;
-  StopSlot cur = mx_decide(st, L, Pin, ts.B, sh, &decide);
-  if (!cur.done && L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
+  const int nblk = (int)gridDim.x;
+  StopSlot cur = mx_decide(st, L, Pin, nblk, sh, &decide);
+  if (!cur.done && (int64_t)L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
     cur.done = 1;
     cur.result = st->last;
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   if (cur.done) return;
-  float *E[2] = {E0, E1};
-  float *EO[3] = {eo0, eo1, eo2};
-  const float *e = L.j >= 1 ? E[L.j % 2] : nullptr;
-  float *out = E[(L.j + 1) % 2];
-  float *eo_out = EO[(L.j + 1) % 3];
-  const float *eo_prev = L.j >= 2 ? EO[(L.j - 1) % 3] : nullptr;   // e_0 = 0 has no slot
+  const float *e = L.j >= 1 ? ((L.j & 1) ? E1 : E0) : nullptr;
+  float *out = (L.j & 1) ? E0 : E1;                                // E[(j+1) % 2]
+  const int r1 = (L.j + 1) % 3, rm = (L.j + 2) % 3;                // slots of e_{j+1}, e_{j-1}
+  float *eo_out = r1 == 0 ? eo0 : (r1 == 1 ? eo1 : eo2);
+  const float *eo_prev = L.j >= 2 ? (rm == 0 ? eo0 : (rm == 1 ? eo1 : eo2)) : nullptr;
   const bool res = L.j >= 1;
-  const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
-  const int nz = g.nz;
-  const bool pz = !g.flow[2];
+  const uint32_t X = sg.X, XY = sg.XY;
+  const int nx = sg.nx, ny = sg.ny, nz = sg.nz;
+  const bool pz = !sg.fz;
   double r2 = 0.0;
   // 2.5D march over the XCD's z-slab with the z-neighbours of e_j in registers
   const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
   const int z0 = (xcd * nz) >> 3, z1 = ((xcd + 1) * nz) >> 3;
-  for (int t = w; z1 > z0 && t < ts.TP; t += ts.W) {
-    const int tx = t % ts.ntx, ty = t / ts.ntx;
-    const int i = tx * ts.bx + (int)threadIdx.x + 2;
-    const int j = ty * ts.by + (int)threadIdx.y + 2;
-    if (i > g.nx + 1 || j > g.ny + 1) continue;
-    const int64_t col = (int64_t)(i - 1) + X * (int64_t)(j - 1);
+  for (int t = w; z1 > z0 && t < sg.TP; t += sg.W) {
+    const int tx = t % sg.ntx, ty = t / sg.ntx;
+    const int i = tx * (int)blockDim.x + (int)threadIdx.x + 2;
+    const int j = ty * (int)blockDim.y + (int)threadIdx.y + 2;
+    if (i > nx + 1 || j > ny + 1) continue;
+    const uint32_t col = (uint32_t)(i - 1) + X * (uint32_t)(j - 1);
     // x/y neighbours: periodic wrap, or the zero correction of a flow guard
-    const bool fe = g.flow[0] && i == g.nx + 1, fw = g.flow[0] && i == 2;
-    const bool fn = g.flow[1] && j == g.ny + 1, fs = g.flow[1] && j == 2;
-    const bool we = !g.flow[0] && i == g.nx + 1, ww = !g.flow[0] && i == 2;   // wrapped
-    const bool wn = !g.flow[1] && j == g.ny + 1, ws = !g.flow[1] && j == 2;
+    const bool fe = sg.fx && i == nx + 1, fw = sg.fx && i == 2;
+    const bool fn = sg.fy && j == ny + 1, fs = sg.fy && j == 2;
+    const bool we = !sg.fx && i == nx + 1, ww = !sg.fx && i == 2;   // wrapped
+    const bool wn = !sg.fy && j == ny + 1, ws = !sg.fy && j == 2;
     const bool edge_xy = we || ww || wn || ws;
     if (!e) {
       for (int k = z0 + 2; k <= z1 + 1; ++k) {
-        const int64_t c = col + XY * (int64_t)(k - 1);
-        const float v = -mc.aff * r32[c];
+        const uint32_t c = col + XY * (uint32_t)(k - 1);
+        const float v = -cf.af * r32[c];
         out[c] = v;
         if (edge_xy || (pz && (k == 2 || k == nz + 1))) eo_out[c] = v;
       }
       continue;
     }
-    const int64_t oe = we ? -(int64_t)(g.nx - 1) : 1;
-    const int64_t ow = ww ? (int64_t)(g.nx - 1) : -1;
-    const int64_t on = wn ? -(int64_t)(g.ny - 1) * X : X;
-    const int64_t os = ws ? (int64_t)(g.ny - 1) * X : -X;
+    const uint32_t oe = we ? 0u - (uint32_t)(nx - 1) : 1u;
+    const uint32_t ow = ww ? (uint32_t)(nx - 1) : 0u - 1u;
+    const uint32_t on = wn ? 0u - (uint32_t)(ny - 1) * X : X;
+    const uint32_t os = ws ? (uint32_t)(ny - 1) * X : 0u - X;
     // value of e_j on 1-based plane kk (guards: wrap or zero)
     auto zval = [&](int kk) -> float {
-      if (kk == 1) return g.flow[2] ? 0.0f : e[col + XY * (int64_t)nz];
-      if (kk == nz + 2) return g.flow[2] ? 0.0f : e[col + XY];
-      return e[col + XY * (int64_t)(kk - 1)];
+      if (kk == 1) return sg.fz ? 0.0f : e[col + XY * (uint32_t)nz];
+      if (kk == nz + 2) return sg.fz ? 0.0f : e[col + XY];
+      return e[col + XY * (uint32_t)(kk - 1)];
     };
     // e_{j-1} at a wrapped neighbour (0 for j = 1)
-    auto prev = [&](int64_t n) -> float { return eo_prev ? eo_prev[n] : 0.0f; };
+    auto prev = [&](uint32_t n) -> float { return eo_prev ? eo_prev[n] : 0.0f; };
     int k = z0 + 2;
     float zm = zval(k - 1), zc = zval(k), zp = zval(k + 1);
 #pragma unroll 2
     for (; k <= z1 + 1; ++k) {
-      const int64_t c = col + XY * (int64_t)(k - 1);
+      const uint32_t c = col + XY * (uint32_t)(k - 1);
       const float ee = fe ? 0.0f : e[c + oe];
       const float ew = fw ? 0.0f : e[c + ow];
       const float en = fn ? 0.0f : e[c + on];
       const float es = fs ? 0.0f : e[c + os];
       const float et = zp, eb = zm;
       const float rc = r32[c];
-      const float v = mc.axf * (ee + ew) + mc.ayf * (en + es) + mc.azf * (et + eb) - mc.aff * rc;
+      // CalcEpotential.f90:74-80 on the correction, with FP32 weights that
+      // sum to 1/2 (kappa carries what rounding leaves, zero on uniform
+      // grids): both ends of the Jacobi spectrum, whose modes decay by ~1e-5
+      // per sweep at 256^3, stay where the FP64 iteration has them.
+      const float v = cf.wx * (ee + ew) + cf.wy * (en + es) + cf.wz * (et + eb) + cf.kappa * zc -
+                      cf.af * rc;
       out[c] = v;
+      const float e2 = 2.0f * zc;
+      float rr = rc - ((ee + ew - e2) * cf.idx2 + (en + es - e2) * cf.idy2 + (et + eb - e2) * cf.idz2);
       const bool wt = pz && k == nz + 1, wb = pz && k == 2;
-      float se = ee, sw = ew, sn = en, ss = es, st_ = et, sb = eb;
       if (edge_xy || wt || wb) {
         eo_out[c] = v;
+        float se = ee, sw = ew, sn = en, ss = es, st_ = et, sb = eb;
         if (we) se = prev(c + oe);
         if (ww) sw = prev(c + ow);
         if (wn) sn = prev(c + on);
         if (ws) ss = prev(c + os);
-        if (wt) st_ = prev(c - XY * (int64_t)(nz - 1));
-        if (wb) sb = prev(c + XY * (int64_t)(nz - 1));
+        if (wt) st_ = prev(c - XY * (uint32_t)(nz - 1));
+        if (wb) sb = prev(c + XY * (uint32_t)(nz - 1));
+        rr = rc - ((se + sw - e2) * cf.idx2 + (sn + ss - e2) * cf.idy2 + (st_ + sb - e2) * cf.idz2);
       }
-      const float e2 = 2.0f * zc;
-      const float rr = rc - ((se + sw - e2) * mc.idx2f + (sn + ss - e2) * mc.idy2f +
-                             (st_ + sb - e2) * mc.idz2f);
       r2 = r2 + (double)rr * (double)rr;
       zm = zc;
       zc = zp;
@@ -728,6 +748,29 @@ int sp_gridify(picard_ctx *c, bool flux) {
   return PICARD_OK;
 }
 
+// The persistent schedule of the solve's kernels with as many blocks per XCD
+// as are resident at once for the sweep kernel (the heaviest), so that the
+// XCD slab assignment is not split over two dispatch rounds.
+TileSched stencil_sched(picard_ctx *c) {
+  const GridDims &g = c->gd;
+  TileSched ts = make_sched(g.nx, g.ny, g.nz);
+  if (c->sweep_wmax <= 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_sweep, 256, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        per_cu <= 0 || cus < 8) {
+      c->sweep_wmax = 256;
+    } else {
+      c->sweep_wmax = (cus / 8) * per_cu;
+    }
+  }
+  if (ts.W > c->sweep_wmax) {
+    ts.W = c->sweep_wmax < 4 ? 4 : c->sweep_wmax;
+    ts.B = 8 * ts.W;
+  }
+  return ts;
+}
+
 int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) {
   const GridDims &g = c->gd;
   const double dx = g.dxyz[0], dy = g.dxyz[1], dz = g.dxyz[2];
@@ -747,14 +790,38 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
   init.last = 2 * (int64_t)maxit + 1;
   PICARD_HIP(hipMemcpyAsync(c->js, &init, sizeof(JacobiState), hipMemcpyHostToDevice, c->stream));
   PICARD_TRY(launch_bnorm(c, maxerr));      // launch 0: b, |b|, the test of ss = 1
-  const TileSched ts = make_sched(g.nx, g.ny, g.nz);
+  const TileSched ts = stencil_sched(c);
+  SweepGeo sg;
+  sg.nx = g.nx; sg.ny = g.ny; sg.nz = g.nz;
+  sg.X = (uint32_t)g.X;
+  sg.XY = (uint32_t)g.X * (uint32_t)g.Y;
+  sg.fx = g.flow[0]; sg.fy = g.flow[1]; sg.fz = g.flow[2];
+  sg.ntx = ts.ntx; sg.TP = ts.TP; sg.W = ts.W;
+  SweepCoef cf{};
+  {
+    // FP32 weights with 2 (wx + wy + wz) = 1 as closely as FP32 allows
+    cf.wx = (float)mc.ax;
+    cf.wy = (float)mc.ay;
+    float best = (float)(0.5 - (double)cf.wx - (double)cf.wy);
+    for (float cand : {best, nextafterf(best, 1.0f), nextafterf(best, 0.0f)}) {
+      const double err = std::fabs(0.5 - ((double)cf.wx + (double)cf.wy + (double)cand));
+      const double cur = std::fabs(0.5 - ((double)cf.wx + (double)cf.wy + (double)best));
+      if (err < cur) best = cand;
+    }
+    cf.wz = best;
+    cf.kappa = (float)(1.0 - 2.0 * ((double)cf.wx + (double)cf.wy + (double)cf.wz));
+    cf.af = mc.aff;
+    cf.idx2 = mc.idx2f;
+    cf.idy2 = mc.idy2f;
+    cf.idz2 = mc.idz2f;
+  }
   const int m = c->mx_inner;
   double *P[2] = {c->s.partials, c->s.partials + c->s.partials_len};
   float *E0 = c->e32[0], *E1 = c->e32[1];
   float *const *EO = c->eo32;
-  int64_t seq = 1, base = 0;
+  int32_t seq = 1, base = 0;
   int pw = 0;                               // partials buffer the next writer uses
-  int64_t check = -1;                       // iterate whose residual the last writer stored
+  int32_t check = -1;                       // iterate whose residual the last writer stored
   StopSlot fin{};
   while (true) {
     if (c->timing) {
@@ -778,17 +845,17 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
     }
     for (int j = 0; j < m; ++j) {
       // the iterate-0 test used b itself (launch 0); later ones use residuals
-      const int64_t chk = (check >= 1) ? check : -1;
+      const int32_t chk = (check >= 1) ? check : -1;
       MxLaunch L{seq, base, j, chk, m};
       const double *pin = P[pw ^ 1];
       double *pout = P[pw];
       if (c->timing)
         hipExtLaunchKernelGGL(k_mx_sweep, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
-                              c->tev[2 * j], c->tev[2 * j + 1], 0, g, mc, ts,
+                              c->tev[2 * j], c->tev[2 * j + 1], 0, sg, cf,
                               (const float *)c->r32, E0, E1, EO[0], EO[1], EO[2], pin, pout, c->js,
                               L);
       else
-        k_mx_sweep<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, mc, ts, c->r32, E0, E1, EO[0],
+        k_mx_sweep<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(sg, cf, c->r32, E0, E1, EO[0],
                                                                EO[1], EO[2], pin, pout, c->js, L);
       ++seq;
       if (j >= 1) {               // wrote the residual of iterate base + j
@@ -799,7 +866,7 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       }
     }
     {
-      const int64_t chk = (check >= 1) ? check : -1;
+      const int32_t chk = (check >= 1) ? check : -1;
       MxLaunch L{seq, base, 0, chk, m};
       k_mx_update<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, ts, c->U, E0, E1, P[pw ^ 1],
                                                               c->js, L);
