@@ -378,6 +378,88 @@ __global__ void __launch_bounds__(64 * kFaceWaves) k_bcpot_x(GridDims g, const d
   }
 }
 
+// x faces with the charge rows staged in LDS: a block's waves share one range
+// of 64 face nodes along y (lanes) and take R face nodes each along z; at
+// wave step t lane l needs charge row j = t + l of plane k, so the block keeps
+// rows t .. t+63 in a ring (row j in slot j % 64, stride Nx + 1 doubles to
+// spread the banks) and loads one new row per step.  Each charge value is
+// read from HBM once per block instead of once per lane; the summation order
+// of every lane is that of k_bcpot_x.
+template <int R>
+__global__ void __launch_bounds__(256) k_bcpot_x_lds(GridDims g, const double *__restrict__ C,
+                                                     const double *__restrict__ D,
+                                                     double *__restrict__ U, double kfac) {
+  extern __shared__ double ring[];
+  const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
+  const int RS = Nx + 1;
+  const int lane = threadIdx.x & 63;
+  const int nw = blockDim.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform per wave
+  const int face = blockIdx.z;
+  const int a0 = blockIdx.x * 64 + 1;                                  // jj of lane 0
+  const int b0 = (blockIdx.y * nw + wave) * R + 1;                     // kk of r = 0
+  const bool wave_on = b0 <= Nz;                                       // still loads rows
+  const int La = Ny - a0 + 1 < 64 ? Ny - a0 + 1 : 64;
+  const bool active = wave_on && lane < La;
+  const int64_t DX = 2 * (int64_t)Nx + 1, DY = 2 * (int64_t)Ny + 1;
+  const int64_t X = g.X;
+  const int di = face == 0 ? Nx + 2 : 1;                               // BCpotentials.f90:60/63
+  double s[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) s[r] = 0.0;
+  for (int k = 1; k <= Nz; ++k) {
+    const double *cplane = C + node_index(g, 2, 2, k + 1);            // row j at (j-1)*X
+    for (int t = 1 - (La - 1); t <= Ny; ++t) {
+      __syncthreads();                                                 // step t-1 done
+      const int jn = t + La - 1;                                       // row first needed now
+      if (jn >= 1 && jn <= Ny) {
+        const double *src = cplane + (int64_t)(jn - 1) * X;
+        double *dst = ring + (jn & 63) * RS;
+        for (int x = threadIdx.x; x < Nx; x += blockDim.x) dst[x] = src[x];
+      }
+      __syncthreads();
+      const int j = t + lane;
+      if (active && j >= 1 && j <= Ny) {
+        const int64_t ydz = (int64_t)(Ny + t - a0) * DX;               // y index Ny+1+j-jj, 0-based
+        const double *drow[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int bb = b0 + r <= Nz ? b0 + r : Nz;
+          drow[r] = D + (int64_t)(di - 1) + ydz + DX * DY * (int64_t)(Nz + k - bb);
+        }
+        const double *row = ring + (j & 63) * RS;
+        // D in blocks of 8 consecutive values (one wide scalar load per row)
+        int i = 0;
+        for (; i + 8 <= Nx; i += 8) {
+          double d[R][8], cv[8];
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) d[r][q] = drow[r][i + q];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) cv[q] = row[i + q];
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[r] = s[r] + cv[q] * d[r][q];
+        }
+        for (; i < Nx; ++i) {
+          const double c = row[i];
+#pragma unroll
+          for (int r = 0; r < R; ++r) s[r] = s[r] + c * drow[r][i];
+        }
+      }
+    }
+  }
+  if (!active) return;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int bb = b0 + r;
+    if (bb > Nz) break;
+    U[node_index(g, face == 0 ? 1 : Nx + 2, a0 + lane + 1, bb + 1)] = s[r] * kfac;
+  }
+}
+
 // y faces (dim 1: lanes along x, b = z) and z faces (dim 2: lanes along x,
 // b = y); skew on the innermost loop i, so the charge reads coalesce as is.
 __global__ void __launch_bounds__(64 * kFaceWaves) k_bcpot_yz(GridDims g, int dim,
@@ -786,7 +868,22 @@ int launch_bc_potentials(picard_ctx *c) {
   const double eps0 = 8.854187817e-12;
   const double kfac = 1.0 / (2.0 * tau * eps0);
   const int per_block = kFaceR * kFaceWaves;
-  if (g.flow[0]) {
+  constexpr int kLdsR = 2, kLdsWaves = 4;
+  const size_t ring_bytes = sizeof(double) * 64 * (size_t)(g.nx + 1);
+  bool lds_ok = g.flow[0] && ring_bytes <= 160 * 1024;
+  if (lds_ok && !c->bcpot_lds_set) {
+    lds_ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bcpot_x_lds<kLdsR>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)ring_bytes) == hipSuccess;
+    (void)hipGetLastError();
+    c->bcpot_lds_set = lds_ok;
+  }
+  if (lds_ok) {
+    const int per_block = kLdsR * kLdsWaves;
+    k_bcpot_x_lds<kLdsR><<<dim3((g.ny + 63) / 64, (g.nz + per_block - 1) / per_block, 2),
+                            64 * kLdsWaves, ring_bytes, c->stream>>>(g, c->C, c->D, c->U, kfac);
+    PICARD_HIP(hipGetLastError());
+  } else if (g.flow[0]) {
     // the third Jacobi buffer is free until CalcEpotential copies U into it
     double *Ct = c->jb[2];
     k_charge_yfirst<<<dim3((g.nx + 31) / 32, (g.ny + 31) / 32, g.nz), 256, 0, c->stream>>>(g, c->C,

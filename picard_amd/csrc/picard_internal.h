@@ -125,6 +125,7 @@ struct picard_ctx {
   int mx_inner = 64;         // split: FP32 sweeps per FP64 refinement
   int sweep_wmax = 0;        // split: resident blocks per XCD of the sweep kernel (0: not yet known)
   int pass_wmax = 0;         // FP64: the same for the Jacobi pass kernel
+  bool bcpot_lds_set = false;  // FP64: the LDS face-sum kernel's LDS limit has been raised
   int64_t next_id = 0;
 
   // grids
