@@ -212,7 +212,9 @@ int create_impl(const picard_geometry *geo, const picard_species *sp, picard_ctx
     PICARD_TRY(dalloc(&c->s.scan, cap + 1));
     for (int q = 0; q < 6; ++q) PICARD_TRY(dalloc(&c->s.work[q], cap + 1));
   }
-  const int64_t nb = ((int64_t)g.S * c->K > c->K + 1 ? (int64_t)g.S * c->K : c->K + 1);
+  // buckets: (species, cell) keys of both deposits and of the split form's
+  // sort, plus the sort's out-of-grid bucket
+  const int64_t nb = (int64_t)g.S * c->K + 1;
   const int64_t scan_n = cap > nb ? cap : nb;
   c->s.scan_tmp_len = scan_tmp_required(scan_n);
   PICARD_TRY(dalloc(&c->s.scan_tmp, c->s.scan_tmp_len));

@@ -16,6 +16,10 @@ namespace picard {
 namespace {
 
 constexpr int kT = 256;
+// threads per block of the persistent stencil kernels (4 waves: a 64 x 4
+// tile; eight blocks per CU at 8 waves per SIMD; 1024-thread blocks measured
+// slower)
+constexpr int kStencilT = 256;
 
 // ---------------------------------------------------------------------------
 // ParticlesGridify (ParticlesGridify.f90:56-293)
@@ -612,7 +616,8 @@ __global__ void __launch_bounds__(kT) k_jacobi_bnorm(GridDims g, const double *_
 // fixed order (no atomics, no fences: the kernel boundary orders the data).
 __device__ __forceinline__ double sum_partials(const double *__restrict__ P, int n, double *sh) {
   double v = 0.0;
-  for (int q = lin_tid(); q < n; q += 256) v = v + P[q];
+  const int nt = (int)(blockDim.x * blockDim.y * blockDim.z);
+  for (int q = lin_tid(); q < n; q += nt) v = v + P[q];
   return block_sum(v, sh);
 }
 
@@ -625,12 +630,12 @@ struct PassGeo {
   int32_t ntx, TP, W;    // tile schedule (tile width = blockDim.x)
 };
 
-__global__ void __launch_bounds__(256, 8) k_jacobi_pass(PassGeo pg, JacobiCoef jc, double *b0, double *b1,
+__global__ void __launch_bounds__(kStencilT, 8) k_jacobi_pass(PassGeo pg, JacobiCoef jc, double *b0, double *b1,
                                                      double *b2, const double *__restrict__ b,
                                                      const double *__restrict__ Pin,
                                                      double *__restrict__ Pout, JacobiState *st,
                                                      int32_t p) {
-  __shared__ double sh[4];
+  __shared__ double sh[kStencilT / 64];
   __shared__ int decide;
   const StopSlot prev = st->slot[(p - 1) & 1];
   const int t0 = lin_tid();
@@ -936,11 +941,11 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
   const int nbn = grid_blocks(g.G, kT);
   PICARD_TRY(launch_bnorm(c, maxerr));
 
-  TileSched ts = make_sched(g.nx, g.ny, g.nz);
+  TileSched ts = make_sched(g.nx, g.ny, g.nz, kStencilT);
   if (c->pass_wmax <= 0) {
     // blocks per XCD resident at once: the XCD slab schedule in one round
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jacobi_pass, 256, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jacobi_pass, kStencilT, 0) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
         per_cu <= 0 || cus < 8)
       c->pass_wmax = 256;

@@ -12,8 +12,9 @@ __device__ __forceinline__ unsigned lin_bid() {
 }
 __device__ __forceinline__ unsigned n_blocks() { return gridDim.x * gridDim.y * gridDim.z; }
 
-// Sum over the block (at most 256 threads): wave64 shuffle tree, then the
-// waves in order.  Result in thread 0; every thread must call it.
+// Sum over the block (at most 1024 threads; `sh` holds one double per
+// wave): wave64 shuffle tree, then the waves in order.  Result in thread 0;
+// every thread must call it.
 __device__ __forceinline__ double block_sum(double v, double *sh) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   const int t = lin_tid();

@@ -18,18 +18,18 @@
 namespace picard {
 
 struct TileSched {
-  int bx, by;        // tile shape (threads per block = bx * by = 256)
+  int bx, by;        // tile shape (threads per block = bx * by)
   int ntx, nty;      // tiles per plane along x, y
   int TP;            // tiles per plane
   int W;             // blocks per XCD
   int B;             // blocks in the launch (8 * W)
 };
 
-inline TileSched make_sched(int nx, int ny, int nz) {
+inline TileSched make_sched(int nx, int ny, int nz, int threads = 256) {
   TileSched s;
   s.bx = 64;
   while (s.bx > 8 && s.bx / 2 >= nx) s.bx /= 2;
-  s.by = 256 / s.bx;
+  s.by = threads / s.bx;
   s.ntx = (nx + s.bx - 1) / s.bx;
   s.nty = (ny + s.by - 1) / s.by;
   s.TP = s.ntx * s.nty;

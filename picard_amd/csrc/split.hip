@@ -27,6 +27,10 @@ namespace picard {
 namespace {
 
 constexpr int kT = 256;
+// threads per block of the persistent stencil kernels (4 waves: a 64 x 4
+// tile; eight blocks per CU at 8 waves per SIMD; 1024-thread blocks measured
+// slower)
+constexpr int kStencilT = 256;
 constexpr double kFix = 1099511627776.0;       // 2^40
 constexpr double kFixInv = 1.0 / 1099511627776.0;
 
@@ -90,7 +94,7 @@ __device__ __forceinline__ void sp_wrap(const GridDims &g, int c[3], const float
   }
 }
 
-// Sort key of a kept particle (its lower node), -1 for a deleted one.
+// Sort key of a kept particle (its lower node and species), -1 for a deleted one.
 __global__ void __launch_bounds__(kT) k_sp_flag(ParticlesSplit p, int64_t n, GridDims g,
                                                  int wrap, int32_t *__restrict__ key,
                                                  int32_t *__restrict__ hist) {
@@ -109,7 +113,10 @@ __global__ void __launch_bounds__(kT) k_sp_flag(ParticlesSplit p, int64_t n, Gri
     p.c[2][q] = (int16_t)c[2];
   }
   const bool in = c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] <= g.nx && c[1] <= g.ny && c[2] <= g.nz;
-  const int32_t k = in ? (int32_t)lin_key(g, c[0], c[1], c[2]) : (int32_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  // key (cell, species): the deposit then sees runs of one cell and species
+  const int32_t cells = (int32_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  const int32_t k = in ? (int32_t)lin_key(g, c[0], c[1], c[2]) * g.S + ((int32_t)p.sp[q] - 1)
+                       : cells * g.S;
   key[q] = k;
   atomicAdd(&hist[k], 1);
 }
@@ -139,41 +146,76 @@ __global__ void __launch_bounds__(kT) k_sp_scatter(ParticlesSplit src, Particles
 constexpr double kFluxFix = 16777216.0;         // 2^24
 constexpr double kFluxFixInv = 1.0 / 16777216.0;
 
+// Particles are sorted by (cell, species), so a thread walks kDepChunk
+// consecutive particles and adds the fixed-point weights of each run of one
+// cell and species in registers, with one atomic per corner when the run
+// ends: integer sums, so the result equals one atomic per particle.
+constexpr int kDepChunk = 16;
+
 template <bool FLUX>
 __global__ void __launch_bounds__(kT) k_sp_deposit(ParticlesSplit p, int64_t n, GridDims g,
                                                     unsigned long long *__restrict__ Nfix,
                                                     unsigned long long *__restrict__ Ffix,
                                                     int32_t *__restrict__ errors) {
-  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (q >= n) return;
-  const int i = p.c[0][q], j = p.c[1][q], k = p.c[2][q];
-  if (i < 0 || j < 0 || k < 0 || i > g.nx || j > g.ny || k > g.nz) {
-    atomicAdd(errors, 1);
-    return;
-  }
-  const float fx = p.f[0][q], fy = p.f[1][q], fz = p.f[2][q];
-  const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
-  const int s = p.sp[q] - 1;
+  const int64_t q0 = ((int64_t)blockIdx.x * kT + threadIdx.x) * kDepChunk;
+  if (q0 >= n) return;
+  const int64_t q1 = q0 + kDepChunk < n ? q0 + kDepChunk : n;
   const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
-  const int64_t base = (int64_t)i + X * j + XY * k;
+  int64_t cur_base = -1;
+  int cur_s = -1;
+  unsigned long long W[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long Wf[FLUX ? 24 : 1] = {};
+  auto flush = [&]() {
+    if (cur_base < 0) return;
 #pragma unroll
-  for (int c = 0; c < 2; ++c)
+    for (int o = 0; o < 8; ++o) {
+      const int64_t node = cur_base + (o & 1) + X * ((o >> 1) & 1) + XY * (o >> 2);
+      atomicAdd(&Nfix[node * g.S + cur_s], W[o]);
+      W[o] = 0;
+      if (FLUX) {
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const float w = wx[a] * wy[b] * wz[c];
-        const unsigned long long W = (unsigned long long)(double(w) * kFix);
-        const int64_t node = base + a + X * b + XY * c;
-        atomicAdd(&Nfix[node * g.S + s], W);
-        if (FLUX) {
-#pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            const long long Wf = llrint((double)w * (double)p.v[d][q] * kFluxFix);
-            atomicAdd(&Ffix[(node * 3 + d) * g.S + s], (unsigned long long)Wf);
-          }
+        for (int d = 0; d < 3; ++d) {
+          atomicAdd(&Ffix[(node * 3 + d) * g.S + cur_s], Wf[3 * o + d]);
+          Wf[3 * o + d] = 0;
         }
       }
+    }
+  };
+  int nerr = 0;
+  for (int64_t q = q0; q < q1; ++q) {
+    const int i = p.c[0][q], j = p.c[1][q], k = p.c[2][q];
+    if (i < 0 || j < 0 || k < 0 || i > g.nx || j > g.ny || k > g.nz) {
+      ++nerr;
+      continue;
+    }
+    const int s = p.sp[q] - 1;
+    const int64_t base = (int64_t)i + X * j + XY * k;
+    if (base != cur_base || s != cur_s) {
+      flush();
+      cur_base = base;
+      cur_s = s;
+    }
+    const float fx = p.f[0][q], fy = p.f[1][q], fz = p.f[2][q];
+    const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+    float v[3] = {0.0f, 0.0f, 0.0f};
+    if (FLUX) {
+      v[0] = p.v[0][q];
+      v[1] = p.v[1][q];
+      v[2] = p.v[2][q];
+    }
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const float w = wx[o & 1] * wy[(o >> 1) & 1] * wz[o >> 2];
+      W[o] += (unsigned long long)(double(w) * kFix);
+      if (FLUX) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          Wf[3 * o + d] += (unsigned long long)llrint((double)w * (double)v[d] * kFluxFix);
+      }
+    }
+  }
+  flush();
+  if (nerr) atomicAdd(errors, nerr);
 }
 
 // Integer guard folds (ParticlesGridify.f90:129-285, z, y, x) and
@@ -275,7 +317,8 @@ __device__ __forceinline__ Nb neighbours(const GridDims &g, int i, int j, int k,
 
 __device__ __forceinline__ double sum_partials(const double *__restrict__ P, int n, double *sh) {
   double v = 0.0;
-  for (int q = lin_tid(); q < n; q += 256) v = v + P[q];
+  const int nt = (int)(blockDim.x * blockDim.y * blockDim.z);
+  for (int q = lin_tid(); q < n; q += nt) v = v + P[q];
   return block_sum(v, sh);
 }
 
@@ -330,7 +373,7 @@ __device__ __forceinline__ StopSlot mx_decide(JacobiState *st, const MxLaunch &L
 // neighbour across a periodic boundary takes U_{base-1} = u - (e_m - e_{m-1}):
 // em and eo hold e_m and e_{m-1} of the previous outer step (null when base
 // is 0, whose test is the |b| test of the first loop iteration).
-__global__ void __launch_bounds__(256) k_mx_residual(GridDims g, MixCoef mc, TileSched ts,
+__global__ void __launch_bounds__(kStencilT) k_mx_residual(GridDims g, MixCoef mc, TileSched ts,
                                                       const double *__restrict__ u,
                                                       const double *__restrict__ b,
                                                       float *__restrict__ r32,
@@ -339,7 +382,7 @@ __global__ void __launch_bounds__(256) k_mx_residual(GridDims g, MixCoef mc, Til
                                                       const double *__restrict__ Pin,
                                                       double *__restrict__ Pout, JacobiState *st,
                                                       MxLaunch L) {
-  __shared__ double sh[4];
+  __shared__ double sh[kStencilT / 64];
   __shared__ int decide;
   const StopSlot cur = mx_decide(st, L, Pin, ts.B, sh, &decide);
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
@@ -385,13 +428,13 @@ __global__ void __launch_bounds__(256) k_mx_residual(GridDims g, MixCoef mc, Til
 // neighbour across a periodic boundary takes e_{j-1}.  e_{j-1} is kept, for
 // the cells of the periodic boundary planes only, in slot (j-1) % 3 of `eo`;
 // this launch leaves e_{j+1} there in slot (j+1) % 3.
-__global__ void __launch_bounds__(256, 8) k_mx_sweep(SweepGeo sg, SweepCoef cf,
+__global__ void __launch_bounds__(kStencilT, 8) k_mx_sweep(SweepGeo sg, SweepCoef cf,
                                                    const float *__restrict__ r32, float *E0, float *E1,
                                                    float *eo0, float *eo1, float *eo2,
                                                    const double *__restrict__ Pin,
                                                    double *__restrict__ Pout, JacobiState *st,
                                                    MxLaunch L) {
-  __shared__ double sh[4];
+  __shared__ double sh[kStencilT / 64];
   __shared__ int decide;
   const int nblk = (int)gridDim.x;
   StopSlot cur = mx_decide(st, L, Pin, nblk, sh, &decide);
@@ -491,11 +534,11 @@ __global__ void __launch_bounds__(256, 8) k_mx_sweep(SweepGeo sg, SweepCoef cf,
 
 // u += e_k: k = m after a full outer step, or the converged (or last)
 // iterate's k when the solve ended inside this step.
-__global__ void __launch_bounds__(256) k_mx_update(GridDims g, TileSched ts, double *__restrict__ u,
+__global__ void __launch_bounds__(kStencilT) k_mx_update(GridDims g, TileSched ts, double *__restrict__ u,
                                                     const float *E0, const float *E1,
                                                     const double *__restrict__ Pin,
                                                     JacobiState *st, MxLaunch L) {
-  __shared__ double sh[4];
+  __shared__ double sh[kStencilT / 64];
   __shared__ int decide;
   const StopSlot prev = st->slot[(L.seq - 1) & 1];
   StopSlot cur = mx_decide(st, L, Pin, ts.B, sh, &decide);
@@ -685,7 +728,7 @@ __global__ void __launch_bounds__(kT) k_sp_to_aos(ParticlesSplit p, int64_t firs
 int sp_sort(picard_ctx *c, int wrap) {
   const GridDims &g = c->gd;
   const int64_t n = c->n;
-  const int64_t K = c->K;                 // cells; bucket K collects out-of-grid keepers
+  const int64_t K = c->K * g.S;           // (cell, species) keys; bucket K: out-of-grid keepers
   PICARD_HIP(hipMemsetAsync(c->s.hist, 0, sizeof(int32_t) * (K + 2), c->stream));
   PICARD_HIP(hipMemsetAsync(c->s.cursor, 0, sizeof(int32_t) * (K + 1), c->stream));
   if (n > 0) {
@@ -727,11 +770,11 @@ int sp_gridify(picard_ctx *c, bool flux) {
   }
   if (c->n > 0) {
     if (flux)
-      k_sp_deposit<true><<<grid_blocks(c->n, kT), kT, 0, c->stream>>>(c->ps, c->n, g, c->Nfix,
-                                                                      c->Ffix, c->s.counters);
+      k_sp_deposit<true><<<grid_blocks((c->n + kDepChunk - 1) / kDepChunk, kT), kT, 0,
+                           c->stream>>>(c->ps, c->n, g, c->Nfix, c->Ffix, c->s.counters);
     else
-      k_sp_deposit<false><<<grid_blocks(c->n, kT), kT, 0, c->stream>>>(c->ps, c->n, g, c->Nfix,
-                                                                       nullptr, c->s.counters);
+      k_sp_deposit<false><<<grid_blocks((c->n + kDepChunk - 1) / kDepChunk, kT), kT, 0,
+                            c->stream>>>(c->ps, c->n, g, c->Nfix, nullptr, c->s.counters);
     PICARD_HIP(hipGetLastError());
   }
   k_sp_fold<<<grid_blocks(g.G, kT), kT, 0, c->stream>>>(g, c->Nfix, c->N, c->C, c->consts);
@@ -753,10 +796,10 @@ int sp_gridify(picard_ctx *c, bool flux) {
 // XCD slab assignment is not split over two dispatch rounds.
 TileSched stencil_sched(picard_ctx *c) {
   const GridDims &g = c->gd;
-  TileSched ts = make_sched(g.nx, g.ny, g.nz);
+  TileSched ts = make_sched(g.nx, g.ny, g.nz, kStencilT);
   if (c->sweep_wmax <= 0) {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_sweep, 256, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_sweep, kStencilT, 0) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
         per_cu <= 0 || cus < 8) {
       c->sweep_wmax = 256;
