@@ -120,8 +120,10 @@ struct picard_ctx {
   picard::ParticlesSplit ps{}, ps2{};
   unsigned long long *Nfix = nullptr;   // split: fixed-point densities
   float *E32 = nullptr;      // split: E as float4 per node
+  // split solve, compact interior layout (rows padded to 4 cells): right-hand
+  // side, ping-pong correction, inner iterates j % 3 on periodic boundary planes
   float *r32 = nullptr, *e32[2] = {nullptr, nullptr};
-  float *eo32[3] = {nullptr, nullptr, nullptr};   // split: inner iterates j % 3 on periodic boundary planes
+  float *eo32[3] = {nullptr, nullptr, nullptr};
   int mx_inner = 64;         // split: FP32 sweeps per FP64 refinement
   int sweep_wmax = 0;        // split: resident blocks per XCD of the sweep kernel (0: not yet known)
   int pass_wmax = 0;         // FP64: the same for the Jacobi pass kernel

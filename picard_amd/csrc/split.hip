@@ -324,38 +324,59 @@ __device__ __forceinline__ double sum_partials(const double *__restrict__ P, int
 
 // One launch of the mixed-precision solve.  Launch number `seq` reads the
 // stop state of launch seq-1 and writes its own (block 0).  If `check` >= 1,
-// the partials `Pin` of the previous launch hold the residual of iterate
-// `check`, and the stop test of loop iteration check+1 is applied to it
-// (CalcEpotential.f90:60-62); every block evaluates the same fixed-order sum,
-// so all reach the same decision without atomics.
+// the `nin` partials `Pin` of the previous launch hold the residual of
+// iterate `check`, and the stop test of loop iteration check+1 is applied to
+// it (CalcEpotential.f90:60-62); every block evaluates the same fixed-order
+// sum, so all reach the same decision without atomics.
 struct MxLaunch {
   int32_t seq;
   int32_t base;          // iterate held by u
   int32_t j;             // inner sweep: e_j -> e_{j+1}
   int32_t check;         // iterate whose residual Pin holds, or -1
   int32_t m;             // inner sweeps of this outer step (update launch)
+  int32_t nin;           // partials in Pin
 };
+
+// Compact FP32 layout of the correction, its right-hand side and the
+// boundary-plane copies: interior cells only, x fastest, rows padded with
+// zeros to a multiple of 4 cells so that a lane moves 4 cells with one
+// 16-byte access.  Cell (x, y, z), 0-based, sits at x + NX y + P z.
+struct CompactDims {
+  uint32_t NX, P;        // row stride (nx rounded up to 4), plane stride NX * ny
+};
+
+// compact index of interior node (i, j, k), 1-based (2..n+1)
+__device__ __forceinline__ uint32_t compact_index(const CompactDims &cd, int i, int j, int k) {
+  return (uint32_t)(i - 2) + cd.NX * (uint32_t)(j - 2) + cd.P * (uint32_t)(k - 2);
+}
+
+constexpr int kSwRows = kStencilT / 64;   // rows of a sweep tile, one wave each
+constexpr int kSwSeg = 256;               // cells of a row segment: 64 lanes x 4
 
 // Uniform geometry of the sweep kernel, kept small (32-bit, no FP64 box
 // data) so it stays in scalar registers.
 struct SweepGeo {
   int32_t nx, ny, nz;
-  uint32_t X, XY;        // node strides of j and k
-  int32_t fx, fy, fz;    // flow flags
-  int32_t ntx, TP, W;    // tile schedule (tile width = blockDim.x)
+  uint32_t NX, P;        // compact strides
+  int32_t px, py, pz;    // periodic flags
+  int32_t ntx, nty, nzc; // tiles: row segments, row groups, z chunks of an XCD slab
+  int32_t TP, W;         // tiles per XCD slab, blocks per XCD
 };
 
 struct SweepCoef {
   float wx, wy, wz, kappa, af, idx2, idy2, idz2;
 };
 
+__device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+__device__ __forceinline__ void st4(float *p, float4 v) { *reinterpret_cast<float4 *>(p) = v; }
+
 __device__ __forceinline__ StopSlot mx_decide(JacobiState *st, const MxLaunch &L,
-                                              const double *__restrict__ Pin, int nblk, double *sh,
+                                              const double *__restrict__ Pin, double *sh,
                                               int *decide) {
   const StopSlot prev = st->slot[(L.seq - 1) & 1];
   StopSlot cur = prev;
   if (!prev.done && L.check >= 1) {
-    const double S = sum_partials(Pin, nblk, sh);
+    const double S = sum_partials(Pin, L.nin, sh);
     if (lin_tid() == 0) *decide = (L.check + 1 <= st->maxit && sqrt(S) <= st->thresh) ? 1 : 0;
     __syncthreads();
     if (*decide) {
@@ -373,7 +394,8 @@ __device__ __forceinline__ StopSlot mx_decide(JacobiState *st, const MxLaunch &L
 // neighbour across a periodic boundary takes U_{base-1} = u - (e_m - e_{m-1}):
 // em and eo hold e_m and e_{m-1} of the previous outer step (null when base
 // is 0, whose test is the |b| test of the first loop iteration).
-__global__ void __launch_bounds__(kStencilT) k_mx_residual(GridDims g, MixCoef mc, TileSched ts,
+__global__ void __launch_bounds__(kStencilT) k_mx_residual(GridDims g, CompactDims cd, MixCoef mc,
+                                                      TileSched ts,
                                                       const double *__restrict__ u,
                                                       const double *__restrict__ b,
                                                       float *__restrict__ r32,
@@ -384,7 +406,7 @@ __global__ void __launch_bounds__(kStencilT) k_mx_residual(GridDims g, MixCoef m
                                                       MxLaunch L) {
   __shared__ double sh[kStencilT / 64];
   __shared__ int decide;
-  const StopSlot cur = mx_decide(st, L, Pin, ts.B, sh, &decide);
+  const StopSlot cur = mx_decide(st, L, Pin, sh, &decide);
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   if (cur.done) return;
   const bool stale = em != nullptr;
@@ -392,22 +414,24 @@ __global__ void __launch_bounds__(kStencilT) k_mx_residual(GridDims g, MixCoef m
   for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int k, bool valid) {
     if (!valid) return;
     const int64_t c = node_index(g, i, j, k);
+    const uint32_t cc = compact_index(cd, i, j, k);
     const Nb nb = neighbours(g, i, j, k, c);
     const double u2 = 2.0 * u[c];
     double ue = u[nb.e], uw = u[nb.w], un = u[nb.n], us = u[nb.s], ut = u[nb.t], ub = u[nb.b];
     const double r = b[c] - ((ue + uw - u2) / mc.dx2 + (un + us - u2) / mc.dy2 +
                              (ut + ub - u2) / mc.dz2);
-    r32[c] = (float)r;
+    r32[cc] = (float)r;
     double rs = r;
     if (stale) {
-      auto prev = [&](int64_t n) { return u[n] - ((double)em[n] - (double)eo[n]); };
+      // n: the wrapped neighbour's node, nc: its compact index
+      auto prev = [&](int64_t n, uint32_t nc) { return u[n] - ((double)em[nc] - (double)eo[nc]); };
       bool any = false;
-      if (!g.flow[0] && i == g.nx + 1) { ue = prev(nb.e); any = true; }
-      if (!g.flow[0] && i == 2) { uw = prev(nb.w); any = true; }
-      if (!g.flow[1] && j == g.ny + 1) { un = prev(nb.n); any = true; }
-      if (!g.flow[1] && j == 2) { us = prev(nb.s); any = true; }
-      if (!g.flow[2] && k == g.nz + 1) { ut = prev(nb.t); any = true; }
-      if (!g.flow[2] && k == 2) { ub = prev(nb.b); any = true; }
+      if (!g.flow[0] && i == g.nx + 1) { ue = prev(nb.e, cc - (uint32_t)(g.nx - 1)); any = true; }
+      if (!g.flow[0] && i == 2) { uw = prev(nb.w, cc + (uint32_t)(g.nx - 1)); any = true; }
+      if (!g.flow[1] && j == g.ny + 1) { un = prev(nb.n, cc - cd.NX * (uint32_t)(g.ny - 1)); any = true; }
+      if (!g.flow[1] && j == 2) { us = prev(nb.s, cc + cd.NX * (uint32_t)(g.ny - 1)); any = true; }
+      if (!g.flow[2] && k == g.nz + 1) { ut = prev(nb.t, cc - cd.P * (uint32_t)(g.nz - 1)); any = true; }
+      if (!g.flow[2] && k == 2) { ub = prev(nb.b, cc + cd.P * (uint32_t)(g.nz - 1)); any = true; }
       if (any)
         rs = b[c] - ((ue + uw - u2) / mc.dx2 + (un + us - u2) / mc.dy2 + (ut + ub - u2) / mc.dz2);
     }
@@ -423,109 +447,158 @@ __global__ void __launch_bounds__(kStencilT) k_mx_residual(GridDims g, MixCoef m
 // Two buffers suffice: the stop test of iterate base+j runs at the start of
 // the launch that would overwrite e_j, before any block writes.
 //
+// A wave owns one row segment of 256 cells, 4 per lane, and marches up a z
+// chunk of its XCD's slab with the z-neighbours in registers; x-neighbours
+// inside the segment come from the adjacent lanes.
+//
 // The residual is that of the stop test, which the reference evaluates with
 // the periodic halo of the previous iterate (CalcEpotential.f90:83-92): a
 // neighbour across a periodic boundary takes e_{j-1}.  e_{j-1} is kept, for
-// the cells of the periodic boundary planes only, in slot (j-1) % 3 of `eo`;
-// this launch leaves e_{j+1} there in slot (j+1) % 3.
-__global__ void __launch_bounds__(kStencilT, 8) k_mx_sweep(SweepGeo sg, SweepCoef cf,
-                                                   const float *__restrict__ r32, float *E0, float *E1,
-                                                   float *eo0, float *eo1, float *eo2,
+// the cells of the periodic boundary planes, in slot (j-1) % 3 of the
+// boundary copies (`eo_prev`, null for j < 2); this launch leaves e_{j+1}
+// there in slot (j+1) % 3 (`eo_out`).  e is e_j (null for j = 0) and out
+// receives e_{j+1}.
+__global__ void __launch_bounds__(kStencilT, 4) k_mx_sweep(SweepGeo sg, SweepCoef cf,
+                                                   const float *__restrict__ r32,
+                                                   const float *__restrict__ e,
+                                                   float *__restrict__ out,
+                                                   float *__restrict__ eo_out,
+                                                   const float *__restrict__ eo_prev,
                                                    const double *__restrict__ Pin,
                                                    double *__restrict__ Pout, JacobiState *st,
                                                    MxLaunch L) {
   __shared__ double sh[kStencilT / 64];
   __shared__ int decide;
-  const int nblk = (int)gridDim.x;
-  StopSlot cur = mx_decide(st, L, Pin, nblk, sh, &decide);
+  StopSlot cur = mx_decide(st, L, Pin, sh, &decide);
   if (!cur.done && (int64_t)L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
     cur.done = 1;
     cur.result = st->last;
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   if (cur.done) return;
-  const float *e = L.j >= 1 ? ((L.j & 1) ? E1 : E0) : nullptr;
-  float *out = (L.j & 1) ? E0 : E1;                                // E[(j+1) % 2]
-  const int r1 = (L.j + 1) % 3, rm = (L.j + 2) % 3;                // slots of e_{j+1}, e_{j-1}
-  float *eo_out = r1 == 0 ? eo0 : (r1 == 1 ? eo1 : eo2);
-  const float *eo_prev = L.j >= 2 ? (rm == 0 ? eo0 : (rm == 1 ? eo1 : eo2)) : nullptr;
   const bool res = L.j >= 1;
-  const uint32_t X = sg.X, XY = sg.XY;
+  const uint32_t NX = sg.NX, P = sg.P;
   const int nx = sg.nx, ny = sg.ny, nz = sg.nz;
-  const bool pz = !sg.fz;
+  const int lane = (int)threadIdx.x;
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   double r2 = 0.0;
-  // 2.5D march over the XCD's z-slab with the z-neighbours of e_j in registers
   const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
   const int z0 = (xcd * nz) >> 3, z1 = ((xcd + 1) * nz) >> 3;
-  for (int t = w; z1 > z0 && t < sg.TP; t += sg.W) {
-    const int tx = t % sg.ntx, ty = t / sg.ntx;
-    const int i = tx * (int)blockDim.x + (int)threadIdx.x + 2;
-    const int j = ty * (int)blockDim.y + (int)threadIdx.y + 2;
-    if (i > nx + 1 || j > ny + 1) continue;
-    const uint32_t col = (uint32_t)(i - 1) + X * (uint32_t)(j - 1);
-    // x/y neighbours: periodic wrap, or the zero correction of a flow guard
-    const bool fe = sg.fx && i == nx + 1, fw = sg.fx && i == 2;
-    const bool fn = sg.fy && j == ny + 1, fs = sg.fy && j == 2;
-    const bool we = !sg.fx && i == nx + 1, ww = !sg.fx && i == 2;   // wrapped
-    const bool wn = !sg.fy && j == ny + 1, ws = !sg.fy && j == 2;
-    const bool edge_xy = we || ww || wn || ws;
+  for (int t = w; t < sg.TP; t += sg.W) {
+    const int zi = t % sg.nzc, tt = t / sg.nzc;
+    const int tx = tt % sg.ntx, ty = tt / sg.ntx;
+    const int ka = z0 + ((z1 - z0) * zi) / sg.nzc, kb = z0 + ((z1 - z0) * (zi + 1)) / sg.nzc;
+    const int j = ty * kSwRows + (int)threadIdx.y;
+    if (j >= ny || kb <= ka) continue;                      // the whole wave
+    const int xs = tx * kSwSeg + 4 * lane;                  // the lane's first cell
+    const bool live = xs < nx;
+    // lanes past the row end mirror the last live lane's cells (loads stay in
+    // bounds, shuffles see defined values) and store nothing
+    const uint32_t x0 = live ? (uint32_t)xs : NX - 4u;
+    const uint32_t rowj = NX * (uint32_t)j, row = x0 + rowj;
+    const bool zn = !sg.py && j == ny - 1, zs = !sg.py && j == 0;   // flow guards: zero
+    const uint32_t rown = x0 + NX * (uint32_t)(j == ny - 1 ? 0 : j + 1);
+    const uint32_t rows = x0 + NX * (uint32_t)(j == 0 ? ny - 1 : j - 1);
+    const bool hasf = xs == 0;                              // the lane holds x = 0
+    const int ql = nx - 1 - xs;                             // x = nx - 1 at cell ql
+    const bool hasl = live && ql < 4;
+    const bool xedge = sg.px && (hasf || hasl);
+    const bool yedge = sg.py && (j == 0 || j == ny - 1);
+    // cell q of the lane is inside the row (the padding past nx - 1 stays 0)
+    auto inrow = [&](int q) { return live && q <= ql; };
     if (!e) {
-      for (int k = z0 + 2; k <= z1 + 1; ++k) {
-        const uint32_t c = col + XY * (uint32_t)(k - 1);
-        const float v = -cf.af * r32[c];
-        out[c] = v;
-        if (edge_xy || (pz && (k == 2 || k == nz + 1))) eo_out[c] = v;
+      for (int k = ka; k < kb; ++k) {
+        const uint32_t c = row + P * (uint32_t)k;
+        const float4 rc = ld4(r32 + c);
+        float4 v;
+        v.x = inrow(0) ? -cf.af * rc.x : 0.0f;
+        v.y = inrow(1) ? -cf.af * rc.y : 0.0f;
+        v.z = inrow(2) ? -cf.af * rc.z : 0.0f;
+        v.w = inrow(3) ? -cf.af * rc.w : 0.0f;
+        if (live) {
+          st4(out + c, v);
+          if (xedge || yedge || (sg.pz && (k == 0 || k == nz - 1))) st4(eo_out + c, v);
+        }
       }
       continue;
     }
-    const uint32_t oe = we ? 0u - (uint32_t)(nx - 1) : 1u;
-    const uint32_t ow = ww ? (uint32_t)(nx - 1) : 0u - 1u;
-    const uint32_t on = wn ? 0u - (uint32_t)(ny - 1) * X : X;
-    const uint32_t os = ws ? (uint32_t)(ny - 1) * X : 0u - X;
-    // value of e_j on 1-based plane kk (guards: wrap or zero)
-    auto zval = [&](int kk) -> float {
-      if (kk == 1) return sg.fz ? 0.0f : e[col + XY * (uint32_t)nz];
-      if (kk == nz + 2) return sg.fz ? 0.0f : e[col + XY];
-      return e[col + XY * (uint32_t)(kk - 1)];
+    // e_j on plane kk (past the ends: the periodic image or the zero guard)
+    auto zval = [&](int kk) -> float4 {
+      if (kk < 0) return sg.pz ? ld4(e + row + P * (uint32_t)(nz - 1)) : zero4;
+      if (kk >= nz) return sg.pz ? ld4(e + row) : zero4;
+      return ld4(e + row + P * (uint32_t)kk);
     };
-    // e_{j-1} at a wrapped neighbour (0 for j = 1)
-    auto prev = [&](uint32_t n) -> float { return eo_prev ? eo_prev[n] : 0.0f; };
-    int k = z0 + 2;
-    float zm = zval(k - 1), zc = zval(k), zp = zval(k + 1);
-#pragma unroll 2
-    for (; k <= z1 + 1; ++k) {
-      const uint32_t c = col + XY * (uint32_t)(k - 1);
-      const float ee = fe ? 0.0f : e[c + oe];
-      const float ew = fw ? 0.0f : e[c + ow];
-      const float en = fn ? 0.0f : e[c + on];
-      const float es = fs ? 0.0f : e[c + os];
-      const float et = zp, eb = zm;
-      const float rc = r32[c];
-      // CalcEpotential.f90:74-80 on the correction, with FP32 weights that
-      // sum to 1/2 (kappa carries what rounding leaves, zero on uniform
-      // grids): both ends of the Jacobi spectrum, whose modes decay by ~1e-5
-      // per sweep at 256^3, stay where the FP64 iteration has them.
-      const float v = cf.wx * (ee + ew) + cf.wy * (en + es) + cf.wz * (et + eb) + cf.kappa * zc -
-                      cf.af * rc;
-      out[c] = v;
-      const float e2 = 2.0f * zc;
-      float rr = rc - ((ee + ew - e2) * cf.idx2 + (en + es - e2) * cf.idy2 + (et + eb - e2) * cf.idz2);
-      const bool wt = pz && k == nz + 1, wb = pz && k == 2;
-      if (edge_xy || wt || wb) {
-        eo_out[c] = v;
-        float se = ee, sw = ew, sn = en, ss = es, st_ = et, sb = eb;
-        if (we) se = prev(c + oe);
-        if (ww) sw = prev(c + ow);
-        if (wn) sn = prev(c + on);
-        if (ws) ss = prev(c + os);
-        if (wt) st_ = prev(c - XY * (uint32_t)(nz - 1));
-        if (wb) sb = prev(c + XY * (uint32_t)(nz - 1));
-        rr = rc - ((se + sw - e2) * cf.idx2 + (sn + ss - e2) * cf.idy2 + (st_ + sb - e2) * cf.idz2);
+    // e_{j-1} (0 for j = 1)
+    auto prev1 = [&](uint32_t n) -> float { return eo_prev ? eo_prev[n] : 0.0f; };
+    auto prev4 = [&](uint32_t n) -> float4 { return eo_prev ? ld4(eo_prev + n) : zero4; };
+    float4 zm = zval(ka - 1), zc = zval(ka), zp = zval(ka + 1);
+    for (int k = ka; k < kb; ++k) {
+      const uint32_t pk = P * (uint32_t)k;
+      const uint32_t c = row + pk;
+      const float4 en = zn ? zero4 : ld4(e + rown + pk);
+      const float4 es = zs ? zero4 : ld4(e + rows + pk);
+      const float4 rc = ld4(r32 + c);
+      // x-neighbours of the lane's end cells: the adjacent lanes, the
+      // adjacent row segment, or the periodic image / zero guard
+      float wv = __shfl(zc.w, (lane + 63) & 63, 64);
+      float ev = __shfl(zc.x, (lane + 1) & 63, 64);
+      if (lane == 0 && xs > 0) wv = e[c - 1];
+      if (lane == 63 && xs + 4 < nx) ev = e[c + 4];
+      if (hasf) wv = sg.px ? e[rowj + pk + (uint32_t)(nx - 1)] : 0.0f;
+      const float wrap_e = (hasl && sg.px) ? e[rowj + pk] : 0.0f;
+      const float ce[4] = {zc.x, zc.y, zc.z, zc.w};
+      const float cn[4] = {en.x, en.y, en.z, en.w}, cs[4] = {es.x, es.y, es.z, es.w};
+      const float ct[4] = {zp.x, zp.y, zp.z, zp.w}, cb[4] = {zm.x, zm.y, zm.z, zm.w};
+      const float cr[4] = {rc.x, rc.y, rc.z, rc.w};
+      float ee[4], ew[4], v[4], rr[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        ew[q] = q == 0 ? wv : ce[q - 1];
+        ee[q] = (hasl && q == ql) ? wrap_e : (q == 3 ? ev : ce[q + 1]);
+        // CalcEpotential.f90:74-80 on the correction, with FP32 weights that
+        // sum to 1/2 (kappa carries what rounding leaves, zero on uniform
+        // grids): both ends of the Jacobi spectrum, whose modes decay by
+        // ~1e-5 per sweep at 256^3, stay where the FP64 iteration has them.
+        v[q] = cf.wx * (ee[q] + ew[q]) + cf.wy * (cn[q] + cs[q]) + cf.wz * (ct[q] + cb[q]) +
+               cf.kappa * ce[q] - cf.af * cr[q];
+        const float e2 = 2.0f * ce[q];
+        rr[q] = cr[q] - ((ee[q] + ew[q] - e2) * cf.idx2 + (cn[q] + cs[q] - e2) * cf.idy2 +
+                         (ct[q] + cb[q] - e2) * cf.idz2);
+        if (!inrow(q)) v[q] = 0.0f;
       }
-      r2 = r2 + (double)rr * (double)rr;
+      const float4 v4 = make_float4(v[0], v[1], v[2], v[3]);
+      if (live) st4(out + c, v4);
+      const bool zedge = sg.pz && (k == 0 || k == nz - 1);
+      if (xedge || yedge || zedge) {
+        if (live) st4(eo_out + c, v4);
+        // stop-test residual with e_{j-1} across the periodic boundaries
+        const float4 pn = (sg.py && j == ny - 1) ? prev4(x0 + pk) : zero4;
+        const float4 ps = (sg.py && j == 0) ? prev4(x0 + NX * (uint32_t)(ny - 1) + pk) : zero4;
+        const float4 pt = (sg.pz && k == nz - 1) ? prev4(row) : zero4;
+        const float4 pb = (sg.pz && k == 0) ? prev4(row + P * (uint32_t)(nz - 1)) : zero4;
+        const float pw = (sg.px && hasf) ? prev1(rowj + pk + (uint32_t)(nx - 1)) : 0.0f;
+        const float pe = (sg.px && hasl) ? prev1(rowj + pk) : 0.0f;
+        const float qn[4] = {pn.x, pn.y, pn.z, pn.w}, qs[4] = {ps.x, ps.y, ps.z, ps.w};
+        const float qt[4] = {pt.x, pt.y, pt.z, pt.w}, qb[4] = {pb.x, pb.y, pb.z, pb.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float se = (sg.px && hasl && q == ql) ? pe : ee[q];
+          const float sw = (sg.px && hasf && q == 0) ? pw : ew[q];
+          const float sn = (sg.py && j == ny - 1) ? qn[q] : cn[q];
+          const float ss = (sg.py && j == 0) ? qs[q] : cs[q];
+          const float st_ = (sg.pz && k == nz - 1) ? qt[q] : ct[q];
+          const float sb = (sg.pz && k == 0) ? qb[q] : cb[q];
+          const float e2 = 2.0f * ce[q];
+          rr[q] = cr[q] - ((se + sw - e2) * cf.idx2 + (sn + ss - e2) * cf.idy2 +
+                           (st_ + sb - e2) * cf.idz2);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (inrow(q)) r2 = r2 + (double)rr[q] * (double)rr[q];
       zm = zc;
       zc = zp;
-      if (k + 1 <= z1 + 1) zp = zval(k + 2);
+      if (k + 1 < kb) zp = zval(k + 2);
     }
   }
   const double sblk = block_sum(r2, sh);
@@ -534,14 +607,14 @@ __global__ void __launch_bounds__(kStencilT, 8) k_mx_sweep(SweepGeo sg, SweepCoe
 
 // u += e_k: k = m after a full outer step, or the converged (or last)
 // iterate's k when the solve ended inside this step.
-__global__ void __launch_bounds__(kStencilT) k_mx_update(GridDims g, TileSched ts, double *__restrict__ u,
-                                                    const float *E0, const float *E1,
-                                                    const double *__restrict__ Pin,
+__global__ void __launch_bounds__(kStencilT) k_mx_update(GridDims g, CompactDims cd, TileSched ts,
+                                                    double *__restrict__ u, const float *E0,
+                                                    const float *E1, const double *__restrict__ Pin,
                                                     JacobiState *st, MxLaunch L) {
   __shared__ double sh[kStencilT / 64];
   __shared__ int decide;
   const StopSlot prev = st->slot[(L.seq - 1) & 1];
-  StopSlot cur = mx_decide(st, L, Pin, ts.B, sh, &decide);
+  StopSlot cur = mx_decide(st, L, Pin, sh, &decide);
   if (!cur.done && L.base + L.m >= st->last) {
     cur.done = 1;
     cur.result = st->last;
@@ -555,7 +628,7 @@ __global__ void __launch_bounds__(kStencilT) k_mx_update(GridDims g, TileSched t
   for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int kk, bool valid) {
     if (!valid) return;
     const int64_t c = node_index(g, i, j, kk);
-    u[c] = u[c] + (double)e[c];
+    u[c] = u[c] + (double)e[compact_index(cd, i, j, kk)];
   });
 }
 
@@ -791,12 +864,8 @@ int sp_gridify(picard_ctx *c, bool flux) {
   return PICARD_OK;
 }
 
-// The persistent schedule of the solve's kernels with as many blocks per XCD
-// as are resident at once for the sweep kernel (the heaviest), so that the
-// XCD slab assignment is not split over two dispatch rounds.
-TileSched stencil_sched(picard_ctx *c) {
-  const GridDims &g = c->gd;
-  TileSched ts = make_sched(g.nx, g.ny, g.nz, kStencilT);
+// Resident blocks per XCD of the sweep kernel (the heaviest of the solve).
+int sweep_wmax(picard_ctx *c) {
   if (c->sweep_wmax <= 0) {
     int per_cu = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_sweep, kStencilT, 0) != hipSuccess ||
@@ -807,11 +876,52 @@ TileSched stencil_sched(picard_ctx *c) {
       c->sweep_wmax = (cus / 8) * per_cu;
     }
   }
-  if (ts.W > c->sweep_wmax) {
-    ts.W = c->sweep_wmax < 4 ? 4 : c->sweep_wmax;
+  return c->sweep_wmax;
+}
+
+// The persistent schedule of the outer-step kernels, with no more blocks per
+// XCD than are resident at once, so that the XCD slab assignment is not split
+// over two dispatch rounds.
+TileSched stencil_sched(picard_ctx *c) {
+  const GridDims &g = c->gd;
+  TileSched ts = make_sched(g.nx, g.ny, g.nz, kStencilT);
+  const int wmax = sweep_wmax(c);
+  if (ts.W > wmax) {
+    ts.W = wmax < 4 ? 4 : wmax;
     ts.B = 8 * ts.W;
   }
   return ts;
+}
+
+// Tiles of the sweep kernel: row segments x 4-row groups x z chunks of an
+// XCD slab.  The slab is cut into chunks only until the tiles give about half
+// the resident blocks per XCD (at 256^3: two chunks, 1024 blocks of 4 waves,
+// beat both one and four chunks; longer z marches re-read fewer planes).
+SweepGeo sweep_sched(picard_ctx *c) {
+  const GridDims &g = c->gd;
+  SweepGeo sg;
+  sg.nx = g.nx;
+  sg.ny = g.ny;
+  sg.nz = g.nz;
+  sg.NX = (uint32_t)(g.nx + 3) & ~3u;
+  sg.P = sg.NX * (uint32_t)g.ny;
+  sg.px = !g.flow[0];
+  sg.py = !g.flow[1];
+  sg.pz = !g.flow[2];
+  sg.ntx = (g.nx + kSwSeg - 1) / kSwSeg;
+  sg.nty = (g.ny + kSwRows - 1) / kSwRows;
+  const int wmax = sweep_wmax(c);
+  const int per_chunk = sg.ntx * sg.nty;
+  int nzc = (wmax + 2 * per_chunk - 1) / (2 * per_chunk);
+  const int zmax = g.nz / 64 > 1 ? g.nz / 64 : 1;         // chunks of at least 8 planes
+  if (nzc > zmax) nzc = zmax;
+  if (nzc < 1) nzc = 1;
+  sg.nzc = nzc;
+  sg.TP = per_chunk * nzc;
+  int W = sg.TP < wmax ? sg.TP : wmax;
+  if (W > (int)(c->s.partials_len / 8)) W = (int)(c->s.partials_len / 8);
+  sg.W = W < 1 ? 1 : W;
+  return sg;
 }
 
 int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) {
@@ -834,12 +944,9 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
   PICARD_HIP(hipMemcpyAsync(c->js, &init, sizeof(JacobiState), hipMemcpyHostToDevice, c->stream));
   PICARD_TRY(launch_bnorm(c, maxerr));      // launch 0: b, |b|, the test of ss = 1
   const TileSched ts = stencil_sched(c);
-  SweepGeo sg;
-  sg.nx = g.nx; sg.ny = g.ny; sg.nz = g.nz;
-  sg.X = (uint32_t)g.X;
-  sg.XY = (uint32_t)g.X * (uint32_t)g.Y;
-  sg.fx = g.flow[0]; sg.fy = g.flow[1]; sg.fz = g.flow[2];
-  sg.ntx = ts.ntx; sg.TP = ts.TP; sg.W = ts.W;
+  const SweepGeo sg = sweep_sched(c);
+  const CompactDims cd{sg.NX, sg.P};
+  const int swB = 8 * sg.W;
   SweepCoef cf{};
   {
     // FP32 weights with 2 (wx + wy + wz) = 1 as closely as FP32 allows
@@ -861,10 +968,12 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
   const int m = c->mx_inner;
   double *P[2] = {c->s.partials, c->s.partials + c->s.partials_len};
   float *E0 = c->e32[0], *E1 = c->e32[1];
+  float *const E[2] = {E0, E1};
   float *const *EO = c->eo32;
   int32_t seq = 1, base = 0;
   int pw = 0;                               // partials buffer the next writer uses
   int32_t check = -1;                       // iterate whose residual the last writer stored
+  int32_t nin = 0;                          // partials the last writer stored
   StopSlot fin{};
   while (true) {
     if (c->timing) {
@@ -877,32 +986,39 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
     // outer residual of iterate `base` (its stop test happens in the next
     // launch); the stale-halo norm needs e_m and e_{m-1} of the previous outer
     {
-      MxLaunch L{seq, base, 0, -1, m};
+      MxLaunch L{seq, base, 0, -1, m, 0};
       const float *em = base > 0 ? c->e32[m & 1] : nullptr;
       const float *eo = base > 0 ? EO[(m - 1) % 3] : nullptr;
-      k_mx_residual<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, mc, ts, c->U, c->b, c->r32, em,
-                                                                eo, P[pw ^ 1], P[pw], c->js, L);
+      k_mx_residual<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, cd, mc, ts, c->U, c->b, c->r32,
+                                                                em, eo, P[pw ^ 1], P[pw], c->js, L);
       ++seq;
       check = base;
+      nin = ts.B;
       pw ^= 1;
     }
     for (int j = 0; j < m; ++j) {
       // the iterate-0 test used b itself (launch 0); later ones use residuals
       const int32_t chk = (check >= 1) ? check : -1;
-      MxLaunch L{seq, base, j, chk, m};
+      MxLaunch L{seq, base, j, chk, m, nin};
       const double *pin = P[pw ^ 1];
       double *pout = P[pw];
+      // e_j in E[j % 2] (none for j = 0), e_{j+1} to E[(j+1) % 2]; boundary
+      // copies of e_{j+1} to slot (j+1) % 3, e_{j-1} from slot (j-1) % 3
+      const float *e = j >= 1 ? E[j & 1] : nullptr;
+      float *out = E[(j + 1) & 1];
+      float *eo_out = EO[(j + 1) % 3];
+      const float *eo_prev = j >= 2 ? EO[(j - 1) % 3] : nullptr;
       if (c->timing)
-        hipExtLaunchKernelGGL(k_mx_sweep, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
+        hipExtLaunchKernelGGL(k_mx_sweep, dim3(swB), dim3(64, kSwRows), 0, c->stream,
                               c->tev[2 * j], c->tev[2 * j + 1], 0, sg, cf,
-                              (const float *)c->r32, E0, E1, EO[0], EO[1], EO[2], pin, pout, c->js,
-                              L);
+                              (const float *)c->r32, e, out, eo_out, eo_prev, pin, pout, c->js, L);
       else
-        k_mx_sweep<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(sg, cf, c->r32, E0, E1, EO[0],
-                                                               EO[1], EO[2], pin, pout, c->js, L);
+        k_mx_sweep<<<swB, dim3(64, kSwRows), 0, c->stream>>>(sg, cf, c->r32, e, out, eo_out,
+                                                             eo_prev, pin, pout, c->js, L);
       ++seq;
       if (j >= 1) {               // wrote the residual of iterate base + j
         check = base + j;
+        nin = swB;
         pw ^= 1;
       } else {
         check = -1;
@@ -910,8 +1026,8 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
     }
     {
       const int32_t chk = (check >= 1) ? check : -1;
-      MxLaunch L{seq, base, 0, chk, m};
-      k_mx_update<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, ts, c->U, E0, E1, P[pw ^ 1],
+      MxLaunch L{seq, base, 0, chk, m, nin};
+      k_mx_update<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, cd, ts, c->U, E0, E1, P[pw ^ 1],
                                                               c->js, L);
       ++seq;
       check = -1;

@@ -89,6 +89,35 @@ def test_split_solve_matches_fp64(flow):
     b.close()
 
 
+@pytest.mark.parametrize("dims,flow", [((13, 10, 9), (True, False, False)),
+                                       ((13, 10, 9), (False, True, False)),
+                                       ((6, 7, 5), (False, False, True)),
+                                       ((262, 6, 8), (False, True, True))])
+def test_split_solve_shapes(dims, flow):
+    """Rows padded past nx (nx not a multiple of 4), periodic and flow x
+    boundaries inside a lane, rows longer than one 256-cell segment."""
+    nx, ny, nz = dims
+    _say(f"solve dims={dims} flow={flow}")
+    cfg = host.Config(nx=nx, ny=ny, nz=nz, flow=flow,
+                      gmin=(-5.0 * nx, -5.0 * ny, -5.0 * nz), gmax=(5.0 * nx, 5.0 * ny, 5.0 * nz),
+                      ppc=(0, 8, 0, 0, 0, 0, 0, 0))
+    d, a, b = _pair(cfg, cap=1024)
+    rng = np.random.default_rng(5)
+    C = rng.normal(size=(nz + 2, ny + 2, nx + 2)) * 1e-16
+    sw = []
+    for dom in (a, b):
+        dom.calc_distance()
+        dom.set_field(_lib.FIELD_C, C)
+        dom.bc_potentials()
+        sw.append(dom.calc_epotential())
+    Ua, Ub = a.field(_lib.FIELD_U), b.field(_lib.FIELD_U)
+    _say(f"sweeps fp64={sw[0]} split={sw[1]}  U rel {_rel(Ub, Ua):.3e}")
+    assert abs(sw[0] - sw[1]) <= 2
+    assert _rel(Ub, Ua) <= 1e-5
+    a.close()
+    b.close()
+
+
 def test_split_steps_error_report():
     """A few whole steps of both forms from the same plasma: errors per step."""
     cfg = _cfg(16, ppc=8)
