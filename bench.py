@@ -210,7 +210,8 @@ def main():
             "traffic": None,
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": avg_s * 1e6,
-            "launches": head["jac_launches"],
+            "timed_launches": head["jac_launches"],
+            "timed_every": 16,
         },
         "cpu_baseline": None,
     }

@@ -197,10 +197,11 @@ int picard_field_read(picard_ctx *ctx, int32_t which, double *host, int64_t coun
 int picard_field_write(picard_ctx *ctx, int32_t which, const double *host, int64_t count);
 
 /* Kernel timing of the Jacobi sweep kernel, for roofline measurement; no
- * reference counterpart (the reference has no timers).  While enabled every
- * sweep launch carries start/stop events stamped by its dispatch packet;
- * picard_timing_read returns the summed kernel time and the launch count
- * since the last enable. */
+ * reference counterpart (the reference has no timers).  While enabled one
+ * sweep launch in 16 carries start/stop events stamped by its dispatch
+ * packet (events around every launch would slow the timed solve);
+ * picard_timing_read returns the summed kernel time and the count of timed
+ * launches since the last enable. */
 int picard_timing_enable(picard_ctx *ctx, int32_t enable);
 int picard_timing_read(picard_ctx *ctx, double *jacobi_ms, int64_t *jacobi_launches);
 

@@ -113,6 +113,7 @@ void release(picard_ctx *c) {
   dfree(c->Nfix);
   dfree(c->E32);
   dfree(c->r32);
+  dfree(c->Uw);
   for (int q = 0; q < 2; ++q) dfree(c->e32[q]);
   for (int q = 0; q < 3; ++q) dfree(c->eo32[q]);
   for (int q = 0; q < 8; ++q) dfree(c->p.r[q]);
@@ -187,13 +188,17 @@ int create_impl(const picard_geometry *geo, const picard_species *sp, picard_ctx
     PICARD_TRY(alloc_split(c->ps2, cap));
     PICARD_TRY(dalloc(&c->Nfix, g.S * g.G));
     PICARD_TRY(dalloc(&c->E32, 4 * g.G));
-    // the solve's FP32 arrays: interior cells, rows padded to 4 cells (zeros)
-    const int64_t nc32 = (int64_t)((g.nx + 3) & ~3) * g.ny * g.nz;
+    // the solve's FP32 arrays: interior cells, rows padded to 4 cells (zeros);
+    // the correction buffers carry one more row of zeros after the last plane
+    const int64_t NX32 = (g.nx + 3) & ~3;
+    const int64_t nc32 = NX32 * g.ny * g.nz;
     PICARD_TRY(dalloc(&c->r32, nc32));
-    for (int q = 0; q < 2; ++q) PICARD_TRY(dalloc(&c->e32[q], nc32));
+    PICARD_TRY(dalloc(&c->Uw, g.G));
+    for (int q = 0; q < 2; ++q) PICARD_TRY(dalloc(&c->e32[q], nc32 + NX32));
     for (int q = 0; q < 3; ++q) PICARD_TRY(dalloc(&c->eo32[q], nc32));
     PICARD_HIP(hipMemsetAsync(c->r32, 0, sizeof(float) * nc32, c->stream));
-    for (int q = 0; q < 2; ++q) PICARD_HIP(hipMemsetAsync(c->e32[q], 0, sizeof(float) * nc32, c->stream));
+    for (int q = 0; q < 2; ++q)
+      PICARD_HIP(hipMemsetAsync(c->e32[q], 0, sizeof(float) * (nc32 + NX32), c->stream));
     for (int q = 0; q < 3; ++q) PICARD_HIP(hipMemsetAsync(c->eo32[q], 0, sizeof(float) * nc32, c->stream));
     PICARD_HIP(hipMemsetAsync(c->E32, 0, sizeof(float) * 4 * g.G, c->stream));
   } else {

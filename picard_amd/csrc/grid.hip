@@ -973,19 +973,22 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
   StopSlot fin{};
   for (;;) {
     const int64_t todo = (last + 1 - (p - 1)) < batch ? (last + 1 - (p - 1)) : batch;
+    // timing: events around one pass in kTimeEvery (event records between
+    // every pair of passes would slow the timed solve itself)
+    const int64_t timed = (todo + kTimeEvery - 1) / kTimeEvery;
     if (c->timing) {
-      while ((int64_t)c->tev.size() < 2 * todo) {
+      while ((int64_t)c->tev.size() < 2 * timed) {
         hipEvent_t e;
         PICARD_HIP(hipEventCreate(&e));
         c->tev.push_back(e);
       }
     }
     for (int64_t q = 0; q < todo; ++q, ++p) {
-      if (c->timing)
+      if (c->timing && q % kTimeEvery == 0)
         hipExtLaunchKernelGGL(k_jacobi_pass, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
-                              c->tev[2 * q], c->tev[2 * q + 1], 0, pg, jc, c->jb[0], c->jb[1],
-                              c->jb[2], (const double *)c->b, (const double *)P[(p - 1) & 1],
-                              P[p & 1], c->js, (int32_t)p);
+                              c->tev[2 * (q / kTimeEvery)], c->tev[2 * (q / kTimeEvery) + 1], 0,
+                              pg, jc, c->jb[0], c->jb[1], c->jb[2], (const double *)c->b,
+                              (const double *)P[(p - 1) & 1], P[p & 1], c->js, (int32_t)p);
       else
         k_jacobi_pass<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(
             pg, jc, c->jb[0], c->jb[1], c->jb[2], c->b, P[(p - 1) & 1], P[p & 1], c->js, (int32_t)p);
@@ -995,12 +998,12 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
                               c->stream));
     PICARD_HIP(hipStreamSynchronize(c->stream));
     if (c->timing) {
-      for (int64_t q = 0; q < todo; ++q) {
+      for (int64_t q = 0; q < timed; ++q) {
         float ms = 0.0f;
         PICARD_HIP(hipEventElapsedTime(&ms, c->tev[2 * q], c->tev[2 * q + 1]));
         c->t_ms += ms;
       }
-      c->t_launches += todo;
+      c->t_launches += timed;
     }
     fin = c->js_host->slot[(p - 1) & 1];
     if (fin.done) break;

@@ -125,6 +125,8 @@ struct picard_ctx {
   float *r32 = nullptr, *e32[2] = {nullptr, nullptr};
   float *eo32[3] = {nullptr, nullptr, nullptr};
   int mx_inner = 64;         // split: FP32 sweeps per FP64 refinement
+  int outer_wmax = 0;        // split: resident blocks per XCD of the outer-step kernel
+  double *Uw = nullptr;      // split: second FP64 potential buffer of the solve
   int sweep_wmax = 0;        // split: resident blocks per XCD of the sweep kernel (0: not yet known)
   int pass_wmax = 0;         // FP64: the same for the Jacobi pass kernel
   bool bcpot_lds_set = false;  // FP64: the LDS face-sum kernel's LDS limit has been raised
@@ -169,7 +171,8 @@ struct picard_ctx {
   int64_t last_sweeps = 0;
 
   // kernel timing of the Jacobi sweep (picard_timing_enable): event pairs
-  // stamped by the dispatch packets themselves
+  // stamped by the dispatch packets themselves, around one sweep in
+  // picard::kTimeEvery
   bool timing = false;
   std::vector<hipEvent_t> tev;
   double t_ms = 0.0;
@@ -177,6 +180,8 @@ struct picard_ctx {
 };
 
 namespace picard {
+
+constexpr int kTimeEvery = 16;   // timed sweeps: one launch in this many
 
 // ---- particles.hip ----
 int launch_push_x(picard_ctx *c, int64_t first, int64_t count, double dt);

@@ -387,56 +387,183 @@ __device__ __forceinline__ StopSlot mx_decide(JacobiState *st, const MxLaunch &L
   return cur;
 }
 
-// Outer step: r = b - A u in FP64 with fresh periodic images, r32 = r (the
-// right-hand side of the inner sweeps); partials of |r_stale|^2 for the stop
-// test of iterate `base`.  The reference evaluates that residual with the
-// periodic halo of the previous iterate (CalcEpotential.f90:83-92), so a
-// neighbour across a periodic boundary takes U_{base-1} = u - (e_m - e_{m-1}):
-// em and eo hold e_m and e_{m-1} of the previous outer step (null when base
-// is 0, whose test is the |b| test of the first loop iteration).
-__global__ void __launch_bounds__(kStencilT) k_mx_residual(GridDims g, CompactDims cd, MixCoef mc,
-                                                      TileSched ts,
-                                                      const double *__restrict__ u,
-                                                      const double *__restrict__ b,
-                                                      float *__restrict__ r32,
-                                                      const float *__restrict__ em,
-                                                      const float *__restrict__ eo,
-                                                      const double *__restrict__ Pin,
-                                                      double *__restrict__ Pout, JacobiState *st,
-                                                      MxLaunch L) {
+// Geometry of the outer-step kernel: guard-layout FP64 strides, compact FP32
+// strides, and a schedule of 64-cell row segments (one cell per lane) x
+// 4-row groups x z chunks of an XCD slab.
+struct OuterGeo {
+  int32_t nx, ny, nz;
+  uint32_t X, XY;        // guard-layout strides
+  uint32_t NX, P;        // compact strides
+  int32_t px, py, pz;    // periodic flags
+  int32_t ntx, nty, nzc, TP, W;
+};
+
+// Outer step of the mixed solve.  u' = u + e_m, the correction the inner
+// sweeps of the previous outer step left in `em` (UPD; the first outer step
+// has none), goes to the other FP64 buffer `ud` (interior nodes; the guards
+// of both buffers hold the same boundary values); r = b - A u' in FP64 with
+// fresh periodic images, r32 = r (the right-hand side of the next inner
+// sweeps); partials of |r_stale|^2 for the stop test of iterate `base`.  The
+// reference evaluates that residual with the periodic halo of the previous
+// iterate (CalcEpotential.f90:83-92), so a neighbour across a periodic
+// boundary takes U_{base-1} = u' - (e_m - e_{m-1}); eo holds e_{m-1}.
+template <bool UPD>
+__global__ void __launch_bounds__(kStencilT, 4) k_mx_outer(OuterGeo og, double dx2, double dy2,
+                                                   double dz2, const double *__restrict__ us,
+                                                   double *__restrict__ ud,
+                                                   const double *__restrict__ b,
+                                                   float *__restrict__ r32,
+                                                   const float *__restrict__ em,
+                                                   const float *__restrict__ eo,
+                                                   const double *__restrict__ Pin,
+                                                   double *__restrict__ Pout, JacobiState *st,
+                                                   MxLaunch L) {
   __shared__ double sh[kStencilT / 64];
   __shared__ int decide;
   const StopSlot cur = mx_decide(st, L, Pin, sh, &decide);
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   if (cur.done) return;
-  const bool stale = em != nullptr;
+  const int nx = og.nx, ny = og.ny, nz = og.nz;
+  const uint32_t X = og.X, XY = og.XY, NX = og.NX, P = og.P;
+  const int lane = (int)threadIdx.x;
   double r2 = 0.0;
-  for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int k, bool valid) {
-    if (!valid) return;
-    const int64_t c = node_index(g, i, j, k);
-    const uint32_t cc = compact_index(cd, i, j, k);
-    const Nb nb = neighbours(g, i, j, k, c);
-    const double u2 = 2.0 * u[c];
-    double ue = u[nb.e], uw = u[nb.w], un = u[nb.n], us = u[nb.s], ut = u[nb.t], ub = u[nb.b];
-    const double r = b[c] - ((ue + uw - u2) / mc.dx2 + (un + us - u2) / mc.dy2 +
-                             (ut + ub - u2) / mc.dz2);
-    r32[cc] = (float)r;
-    double rs = r;
-    if (stale) {
-      // n: the wrapped neighbour's node, nc: its compact index
-      auto prev = [&](int64_t n, uint32_t nc) { return u[n] - ((double)em[nc] - (double)eo[nc]); };
-      bool any = false;
-      if (!g.flow[0] && i == g.nx + 1) { ue = prev(nb.e, cc - (uint32_t)(g.nx - 1)); any = true; }
-      if (!g.flow[0] && i == 2) { uw = prev(nb.w, cc + (uint32_t)(g.nx - 1)); any = true; }
-      if (!g.flow[1] && j == g.ny + 1) { un = prev(nb.n, cc - cd.NX * (uint32_t)(g.ny - 1)); any = true; }
-      if (!g.flow[1] && j == 2) { us = prev(nb.s, cc + cd.NX * (uint32_t)(g.ny - 1)); any = true; }
-      if (!g.flow[2] && k == g.nz + 1) { ut = prev(nb.t, cc - cd.P * (uint32_t)(g.nz - 1)); any = true; }
-      if (!g.flow[2] && k == 2) { ub = prev(nb.b, cc + cd.P * (uint32_t)(g.nz - 1)); any = true; }
-      if (any)
-        rs = b[c] - ((ue + uw - u2) / mc.dx2 + (un + us - u2) / mc.dy2 + (ut + ub - u2) / mc.dz2);
+  const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
+  const int z0 = (xcd * nz) >> 3, z1 = ((xcd + 1) * nz) >> 3;
+  for (int t = w; t < og.TP; t += og.W) {
+    const int zi = t % og.nzc, tt = t / og.nzc;
+    const int tx = tt % og.ntx, ty = tt / og.ntx;
+    const int ka = z0 + ((z1 - z0) * zi) / og.nzc, kb = z0 + ((z1 - z0) * (zi + 1)) / og.nzc;
+    const int j = ty * kSwRows + (int)threadIdx.y;
+    if (j >= ny || kb <= ka) continue;                      // the whole wave
+    const int i0 = tx * 64 + lane;
+    const bool live = i0 < nx;                              // lanes past the row end store nothing
+    const int i = live ? i0 : nx - 1;
+    // node of (i, j, plane 0 of the guard layout) and compact cell (i, j, 0)
+    const uint32_t g0 = (uint32_t)(i + 1) + X * (uint32_t)(j + 1);
+    const uint32_t c0 = (uint32_t)i + NX * (uint32_t)j;
+    // neighbour offsets: node, compact cell, and whether the neighbour is an
+    // interior cell (periodic image) or a flow guard (boundary value, no e)
+    const bool wE = og.px && i == nx - 1, wW = og.px && i == 0;     // wrapped
+    const bool gE = !og.px && i == nx - 1, gW = !og.px && i == 0;   // flow guard
+    const uint32_t dgE = wE ? 0u - (uint32_t)(nx - 1) : 1u, dcE = wE ? 0u - (uint32_t)(nx - 1) : 1u;
+    const uint32_t dgW = wW ? (uint32_t)(nx - 1) : 0u - 1u, dcW = wW ? (uint32_t)(nx - 1) : 0u - 1u;
+    const bool wN = og.py && j == ny - 1, wS = og.py && j == 0;
+    const bool gN = !og.py && j == ny - 1, gS = !og.py && j == 0;
+    const uint32_t dgN = wN ? 0u - (uint32_t)(ny - 1) * X : X;
+    const uint32_t dcN = wN ? 0u - (uint32_t)(ny - 1) * NX : NX;
+    const uint32_t dgS = wS ? (uint32_t)(ny - 1) * X : 0u - X;
+    const uint32_t dcS = wS ? (uint32_t)(ny - 1) * NX : 0u - NX;
+    // x-neighbours come from the adjacent lanes, except at the segment ends
+    // and the domain boundary, where they are read
+    const bool xe_mem = lane == 63 || i == nx - 1, xw_mem = lane == 0 || i == 0;
+    // u and e on plane kk of the z-window, -1 <= kk <= nz (past the ends: the
+    // periodic image, or the guard's boundary value with e = 0); u' = u + e
+    struct Zv {
+      double u;
+      float e;
+    };
+    auto zval = [&](int kk) -> Zv {
+      Zv z;
+      z.e = 0.0f;
+      const bool guard = !og.pz && (kk < 0 || kk >= nz);
+      const int kw = guard ? kk : (kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk));
+      if (UPD && !guard) z.e = em[c0 + P * (uint32_t)kw];
+      z.u = us[g0 + XY * (uint32_t)(kw + 1)];
+      return z;
+    };
+    // the rest of plane kk's inputs; loads that may be skipped go first (see
+    // the sweep kernel); e = 0 where a flow guard is read
+    struct OIn {
+      double un, us, b, uxe, uxw;
+      float en, es, exe, exw;
+      float qxe, qxw, qn, qs, qz;    // e_{m-1} at the neighbours across periodic boundaries
+    };
+    auto plane_in = [&](int kk) -> OIn {
+      const uint32_t gk = g0 + XY * (uint32_t)(kk + 1), ck = c0 + P * (uint32_t)kk;
+      OIn a;
+      a.uxe = 0.0;
+      a.uxw = 0.0;
+      a.en = 0.0f;
+      a.es = 0.0f;
+      a.exe = 0.0f;
+      a.exw = 0.0f;
+      a.qxe = 0.0f;
+      a.qxw = 0.0f;
+      a.qn = 0.0f;
+      a.qs = 0.0f;
+      a.qz = 0.0f;
+      if (UPD) {
+        if (wE) a.qxe = eo[ck + dcE];
+        if (wW) a.qxw = eo[ck + dcW];
+        if (wN) a.qn = eo[ck + dcN];
+        if (wS) a.qs = eo[ck + dcS];
+        if (og.pz && kk == nz - 1) a.qz = eo[ck - P * (uint32_t)(nz - 1)];
+        else if (og.pz && kk == 0) a.qz = eo[ck + P * (uint32_t)(nz - 1)];
+        if (xe_mem && !gE) a.exe = em[ck + dcE];
+        if (xw_mem && !gW) a.exw = em[ck + dcW];
+        if (!gN) a.en = em[ck + dcN];
+        if (!gS) a.es = em[ck + dcS];
+      }
+      if (xe_mem) a.uxe = us[gk + dgE];
+      if (xw_mem) a.uxw = us[gk + dgW];
+      a.un = us[gk + dgN];
+      a.us = us[gk + dgS];
+      a.b = b[gk];
+      return a;
+    };
+    // plane k from its inputs and the z-window (zb, zc, zt) = planes k-1, k, k+1
+    auto do_plane = [&](int k, const OIn &a, const Zv &zb, const Zv &zc, const Zv &zt) {
+      const uint32_t gk = g0 + XY * (uint32_t)(k + 1), ck = c0 + P * (uint32_t)k;
+      const double uc = UPD ? zc.u + (double)zc.e : zc.u;
+      const double ut = UPD ? zt.u + (double)zt.e : zt.u;
+      const double ub = UPD ? zb.u + (double)zb.e : zb.u;
+      double ue = __shfl(uc, (lane + 1) & 63, 64), uw = __shfl(uc, (lane + 63) & 63, 64);
+      if (xe_mem) ue = UPD ? a.uxe + (double)a.exe : a.uxe;
+      if (xw_mem) uw = UPD ? a.uxw + (double)a.exw : a.uxw;
+      const double un = UPD ? a.un + (double)a.en : a.un;
+      const double us_ = UPD ? a.us + (double)a.es : a.us;
+      const double u2 = 2.0 * uc;
+      const double r = a.b - ((ue + uw - u2) / dx2 + (un + us_ - u2) / dy2 + (ut + ub - u2) / dz2);
+      double rs = r;
+      const bool wT = og.pz && k == nz - 1, wB = og.pz && k == 0;
+      if (UPD && (wE || wW || wN || wS || wT || wB)) {
+        const double se = wE ? ue - ((double)a.exe - (double)a.qxe) : ue;
+        const double sw = wW ? uw - ((double)a.exw - (double)a.qxw) : uw;
+        const double sn = wN ? un - ((double)a.en - (double)a.qn) : un;
+        const double ss = wS ? us_ - ((double)a.es - (double)a.qs) : us_;
+        const double st_ = wT ? ut - ((double)zt.e - (double)a.qz) : ut;
+        const double sb = wB ? ub - ((double)zb.e - (double)a.qz) : ub;
+        rs = a.b - ((se + sw - u2) / dx2 + (sn + ss - u2) / dy2 + (st_ + sb - u2) / dz2);
+      }
+      if (live) {
+        ud[gk] = uc;
+        r32[ck] = (float)r;
+        r2 = r2 + rs * rs;
+      }
+    };
+    // four planes per pass, as in the sweep kernel
+    auto zl = [&](int kk) { return zval(kk < kb ? kk : kb); };
+    auto pl = [&](int kk) { return plane_in(kk < kb ? kk : kb - 1); };
+    Zv z0 = zl(ka - 1), z1 = zl(ka), z2 = zl(ka + 1), z3 = zl(ka + 2);
+    OIn A = pl(ka), B = pl(ka + 1);
+    for (int k = ka; k < kb; k += 4) {
+      do_plane(k, A, z0, z1, z2);
+      if (k + 1 >= kb) break;
+      z0 = zl(k + 3);
+      A = pl(k + 2);
+      do_plane(k + 1, B, z1, z2, z3);
+      if (k + 2 >= kb) break;
+      z1 = zl(k + 4);
+      B = pl(k + 3);
+      do_plane(k + 2, A, z2, z3, z0);
+      if (k + 3 >= kb) break;
+      z2 = zl(k + 5);
+      A = pl(k + 4);
+      do_plane(k + 3, B, z3, z0, z1);
+      z3 = zl(k + 6);
+      B = pl(k + 5);
     }
-    r2 = r2 + rs * rs;
-  });
+  }
   const double sblk = block_sum(r2, sh);
   if (lin_tid() == 0) Pout[blockIdx.x] = sblk;
 }
@@ -448,8 +575,12 @@ __global__ void __launch_bounds__(kStencilT) k_mx_residual(GridDims g, CompactDi
 // the launch that would overwrite e_j, before any block writes.
 //
 // A wave owns one row segment of 256 cells, 4 per lane, and marches up a z
-// chunk of its XCD's slab with the z-neighbours in registers; x-neighbours
-// inside the segment come from the adjacent lanes.
+// chunk of its XCD's slab; x-neighbours inside the segment come from the
+// adjacent lanes.  A flow guard, and the z-window of lanes past the row end,
+// read the zero row stored after the last plane of E, so that no loaded value
+// needs a select; padding cells past x = nx - 1 are kept at zero, which makes
+// them the flow guard of x = nx - 1.  PX: x periodic; PAD: nx is not a
+// multiple of 4.
 //
 // The residual is that of the stop test, which the reference evaluates with
 // the periodic halo of the previous iterate (CalcEpotential.f90:83-92): a
@@ -458,6 +589,7 @@ __global__ void __launch_bounds__(kStencilT) k_mx_residual(GridDims g, CompactDi
 // boundary copies (`eo_prev`, null for j < 2); this launch leaves e_{j+1}
 // there in slot (j+1) % 3 (`eo_out`).  e is e_j (null for j = 0) and out
 // receives e_{j+1}.
+template <bool PX, bool PAD>
 __global__ void __launch_bounds__(kStencilT, 4) k_mx_sweep(SweepGeo sg, SweepCoef cf,
                                                    const float *__restrict__ r32,
                                                    const float *__restrict__ e,
@@ -481,6 +613,7 @@ __global__ void __launch_bounds__(kStencilT, 4) k_mx_sweep(SweepGeo sg, SweepCoe
   const int nx = sg.nx, ny = sg.ny, nz = sg.nz;
   const int lane = (int)threadIdx.x;
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const uint32_t zrow = P * (uint32_t)nz;                   // the zero row of E
   double r2 = 0.0;
   const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
   const int z0 = (xcd * nz) >> 3, z1 = ((xcd + 1) * nz) >> 3;
@@ -491,21 +624,18 @@ __global__ void __launch_bounds__(kStencilT, 4) k_mx_sweep(SweepGeo sg, SweepCoe
     const int j = ty * kSwRows + (int)threadIdx.y;
     if (j >= ny || kb <= ka) continue;                      // the whole wave
     const int xs = tx * kSwSeg + 4 * lane;                  // the lane's first cell
-    const bool live = xs < nx;
-    // lanes past the row end mirror the last live lane's cells (loads stay in
-    // bounds, shuffles see defined values) and store nothing
-    const uint32_t x0 = live ? (uint32_t)xs : NX - 4u;
+    const bool live = xs < nx;                              // lanes past the row end store nothing
+    const uint32_t x0 = live ? (uint32_t)xs : 0u;
     const uint32_t rowj = NX * (uint32_t)j, row = x0 + rowj;
-    const bool zn = !sg.py && j == ny - 1, zs = !sg.py && j == 0;   // flow guards: zero
+    const bool zn = !sg.py && j == ny - 1, zs = !sg.py && j == 0;   // flow guards
     const uint32_t rown = x0 + NX * (uint32_t)(j == ny - 1 ? 0 : j + 1);
     const uint32_t rows = x0 + NX * (uint32_t)(j == 0 ? ny - 1 : j - 1);
-    const bool hasf = xs == 0;                              // the lane holds x = 0
     const int ql = nx - 1 - xs;                             // x = nx - 1 at cell ql
-    const bool hasl = live && ql < 4;
-    const bool xedge = sg.px && (hasf || hasl);
-    const bool yedge = sg.py && (j == 0 || j == ny - 1);
-    // cell q of the lane is inside the row (the padding past nx - 1 stays 0)
-    auto inrow = [&](int q) { return live && q <= ql; };
+    const bool hasf = PX && xs == 0;                        // the lane holds x = 0 (periodic x)
+    const bool hasl = PX && live && ql < 4;                 // ... x = nx - 1
+    const bool yN = sg.py && j == ny - 1, yS = sg.py && j == 0;
+    const bool edge_row = hasf || hasl || yN || yS;
+    auto inrow = [&](int q) { return live && (!PAD || q <= ql); };
     if (!e) {
       for (int k = ka; k < kb; ++k) {
         const uint32_t c = row + P * (uint32_t)k;
@@ -517,96 +647,138 @@ __global__ void __launch_bounds__(kStencilT, 4) k_mx_sweep(SweepGeo sg, SweepCoe
         v.w = inrow(3) ? -cf.af * rc.w : 0.0f;
         if (live) {
           st4(out + c, v);
-          if (xedge || yedge || (sg.pz && (k == 0 || k == nz - 1))) st4(eo_out + c, v);
+          if (edge_row || (sg.pz && (k == 0 || k == nz - 1))) st4(eo_out + c, v);
         }
       }
       continue;
     }
-    // e_j on plane kk (past the ends: the periodic image or the zero guard)
+    // e_j on plane kk, -1 <= kk <= nz (past the ends: the periodic image or
+    // the zero row)
     auto zval = [&](int kk) -> float4 {
-      if (kk < 0) return sg.pz ? ld4(e + row + P * (uint32_t)(nz - 1)) : zero4;
-      if (kk >= nz) return sg.pz ? ld4(e + row) : zero4;
-      return ld4(e + row + P * (uint32_t)kk);
+      const int kw = kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk);
+      const bool zero = !live || (!sg.pz && (kk < 0 || kk >= nz));
+      return ld4(e + (zero ? zrow + x0 : row + P * (uint32_t)kw));
     };
-    // e_{j-1} (0 for j = 1)
-    auto prev1 = [&](uint32_t n) -> float { return eo_prev ? eo_prev[n] : 0.0f; };
-    auto prev4 = [&](uint32_t n) -> float4 { return eo_prev ? ld4(eo_prev + n) : zero4; };
-    float4 zm = zval(ka - 1), zc = zval(ka), zp = zval(ka + 1);
-    for (int k = ka; k < kb; ++k) {
-      const uint32_t pk = P * (uint32_t)k;
-      const uint32_t c = row + pk;
-      const float4 en = zn ? zero4 : ld4(e + rown + pk);
-      const float4 es = zs ? zero4 : ld4(e + rows + pk);
-      const float4 rc = ld4(r32 + c);
-      // x-neighbours of the lane's end cells: the adjacent lanes, the
-      // adjacent row segment, or the periodic image / zero guard
+    // what plane kk needs besides the z-window: y-neighbours, right-hand
+    // side, the x-neighbours no lane of the wave holds (lane 0: the previous
+    // segment's last cell or the image of x = nx - 1; lane 63: the next
+    // segment's first cell; the lane with x = nx - 1: the image of x = 0), and
+    // e_{j-1} (0 for j = 1) at the neighbours across periodic boundaries
+    struct PlaneIn {
+      float4 n, s, r, py, pz;
+      float w, eseg, ewrap, pw, pe;
+    };
+    // Loads that may be skipped go first: a wait for this group's last load
+    // then also covers them, and the waits of later groups can count the
+    // unconditional loads exactly.
+    auto plane_in = [&](int kk) -> PlaneIn {
+      const uint32_t pk = P * (uint32_t)kk;
+      PlaneIn a;
+      a.w = 0.0f;
+      a.eseg = 0.0f;
+      a.ewrap = 0.0f;
+      a.py = zero4;
+      a.pz = zero4;
+      a.pw = 0.0f;
+      a.pe = 0.0f;
+      if (eo_prev) {
+        if (yN) a.py = ld4(eo_prev + x0 + pk);                                   // row 0
+        else if (yS) a.py = ld4(eo_prev + x0 + NX * (uint32_t)(ny - 1) + pk);    // row ny - 1
+        if (sg.pz && kk == nz - 1) a.pz = ld4(eo_prev + row);                    // plane 0
+        else if (sg.pz && kk == 0) a.pz = ld4(eo_prev + row + P * (uint32_t)(nz - 1));
+        if (hasf) a.pw = eo_prev[rowj + pk + (uint32_t)(nx - 1)];
+        if (hasl) a.pe = eo_prev[rowj + pk];
+      }
+      if (sg.ntx > 1) {
+        if (lane == 0 && xs > 0) a.w = e[row + pk - 1u];
+        if (lane == 63 && xs + 4 < nx) a.eseg = e[row + pk + 4u];
+      }
+      if (hasf) a.w = e[rowj + pk + (uint32_t)(nx - 1)];
+      if (hasl) a.ewrap = e[rowj + pk];
+      a.n = ld4(e + (zn ? zrow + x0 : rown + pk));
+      a.s = ld4(e + (zs ? zrow + x0 : rows + pk));
+      a.r = ld4(r32 + row + pk);
+      return a;
+    };
+    // plane k from its inputs and the z-window (zb, zc, zt) = planes k-1, k, k+1
+    auto do_plane = [&](int k, const PlaneIn &a, const float4 &zb, const float4 &zc,
+                        const float4 &zt) {
+      const uint32_t c = row + P * (uint32_t)k;
       float wv = __shfl(zc.w, (lane + 63) & 63, 64);
       float ev = __shfl(zc.x, (lane + 1) & 63, 64);
-      if (lane == 0 && xs > 0) wv = e[c - 1];
-      if (lane == 63 && xs + 4 < nx) ev = e[c + 4];
-      if (hasf) wv = sg.px ? e[rowj + pk + (uint32_t)(nx - 1)] : 0.0f;
-      const float wrap_e = (hasl && sg.px) ? e[rowj + pk] : 0.0f;
+      if (lane == 0) wv = a.w;
+      if (lane == 63) ev = a.eseg;
+      const bool zT = sg.pz && k == nz - 1, zB = sg.pz && k == 0;
       const float ce[4] = {zc.x, zc.y, zc.z, zc.w};
-      const float cn[4] = {en.x, en.y, en.z, en.w}, cs[4] = {es.x, es.y, es.z, es.w};
-      const float ct[4] = {zp.x, zp.y, zp.z, zp.w}, cb[4] = {zm.x, zm.y, zm.z, zm.w};
-      const float cr[4] = {rc.x, rc.y, rc.z, rc.w};
-      float ee[4], ew[4], v[4], rr[4];
+      const float cn[4] = {a.n.x, a.n.y, a.n.z, a.n.w}, cs[4] = {a.s.x, a.s.y, a.s.z, a.s.w};
+      const float ct[4] = {zt.x, zt.y, zt.z, zt.w}, cb[4] = {zb.x, zb.y, zb.z, zb.w};
+      const float cr[4] = {a.r.x, a.r.y, a.r.z, a.r.w};
+      const float qy[4] = {a.py.x, a.py.y, a.py.z, a.py.w};
+      const float qz[4] = {a.pz.x, a.pz.y, a.pz.z, a.pz.w};
+      float v[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        ew[q] = q == 0 ? wv : ce[q - 1];
-        ee[q] = (hasl && q == ql) ? wrap_e : (q == 3 ? ev : ce[q + 1]);
+        const float ew = q == 0 ? wv : ce[q - 1];
+        float ee = q == 3 ? ev : ce[q + 1];
+        if (hasl && q == ql) ee = a.ewrap;
         // CalcEpotential.f90:74-80 on the correction, with FP32 weights that
         // sum to 1/2 (kappa carries what rounding leaves, zero on uniform
         // grids): both ends of the Jacobi spectrum, whose modes decay by
         // ~1e-5 per sweep at 256^3, stay where the FP64 iteration has them.
-        v[q] = cf.wx * (ee[q] + ew[q]) + cf.wy * (cn[q] + cs[q]) + cf.wz * (ct[q] + cb[q]) +
+        v[q] = cf.wx * (ee + ew) + cf.wy * (cn[q] + cs[q]) + cf.wz * (ct[q] + cb[q]) +
                cf.kappa * ce[q] - cf.af * cr[q];
+        if (PAD && q > ql) v[q] = 0.0f;
+        // stop-test residual: e_{j-1} across the periodic boundaries
+        const float se = (hasl && q == ql) ? a.pe : ee;
+        const float sw = (hasf && q == 0) ? a.pw : ew;
+        const float sn = yN ? qy[q] : cn[q];
+        const float ss = yS ? qy[q] : cs[q];
+        const float st_ = zT ? qz[q] : ct[q];
+        const float sb = zB ? qz[q] : cb[q];
         const float e2 = 2.0f * ce[q];
-        rr[q] = cr[q] - ((ee[q] + ew[q] - e2) * cf.idx2 + (cn[q] + cs[q] - e2) * cf.idy2 +
-                         (ct[q] + cb[q] - e2) * cf.idz2);
-        if (!inrow(q)) v[q] = 0.0f;
+        const float rr = cr[q] - ((se + sw - e2) * cf.idx2 + (sn + ss - e2) * cf.idy2 +
+                                  (st_ + sb - e2) * cf.idz2);
+        if (inrow(q)) r2 = r2 + (double)rr * (double)rr;
       }
       const float4 v4 = make_float4(v[0], v[1], v[2], v[3]);
-      if (live) st4(out + c, v4);
-      const bool zedge = sg.pz && (k == 0 || k == nz - 1);
-      if (xedge || yedge || zedge) {
-        if (live) st4(eo_out + c, v4);
-        // stop-test residual with e_{j-1} across the periodic boundaries
-        const float4 pn = (sg.py && j == ny - 1) ? prev4(x0 + pk) : zero4;
-        const float4 ps = (sg.py && j == 0) ? prev4(x0 + NX * (uint32_t)(ny - 1) + pk) : zero4;
-        const float4 pt = (sg.pz && k == nz - 1) ? prev4(row) : zero4;
-        const float4 pb = (sg.pz && k == 0) ? prev4(row + P * (uint32_t)(nz - 1)) : zero4;
-        const float pw = (sg.px && hasf) ? prev1(rowj + pk + (uint32_t)(nx - 1)) : 0.0f;
-        const float pe = (sg.px && hasl) ? prev1(rowj + pk) : 0.0f;
-        const float qn[4] = {pn.x, pn.y, pn.z, pn.w}, qs[4] = {ps.x, ps.y, ps.z, ps.w};
-        const float qt[4] = {pt.x, pt.y, pt.z, pt.w}, qb[4] = {pb.x, pb.y, pb.z, pb.w};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const float se = (sg.px && hasl && q == ql) ? pe : ee[q];
-          const float sw = (sg.px && hasf && q == 0) ? pw : ew[q];
-          const float sn = (sg.py && j == ny - 1) ? qn[q] : cn[q];
-          const float ss = (sg.py && j == 0) ? qs[q] : cs[q];
-          const float st_ = (sg.pz && k == nz - 1) ? qt[q] : ct[q];
-          const float sb = (sg.pz && k == 0) ? qb[q] : cb[q];
-          const float e2 = 2.0f * ce[q];
-          rr[q] = cr[q] - ((se + sw - e2) * cf.idx2 + (sn + ss - e2) * cf.idy2 +
-                           (st_ + sb - e2) * cf.idz2);
-        }
+      if (live) {
+        st4(out + c, v4);
+        if (edge_row || zT || zB) st4(eo_out + c, v4);
       }
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (inrow(q)) r2 = r2 + (double)rr[q] * (double)rr[q];
-      zm = zc;
-      zc = zp;
-      if (k + 1 < kb) zp = zval(k + 2);
+    };
+    // Four planes per pass: the window registers z0..z3 rotate by name, and
+    // each load goes out two planes before its use, right after the plane
+    // that last read its register.
+    auto zl = [&](int kk) { return zval(kk < kb ? kk : kb); };
+    auto pl = [&](int kk) { return plane_in(kk < kb ? kk : kb - 1); };
+    float4 z0 = zl(ka - 1), z1 = zl(ka), z2 = zl(ka + 1), z3 = zl(ka + 2);
+    PlaneIn A = pl(ka), B = pl(ka + 1);
+    for (int k = ka; k < kb; k += 4) {
+      do_plane(k, A, z0, z1, z2);
+      if (k + 1 >= kb) break;
+      z0 = zl(k + 3);
+      A = pl(k + 2);
+      do_plane(k + 1, B, z1, z2, z3);
+      if (k + 2 >= kb) break;
+      z1 = zl(k + 4);
+      B = pl(k + 3);
+      do_plane(k + 2, A, z2, z3, z0);
+      if (k + 3 >= kb) break;
+      z2 = zl(k + 5);
+      A = pl(k + 4);
+      do_plane(k + 3, B, z3, z0, z1);
+      z3 = zl(k + 6);
+      B = pl(k + 5);
     }
   }
   const double sblk = block_sum(r2, sh);
   if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
 }
 
-// u += e_k: k = m after a full outer step, or the converged (or last)
-// iterate's k when the solve ended inside this step.
+// End of an outer step: the stop test of its last inner iterate and, when
+// the solve ended inside this step (converged, or the last iterate reached),
+// u += e_k for that iterate's k; otherwise nothing (the next outer step
+// applies e_m).
 __global__ void __launch_bounds__(kStencilT) k_mx_update(GridDims g, CompactDims cd, TileSched ts,
                                                     double *__restrict__ u, const float *E0,
                                                     const float *E1, const double *__restrict__ Pin,
@@ -622,7 +794,8 @@ __global__ void __launch_bounds__(kStencilT) k_mx_update(GridDims g, CompactDims
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   // solves that ended before this outer step have nothing left to apply
   if (prev.done && prev.result < L.base) return;
-  const int64_t k = cur.done ? cur.result - L.base : (int64_t)L.m;
+  if (!cur.done) return;
+  const int64_t k = cur.result - L.base;
   if (k <= 0) return;
   const float *e = (k & 1) ? E1 : E0;
   for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int kk, bool valid) {
@@ -864,11 +1037,22 @@ int sp_gridify(picard_ctx *c, bool flux) {
   return PICARD_OK;
 }
 
+// One sweep launch (timed with events t0/t1 when given).
+template <bool PX>
+void launch_sweep(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t t1,
+                  const SweepGeo &sg, const SweepCoef &cf, const float *r32, const float *e,
+                  float *out, float *eo_out, const float *eo_prev, const double *pin, double *pout,
+                  JacobiState *js, const MxLaunch &L) {
+  auto fn = pad ? k_mx_sweep<PX, true> : k_mx_sweep<PX, false>;
+  hipExtLaunchKernelGGL(fn, dim3(blocks), dim3(64, kSwRows), 0, s, t0, t1, 0, sg, cf, r32, e,
+                        out, eo_out, eo_prev, pin, pout, js, L);
+}
+
 // Resident blocks per XCD of the sweep kernel (the heaviest of the solve).
 int sweep_wmax(picard_ctx *c) {
   if (c->sweep_wmax <= 0) {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_sweep, kStencilT, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_sweep<false, false>, kStencilT, 0) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
         per_cu <= 0 || cus < 8) {
       c->sweep_wmax = 256;
@@ -918,10 +1102,55 @@ SweepGeo sweep_sched(picard_ctx *c) {
   if (nzc < 1) nzc = 1;
   sg.nzc = nzc;
   sg.TP = per_chunk * nzc;
+  // equal tiles per block: the fewest blocks that need no more passes
   int W = sg.TP < wmax ? sg.TP : wmax;
+  W = (sg.TP + (sg.TP + W - 1) / W - 1) / ((sg.TP + W - 1) / W);
   if (W > (int)(c->s.partials_len / 8)) W = (int)(c->s.partials_len / 8);
   sg.W = W < 1 ? 1 : W;
   return sg;
+}
+
+// Tiles of the outer-step kernel: 64-cell row segments x 4-row groups x z
+// chunks of an XCD slab, about half the resident blocks per XCD, as the sweep.
+OuterGeo outer_sched(picard_ctx *c) {
+  const GridDims &g = c->gd;
+  if (c->outer_wmax <= 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_outer<true>, kStencilT, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        per_cu <= 0 || cus < 8) {
+      c->outer_wmax = 128;
+    } else {
+      c->outer_wmax = (cus / 8) * per_cu;
+    }
+  }
+  OuterGeo og;
+  og.nx = g.nx;
+  og.ny = g.ny;
+  og.nz = g.nz;
+  og.X = (uint32_t)g.X;
+  og.XY = (uint32_t)g.X * (uint32_t)g.Y;
+  og.NX = (uint32_t)(g.nx + 3) & ~3u;
+  og.P = og.NX * (uint32_t)g.ny;
+  og.px = !g.flow[0];
+  og.py = !g.flow[1];
+  og.pz = !g.flow[2];
+  og.ntx = (g.nx + 63) / 64;
+  og.nty = (g.ny + kSwRows - 1) / kSwRows;
+  const int wmax = c->outer_wmax;
+  const int per_chunk = og.ntx * og.nty;
+  int nzc = (wmax + 2 * per_chunk - 1) / (2 * per_chunk);
+  const int zmax = g.nz / 64 > 1 ? g.nz / 64 : 1;
+  if (nzc > zmax) nzc = zmax;
+  if (nzc < 1) nzc = 1;
+  og.nzc = nzc;
+  og.TP = per_chunk * nzc;
+  // equal tiles per block: the fewest blocks that need no more passes
+  int W = og.TP < wmax ? og.TP : wmax;
+  W = (og.TP + (og.TP + W - 1) / W - 1) / ((og.TP + W - 1) / W);
+  if (W > (int)(c->s.partials_len / 8)) W = (int)(c->s.partials_len / 8);
+  og.W = W < 1 ? 1 : W;
+  return og;
 }
 
 int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) {
@@ -945,8 +1174,15 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
   PICARD_TRY(launch_bnorm(c, maxerr));      // launch 0: b, |b|, the test of ss = 1
   const TileSched ts = stencil_sched(c);
   const SweepGeo sg = sweep_sched(c);
+  const OuterGeo og = outer_sched(c);
   const CompactDims cd{sg.NX, sg.P};
-  const int swB = 8 * sg.W;
+  const int swB = 8 * sg.W, ouB = 8 * og.W;
+  const bool pad = (g.nx & 3) != 0;
+  // u ping-pongs between U and the work buffer (outer step s reads Ub[s % 2]
+  // and writes Ub[(s + 1) % 2]); both carry the boundary values in the guards
+  double *Ub[2] = {c->U, c->Uw};
+  PICARD_HIP(hipMemcpyAsync(c->Uw, c->U, sizeof(double) * g.G, hipMemcpyDeviceToDevice, c->stream));
+  int outer = 0;
   SweepCoef cf{};
   {
     // FP32 weights with 2 (wx + wy + wz) = 1 as closely as FP32 allows
@@ -976,8 +1212,11 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
   int32_t nin = 0;                          // partials the last writer stored
   StopSlot fin{};
   while (true) {
+    // timing: events around one sweep in kTimeEvery (event records between
+    // every pair of sweeps would slow the timed solve itself)
+    const int timed = (m + kTimeEvery - 1) / kTimeEvery;
     if (c->timing) {
-      while ((int64_t)c->tev.size() < 2 * m) {
+      while ((int64_t)c->tev.size() < 2 * timed) {
         hipEvent_t e;
         PICARD_HIP(hipEventCreate(&e));
         c->tev.push_back(e);
@@ -985,15 +1224,20 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
     }
     // outer residual of iterate `base` (its stop test happens in the next
     // launch); the stale-halo norm needs e_m and e_{m-1} of the previous outer
+    double *usrc = Ub[outer & 1], *udst = Ub[(outer + 1) & 1];
     {
       MxLaunch L{seq, base, 0, -1, m, 0};
-      const float *em = base > 0 ? c->e32[m & 1] : nullptr;
-      const float *eo = base > 0 ? EO[(m - 1) % 3] : nullptr;
-      k_mx_residual<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, cd, mc, ts, c->U, c->b, c->r32,
-                                                                em, eo, P[pw ^ 1], P[pw], c->js, L);
+      if (base > 0)
+        k_mx_outer<true><<<ouB, dim3(64, kSwRows), 0, c->stream>>>(
+            og, mc.dx2, mc.dy2, mc.dz2, usrc, udst, c->b, c->r32, c->e32[m & 1], EO[(m - 1) % 3],
+            P[pw ^ 1], P[pw], c->js, L);
+      else
+        k_mx_outer<false><<<ouB, dim3(64, kSwRows), 0, c->stream>>>(
+            og, mc.dx2, mc.dy2, mc.dz2, usrc, udst, c->b, c->r32, nullptr, nullptr, P[pw ^ 1],
+            P[pw], c->js, L);
       ++seq;
       check = base;
-      nin = ts.B;
+      nin = ouB;
       pw ^= 1;
     }
     for (int j = 0; j < m; ++j) {
@@ -1008,13 +1252,15 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       float *out = E[(j + 1) & 1];
       float *eo_out = EO[(j + 1) % 3];
       const float *eo_prev = j >= 2 ? EO[(j - 1) % 3] : nullptr;
-      if (c->timing)
-        hipExtLaunchKernelGGL(k_mx_sweep, dim3(swB), dim3(64, kSwRows), 0, c->stream,
-                              c->tev[2 * j], c->tev[2 * j + 1], 0, sg, cf,
-                              (const float *)c->r32, e, out, eo_out, eo_prev, pin, pout, c->js, L);
+      const bool tj = c->timing && j % kTimeEvery == 0;
+      hipEvent_t t0 = tj ? c->tev[2 * (j / kTimeEvery)] : nullptr;
+      hipEvent_t t1 = tj ? c->tev[2 * (j / kTimeEvery) + 1] : nullptr;
+      if (sg.px)
+        launch_sweep<true>(pad, swB, c->stream, t0, t1, sg, cf, c->r32, e, out, eo_out, eo_prev,
+                           pin, pout, c->js, L);
       else
-        k_mx_sweep<<<swB, dim3(64, kSwRows), 0, c->stream>>>(sg, cf, c->r32, e, out, eo_out,
-                                                             eo_prev, pin, pout, c->js, L);
+        launch_sweep<false>(pad, swB, c->stream, t0, t1, sg, cf, c->r32, e, out, eo_out, eo_prev,
+                            pin, pout, c->js, L);
       ++seq;
       if (j >= 1) {               // wrote the residual of iterate base + j
         check = base + j;
@@ -1027,7 +1273,7 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
     {
       const int32_t chk = (check >= 1) ? check : -1;
       MxLaunch L{seq, base, 0, chk, m, nin};
-      k_mx_update<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, cd, ts, c->U, E0, E1, P[pw ^ 1],
+      k_mx_update<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, cd, ts, udst, E0, E1, P[pw ^ 1],
                                                               c->js, L);
       ++seq;
       check = -1;
@@ -1037,16 +1283,22 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
                               c->stream));
     PICARD_HIP(hipStreamSynchronize(c->stream));
     if (c->timing) {
-      for (int q = 0; q < m; ++q) {
+      for (int q = 0; q < timed; ++q) {
         float ms = 0.0f;
         PICARD_HIP(hipEventElapsedTime(&ms, c->tev[2 * q], c->tev[2 * q + 1]));
         c->t_ms += ms;
       }
-      c->t_launches += m;
+      c->t_launches += timed;
     }
     fin = c->js_host->slot[(seq - 1) & 1];
-    if (fin.done) break;
+    if (fin.done) {
+      if (udst != c->U)
+        PICARD_HIP(hipMemcpyAsync(c->U, udst, sizeof(double) * g.G, hipMemcpyDeviceToDevice,
+                                  c->stream));
+      break;
+    }
     base += m;
+    ++outer;
   }
   k_mx_halo<<<grid_blocks(g.G, kT), kT, 0, c->stream>>>(g, c->U);
   PICARD_HIP(hipGetLastError());
