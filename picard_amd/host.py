@@ -95,6 +95,62 @@ def config2(n: int = 256, **kw) -> Config:
     return cfg
 
 
+def config3(n: int = 512, **kw) -> Config:
+    """The mixed-precision Poisson case: n^3 cells of 10 m, every boundary
+    open (the BCpotentials face potential on x, y and z), solved to relative
+    residual 1e-10; the charge is poisson_charge()."""
+    half = 5.0 * n
+    base = dict(nx=n, ny=n, nz=n, flow=(True, True, True), gmin=(-half,) * 3, gmax=(half,) * 3,
+                maxerr=1.0e-10)
+    base.update(kw)
+    return Config(**base)
+
+
+def poisson_charge(cfg: Config, seed: int = 20261205, blobs: int = 16,
+                   Q: float = HASER_Q_STATED, vn: float = 1.0e3, L: float = 1.0e6) -> np.ndarray:
+    """Charge per cell [C] of the Poisson case, the array CalcEpotential reads
+    (b = -C / (dV eps0), CalcEpotential.f90:50), [Z+2, Y+2, X+2] with zero
+    guards.  `blobs` Gaussian blobs (centres uniform in the interior, sigma
+    uniform in 4-20 cells, amplitude uniform in +-1e-3 e n of the plasma
+    density n) plus e times the Haser ion density Q / (4 pi vn r^2) exp(-r/L)
+    of a source at the origin, r taken as ParticlesProduce.f90:46-58 takes it:
+    the cell centre's distance, or, where (4 r)^3 < dV, the mean distance of
+    the 8 points at the quarter offsets of the cell."""
+    d = derive(cfg)
+    rng = np.random.default_rng(seed)
+    dx, dy, dz = (float(v) for v in d.dxyz[:3])
+    x0 = cfg.gmin[0] + dx * np.arange(cfg.nx)          # lower cell edges
+    y0 = cfg.gmin[1] + dy * np.arange(cfg.ny)
+    z0 = cfg.gmin[2] + dz * np.arange(cfg.nz)
+    Z, Y, X = np.meshgrid(z0, y0, x0, indexing="ij")
+    xc, yc, zc = X + 0.5 * dx, Y + 0.5 * dy, Z + 0.5 * dz
+    amp = 1.0e-3 * EC * float(cfg.density[1])
+    rho = np.zeros_like(xc)
+    for _ in range(blobs):
+        cx = cfg.gmin[0] + dx * rng.uniform(0.0, cfg.nx)
+        cy = cfg.gmin[1] + dy * rng.uniform(0.0, cfg.ny)
+        cz = cfg.gmin[2] + dz * rng.uniform(0.0, cfg.nz)
+        sig = rng.uniform(4.0, 20.0) * dx
+        a = rng.uniform(-amp, amp)
+        rho += a * np.exp(-((xc - cx) ** 2 + (yc - cy) ** 2 + (zc - cz) ** 2) / (2.0 * sig * sig))
+    r2 = xc * xc + yc * yc + zc * zc
+    r1 = np.sqrt(r2)
+    near = (4.0 * r1) ** 3 < d.dV
+    if near.any():
+        acc = np.zeros(int(near.sum()))
+        for fx, fy, fz in ((0.25, 0.25, 0.25), (0.25, 0.25, 0.75), (0.25, 0.75, 0.25),
+                           (0.75, 0.25, 0.25), (0.25, 0.75, 0.75), (0.75, 0.25, 0.75),
+                           (0.75, 0.75, 0.25), (0.75, 0.75, 0.75)):        # the reference's order
+            acc = acc + np.sqrt((X[near] + fx * dx) ** 2 + (Y[near] + fy * dy) ** 2 +
+                                (Z[near] + fz * dz) ** 2)
+        r1[near] = acc * 0.125
+        r2[near] = r1[near] ** 2
+    rho += EC * Q / (2.0 * TAU * r2 * vn) * np.exp(-r1 / L)
+    C = np.zeros((cfg.nz + 2, cfg.ny + 2, cfg.nx + 2))
+    C[1:-1, 1:-1, 1:-1] = rho * d.dV
+    return C
+
+
 @dataclass
 class Derived:
     dxyz: np.ndarray

@@ -188,3 +188,31 @@ def test_philox_known_answers():
     for ctr, key, want in cases:
         got = tuple(int(w) for w in philox4x32(*ctr, key))
         assert got == want, [hex(v) for v in got]
+
+
+def test_poisson_charge_haser_term_matches_oracle_distance():
+    """The Poisson case's Haser term per cell equals e Q/(4 pi vn r^2) exp(-r/L) dV
+    with r from the oracle's restatement of ParticlesProduce.f90:41-59 (itself
+    pinned against the reference build); the box is symmetric about the source,
+    so the charge is too; the blobs add a bounded part."""
+    from oracle import haser
+    cfg = host.config3(10)
+    d = host.derive(cfg)
+    dx, dy, dz = (float(v) for v in d.dxyz[:3])
+    C = host.poisson_charge(cfg, blobs=0)
+    Q, vn, L = host.HASER_Q_STATED, 1.0e3, 1.0e6
+    ref = np.zeros_like(C)
+    for k in range(cfg.nz):
+        for j in range(cfg.ny):
+            for i in range(cfg.nx):
+                r1, r2 = haser.cell_distance(cfg.gmin[0] + dx * i, cfg.gmin[1] + dy * j,
+                                             cfg.gmin[2] + dz * k, dx, dy, dz)
+                ref[k + 1, j + 1, i + 1] = (host.EC * Q / (2.0 * host.TAU * r2 * vn) *
+                                            np.exp(-r1 / L) * d.dV)
+    np.testing.assert_allclose(C, ref, rtol=1e-14, atol=0)
+    inner = C[1:-1, 1:-1, 1:-1]
+    np.testing.assert_allclose(inner, inner[::-1, ::-1, ::-1], rtol=1e-14)
+    Cb = host.poisson_charge(cfg)
+    amp = 1.0e-3 * host.EC * cfg.density[1] * d.dV
+    assert np.all(np.abs(Cb - C) <= 16 * amp * (1 + 1e-12))
+    assert np.any(Cb != C)
