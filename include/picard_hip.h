@@ -204,6 +204,11 @@ int picard_field_write(picard_ctx *ctx, int32_t which, const double *host, int64
  * launches since the last enable. */
 int picard_timing_enable(picard_ctx *ctx, int32_t enable);
 int picard_timing_read(picard_ctx *ctx, double *jacobi_ms, int64_t *jacobi_launches);
+/* FP32-split solve: `inner` FP32 sweeps per FP64 refinement step, 2..4096
+ * (default 128).  No reference counterpart; the stop rule is applied to every
+ * iterate whatever the choice, and the solution agrees with the FP64 form's
+ * to the split form's tolerance for each. */
+int picard_split_solver(picard_ctx *ctx, int32_t inner);
 
 /* One whole main-loop body after BCinflow (picard.f90:564-631, no output
  * step): AdvancePosition, append `produced` (ParticlesProduce's particles,

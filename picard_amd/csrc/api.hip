@@ -595,6 +595,17 @@ int picard_timing_read(picard_ctx *c, double *jacobi_ms, int64_t *jacobi_launche
   return PICARD_OK;
 }
 
+int picard_split_solver(picard_ctx *c, int32_t inner) {
+  CTX_CHECK(c);
+  if (inner < 2 || inner > 4096) {
+    set_error("picard_split_solver: inner sweeps per refinement must be in [2, 4096], got " +
+              std::to_string(inner));
+    return PICARD_ERR_ARG;
+  }
+  c->mx_inner = inner;
+  return PICARD_OK;
+}
+
 int picard_step(picard_ctx *c, double dt, double maxerr, int32_t maxit, const double *produced,
                 const int32_t *produced_species, int64_t nproduced, picard_step_stats *stats) {
   CTX_CHECK(c);

@@ -975,7 +975,7 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
     const int64_t todo = (last + 1 - (p - 1)) < batch ? (last + 1 - (p - 1)) : batch;
     // timing: events around one pass in kTimeEvery (event records between
     // every pair of passes would slow the timed solve itself)
-    const int64_t timed = (todo + kTimeEvery - 1) / kTimeEvery;
+    const int64_t timed = (todo + kTimeEvery / 2 - 1) / kTimeEvery;   // passes q = 8, 24, ...
     if (c->timing) {
       while ((int64_t)c->tev.size() < 2 * timed) {
         hipEvent_t e;
@@ -984,7 +984,7 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
       }
     }
     for (int64_t q = 0; q < todo; ++q, ++p) {
-      if (c->timing && q % kTimeEvery == 0)
+      if (c->timing && q % kTimeEvery == kTimeEvery / 2)
         hipExtLaunchKernelGGL(k_jacobi_pass, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
                               c->tev[2 * (q / kTimeEvery)], c->tev[2 * (q / kTimeEvery) + 1], 0,
                               pg, jc, c->jb[0], c->jb[1], c->jb[2], (const double *)c->b,

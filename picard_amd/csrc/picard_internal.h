@@ -124,7 +124,7 @@ struct picard_ctx {
   // side, ping-pong correction, inner iterates j % 3 on periodic boundary planes
   float *r32 = nullptr, *e32[2] = {nullptr, nullptr};
   float *eo32[3] = {nullptr, nullptr, nullptr};
-  int mx_inner = 64;         // split: FP32 sweeps per FP64 refinement
+  int mx_inner = 128;        // split: FP32 sweeps per FP64 refinement
   int outer_wmax = 0;        // split: resident blocks per XCD of the outer-step kernel
   double *Uw = nullptr;      // split: second FP64 potential buffer of the solve
   int sweep_wmax = 0;        // split: resident blocks per XCD of the sweep kernel (0: not yet known)

@@ -715,7 +715,7 @@ __global__ void __launch_bounds__(kStencilT, 4) k_mx_sweep(SweepGeo sg, SweepCoe
       const float cr[4] = {a.r.x, a.r.y, a.r.z, a.r.w};
       const float qy[4] = {a.py.x, a.py.y, a.py.z, a.py.w};
       const float qz[4] = {a.pz.x, a.pz.y, a.pz.z, a.pz.w};
-      float v[4];
+      float v[4], sq = 0.0f;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const float ew = q == 0 ? wv : ce[q - 1];
@@ -738,8 +738,9 @@ __global__ void __launch_bounds__(kStencilT, 4) k_mx_sweep(SweepGeo sg, SweepCoe
         const float e2 = 2.0f * ce[q];
         const float rr = cr[q] - ((se + sw - e2) * cf.idx2 + (sn + ss - e2) * cf.idy2 +
                                   (st_ + sb - e2) * cf.idz2);
-        if (inrow(q)) r2 = r2 + (double)rr * (double)rr;
+        if (inrow(q)) sq = sq + rr * rr;       // the lane's 4 squares in FP32
       }
+      r2 = r2 + (double)sq;
       const float4 v4 = make_float4(v[0], v[1], v[2], v[3]);
       if (live) {
         st4(out + c, v4);
@@ -1079,8 +1080,9 @@ TileSched stencil_sched(picard_ctx *c) {
 
 // Tiles of the sweep kernel: row segments x 4-row groups x z chunks of an
 // XCD slab.  The slab is cut into chunks only until the tiles give about half
-// the resident blocks per XCD (at 256^3: two chunks, 1024 blocks of 4 waves,
-// beat both one and four chunks; longer z marches re-read fewer planes).
+// the resident blocks per XCD (at 256^3 and 4 blocks per CU: one chunk, 512
+// blocks of 4 waves marching 32 planes, measured 31 us against 38 us for two
+// chunks and 1024 blocks).
 SweepGeo sweep_sched(picard_ctx *c) {
   const GridDims &g = c->gd;
   SweepGeo sg;
@@ -1212,9 +1214,10 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
   int32_t nin = 0;                          // partials the last writer stored
   StopSlot fin{};
   while (true) {
-    // timing: events around one sweep in kTimeEvery (event records between
-    // every pair of sweeps would slow the timed solve itself)
-    const int timed = (m + kTimeEvery - 1) / kTimeEvery;
+    // timing: events around one sweep in kTimeEvery, mid-way between outer
+    // steps (event records between every pair of sweeps would slow the timed
+    // solve itself)
+    const int timed = (m + kTimeEvery / 2 - 1) / kTimeEvery;     // sweeps j = 8, 24, ...
     if (c->timing) {
       while ((int64_t)c->tev.size() < 2 * timed) {
         hipEvent_t e;
@@ -1227,10 +1230,13 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
     double *usrc = Ub[outer & 1], *udst = Ub[(outer + 1) & 1];
     {
       MxLaunch L{seq, base, 0, -1, m, 0};
+      // e_m and the boundary copies of e_{m-1} of the previous outer step
+      const float *em = E[m & 1];
+      const float *eo = EO[(m - 1) % 3];
       if (base > 0)
         k_mx_outer<true><<<ouB, dim3(64, kSwRows), 0, c->stream>>>(
-            og, mc.dx2, mc.dy2, mc.dz2, usrc, udst, c->b, c->r32, c->e32[m & 1], EO[(m - 1) % 3],
-            P[pw ^ 1], P[pw], c->js, L);
+            og, mc.dx2, mc.dy2, mc.dz2, usrc, udst, c->b, c->r32, em, eo, P[pw ^ 1], P[pw],
+            c->js, L);
       else
         k_mx_outer<false><<<ouB, dim3(64, kSwRows), 0, c->stream>>>(
             og, mc.dx2, mc.dy2, mc.dz2, usrc, udst, c->b, c->r32, nullptr, nullptr, P[pw ^ 1],
@@ -1252,7 +1258,7 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       float *out = E[(j + 1) & 1];
       float *eo_out = EO[(j + 1) % 3];
       const float *eo_prev = j >= 2 ? EO[(j - 1) % 3] : nullptr;
-      const bool tj = c->timing && j % kTimeEvery == 0;
+      const bool tj = c->timing && j % kTimeEvery == kTimeEvery / 2;
       hipEvent_t t0 = tj ? c->tev[2 * (j / kTimeEvery)] : nullptr;
       hipEvent_t t1 = tj ? c->tev[2 * (j / kTimeEvery) + 1] : nullptr;
       if (sg.px)

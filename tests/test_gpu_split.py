@@ -182,3 +182,45 @@ def test_split_bc_potentials_fft(dims, flow):
     assert np.abs(b.field(_lib.FIELD_U) + 2.0 * Ua).max() <= 1e-12 * 2.0 * np.abs(Ua).max()
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("dims,flow", [((16, 16, 16), (True, False, False)),
+                                       ((13, 10, 9), (False, True, False)),
+                                       ((20, 9, 11), (True, True, True))])
+def test_split_inner_length_keeps_solution(dims, flow):
+    """The number of FP32 sweeps per FP64 refinement step changes where the
+    refinement happens, not the iteration: every choice stops within two
+    sweeps of the FP64 form at the same solution (1e-5)."""
+    nx, ny, nz = dims
+    cfg = host.Config(nx=nx, ny=ny, nz=nz, flow=flow,
+                      gmin=(-5.0 * nx, -5.0 * ny, -5.0 * nz), gmax=(5.0 * nx, 5.0 * ny, 5.0 * nz),
+                      ppc=(0, 8, 0, 0, 0, 0, 0, 0))
+    d = host.derive(cfg)
+    rng = np.random.default_rng(nx * 100 + ny)
+    C = rng.normal(size=(nz + 2, ny + 2, nx + 2)) * 1e-16
+    out = {}
+    for prec, inner in ((_lib.PICARD_FP64, None), (_lib.PICARD_FP32_SPLIT, 2),
+                        (_lib.PICARD_FP32_SPLIT, 7), (_lib.PICARD_FP32_SPLIT, 64),
+                        (_lib.PICARD_FP32_SPLIT, 128)):
+        dom = host.Domain(cfg, max_particles=1024, d=d, precision=prec)
+        if inner:
+            dom.split_solver(inner=inner)
+        dom.calc_distance()
+        dom.set_field(_lib.FIELD_C, C)
+        dom.bc_potentials()
+        out[inner] = (dom.calc_epotential(), dom.field(_lib.FIELD_U))
+        dom.close()
+    s0, U0 = out[None]
+    for inner in (2, 7, 64, 128):
+        s1, U1 = out[inner]
+        _say(f"dims={dims} inner={inner}: sweeps {s1} (fp64 {s0}) U rel {_rel(U1, U0):.2e}")
+        assert abs(s1 - s0) <= 2
+        assert _rel(U1, U0) <= 1e-5
+
+
+def test_split_solver_rejects_bad_inner():
+    cfg = _cfg(8)
+    dom = host.Domain(cfg, max_particles=1024, precision=_lib.PICARD_FP32_SPLIT)
+    with pytest.raises(RuntimeError):
+        dom.split_solver(inner=1)
+    dom.close()
