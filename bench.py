@@ -3,6 +3,8 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 256]
                     [--form both|split|fp64] [--no-cpu-baseline]
+    python bench.py --solve-only SWEEPS [--form split|fp64]   (profiling)
+    python bench.py --copy-cal                                (profiling)
 
 Workload: 256^3 cells of 10 m, H+ and e- at 32 per cell (1.07e9 particles),
 drifting Maxwellian 400 km/s at 10 eV, B = (0, 0, 5 nT), x in/outflow, y/z
@@ -131,6 +133,53 @@ def cpu_baseline(budget_s=25.0):
                        f"restatement in reference order, FP64, 1 thread, {el:.1f} s")
 
 
+def solve_only(cfg, form, sweeps):
+    """Profiling mode: one field solve of the workload's grid on a seeded
+    random charge, capped at `sweeps` sweeps (no particles), so that counter
+    passes over every dispatch stay short."""
+    from picard_amd import _lib, host
+    d = host.derive(cfg)
+    prec = _lib.PICARD_FP32_SPLIT if form == "split" else _lib.PICARD_FP64
+    dom = host.Domain(cfg, max_particles=1024, d=d, precision=prec)
+    dom.calc_distance()
+    rng = np.random.default_rng(SEED)
+    dom.set_field(_lib.FIELD_C, rng.normal(size=(cfg.nz + 2, cfg.ny + 2, cfg.nx + 2)) * 1e-16)
+    dom.bc_potentials()
+    dom.synchronize()
+    t0 = time.perf_counter()
+    sw = dom.calc_epotential(maxerr=1e-30, maxit=max(sweeps // 2, 1))
+    dom.synchronize()
+    el = time.perf_counter() - t0
+    dom.close()
+    print(json.dumps({"mode": "solve-only", "form": form, "grid": [cfg.nx, cfg.ny, cfg.nz],
+                      "sweeps": sw, "seconds": el}), flush=True)
+
+
+def copy_calibration():
+    """Counter calibration: three device copies of 1 GiB (read 1 GiB, write
+    1 GiB each), the plain copy kernel the HBM byte counters are scaled by."""
+    import torch
+    a = torch.zeros(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": "copy-cal", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
+                      "copies": 3}), flush=True)
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/hbm_traffic_256cube.json: FETCH_SIZE and WRITE_SIZE per
+    dispatch, each scaled by what it reads for a 1 GiB copy), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_256cube.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +188,10 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--form", choices=("both", "split", "fp64"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--solve-only", type=int, default=0, metavar="SWEEPS",
+                    help="profiling: one capped field solve of --form (split or fp64) only")
+    ap.add_argument("--copy-cal", action="store_true",
+                    help="profiling: the 1 GiB copies the byte counters are scaled by")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the domain "
@@ -147,6 +200,12 @@ def main():
         sys.exit(2)
     from picard_amd import _lib, host
     cfg = host.config2(args.grid)
+    if args.copy_cal:
+        copy_calibration()
+        return
+    if args.solve_only:
+        solve_only(cfg, "fp64" if args.form == "fp64" else "split", args.solve_only)
+        return
     runs = {}
     if args.form in ("both", "split"):
         runs["split"] = run_form(cfg, _lib.PICARD_FP32_SPLIT, args.steps, args.warmup)
@@ -207,7 +266,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_from_profiles("k_mx_sweep" if headname == "split"
+                                             else "k_jacobi_pass"),
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": avg_s * 1e6,
             "timed_launches": head["jac_launches"],
