@@ -20,6 +20,7 @@ from __future__ import annotations
 import ctypes as ct
 import os
 import subprocess
+import sys
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -430,3 +431,52 @@ def ref_dump_fields(U, N, F, E, nx, ny, nz, iteration, species, myid, path):
             with open(os.path.join(d, name), "rb") as f:
                 out[name] = f.read()
     return out
+
+
+def _byref_d(x):
+    return ct.byref(ct.c_double(float(x)))
+
+
+def ref_write_warnings(cfg, d, path, procs=(1, 1, 1)):
+    """WriteWarnings(mass, charge, eta, kelvin, density, v0, ve, dt, n2p,
+    dxyz, B0, Nx, Ny, Nz, xmin..zmax, iprocs, jprocs, kprocs, gyration,
+    xflow, yflow, zflow, Qn, vn, Ek, nu_i, nu_d, ppc) (WriteWarnings.f90:4-...)
+    as picard.f90:488-489 calls it, writing under `path`.  Returns the text it
+    printed on standard output and the bytes of save/parameters.dat and
+    save/warnings.dat.  Logicals are passed as 4-byte integers 1/0."""
+    import tempfile
+    R = ref()
+    lg = lambda b: ct.byref(ct.c_int32(1 if b else 0))
+    v0 = np.ascontiguousarray(d.v0, np.float64)           # (8, 4) C == v0(4, 8)
+    args = [_d(np.ascontiguousarray(d.mass, np.float64)), _d(np.ascontiguousarray(d.charge, np.float64)),
+            _d(np.ascontiguousarray(d.eta, np.float64)), _d(np.ascontiguousarray(d.kelvin, np.float64)),
+            _d(np.ascontiguousarray(d.density, np.float64)), _d(v0),
+            _d(np.ascontiguousarray(d.ve, np.float64)), _byref_d(d.dt),
+            _d(np.ascontiguousarray(d.n2p, np.float64)), _d(np.ascontiguousarray(d.dxyz[:3], np.float64)),
+            _d(np.ascontiguousarray(d.B0, np.float64)),
+            _byref_i(cfg.nx), _byref_i(cfg.ny), _byref_i(cfg.nz),
+            _byref_d(cfg.gmin[0]), _byref_d(cfg.gmax[0]), _byref_d(cfg.gmin[1]),
+            _byref_d(cfg.gmax[1]), _byref_d(cfg.gmin[2]), _byref_d(cfg.gmax[2]),
+            _byref_i(procs[0]), _byref_i(procs[1]), _byref_i(procs[2]),
+            _i(np.array([1 if g else 0 for g in cfg.gyration], np.int32)),
+            lg(cfg.flow[0]), lg(cfg.flow[1]), lg(cfg.flow[2]),
+            _d(_f8(cfg.Qn)), _d(_f8(cfg.vn)), _d(_f8(cfg.Ek)), _d(_f8(cfg.nu_i)), _d(_f8(cfg.nu_d)),
+            _i(_i8(d.ppc))]
+    keep = args                                            # arrays stay alive for the call
+    with tempfile.TemporaryFile() as cap:
+        sys.stdout.flush()
+        saved = os.dup(1)
+        os.dup2(cap.fileno(), 1)
+        try:
+            _in_dir(path, lambda: R.writewarnings_(*keep))
+            R.picard_ref_flush()
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
+        cap.seek(0)
+        text = cap.read()
+    out = []
+    for name in ("parameters.dat", "warnings.dat"):
+        with open(os.path.join(path, "save", name), "rb") as f:
+            out.append(f.read())
+    return text, out[0], out[1]

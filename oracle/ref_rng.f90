@@ -22,3 +22,9 @@ subroutine picard_ref_uniforms(out, n) bind(C, name="picard_ref_uniforms")
   real(c_double), intent(out) :: out(n)
   call random_number(out)
 end subroutine picard_ref_uniforms
+
+! Flush the runtime's standard output (the reference's list-directed
+! summaries), so a caller that redirects file descriptor 1 receives them.
+subroutine picard_ref_flush() bind(C, name='picard_ref_flush')
+  flush(6)
+end subroutine picard_ref_flush

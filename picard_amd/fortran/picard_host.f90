@@ -49,6 +49,11 @@ program picard_host
   Lx_min = cfg%xmin; Lx_max = cfg%xmax; Ly_min = cfg%ymin; Ly_max = cfg%ymax
   Lz_min = cfg%zmin; Lz_max = cfg%zmax
 
+  ! the run's parameters, as picard.f90:488-489 writes them at setup
+  call dump_parameters(dc%n2p, dc%mass, dc%charge, dc%density, dc%kelvin, cfg%Qn, cfg%vn, &
+                       cfg%Ek, cfg%nu_i, cfg%nu_d, dc%ppc, cfg%gyration, dc%ve, dc%B0, dc%dxyz, &
+                       [cfg%xmin, cfg%xmax, cfg%ymin, cfg%ymax, cfg%zmin, cfg%zmax], dc%dt, &
+                       [cfg%nx, cfg%ny, cfg%nz], [1, 1, 1], [cfg%xflow, cfg%yflow, cfg%zflow])
   call fill_structs(cfg, dc, max_per_proc, geo, spc)
   call picard_dropin_init(geo, spc, real_particles, int_particles, max_per_proc)
 

@@ -3,11 +3,12 @@
 ! descriptor E10.4, particles seven values per record (positions, velocities,
 ! species), fields one value per record in array order with the guard layers,
 ! files save/particles/particles_IIIII_RRRRR and save/fields/<name>_IIIII_RRRRR
-! (name UE, Ex, Ey, Ez, N<s>, X<s>, Y<s>, Z<s> for s = 1..species).
+! (name UE, Ex, Ey, Ez, N<s>, X<s>, Y<s>, Z<s> for s = 1..species); and the
+! run's save/parameters.dat (WriteWarnings.f90:130-256).
 module picard_output
   implicit none
   private
-  public :: dump_particles, dump_fields
+  public :: dump_particles, dump_fields, dump_parameters
 
 contains
 
@@ -76,5 +77,74 @@ contains
       end do
     end do
   end subroutine dump_fields
+
+  ! save/parameters.dat as the reference writes it at setup
+  ! (WriteWarnings.f90:130-256, picard.f90:488-489): one `name = value`
+  ! record per quantity in the reference's order, reals in E12.6, integers in
+  ! I4, logicals list-directed.  bounds = xmin, xmax, ymin, ymax, zmin, zmax;
+  ! ve is the bulk velocity v0 of the file.
+  subroutine dump_parameters(n2p, mass, charge, density, kelvin, Qn, vn, Ek, nu_i, nu_d, ppc, &
+                             gyration, ve, B0, dxyz, bounds, dt, nxyz, procs, flow)
+    real*8, intent(in) :: n2p(8), mass(8), charge(8), density(8), kelvin(8), Qn(8), vn(8), &
+                          Ek(8), nu_i(8), nu_d(8), ve(4), B0(4), dxyz(3), bounds(6), dt
+    integer, intent(in) :: ppc(8), nxyz(3), procs(3)
+    logical, intent(in) :: gyration(8), flow(3)
+    character(len=6), parameter :: bname(6) = [character(len=6) :: 'xmin', 'xmax', 'ymin', &
+                                                'ymax', 'zmin', 'zmax']
+    character(len=6), parameter :: iname(6) = [character(len=6) :: 'Nx', 'Ny', 'Nz', 'iprocs', &
+                                                'jprocs', 'kprocs']
+    character(len=1), parameter :: axis(3) = ['x', 'y', 'z']
+    integer :: u, s, q(6)
+    call execute_command_line('mkdir -p save')
+    open(newunit=u, file='save/parameters.dat', status='replace', action='write')
+    call per_species(u, 'n2p', n2p)
+    call per_species(u, 'mass', mass)
+    call per_species(u, 'charge', charge)
+    call per_species(u, 'density', density)
+    call per_species(u, 'kelvin', kelvin)
+    call per_species(u, 'Qn', Qn)
+    call per_species(u, 'vn', vn)
+    call per_species(u, 'Ek', Ek)
+    call per_species(u, 'nui', nu_i)
+    call per_species(u, 'nud', nu_d)
+    do s = 1, 8
+      write(u, '(a,i1,a,I4)') 'ppc_', s, ' = ', ppc(s)
+    end do
+    do s = 1, 8
+      write(u, *) 'gyration_'//achar(48 + s)//' = ', gyration(s)
+    end do
+    do s = 1, 3
+      write(u, '(a,i1,a,E12.6)') 'v0_', s, ' = ', ve(s)
+    end do
+    do s = 1, 3
+      write(u, '(a,i1,a,E12.6)') 'B0_', s, ' = ', B0(s)
+    end do
+    do s = 1, 3
+      write(u, '(a,i1,a,E12.6)') 'dxyz_', s, ' = ', dxyz(s)
+    end do
+    do s = 1, 6
+      write(u, '(a,a,E12.6)') trim(bname(s)), ' = ', bounds(s)
+    end do
+    write(u, '(a,E12.6)') 'dt = ', dt
+    q(1:3) = nxyz
+    q(4:6) = procs
+    do s = 1, 6
+      write(u, '(a,a,I4)') trim(iname(s)), ' = ', q(s)
+    end do
+    do s = 1, 3
+      write(u, *) axis(s)//'flow = ', flow(s)
+    end do
+    close(u)
+  contains
+    subroutine per_species(u, name, a)
+      integer, intent(in) :: u
+      character(len=*), intent(in) :: name
+      real*8, intent(in) :: a(8)
+      integer :: k
+      do k = 1, 8
+        write(u, '(a,a,i1,a,E12.6)') name, '_', k, ' = ', a(k)
+      end do
+    end subroutine per_species
+  end subroutine dump_parameters
 
 end module picard_output

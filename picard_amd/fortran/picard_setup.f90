@@ -29,6 +29,14 @@ module picard_setup
     real*8 :: v0(3) = [4d5, 0d0, 0d0]
     real*8 :: B0(3) = [0d0, 0d0, 5d-9]
     logical :: gyration(8) = .true., fluid(8) = .false.
+    ! Haser parameters (picard.f90:255-300: an H2O+ source as species 3; index
+    ! 1 is never set there and reads zero); this host writes them to
+    ! save/parameters.dat and does not run the source
+    real*8 :: Qn(8) = [0d0, 0d0, (400.0d0)**(-3) * 1.99d26, 0d0, 0d0, 0d0, 0d0, 0d0]
+    real*8 :: vn(8) = [0d0, 0d0, 629d0, 0d0, 0d0, 0d0, 0d0, 0d0]
+    real*8 :: Ek(8) = [0d0, 0d0, 1.52d1 * EC, 0d0, 0d0, 0d0, 0d0, 0d0]
+    real*8 :: nu_i(8) = [0d0, 0d0, 1.26d-7, 0d0, 0d0, 0d0, 0d0, 0d0]
+    real*8 :: nu_d(8) = [0d0, 0d0, 1.26d-7, 0d0, 0d0, 0d0, 0d0, 0d0]
     ! output (picard.f90 section 1.1): dumps when dmod(time, interval) < dt,
     ! the diagnostics line every write_output_interval steps
     logical :: write_fields = .false., write_particles = .false.
@@ -51,19 +59,21 @@ contains
     integer :: nx, ny, nz, species, steps, maxit, seed, ppc(8), u, ios
     logical :: xflow, yflow, zflow, gyration(8), fluid(8)
     real*8 :: xmin, xmax, ymin, ymax, zmin, zmax, dt_wpe, maxerr, mass(8), density(8), kelvin(8), &
-              v0(3), B0(3), write_fields_interval, write_particles_interval
+              v0(3), B0(3), write_fields_interval, write_particles_interval, Qn(8), vn(8), Ek(8), &
+              nu_i(8), nu_d(8)
     logical :: write_fields, write_particles
     integer :: write_output_interval
     namelist /picard_run/ nx, ny, nz, species, steps, maxit, seed, ppc, xflow, yflow, zflow, gyration, &
                           fluid, xmin, xmax, ymin, ymax, zmin, zmax, dt_wpe, maxerr, mass, density, &
                           kelvin, v0, B0, write_fields, write_particles, write_fields_interval, &
-                          write_particles_interval, write_output_interval
+                          write_particles_interval, write_output_interval, Qn, vn, Ek, nu_i, nu_d
     nx = cfg%nx; ny = cfg%ny; nz = cfg%nz; species = cfg%species; steps = cfg%steps
     maxit = cfg%maxit; seed = cfg%seed; ppc = cfg%ppc; xflow = cfg%xflow; yflow = cfg%yflow
     zflow = cfg%zflow; gyration = cfg%gyration; fluid = cfg%fluid; xmin = cfg%xmin; xmax = cfg%xmax
     ymin = cfg%ymin; ymax = cfg%ymax; zmin = cfg%zmin; zmax = cfg%zmax; dt_wpe = cfg%dt_wpe
     maxerr = cfg%maxerr; mass = cfg%mass; density = cfg%density; kelvin = cfg%kelvin; v0 = cfg%v0
     B0 = cfg%B0
+    Qn = cfg%Qn; vn = cfg%vn; Ek = cfg%Ek; nu_i = cfg%nu_i; nu_d = cfg%nu_d
     write_fields = cfg%write_fields; write_particles = cfg%write_particles
     write_fields_interval = cfg%write_fields_interval
     write_particles_interval = cfg%write_particles_interval
@@ -81,6 +91,7 @@ contains
     cfg%ymin = ymin; cfg%ymax = ymax; cfg%zmin = zmin; cfg%zmax = zmax; cfg%dt_wpe = dt_wpe
     cfg%maxerr = maxerr; cfg%mass = mass; cfg%density = density; cfg%kelvin = kelvin; cfg%v0 = v0
     cfg%B0 = B0
+    cfg%Qn = Qn; cfg%vn = vn; cfg%Ek = Ek; cfg%nu_i = nu_i; cfg%nu_d = nu_d
     cfg%write_fields = write_fields; cfg%write_particles = write_particles
     cfg%write_fields_interval = write_fields_interval
     cfg%write_particles_interval = write_particles_interval

@@ -20,23 +20,30 @@ import os
 import numpy as np
 
 
-def format_e10_4(values) -> np.ndarray:
-    """E10.4 strings of an array of float64 (NaN and infinities are not
-    produced by the time step and are not handled)."""
+def format_e(values, d: int) -> np.ndarray:
+    """Ew.d strings (w = d + 6) of an array of float64: mantissa 0.<d digits>
+    rounded to nearest on the exact binary value, exponent 'E+dd' or, past
+    two digits, '+ddd'; negative values drop the leading zero.  NaN and
+    infinities are not produced by the time step and are not handled."""
     a = np.asarray(values, np.float64).ravel()
-    sci = np.char.mod("%.3e", np.abs(a))            # 'd.ddde+XX', correctly rounded
-    out = np.empty(a.size, dtype="<U10")
+    sci = np.char.mod(f"%.{d - 1}e", np.abs(a))     # 'd.ddd...e+XX', correctly rounded
+    out = np.empty(a.size, dtype=f"<U{d + 6}")
     for q, (txt, v) in enumerate(zip(sci.tolist(), a.tolist())):
         mant, ex = txt.split("e")
-        digits = mant[0] + mant[2:5]
+        digits = mant[0] + mant[2:d + 1]
         e = int(ex) + 1
         if v == 0.0:
-            digits, e = "0000", 0
+            digits, e = "0" * d, 0
         expo = f"E{'+' if e >= 0 else '-'}{abs(e):02d}" if abs(e) <= 99 else \
             f"{'+' if e >= 0 else '-'}{abs(e):03d}"
         neg = v < 0.0 or (v == 0.0 and np.signbit(v))
         out[q] = ("-." if neg else "0.") + digits + expo
     return out
+
+
+def format_e10_4(values) -> np.ndarray:
+    """E10.4 strings of an array of float64 (see format_e)."""
+    return format_e(values, 4)
 
 
 def particles_text(rp, sp) -> str:
@@ -83,3 +90,61 @@ def write_fields(path, U, N, F, E, iteration: int, rank: int = 0) -> list:
             f.write(field_text(arr))
         names.append(name)
     return names
+
+
+def _i4(v: int) -> str:
+    """I4: right-justified in 4 columns, asterisks when it does not fit."""
+    t = str(int(v))
+    return t.rjust(4) if len(t) <= 4 else "****"
+
+
+def parameters_text(cfg, d, procs=(1, 1, 1), haser=None) -> str:
+    """save/parameters.dat of WriteWarnings (WriteWarnings.f90:130-256): one
+    `name = value` record per quantity, reals in E12.6, integers in I4, and
+    the logicals in the Fortran runtime's list-directed form (' name =  T').
+    `d` is the derived constants (host.derive), `haser` the Haser arrays
+    (Qn, vn, Ek, nu_i, nu_d; default: the Config's)."""
+    h = haser or dict(Qn=cfg.Qn, vn=cfg.vn, Ek=cfg.Ek, nu_i=cfg.nu_i, nu_d=cfg.nu_d)
+    lines = []
+
+    def reals(name, vals):
+        for s, t in enumerate(format_e(np.asarray(vals, np.float64)[:8], 6).tolist()):
+            lines.append(f"{name}_{s + 1} = {t}")
+
+    reals("n2p", d.n2p)
+    reals("mass", d.mass)
+    reals("charge", d.charge)
+    reals("density", d.density)
+    reals("kelvin", d.kelvin)
+    reals("Qn", h["Qn"])
+    reals("vn", h["vn"])
+    reals("Ek", h["Ek"])
+    reals("nui", h["nu_i"])
+    reals("nud", h["nu_d"])
+    for s in range(8):
+        lines.append(f"ppc_{s + 1} = {_i4(d.ppc[s])}")
+    for s in range(8):
+        lines.append(f" gyration_{s + 1} =  {'T' if cfg.gyration[s] else 'F'}")
+    for name, vals in (("v0", d.ve[:3]), ("B0", d.B0[:3]), ("dxyz", d.dxyz[:3])):
+        for c, t in enumerate(format_e(vals, 6).tolist()):
+            lines.append(f"{name}_{c + 1} = {t}")
+    bounds = (cfg.gmin[0], cfg.gmax[0], cfg.gmin[1], cfg.gmax[1], cfg.gmin[2], cfg.gmax[2], d.dt)
+    for name, t in zip(("xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "dt"),
+                       format_e(bounds, 6).tolist()):
+        lines.append(f"{name} = {t}")
+    for name, v in zip(("Nx", "Ny", "Nz", "iprocs", "jprocs", "kprocs"),
+                       (cfg.nx, cfg.ny, cfg.nz) + tuple(procs)):
+        lines.append(f"{name} = {_i4(v)}")
+    for name, f in zip(("xflow", "yflow", "zflow"), cfg.flow):
+        lines.append(f" {name} =  {'T' if f else 'F'}")
+    return "".join(line + "\n" for line in lines)
+
+
+def write_parameters(path, cfg, d, procs=(1, 1, 1), haser=None) -> str:
+    """save/parameters.dat under `path` (WriteWarnings.f90:130-256, written
+    once at setup, picard.f90:488-489); returns the file name."""
+    os.makedirs(os.path.join(path, "save"), exist_ok=True)
+    name = os.path.join(path, "save", "parameters.dat")
+    with open(name, "w") as f:
+        f.write(parameters_text(cfg, d, procs, haser))
+    return name

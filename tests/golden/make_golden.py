@@ -207,5 +207,60 @@ def produce_cases():
     np.savez_compressed(os.path.join(HERE, "ref_particles_produce.npz"), **d_out)
 
 
+def parameters_cases():
+    """WriteWarnings' save/parameters.dat (WriteWarnings.f90:130-256) for
+    three setups: the 32^3 case with picard.f90's own Haser parameters
+    (picard.f90:255-300), the 256^3 workload, and one with mixed gyration
+    flags, an I4 field that overflows, three-digit exponents, negative values
+    and a 2 x 1 x 3 process grid.  Stores each setup's inputs (JSON) and the
+    file's bytes."""
+    import json
+    import tempfile
+    from picard_amd import host
+    ec = host.EC
+    ref_haser = dict(Qn=[0.0, 0.0, 400.0 ** (-3) * 1.99e26, 0, 0, 0, 0, 0],
+                     vn=[0.0, 0.0, 629.0, 0, 0, 0, 0, 0],
+                     Ek=[0.0, 0.0, 1.0 * 1.52e1 * ec, 0, 0, 0, 0, 0],
+                     nu_i=[0.0, 0.0, 1.26e-7, 0, 0, 0, 0, 0],
+                     nu_d=[0.0, 0.0, 1.26e-7, 0, 0, 0, 0, 0])
+    cases = [
+        (dict(), ref_haser, (1, 1, 1)),
+        (dict(n=256), None, (1, 1, 1)),
+        (dict(nx=12345, ny=7, nz=3, species=4, flow=(False, True, True),
+              gmin=(-1.0e-3, -5.0, 2.0), gmax=(5.0e-3, 5.0, 9.0e5),
+              ppc=(0, 3000, 7, 11, 0, 0, 0, 0), v0=(-4.0e5, 2.5e3, -1.0e-200),
+              B0=(-1.0e-9, 3.0e-301, 7.0e-9),
+              charge3=(-2.0 * ec, 3.0 * ec, ec, ec, ec, ec),
+              gyration=(True, False, True, False, False, True, True, False)),
+         dict(Qn=[1e-300, 2.5e28, 0, 0, 0, 0, 0, 0], vn=[0, -1.5, 629.0, 0, 0, 0, 0, 0],
+              Ek=[0] * 8, nu_i=[0, 0, 0, 1e+300, 0, 0, 0, 0], nu_d=[0] * 8), (2, 1, 3)),
+    ]
+    out = {}
+    for t, (kw, haser, procs) in enumerate(cases):
+        if kw.get("n"):
+            cfg = host.config2(kw["n"])
+        else:
+            cfg = host.Config(**kw)
+        if haser:
+            for k, v in haser.items():
+                setattr(cfg, k, tuple(float(x) for x in v))
+        d = host.derive(cfg)
+        with tempfile.TemporaryDirectory() as tmp:
+            _, par, _ = oracle.ref_write_warnings(cfg, d, tmp, procs)
+        desc = {k: (list(v) if isinstance(v, tuple) else v) for k, v in vars(cfg).items()}
+        out[f"config{t}"] = np.array(json.dumps(desc))
+        out[f"procs{t}"] = np.array(procs, np.int32)
+        out[f"parameters{t}"] = np.frombuffer(par, np.uint8)
+        print(f"parameters case {t}: {len(par)} bytes")
+    out["ncase"] = np.int64(len(cases))
+    np.savez_compressed(os.path.join(HERE, "ref_parameters.npz"), **out)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["parameters"]:
+        if not oracle.ref_available():
+            raise SystemExit("oracle/_ref is not built; run `make -C oracle` where the reference is")
+        parameters_cases()
+    else:
+        main()
+        parameters_cases()

@@ -10,9 +10,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 pytestmark = pytest.mark.gpu
 
 
-def _run(cmd, timeout):
+def _run(cmd, timeout, cwd=None):
     print(f"[gpu-fortran] running {' '.join(cmd)}", flush=True)
-    out = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, cwd=cwd)
     print(out.stdout[-3000:], out.stderr[-2000:], flush=True)
     return out
 
@@ -31,8 +31,19 @@ def test_fortran_host_runs(tmp_path):
     nml = tmp_path / "run.nml"
     nml.write_text("&picard_run\n nx=16, ny=16, nz=16, steps=3, xmin=-80, xmax=80, ymin=-80, "
                    "ymax=80, zmin=-80, zmax=80 /\n")
-    out = _run([exe, str(nml)], 240)
+    out = _run([exe, str(nml)], 240, cwd=str(tmp_path))
     assert out.returncode == 0
     lines = [ln for ln in out.stdout.splitlines() if ln.startswith("step")]
     assert len(lines) == 3
     assert "particle-steps/s" in out.stdout
+    # save/parameters.dat as the reference writes it at setup: the Python
+    # writer (pinned byte for byte against the reference's WriteWarnings) on
+    # the same run, with picard.f90's Haser parameters
+    from picard_amd import host, output
+    ec = host.EC
+    cfg = host.Config(nx=16, ny=16, nz=16, gmin=(-80.0,) * 3, gmax=(80.0,) * 3,
+                      Qn=(0.0, 0.0, 400.0 ** (-3) * 1.99e26, 0, 0, 0, 0, 0),
+                      vn=(0.0, 0.0, 629.0, 0, 0, 0, 0, 0), Ek=(0.0, 0.0, 1.52e1 * ec, 0, 0, 0, 0, 0),
+                      nu_i=(0.0, 0.0, 1.26e-7, 0, 0, 0, 0, 0), nu_d=(0.0, 0.0, 1.26e-7, 0, 0, 0, 0, 0))
+    with open(tmp_path / "save" / "parameters.dat") as f:
+        assert f.read() == output.parameters_text(cfg, host.derive(cfg))

@@ -70,3 +70,75 @@ def test_fortran_host_dumps_match_reference(tmp_path):
     for name in z["field_names"].tolist():
         with open(tmp_path / "save" / "fields" / name, "rb") as f:
             assert f.read() == z["txt_" + name].tobytes(), name
+
+
+PARAMS = os.path.join(os.path.dirname(__file__), "golden", "ref_parameters.npz")
+
+
+def _parameter_cases():
+    import json
+    from picard_amd import host
+    if not os.path.exists(PARAMS):
+        pytest.skip("golden fixture ref_parameters.npz missing")
+    z = np.load(PARAMS, allow_pickle=False)
+    for t in range(int(z["ncase"])):
+        desc = json.loads(str(z[f"config{t}"]))
+        cfg = host.Config(**{k: (tuple(v) if isinstance(v, list) else v) for k, v in desc.items()})
+        yield t, cfg, tuple(int(v) for v in z[f"procs{t}"]), z[f"parameters{t}"].tobytes()
+
+
+def test_parameters_dat_matches_reference(tmp_path):
+    """save/parameters.dat (WriteWarnings.f90:130-256) byte for byte against
+    the reference's routine compiled with amdflang: the 32^3 case with
+    picard.f90's Haser parameters, the 256^3 workload, and a case with an I4
+    overflow, three-digit exponents, negative values and mixed flags."""
+    from picard_amd import host
+    n = 0
+    for t, cfg, procs, want in _parameter_cases():
+        d = host.derive(cfg)
+        name = output.write_parameters(str(tmp_path / f"c{t}"), cfg, d, procs)
+        with open(name, "rb") as f:
+            assert f.read() == want, f"case {t}"
+        n += 1
+    assert n == 3
+
+
+def test_e12_6_known_answers():
+    cases = [(0.625e9, "0.625000E+09"), (-1.60217646e-19, "-.160218E-18"), (0.0, "0.000000E+00"),
+             (1e-300, "0.100000-299"), (9.9999995e5, "0.100000E+07"), (7.927266729303253e-07,
+                                                                      "0.792727E-06")]
+    assert output.format_e([v for v, _ in cases], 6).tolist() == [t for _, t in cases]
+
+
+def test_fortran_host_parameters_match_reference(tmp_path):
+    """The Fortran host's parameters.dat writer (dump_parameters in
+    picard_amd/fortran/picard_output.f90) gives the reference's bytes."""
+    import subprocess
+    from picard_amd import host
+    if not os.path.exists(OUTPUT_CHECK):
+        pytest.skip("tests/fortran/output_check is not built")
+    n = 0
+    for t, cfg, procs, want in _parameter_cases():
+        d = host.derive(cfg)
+        w = tmp_path / f"c{t}"
+        w.mkdir()
+        reals = np.concatenate([np.asarray(v, np.float64)[:8] for v in
+                                (d.n2p, d.mass, d.charge, d.density, d.kelvin, cfg.Qn, cfg.vn,
+                                 cfg.Ek, cfg.nu_i, cfg.nu_d)] +
+                               [np.asarray(d.ve, np.float64)[:4], np.asarray(d.B0, np.float64)[:4],
+                                np.asarray(d.dxyz, np.float64)[:3],
+                                np.array([cfg.gmin[0], cfg.gmax[0], cfg.gmin[1], cfg.gmax[1],
+                                          cfg.gmin[2], cfg.gmax[2], d.dt])])
+        ints = np.concatenate([np.asarray(d.ppc, np.int32)[:8],
+                               np.array([1 if g else 0 for g in cfg.gyration], np.int32),
+                               np.array([cfg.nx, cfg.ny, cfg.nz], np.int32),
+                               np.array(procs, np.int32),
+                               np.array([1 if f else 0 for f in cfg.flow], np.int32)])
+        with open(w / "params.bin", "wb") as f:
+            f.write(reals.tobytes())
+            f.write(ints.tobytes())
+        subprocess.run([OUTPUT_CHECK, str(w), "params"], check=True, timeout=60)
+        with open(w / "save" / "parameters.dat", "rb") as f:
+            assert f.read() == want, f"case {t}"
+        n += 1
+    assert n == 3
