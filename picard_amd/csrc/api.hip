@@ -129,8 +129,12 @@ void release(picard_ctx *c) {
   dfree(c->F);
   dfree(c->Fraw);
   dfree(c->Ffix);
-  for (int q = 0; q < 3; ++q) dfree(c->jb[q]);
-  dfree(c->b);
+  for (int q = 0; q < 3; ++q) {
+    dfree(c->jb_mem[q]);
+    c->jb[q] = nullptr;
+  }
+  dfree(c->b_mem);
+  c->b = nullptr;
   dfree(c->consts);
   dfree(c->js);
   dfree(c->s.flags);
@@ -212,9 +216,15 @@ int create_impl(const picard_geometry *geo, const picard_species *sp, picard_ctx
   PICARD_TRY(dalloc(&c->C, g.G));
   const int64_t DN = (2 * (int64_t)g.nx + 1) * (2 * (int64_t)g.ny + 1) * (2 * (int64_t)g.nz + 1);
   PICARD_TRY(dalloc(&c->D, DN));
+  // the Jacobi iterates and b start one element into their allocations, so
+  // that node pairs (i, i+1) with i even are 16-byte aligned when nx is even
   if (!c->split)
-    for (int q = 0; q < 3; ++q) PICARD_TRY(dalloc(&c->jb[q], g.G));
-  PICARD_TRY(dalloc(&c->b, g.G));
+    for (int q = 0; q < 3; ++q) {
+      PICARD_TRY(dalloc(&c->jb_mem[q], g.G + 2));
+      c->jb[q] = c->jb_mem[q] + 1;
+    }
+  PICARD_TRY(dalloc(&c->b_mem, g.G + 2));
+  c->b = c->b_mem + 1;
   PICARD_TRY(dalloc(&c->consts, 16));
   PICARD_TRY(dalloc(&c->js, 1));
   if (!c->split) {
@@ -238,6 +248,7 @@ int create_impl(const picard_geometry *geo, const picard_species *sp, picard_ctx
   const int64_t pb3 = interior_launch(g.nx, g.ny, g.nz).blocks();
   if (pb3 > pb) pb = pb3;
   if (make_sched(g.nx, g.ny, g.nz).B > pb) pb = make_sched(g.nx, g.ny, g.nz).B;
+  if (pb < 8 * 256) pb = 8 * 256;                 // the Jacobi pass kernels: at most 256 blocks per XCD
   c->s.partials_len = pb;
   PICARD_TRY(dalloc(&c->s.partials, 4 * pb));     // two launches' worth (read + write) and scratch
   PICARD_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->js_host), sizeof(JacobiState)));

@@ -732,6 +732,210 @@ __global__ void __launch_bounds__(kStencilT, 8) k_jacobi_pass(PassGeo pg, Jacobi
   if (t0 == 0) Pout[blockIdx.x] = sblk;
 }
 
+// The same pass with two cells per lane, for even nx: a wave owns 128 cells
+// of a row, each lane's pair moves with one 16-byte access (the iterate
+// buffers and b start one element past a 16-byte boundary, so that the pairs
+// (i, i+1), i even, are aligned), and the z-march is software-pipelined: four
+// planes per pass with the z-window registers rotating by name and each
+// plane's loads issued two planes ahead.  Each lane reads its pair's two
+// x-neighbours itself (8-byte loads from the lines its wave already touches):
+// no cross-lane exchange, which the compiler may not keep convergent across
+// the lane-dependent branches of the boundary loads.  The stale-halo values
+// of U_{p-2} on the y and z boundaries are read where they are used (rows 2
+// and ny+1, planes 2 and nz+1: whole waves, rarely); those of the x boundary
+// travel with the plane's loads.  Every cell's arithmetic is k_jacobi_pass's,
+// so the iterates are the same.
+// A 16-byte store marked non-temporal: U_p is not read again in this pass,
+// and the 400 MB a pass moves at 256^3 do not fit the last-level cache anyway,
+// so the hint keeps the stream from displacing lines still to be read
+// (measured: 93 us per pass at 256^3 against 102 us with plain stores).
+__device__ __forceinline__ void st2_stream(double *p, double a, double b) {
+  typedef double d2v __attribute__((ext_vector_type(2)));
+  const d2v t = {a, b};
+  __builtin_nontemporal_store(t, reinterpret_cast<d2v *>(p));
+}
+
+template <bool PX>
+__global__ void __launch_bounds__(kStencilT, 4) k_jacobi_pass2(PassGeo pg, JacobiCoef jc,
+                                                       double *b0, double *b1, double *b2,
+                                                       const double *__restrict__ b,
+                                                       const double *__restrict__ Pin,
+                                                       double *__restrict__ Pout, JacobiState *st,
+                                                       int32_t p) {
+  __shared__ double sh[kStencilT / 64];
+  __shared__ int decide;
+  const StopSlot prev = st->slot[(p - 1) & 1];
+  const int t0 = lin_tid();
+  StopSlot cur = prev;
+  if (!prev.done) {
+    if (p >= 3) {
+      const double S = sum_partials(Pin, (int)gridDim.x, sh);
+      if (t0 == 0) decide = (p - 1 <= st->maxit && sqrt(S) <= st->thresh) ? 1 : 0;
+      __syncthreads();
+      if (decide) {
+        cur.done = 1;
+        cur.result = p - 2;
+      }
+    }
+    if (!cur.done && p > st->last) {      // all 2*maxit+1 sweeps done (:58)
+      cur.done = 1;
+      cur.result = st->last;
+    }
+  }
+  if (blockIdx.x == 0 && t0 == 0) st->slot[p & 1] = cur;
+  if (cur.done) return;
+
+  const int r = p % 3;
+  const double *curb = r == 1 ? b0 : (r == 2 ? b1 : b2);           // U_{p-1} in bufs[(p-1)%3]
+  double *out = r == 0 ? b0 : (r == 1 ? b1 : b2);                  // U_p in bufs[p%3]
+  const double *old = p >= 2 ? (r == 2 ? b0 : (r == 0 ? b1 : b2)) : nullptr;   // U_{p-2}
+  const bool wrap_cur = (p - 1) >= 1;              // halo of U_{p-1} = wrapped interior
+  const bool wrap_old = (p - 2) >= 1;
+  const bool res = p >= 2;
+  const bool py = pg.py, pz = pg.pz;
+  const uint32_t X = pg.X, XY = pg.XY;
+  const int nx = pg.nx, ny = pg.ny, nz = pg.nz;
+  const int lane = (int)threadIdx.x;
+  // the index is formed in 32 bits first: offsets below are unsigned and wrap
+  auto ld2 = [](const double *base, uint32_t idx) {
+    return *reinterpret_cast<const double2 *>(base + idx);
+  };
+  auto zplane = [&](int kk, bool wrap) -> uint32_t {
+    if (pz && wrap) {
+      if (kk == 1) return (uint32_t)nz;
+      if (kk == nz + 2) return 1u;
+    }
+    return (uint32_t)(kk - 1);
+  };
+  // offsets from a cell to its stale-halo neighbours in U_{p-2}
+  const uint32_t rwo = wrap_old ? (uint32_t)(nx - 1) : 0u - 1u;            // west of i = 2
+  const uint32_t reo = wrap_old ? 0u - (uint32_t)(nx - 1) : 1u;            // east of i = nx+1
+  const uint32_t rno = wrap_old ? 0u - (uint32_t)(ny - 1) * X : X;         // north of j = ny+1
+  const uint32_t rso = wrap_old ? (uint32_t)(ny - 1) * X : 0u - X;         // south of j = 2
+  const uint32_t rto = wrap_old ? 0u - (uint32_t)(nz - 1) * XY : XY;       // top of k = nz+1
+  const uint32_t rbo = wrap_old ? (uint32_t)(nz - 1) * XY : 0u - XY;       // bottom of k = 2
+  double r2 = 0.0;
+  const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
+  const int z0 = (xcd * nz) >> 3, z1 = ((xcd + 1) * nz) >> 3;
+  for (int t = w; z1 > z0 && t < pg.TP; t += pg.W) {
+    const int tx = t % pg.ntx, ty = t / pg.ntx;
+    const int j = ty * (int)blockDim.y + (int)threadIdx.y + 2;
+    if (j > ny + 1) continue;                                // the whole wave
+    const int ia = tx * 128 + 2 * lane + 2;                  // the pair (ia, ia+1), 1-based
+    const bool live = ia <= nx;
+    const int i0 = live ? ia : nx;                           // past the row end: the last pair again
+    const uint32_t col = (uint32_t)(i0 - 1) + X * (uint32_t)(j - 1);
+    const bool hasf = i0 == 2, hasl = i0 == nx;              // the pair holds i = 2 / i = nx+1
+    const uint32_t ow = (PX && wrap_cur && hasf) ? (uint32_t)(nx - 1) : 0u - 1u;   // from i0
+    const uint32_t oe = (PX && wrap_cur && hasl) ? 0u - (uint32_t)(nx - 1) : 1u;   // from i0+1
+    const uint32_t on = (py && wrap_cur && j == ny + 1) ? 0u - (uint32_t)(ny - 1) * X : X;
+    const uint32_t os = (py && wrap_cur && j == 2) ? (uint32_t)(ny - 1) * X : 0u - X;
+    const bool yN = py && j == ny + 1, yS = py && j == 2;
+    struct PIn {
+      double2 n, s, bc;
+      double w, e, sw, se;
+    };
+    auto zval = [&](int kk) -> double2 { return ld2(curb, col + XY * zplane(kk, wrap_cur)); };
+    auto plane_in = [&](int kk) -> PIn {
+      const uint32_t c = col + XY * (uint32_t)(kk - 1);
+      PIn a;
+      a.sw = 0.0;
+      a.se = 0.0;
+      if (PX && res && hasf) a.sw = old[c + rwo];
+      if (PX && res && hasl) a.se = old[c + 1 + reo];
+      a.w = curb[c + ow];
+      a.e = curb[c + 1 + oe];
+      a.n = ld2(curb, c + on);
+      a.s = ld2(curb, c + os);
+      a.bc = ld2(b, c);
+      return a;
+    };
+    // plane kk from its loads and the z-window (zb, zc, zt) = planes kk-1, kk, kk+1
+    auto do_plane = [&](int kk, const PIn &a, const double2 &zb, const double2 &zc,
+                        const double2 &zt) {
+      const uint32_t c = col + XY * (uint32_t)(kk - 1);
+      const double wv = a.w, ev = a.e;
+      const double uc[2] = {zc.x, zc.y};
+      const double un[2] = {a.n.x, a.n.y}, us[2] = {a.s.x, a.s.y};
+      const double ut[2] = {zt.x, zt.y}, ub[2] = {zb.x, zb.y};
+      const double bc[2] = {a.bc.x, a.bc.y};
+      double v[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const double uw = q == 0 ? wv : uc[0];
+        const double ue = q == 1 ? ev : uc[1];
+        // CalcEpotential.f90:74-80
+        v[q] = jc.ax * (ue + uw) + jc.ay * (un[q] + us[q]) + jc.az * (ut[q] + ub[q]) -
+               jc.af * bc[q];
+      }
+      if (live) st2_stream(out + c, v[0], v[1]);
+      if (res) {
+        // residual of U_{p-1} with the halo of U_{p-2} across periodic boundaries
+        double rn[2] = {un[0], un[1]}, rs[2] = {us[0], us[1]};
+        double rt[2] = {ut[0], ut[1]}, rb[2] = {ub[0], ub[1]};
+        if (yN || yS || (pz && (kk == nz + 1 || kk == 2))) {
+          if (yN) {
+            const double2 q2 = ld2(old, c + rno);
+            rn[0] = q2.x;
+            rn[1] = q2.y;
+          }
+          if (yS) {
+            const double2 q2 = ld2(old, c + rso);
+            rs[0] = q2.x;
+            rs[1] = q2.y;
+          }
+          if (pz && kk == nz + 1) {
+            const double2 q2 = ld2(old, c + rto);
+            rt[0] = q2.x;
+            rt[1] = q2.y;
+          }
+          if (pz && kk == 2) {
+            const double2 q2 = ld2(old, c + rbo);
+            rb[0] = q2.x;
+            rb[1] = q2.y;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const double uw = q == 0 ? wv : uc[0];
+          const double ue = q == 1 ? ev : uc[1];
+          const double re = (PX && hasl && q == 1) ? a.se : ue;
+          const double rw = (PX && hasf && q == 0) ? a.sw : uw;
+          const double u2 = 2.0 * uc[q];
+          // CalcEpotential.f90:83-92
+          const double rr = bc[q] - ((re + rw - u2) / jc.dx2 + (rn[q] + rs[q] - u2) / jc.dy2 +
+                                     (rt[q] + rb[q] - u2) / jc.dz2);
+          if (live) r2 = r2 + rr * rr;
+        }
+      }
+    };
+    const int ka = z0 + 2, kb = z1 + 2;                      // planes [ka, kb), 1-based
+    auto zl = [&](int kk) { return zval(kk < kb ? kk : kb); };
+    auto pl = [&](int kk) { return plane_in(kk < kb ? kk : kb - 1); };
+    double2 w0 = zl(ka - 1), w1 = zl(ka), w2 = zl(ka + 1), w3 = zl(ka + 2);
+    PIn A = pl(ka), B = pl(ka + 1);
+    for (int k = ka; k < kb; k += 4) {
+      do_plane(k, A, w0, w1, w2);
+      if (k + 1 >= kb) break;
+      w0 = zl(k + 3);
+      A = pl(k + 2);
+      do_plane(k + 1, B, w1, w2, w3);
+      if (k + 2 >= kb) break;
+      w1 = zl(k + 4);
+      B = pl(k + 3);
+      do_plane(k + 2, A, w2, w3, w0);
+      if (k + 3 >= kb) break;
+      w2 = zl(k + 5);
+      A = pl(k + 4);
+      do_plane(k + 3, B, w3, w0, w1);
+      w3 = zl(k + 6);
+      B = pl(k + 5);
+    }
+  }
+  const double sblk = block_sum(r2, sh);
+  if (t0 == 0) Pout[blockIdx.x] = sblk;
+}
+
 // After q >= 1 sweeps: U(node) = U_q at the composite periodic image of the
 // node (z, then y, then x copies of whole planes, CalcEpotential.f90:100-239);
 // nodes whose image lies on a flow face keep the input value found there.
@@ -941,8 +1145,33 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
   const int nbn = grid_blocks(g.G, kT);
   PICARD_TRY(launch_bnorm(c, maxerr));
 
+  // even nx: two cells per lane, tiles of 128 x 4 cells
+  const bool two = g.nx % 2 == 0;
   TileSched ts = make_sched(g.nx, g.ny, g.nz, kStencilT);
-  if (c->pass_wmax <= 0) {
+  if (two) {
+    ts.bx = 64;
+    ts.by = kStencilT / 64;
+    ts.ntx = (g.nx + 127) / 128;
+    ts.nty = (g.ny + ts.by - 1) / ts.by;
+    ts.TP = ts.ntx * ts.nty;
+    ts.W = ts.TP < 256 ? (ts.TP < 4 ? 4 : ts.TP) : 256;
+    ts.B = 8 * ts.W;
+    if (c->pass2_wmax <= 0) {
+      int per_cu = 0, cus = 0;
+      const void *kf = g.flow[0] ? reinterpret_cast<const void *>(k_jacobi_pass2<false>)
+                                 : reinterpret_cast<const void *>(k_jacobi_pass2<true>);
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, kStencilT, 0) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+          per_cu <= 0 || cus < 8)
+        c->pass2_wmax = 128;
+      else
+        c->pass2_wmax = (cus / 8) * per_cu;
+    }
+    if (ts.W > c->pass2_wmax) {
+      ts.W = c->pass2_wmax < 4 ? 4 : c->pass2_wmax;
+      ts.B = 8 * ts.W;
+    }
+  } else if (c->pass_wmax <= 0) {
     // blocks per XCD resident at once: the XCD slab schedule in one round
     int per_cu = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jacobi_pass, kStencilT, 0) != hipSuccess ||
@@ -952,7 +1181,7 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
     else
       c->pass_wmax = (cus / 8) * per_cu;
   }
-  if (ts.W > c->pass_wmax) {
+  if (!two && ts.W > c->pass_wmax) {
     ts.W = c->pass_wmax < 4 ? 4 : c->pass_wmax;
     ts.B = 8 * ts.W;
   }
@@ -984,14 +1213,22 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
       }
     }
     for (int64_t q = 0; q < todo; ++q, ++p) {
-      if (c->timing && q % kTimeEvery == kTimeEvery / 2)
-        hipExtLaunchKernelGGL(k_jacobi_pass, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
-                              c->tev[2 * (q / kTimeEvery)], c->tev[2 * (q / kTimeEvery) + 1], 0,
-                              pg, jc, c->jb[0], c->jb[1], c->jb[2], (const double *)c->b,
-                              (const double *)P[(p - 1) & 1], P[p & 1], c->js, (int32_t)p);
+      const bool stamp = c->timing && q % kTimeEvery == kTimeEvery / 2;
+      hipEvent_t e0 = stamp ? c->tev[2 * (q / kTimeEvery)] : nullptr;
+      hipEvent_t e1 = stamp ? c->tev[2 * (q / kTimeEvery) + 1] : nullptr;
+      const double *pin = P[(p - 1) & 1];
+      if (two && g.flow[0])
+        hipExtLaunchKernelGGL(k_jacobi_pass2<false>, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
+                              e0, e1, 0, pg, jc, c->jb[0], c->jb[1], c->jb[2],
+                              (const double *)c->b, pin, P[p & 1], c->js, (int32_t)p);
+      else if (two)
+        hipExtLaunchKernelGGL(k_jacobi_pass2<true>, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
+                              e0, e1, 0, pg, jc, c->jb[0], c->jb[1], c->jb[2],
+                              (const double *)c->b, pin, P[p & 1], c->js, (int32_t)p);
       else
-        k_jacobi_pass<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(
-            pg, jc, c->jb[0], c->jb[1], c->jb[2], c->b, P[(p - 1) & 1], P[p & 1], c->js, (int32_t)p);
+        hipExtLaunchKernelGGL(k_jacobi_pass, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
+                              e0, e1, 0, pg, jc, c->jb[0], c->jb[1], c->jb[2],
+                              (const double *)c->b, pin, P[p & 1], c->js, (int32_t)p);
     }
     PICARD_HIP(hipGetLastError());
     PICARD_HIP(hipMemcpyAsync(c->js_host, c->js, sizeof(JacobiState), hipMemcpyDeviceToHost,

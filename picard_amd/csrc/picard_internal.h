@@ -129,13 +129,16 @@ struct picard_ctx {
   double *Uw = nullptr;      // split: second FP64 potential buffer of the solve
   int sweep_wmax = 0;        // split: resident blocks per XCD of the sweep kernel (0: not yet known)
   int pass_wmax = 0;         // FP64: the same for the Jacobi pass kernel
+  int pass2_wmax = 0;        // FP64: the same for the two-cells-per-lane pass kernel
   bool bcpot_lds_set = false;  // FP64: the LDS face-sum kernel's LDS limit has been raised
   int64_t next_id = 0;
 
   // grids
   double *U = nullptr, *E = nullptr, *N = nullptr, *Nraw = nullptr, *C = nullptr, *D = nullptr;
-  double *jb[3] = {nullptr, nullptr, nullptr};   // Jacobi iterate buffers
+  double *jb[3] = {nullptr, nullptr, nullptr};   // Jacobi iterate buffers (G nodes each)
   double *b = nullptr;
+  double *jb_mem[3] = {nullptr, nullptr, nullptr};   // their allocations: jb[q] = jb_mem[q] + 1
+  double *b_mem = nullptr;                           // b = b_mem + 1
   double *consts = nullptr;  // device: n2p[8], charge[8]
   int64_t K = 0;             // deposit cells per species: (nx+1)(ny+1)(nz+1)
   bool have_D = false;

@@ -350,7 +350,9 @@ __device__ __forceinline__ uint32_t compact_index(const CompactDims &cd, int i, 
   return (uint32_t)(i - 2) + cd.NX * (uint32_t)(j - 2) + cd.P * (uint32_t)(k - 2);
 }
 
-constexpr int kSwRows = kStencilT / 64;   // rows of a sweep tile, one wave each
+constexpr int kSwThreads = 256;           // threads per block of the sweep kernel
+constexpr int kSwRows = kSwThreads / 64;  // rows of a sweep tile, one wave each
+constexpr int kOutRows = kStencilT / 64;  // rows of an outer-step tile, one wave each
 constexpr int kSwSeg = 256;               // cells of a row segment: 64 lanes x 4
 
 // Uniform geometry of the sweep kernel, kept small (32-bit, no FP64 box
@@ -433,7 +435,7 @@ __global__ void __launch_bounds__(kStencilT, 4) k_mx_outer(OuterGeo og, double d
     const int zi = t % og.nzc, tt = t / og.nzc;
     const int tx = tt % og.ntx, ty = tt / og.ntx;
     const int ka = z0 + ((z1 - z0) * zi) / og.nzc, kb = z0 + ((z1 - z0) * (zi + 1)) / og.nzc;
-    const int j = ty * kSwRows + (int)threadIdx.y;
+    const int j = ty * kOutRows + (int)threadIdx.y;
     if (j >= ny || kb <= ka) continue;                      // the whole wave
     const int i0 = tx * 64 + lane;
     const bool live = i0 < nx;                              // lanes past the row end store nothing
@@ -590,7 +592,7 @@ __global__ void __launch_bounds__(kStencilT, 4) k_mx_outer(OuterGeo og, double d
 // there in slot (j+1) % 3 (`eo_out`).  e is e_j (null for j = 0) and out
 // receives e_{j+1}.
 template <bool PX, bool PAD>
-__global__ void __launch_bounds__(kStencilT, 4) k_mx_sweep(SweepGeo sg, SweepCoef cf,
+__global__ void __launch_bounds__(kSwThreads, 1024 / kSwThreads) k_mx_sweep(SweepGeo sg, SweepCoef cf,
                                                    const float *__restrict__ r32,
                                                    const float *__restrict__ e,
                                                    float *__restrict__ out,
@@ -599,7 +601,7 @@ __global__ void __launch_bounds__(kStencilT, 4) k_mx_sweep(SweepGeo sg, SweepCoe
                                                    const double *__restrict__ Pin,
                                                    double *__restrict__ Pout, JacobiState *st,
                                                    MxLaunch L) {
-  __shared__ double sh[kStencilT / 64];
+  __shared__ double sh[kSwThreads / 64];
   __shared__ int decide;
   StopSlot cur = mx_decide(st, L, Pin, sh, &decide);
   if (!cur.done && (int64_t)L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
@@ -1053,7 +1055,7 @@ void launch_sweep(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t
 int sweep_wmax(picard_ctx *c) {
   if (c->sweep_wmax <= 0) {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_sweep<false, false>, kStencilT, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_sweep<false, false>, kSwThreads, 0) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
         per_cu <= 0 || cus < 8) {
       c->sweep_wmax = 256;
@@ -1138,7 +1140,7 @@ OuterGeo outer_sched(picard_ctx *c) {
   og.py = !g.flow[1];
   og.pz = !g.flow[2];
   og.ntx = (g.nx + 63) / 64;
-  og.nty = (g.ny + kSwRows - 1) / kSwRows;
+  og.nty = (g.ny + kOutRows - 1) / kOutRows;
   const int wmax = c->outer_wmax;
   const int per_chunk = og.ntx * og.nty;
   int nzc = (wmax + 2 * per_chunk - 1) / (2 * per_chunk);
@@ -1234,11 +1236,11 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       const float *em = E[m & 1];
       const float *eo = EO[(m - 1) % 3];
       if (base > 0)
-        k_mx_outer<true><<<ouB, dim3(64, kSwRows), 0, c->stream>>>(
+        k_mx_outer<true><<<ouB, dim3(64, kOutRows), 0, c->stream>>>(
             og, mc.dx2, mc.dy2, mc.dz2, usrc, udst, c->b, c->r32, em, eo, P[pw ^ 1], P[pw],
             c->js, L);
       else
-        k_mx_outer<false><<<ouB, dim3(64, kSwRows), 0, c->stream>>>(
+        k_mx_outer<false><<<ouB, dim3(64, kOutRows), 0, c->stream>>>(
             og, mc.dx2, mc.dy2, mc.dz2, usrc, udst, c->b, c->r32, nullptr, nullptr, P[pw ^ 1],
             P[pw], c->js, L);
       ++seq;

@@ -117,12 +117,15 @@ def test_distance_potentials_efield(flow):
 @pytest.mark.parametrize("flow", [(True, False, False), (True, True, False), (True, True, True),
                                   (False, True, False)])
 @pytest.mark.parametrize("case", ["bcpot", "random_halo", "maxit"])
-def test_epotential(flow, case):
-    _say(f"epotential flow={flow} case={case}")
+@pytest.mark.parametrize("nx", [8, 7])
+def test_epotential(flow, case, nx):
+    # even nx runs the two-cells-per-lane pass, odd nx the one-cell pass
+    _say(f"epotential flow={flow} case={case} nx={nx}")
     cfg = small_config(8, flow=flow)
+    cfg.nx = nx
     cfg.nz = 6
-    cfg.gmin = (-40.0, -40.0, -30.0)
-    cfg.gmax = (40.0, 40.0, 30.0)
+    cfg.gmin = (-5.0 * nx, -40.0, -30.0)
+    cfg.gmax = (5.0 * nx, 40.0, 30.0)
     g = oracle_grid(cfg)
     rng = np.random.default_rng(9)
     C = rng.normal(size=g.shape) * 1e-16
@@ -145,6 +148,34 @@ def test_epotential(flow, case):
         assert sw == 2 * maxit + 1
     _eq(dom.field(_lib.FIELD_U), U, "U after CalcEpotential")
     dom.close()
+
+
+@pytest.mark.parametrize("flow", [(False, True, True), (False, False, False), (True, True, True)])
+@pytest.mark.parametrize("nx", [130, 256, 258])
+def test_epotential_long_rows(flow, nx):
+    # rows longer than one wave's 128 cells: the pairs at either end of a
+    # wave's segment take their x-neighbours from memory, and a segment may
+    # hold a single live pair (nx = 130, 258)
+    _say(f"epotential long rows flow={flow} nx={nx}")
+    cfg = small_config(6, flow=flow)
+    cfg.nx, cfg.ny, cfg.nz = nx, 5, 4
+    cfg.gmin = (-5.0 * nx, -25.0, -20.0)
+    cfg.gmax = (5.0 * nx, 25.0, 20.0)
+    g = oracle_grid(cfg)
+    rng = np.random.default_rng(nx)
+    C = rng.normal(size=g.shape) * 1e-16
+    D = oracle.calc_distance(cfg.nx, cfg.ny, cfg.nz, g.dxyz, flow)
+    U0 = oracle.bc_potentials(g, C, D) + rng.normal(size=g.shape) * 1e-3
+    for maxerr, maxit in ((1e-12, 9), (1e-4, 2 ** 20)):
+        dom = _domain(cfg)
+        dom.set_field(_lib.FIELD_C, C)
+        dom.set_field(_lib.FIELD_U, U0)
+        sw = dom.calc_epotential(maxerr, maxit)
+        U = U0.copy()
+        ct_sweeps, _ = oracle.calc_epotential(g, U, C, maxerr, maxit)
+        assert sw == ct_sweeps, (sw, ct_sweeps)
+        _eq(dom.field(_lib.FIELD_U), U, f"U after CalcEpotential (maxit {maxit})")
+        dom.close()
 
 
 @pytest.mark.parametrize("mode", ["boris", "fluid", "nogyration"])
