@@ -209,6 +209,12 @@ int picard_timing_read(picard_ctx *ctx, double *jacobi_ms, int64_t *jacobi_launc
  * iterate whatever the choice, and the solution agrees with the FP64 form's
  * to the split form's tolerance for each. */
 int picard_split_solver(picard_ctx *ctx, int32_t inner);
+/* FP32-split solve: inner sweeps per kernel launch, 1 or 2 (default 2).  With
+ * 2, one launch performs two sweeps with the traffic of one, on grids whose
+ * rows are at most 256 cells and with an even inner length (otherwise one
+ * sweep per launch); the iterates are the same bit for bit, and the stop rule
+ * is then applied to every second iterate.  No reference counterpart. */
+int picard_split_sweeps_per_launch(picard_ctx *ctx, int32_t sweeps);
 
 /* One whole main-loop body after BCinflow (picard.f90:564-631, no output
  * step): AdvancePosition, append `produced` (ParticlesProduce's particles,

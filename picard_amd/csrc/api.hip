@@ -617,6 +617,17 @@ int picard_split_solver(picard_ctx *c, int32_t inner) {
   return PICARD_OK;
 }
 
+int picard_split_sweeps_per_launch(picard_ctx *c, int32_t sweeps) {
+  CTX_CHECK(c);
+  if (sweeps != 1 && sweeps != 2) {
+    set_error("picard_split_sweeps_per_launch: 1 or 2 sweeps per launch, got " +
+              std::to_string(sweeps));
+    return PICARD_ERR_ARG;
+  }
+  c->mx_pairs = sweeps == 2;
+  return PICARD_OK;
+}
+
 int picard_step(picard_ctx *c, double dt, double maxerr, int32_t maxit, const double *produced,
                 const int32_t *produced_species, int64_t nproduced, picard_step_stats *stats) {
   CTX_CHECK(c);

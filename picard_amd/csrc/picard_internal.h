@@ -125,6 +125,8 @@ struct picard_ctx {
   float *r32 = nullptr, *e32[2] = {nullptr, nullptr};
   float *eo32[3] = {nullptr, nullptr, nullptr};
   int mx_inner = 128;        // split: FP32 sweeps per FP64 refinement
+  bool mx_pairs = true;      // split: two sweeps per launch where the grid allows
+  int pair_wmax = 0;         // split: resident blocks per XCD of the two-sweep kernel
   int outer_wmax = 0;        // split: resident blocks per XCD of the outer-step kernel
   double *Uw = nullptr;      // split: second FP64 potential buffer of the solve
   int sweep_wmax = 0;        // split: resident blocks per XCD of the sweep kernel (0: not yet known)

@@ -335,6 +335,7 @@ struct MxLaunch {
   int32_t check;         // iterate whose residual Pin holds, or -1
   int32_t m;             // inner sweeps of this outer step (update launch)
   int32_t nin;           // partials in Pin
+  int32_t pairs;         // two sweeps per launch: e_k lives in E[((k + 1) / 2) % 2]
 };
 
 // Compact FP32 layout of the correction, its right-hand side and the
@@ -778,6 +779,281 @@ __global__ void __launch_bounds__(kSwThreads, 1024 / kSwThreads) k_mx_sweep(Swee
   if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
 }
 
+// Two inner sweeps per launch, e_j -> e_{j+2}, for rows of one segment
+// (nx <= 256).  Each wave owns a row y and marches up a z chunk as in
+// k_mx_sweep; at every plane step p it computes the first sweep, e_{j+1}, on
+// rows y-1, y and y+1 at plane p (the neighbouring rows again, rather than
+// sharing them through LDS with a barrier per plane), and the second sweep,
+// e_{j+2}, on row y at plane p-1 from the e_{j+1} planes p-2..p it holds.  The
+// unique traffic per launch is that of one sweep (e_j and r32 read, e_{j+2}
+// written); e_j of rows y-2..y+2 is read by five waves, mostly from cache.
+// Every cell's arithmetic is k_mx_sweep's, so e_{j+2} is bit for bit what two
+// launches of it give.  The stop-test residual is that of e_j (taken from the
+// first sweep's centre row, with the stale halo of e_{j-1}), so iterates are
+// tested every second sweep; the boundary copies of e_{j+1} go to `eo_out`,
+// for the residual of e_{j+2} in the next launch.  x-neighbours come from DPP
+// wave shifts and the periodic image of x from readlane; no load depends on
+// the lane beyond its address, so the wave stays convergent where it
+// exchanges values.  For j = 0 (e_0 = 0) the first sweep is -af r and every
+// e_j read goes to the zero row of `e`, which then only has to be a buffer of
+// the solve.
+template <bool PX, bool PAD>
+__global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoef cf,
+                                                  const float *__restrict__ r32,
+                                                  const float *__restrict__ e,
+                                                  float *__restrict__ out,
+                                                  float *__restrict__ eo_out,
+                                                  const float *__restrict__ eo_prev,
+                                                  const double *__restrict__ Pin,
+                                                  double *__restrict__ Pout, JacobiState *st,
+                                                  MxLaunch L) {
+  __shared__ double sh[kSwThreads / 64];
+  __shared__ int decide;
+  StopSlot cur = mx_decide(st, L, Pin, sh, &decide);
+  if (!cur.done && (int64_t)L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
+    cur.done = 1;
+    cur.result = st->last;
+  }
+  if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
+  if (cur.done) return;
+  const bool res = L.j >= 1;
+  const bool first = L.j == 0;
+  const uint32_t NX = sg.NX, P = sg.P;
+  const int nx = sg.nx, ny = sg.ny, nz = sg.nz;
+  const int lane = (int)threadIdx.x;
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const uint32_t zrow = P * (uint32_t)nz;                   // the zero row of E
+  const int Lw = (nx - 1) >> 2, qw = (nx - 1) & 3;          // lane and cell of x = nx - 1
+  double r2 = 0.0;
+  const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
+  const int z0 = (xcd * nz) >> 3, z1 = ((xcd + 1) * nz) >> 3;
+  auto shr = [](float v) {      // from the lane below (0 into lane 0)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
+  };
+  auto shl = [](float v) {      // from the lane above (0 into lane 63)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
+  };
+  auto bcast = [](float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  };
+  // periodic image of plane pp, -2 <= pp <= nz + 1 (also for nz = 1)
+  auto wrapz = [&](int pp) {
+    int kk = pp < 0 ? pp + nz : pp;
+    kk = kk < 0 ? kk + nz : kk;
+    kk = kk >= nz ? kk - nz : kk;
+    return kk >= nz ? kk - nz : kk;
+  };
+  for (int t = w; t < sg.TP; t += sg.W) {
+    const int zi = t % sg.nzc, ty = t / sg.nzc;            // one segment per row
+    const int ka = z0 + ((z1 - z0) * zi) / sg.nzc, kb = z0 + ((z1 - z0) * (zi + 1)) / sg.nzc;
+    const int y = ty * kSwRows + (int)threadIdx.y;
+    if (y >= ny || kb <= ka) continue;                      // the whole wave
+    const int xs = 4 * lane;                                // the lane's first cell
+    const bool live = xs < nx;                              // lanes past the row end store nothing
+    const uint32_t x0 = live ? (uint32_t)xs : 0u;
+    const int ql = nx - 1 - xs;                             // x = nx - 1 at cell ql
+    const bool hasf = PX && xs == 0;                        // the lane holds x = 0 (periodic x)
+    const bool hasl = PX && live && ql < 4;                 // ... x = nx - 1
+    const bool yN = sg.py && y == ny - 1, yS = sg.py && y == 0;
+    const bool edge_row = hasf || hasl || yN || yS;
+    auto inrow = [&](int q) { return live && (!PAD || q <= ql); };
+    // rows y-2..y+2: offset and whether the row is in the grid (or its
+    // periodic image); a flow guard row reads the zero row
+    uint32_t ro[5];
+    bool rv[5];
+#pragma unroll
+    for (int d = 0; d < 5; ++d) {
+      int yy = y + d - 2;
+      bool ok = true;
+      if (yy < 0 || yy >= ny) {
+        if (sg.py) {
+          yy = yy % ny;
+          if (yy < 0) yy += ny;
+        } else {
+          ok = false;
+          yy = y;
+        }
+      }
+      ro[d] = NX * (uint32_t)yy;
+      rv[d] = ok;
+    }
+    auto pvalid = [&](int pp) { return sg.pz | ((pp >= 0) & (pp < nz)); };
+    // per row: the lane's address of its cells, and whether e_j is read there
+    // (else the zero row); the flags are combined without short-circuit so
+    // that every load is one select, not a branch
+    uint32_t ra[5];
+    bool rok[5];
+#pragma unroll
+    for (int d = 0; d < 5; ++d) {
+      ra[d] = x0 + ro[d];
+      rok[d] = (!first) & live & rv[d];
+    }
+    const uint32_t zr = zrow + x0;
+    // e_j on row y+d-2 at plane pp (plane offset o, read when v)
+    auto e0at = [&](int d, uint32_t o, bool v) -> float4 {
+      return ld4(e + ((v & rok[d]) ? ra[d] + o : zr));
+    };
+    auto poff = [&](int pp) { return P * (uint32_t)wrapz(pp); };
+    // what plane step p reads: e_j on rows y-1..y+1 at plane p+1 and on rows
+    // y-2, y+2 at plane p, r32 on rows y-1..y+1 at plane p, and e_{j-1} at
+    // the neighbours of row y across periodic boundaries (the stop test)
+    struct StepIn {
+      float4 c0, c1, c2, f0, f1, r0, r1, r2, py, pz;
+      float pw, pe;
+    };
+    auto step_in = [&](int p) -> StepIn {
+      StepIn a;
+      a.py = zero4;
+      a.pz = zero4;
+      a.pw = 0.0f;
+      a.pe = 0.0f;
+      if (res && p >= ka && p < kb) {
+        const uint32_t pk = P * (uint32_t)p;
+        if (yN) a.py = ld4(eo_prev + x0 + pk);                                   // row 0
+        else if (yS) a.py = ld4(eo_prev + x0 + NX * (uint32_t)(ny - 1) + pk);    // row ny - 1
+        if (sg.pz && p == nz - 1) a.pz = ld4(eo_prev + x0 + ro[2]);              // plane 0
+        else if (sg.pz && p == 0) a.pz = ld4(eo_prev + x0 + ro[2] + P * (uint32_t)(nz - 1));
+        if (PX) {
+          a.pw = eo_prev[ro[2] + pk + (uint32_t)(nx - 1)];
+          a.pe = eo_prev[ro[2] + pk];
+        }
+      }
+      const uint32_t o0 = poff(p), o1 = poff(p + 1);
+      const bool v0 = pvalid(p), v1 = pvalid(p + 1);
+      a.c0 = e0at(1, o1, v1);
+      a.c1 = e0at(2, o1, v1);
+      a.c2 = e0at(3, o1, v1);
+      a.f0 = e0at(0, o0, v0);
+      a.f1 = e0at(4, o0, v0);
+      // r32 at any cell where the sweep is not used (guard rows and planes)
+      const uint32_t orr = v0 ? o0 : P * (uint32_t)ka;
+      a.r0 = ld4(r32 + ra[1] + orr);
+      a.r1 = ld4(r32 + ra[2] + orr);
+      a.r2 = ld4(r32 + ra[3] + orr);
+      return a;
+    };
+    // one Jacobi sweep of a row at one plane (k_mx_sweep's arithmetic):
+    // centre c, z-neighbours zb/zt, y-neighbours n/s, right-hand side r
+    auto sweep_row = [&](const float4 &c, const float4 &zb, const float4 &zt, const float4 &n,
+                         const float4 &s, const float4 &r, bool zero, bool first1) -> float4 {
+      const float ce[4] = {c.x, c.y, c.z, c.w};
+      const float cn[4] = {n.x, n.y, n.z, n.w}, cs[4] = {s.x, s.y, s.z, s.w};
+      const float ct[4] = {zt.x, zt.y, zt.z, zt.w}, cb[4] = {zb.x, zb.y, zb.z, zb.w};
+      const float cr[4] = {r.x, r.y, r.z, r.w};
+      const float wv = shr(c.w), ev = shl(c.x);
+      float wrapW = 0.0f, wrapE = 0.0f;
+      if (PX) {
+        const float lastc = qw == 0 ? c.x : (qw == 1 ? c.y : (qw == 2 ? c.z : c.w));
+        wrapW = bcast(lastc, Lw);
+        wrapE = bcast(c.x, 0);
+      }
+      float v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float ew = q == 0 ? wv : ce[q - 1];
+        if (hasf && q == 0) ew = wrapW;
+        float ee = q == 3 ? ev : ce[q + 1];
+        if (hasl && q == ql) ee = wrapE;
+        v[q] = cf.wx * (ee + ew) + cf.wy * (cn[q] + cs[q]) + cf.wz * (ct[q] + cb[q]) +
+               cf.kappa * ce[q] - cf.af * cr[q];
+        if (first1) v[q] = -cf.af * cr[q];     // from e_0 = 0, as k_mx_sweep writes e_1
+        if (!inrow(q) || zero) v[q] = 0.0f;
+      }
+      return make_float4(v[0], v[1], v[2], v[3]);
+    };
+    // the first sweep of row y at plane p, with the stop-test residual of e_j there
+    auto sweep_centre = [&](const float4 &c, const float4 &zb, const float4 &zt, const float4 &n,
+                            const float4 &s, const float4 &r, const StepIn &a, int p,
+                            bool zero, bool test) -> float4 {
+      const float ce[4] = {c.x, c.y, c.z, c.w};
+      const float cn[4] = {n.x, n.y, n.z, n.w}, cs[4] = {s.x, s.y, s.z, s.w};
+      const float ct[4] = {zt.x, zt.y, zt.z, zt.w}, cb[4] = {zb.x, zb.y, zb.z, zb.w};
+      const float cr[4] = {r.x, r.y, r.z, r.w};
+      const float qy[4] = {a.py.x, a.py.y, a.py.z, a.py.w};
+      const float qz[4] = {a.pz.x, a.pz.y, a.pz.z, a.pz.w};
+      const bool zT = sg.pz && p == nz - 1, zB = sg.pz && p == 0;
+      const float wv = shr(c.w), ev = shl(c.x);
+      float wrapW = 0.0f, wrapE = 0.0f;
+      if (PX) {
+        const float lastc = qw == 0 ? c.x : (qw == 1 ? c.y : (qw == 2 ? c.z : c.w));
+        wrapW = bcast(lastc, Lw);
+        wrapE = bcast(c.x, 0);
+      }
+      float v[4], sq = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float ew = q == 0 ? wv : ce[q - 1];
+        if (hasf && q == 0) ew = wrapW;
+        float ee = q == 3 ? ev : ce[q + 1];
+        if (hasl && q == ql) ee = wrapE;
+        v[q] = cf.wx * (ee + ew) + cf.wy * (cn[q] + cs[q]) + cf.wz * (ct[q] + cb[q]) +
+               cf.kappa * ce[q] - cf.af * cr[q];
+        if (first) v[q] = -cf.af * cr[q];      // from e_0 = 0, as k_mx_sweep writes e_1
+        if (!inrow(q) || zero) v[q] = 0.0f;
+        // stop-test residual: e_{j-1} across the periodic boundaries
+        const float se = (hasl && q == ql) ? a.pe : ee;
+        const float sw = (hasf && q == 0) ? a.pw : ew;
+        const float sn = yN ? qy[q] : cn[q];
+        const float ss = yS ? qy[q] : cs[q];
+        const float st_ = zT ? qz[q] : ct[q];
+        const float sb = zB ? qz[q] : cb[q];
+        const float e2 = 2.0f * ce[q];
+        const float rr = cr[q] - ((se + sw - e2) * cf.idx2 + (sn + ss - e2) * cf.idy2 +
+                                  (st_ + sb - e2) * cf.idz2);
+        if (inrow(q)) sq = sq + rr * rr;       // the lane's 4 squares in FP32
+      }
+      if (test) r2 = r2 + (double)sq;
+      return make_float4(v[0], v[1], v[2], v[3]);
+    };
+    // windows: e_j on rows y-1..y+1 at planes p-1 (m*) and p (c*); e_{j+1} on
+    // row y at planes p-2 (u0) and p-1 (u1), on rows y+-1 at plane p-1 (un, us)
+    const int pa = ka - 1, pb = kb;                         // plane steps pa..pb
+    const uint32_t oam = poff(pa - 1), oa = poff(pa);
+    const bool vam = pvalid(pa - 1), va = pvalid(pa);
+    float4 m0 = e0at(1, oam, vam), m1 = e0at(2, oam, vam), m2 = e0at(3, oam, vam);
+    float4 c0 = e0at(1, oa, va), c1 = e0at(2, oa, va), c2 = e0at(3, oa, va);
+    float4 u0 = zero4, u1 = zero4, un = zero4, us = zero4, rprev = zero4;
+    auto do_step = [&](int p, const StepIn &a) {
+      const bool pv = pvalid(p);
+      const bool in = p >= ka && p < kb;
+      // first sweep at plane p: rows y-1 (south), y, y+1 (north)
+      const float4 vs = sweep_row(c0, m0, a.c0, c1, a.f0, a.r0, !pv || !rv[1], first);
+      const float4 vc = sweep_centre(c1, m1, a.c1, c2, c0, a.r1, a, p, !pv, res && in);
+      const float4 vn = sweep_row(c2, m2, a.c2, a.f1, c1, a.r2, !pv || !rv[3], first);
+      if (in && live && (edge_row || (sg.pz && (p == 0 || p == nz - 1))))
+        st4(eo_out + x0 + ro[2] + P * (uint32_t)p, vc);
+      // second sweep at plane p-1
+      if (p - 1 >= ka) {
+        const float4 v2 = sweep_row(u1, u0, vc, un, us, rprev, false, false);
+        if (live) st4(out + x0 + ro[2] + P * (uint32_t)(p - 1), v2);
+      }
+      m0 = c0;
+      m1 = c1;
+      m2 = c2;
+      c0 = a.c0;
+      c1 = a.c1;
+      c2 = a.c2;
+      u0 = u1;
+      u1 = vc;
+      un = vn;
+      us = vs;
+      rprev = a.r1;
+    };
+    auto si = [&](int p) { return step_in(p < pb ? p : pb); };
+    StepIn A = si(pa);
+    for (int p = pa;; p += 2) {
+      StepIn B = si(p + 1);
+      do_step(p, A);
+      if (p + 1 > pb) break;
+      A = si(p + 2);
+      do_step(p + 1, B);
+      if (p + 2 > pb) break;
+    }
+  }
+  const double sblk = block_sum(r2, sh);
+  if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
+}
+
 // End of an outer step: the stop test of its last inner iterate and, when
 // the solve ended inside this step (converged, or the last iterate reached),
 // u += e_k for that iterate's k; otherwise nothing (the next outer step
@@ -800,7 +1076,8 @@ __global__ void __launch_bounds__(kStencilT) k_mx_update(GridDims g, CompactDims
   if (!cur.done) return;
   const int64_t k = cur.result - L.base;
   if (k <= 0) return;
-  const float *e = (k & 1) ? E1 : E0;
+  const int64_t bi = L.pairs ? ((k + 1) >> 1) & 1 : k & 1;
+  const float *e = bi ? E1 : E0;
   for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int kk, bool valid) {
     if (!valid) return;
     const int64_t c = node_index(g, i, j, kk);
@@ -1085,7 +1362,7 @@ TileSched stencil_sched(picard_ctx *c) {
 // the resident blocks per XCD (at 256^3 and 4 blocks per CU: one chunk, 512
 // blocks of 4 waves marching 32 planes, measured 31 us against 38 us for two
 // chunks and 1024 blocks).
-SweepGeo sweep_sched(picard_ctx *c) {
+SweepGeo sweep_geo(picard_ctx *c, int wmax) {
   const GridDims &g = c->gd;
   SweepGeo sg;
   sg.nx = g.nx;
@@ -1098,7 +1375,6 @@ SweepGeo sweep_sched(picard_ctx *c) {
   sg.pz = !g.flow[2];
   sg.ntx = (g.nx + kSwSeg - 1) / kSwSeg;
   sg.nty = (g.ny + kSwRows - 1) / kSwRows;
-  const int wmax = sweep_wmax(c);
   const int per_chunk = sg.ntx * sg.nty;
   int nzc = (wmax + 2 * per_chunk - 1) / (2 * per_chunk);
   const int zmax = g.nz / 64 > 1 ? g.nz / 64 : 1;         // chunks of at least 8 planes
@@ -1112,6 +1388,35 @@ SweepGeo sweep_sched(picard_ctx *c) {
   if (W > (int)(c->s.partials_len / 8)) W = (int)(c->s.partials_len / 8);
   sg.W = W < 1 ? 1 : W;
   return sg;
+}
+
+SweepGeo sweep_sched(picard_ctx *c) { return sweep_geo(c, sweep_wmax(c)); }
+
+// The two-sweep kernel's schedule: the same tiles, sized by its own
+// occupancy (it holds more registers per wave).
+SweepGeo pair_sched(picard_ctx *c) {
+  if (c->pair_wmax <= 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_pair<false, false>, kSwThreads, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        per_cu <= 0 || cus < 8) {
+      c->pair_wmax = 64;
+    } else {
+      c->pair_wmax = (cus / 8) * per_cu;
+    }
+  }
+  return sweep_geo(c, c->pair_wmax);
+}
+
+// One two-sweep launch (timed with events t0/t1 when given).
+template <bool PX>
+void launch_pair(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t t1,
+                 const SweepGeo &sg, const SweepCoef &cf, const float *r32, const float *e,
+                 float *out, float *eo_out, const float *eo_prev, const double *pin, double *pout,
+                 JacobiState *js, const MxLaunch &L) {
+  auto fn = pad ? k_mx_pair<PX, true> : k_mx_pair<PX, false>;
+  hipExtLaunchKernelGGL(fn, dim3(blocks), dim3(64, kSwRows), 0, s, t0, t1, 0, sg, cf, r32, e,
+                        out, eo_out, eo_prev, pin, pout, js, L);
 }
 
 // Tiles of the outer-step kernel: 64-cell row segments x 4-row groups x z
@@ -1206,9 +1511,19 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
     cf.idz2 = mc.idz2f;
   }
   const int m = c->mx_inner;
+  // two sweeps per launch where rows are one segment and m is even
+  const bool pairs = c->mx_pairs && sg.ntx == 1 && m % 2 == 0;
+  const SweepGeo pgeo = pairs ? pair_sched(c) : sg;
+  const int prB = 8 * pgeo.W;
   double *P[2] = {c->s.partials, c->s.partials + c->s.partials_len};
   float *E0 = c->e32[0], *E1 = c->e32[1];
   float *const E[2] = {E0, E1};
+  // the buffer of e_k: E[k % 2] one sweep per launch; with two, a launch
+  // reads E[(k/2) % 2] and writes the other, so e_k (k even) is in
+  // E[(k/2) % 2] and the odd iterate a single sweep leaves at the end of
+  // the solve is in the buffer after it
+  auto ebuf = [&](int k) -> float * { return pairs ? E[((k + 1) / 2) & 1] : E[k & 1]; };
+  const int64_t last_it = init.last;
   float *const *EO = c->eo32;
   int32_t seq = 1, base = 0;
   int pw = 0;                               // partials buffer the next writer uses
@@ -1231,9 +1546,9 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
     // launch); the stale-halo norm needs e_m and e_{m-1} of the previous outer
     double *usrc = Ub[outer & 1], *udst = Ub[(outer + 1) & 1];
     {
-      MxLaunch L{seq, base, 0, -1, m, 0};
+      MxLaunch L{seq, base, 0, -1, m, 0, pairs ? 1 : 0};
       // e_m and the boundary copies of e_{m-1} of the previous outer step
-      const float *em = E[m & 1];
+      const float *em = ebuf(m);
       const float *eo = EO[(m - 1) % 3];
       if (base > 0)
         k_mx_outer<true><<<ouB, dim3(64, kOutRows), 0, c->stream>>>(
@@ -1248,22 +1563,33 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       nin = ouB;
       pw ^= 1;
     }
-    for (int j = 0; j < m; ++j) {
+    int nrec = 0;                           // timed launches of this outer step
+    for (int j = 0; j < m;) {
       // the iterate-0 test used b itself (launch 0); later ones use residuals
       const int32_t chk = (check >= 1) ? check : -1;
-      MxLaunch L{seq, base, j, chk, m, nin};
+      MxLaunch L{seq, base, j, chk, m, nin, pairs ? 1 : 0};
       const double *pin = P[pw ^ 1];
       double *pout = P[pw];
-      // e_j in E[j % 2] (none for j = 0), e_{j+1} to E[(j+1) % 2]; boundary
-      // copies of e_{j+1} to slot (j+1) % 3, e_{j-1} from slot (j-1) % 3
-      const float *e = j >= 1 ? E[j & 1] : nullptr;
-      float *out = E[(j + 1) & 1];
+      // two sweeps, unless the first of them reaches iterate 2*maxit+1 (the
+      // solve's last, which must exist)
+      const bool two = pairs && j + 2 <= m && base + j + 1 != last_it;
+      // e_j (none for j = 0) -> e_{j+1} or e_{j+2}; boundary copies of
+      // e_{j+1} to slot (j+1) % 3, e_{j-1} from slot (j-1) % 3
+      const float *e = j >= 1 || two ? ebuf(j) : nullptr;
+      float *out = ebuf(j + (two ? 2 : 1));
       float *eo_out = EO[(j + 1) % 3];
       const float *eo_prev = j >= 2 ? EO[(j - 1) % 3] : nullptr;
-      const bool tj = c->timing && j % kTimeEvery == kTimeEvery / 2;
-      hipEvent_t t0 = tj ? c->tev[2 * (j / kTimeEvery)] : nullptr;
-      hipEvent_t t1 = tj ? c->tev[2 * (j / kTimeEvery) + 1] : nullptr;
-      if (sg.px)
+      const bool tj = c->timing && j % kTimeEvery == kTimeEvery / 2 && nrec < timed;
+      hipEvent_t t0 = tj ? c->tev[2 * nrec] : nullptr;
+      hipEvent_t t1 = tj ? c->tev[2 * nrec + 1] : nullptr;
+      if (tj) ++nrec;
+      if (two && sg.px)
+        launch_pair<true>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out, eo_prev,
+                          pin, pout, c->js, L);
+      else if (two)
+        launch_pair<false>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out, eo_prev,
+                           pin, pout, c->js, L);
+      else if (sg.px)
         launch_sweep<true>(pad, swB, c->stream, t0, t1, sg, cf, c->r32, e, out, eo_out, eo_prev,
                            pin, pout, c->js, L);
       else
@@ -1272,15 +1598,16 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       ++seq;
       if (j >= 1) {               // wrote the residual of iterate base + j
         check = base + j;
-        nin = swB;
+        nin = two ? prB : swB;
         pw ^= 1;
       } else {
         check = -1;
       }
+      j += two ? 2 : 1;
     }
     {
       const int32_t chk = (check >= 1) ? check : -1;
-      MxLaunch L{seq, base, 0, chk, m, nin};
+      MxLaunch L{seq, base, 0, chk, m, nin, pairs ? 1 : 0};
       k_mx_update<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, cd, ts, udst, E0, E1, P[pw ^ 1],
                                                               c->js, L);
       ++seq;
@@ -1291,12 +1618,12 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
                               c->stream));
     PICARD_HIP(hipStreamSynchronize(c->stream));
     if (c->timing) {
-      for (int q = 0; q < timed; ++q) {
+      for (int q = 0; q < nrec; ++q) {
         float ms = 0.0f;
         PICARD_HIP(hipEventElapsedTime(&ms, c->tev[2 * q], c->tev[2 * q + 1]));
         c->t_ms += ms;
       }
-      c->t_launches += timed;
+      c->t_launches += nrec;
     }
     fin = c->js_host->slot[(seq - 1) & 1];
     if (fin.done) {

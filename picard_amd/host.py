@@ -668,6 +668,11 @@ class Domain:
         """FP32 sweeps per FP64 refinement step of the FP32-split solve."""
         check("picard_split_solver", lib().picard_split_solver(self.h, inner))
 
+    def split_sweeps_per_launch(self, sweeps: int = 2):
+        """Inner sweeps per launch of the FP32-split solve (1 or 2)."""
+        check("picard_split_sweeps_per_launch",
+              lib().picard_split_sweeps_per_launch(self.h, sweeps))
+
     def timing_read(self):
         ms, n = ct.c_double(), ct.c_int64()
         check("picard_timing_read", lib().picard_timing_read(self.h, ct.byref(ms), ct.byref(n)))

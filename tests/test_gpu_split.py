@@ -224,3 +224,58 @@ def test_split_solver_rejects_bad_inner():
     with pytest.raises(RuntimeError):
         dom.split_solver(inner=1)
     dom.close()
+
+
+def _split_solve(cfg, d, C, sweeps_per_launch, maxerr, maxit, inner=128):
+    dom = host.Domain(cfg, max_particles=1024, d=d, precision=_lib.PICARD_FP32_SPLIT)
+    dom.split_solver(inner=inner)
+    dom.split_sweeps_per_launch(sweeps_per_launch)
+    dom.calc_distance()
+    dom.set_field(_lib.FIELD_C, C)
+    dom.bc_potentials()
+    dom.timing(True)
+    sw = dom.calc_epotential(maxerr, maxit)
+    U = dom.field(_lib.FIELD_U)
+    dom.timing(False)
+    dom.close()
+    return sw, U
+
+
+@pytest.mark.parametrize("dims,flow", [((32, 24, 20), (True, False, False)),
+                                       ((30, 17, 9), (False, False, False)),
+                                       ((64, 5, 3), (False, True, False)),
+                                       ((256, 8, 12), (True, False, True)),
+                                       ((13, 6, 40), (False, True, True))])
+def test_split_two_sweeps_per_launch_same_iterates(dims, flow):
+    """Two sweeps per launch (rows of at most 256 cells) give the iterates of
+    two single-sweep launches bit for bit: with every stop test failing, both
+    end at iterate 2*maxit+1 with the same potential, also when that iterate
+    ends an odd number of sweeps into a refinement step (a single sweep then
+    finishes it).  With the stop rule live, the two-sweep solve tests every
+    second iterate and stops at most one sweep later."""
+    nx, ny, nz = dims
+    cfg = host.Config(nx=nx, ny=ny, nz=nz, flow=flow,
+                      gmin=(-5.0 * nx, -5.0 * ny, -5.0 * nz), gmax=(5.0 * nx, 5.0 * ny, 5.0 * nz),
+                      ppc=(0, 8, 0, 0, 0, 0, 0, 0))
+    d = host.derive(cfg)
+    rng = np.random.default_rng(nx * 1000 + ny * 10 + nz)
+    C = rng.normal(size=(nz + 2, ny + 2, nx + 2)) * 1e-16
+    for maxit, inner in ((7, 128), (70, 128), (45, 16)):
+        s1, U1 = _split_solve(cfg, d, C, 1, 1e-30, maxit, inner)
+        s2, U2 = _split_solve(cfg, d, C, 2, 1e-30, maxit, inner)
+        _say(f"dims={dims} flow={flow} maxit={maxit} inner={inner}: sweeps {s1} / {s2}")
+        assert s1 == s2 == 2 * maxit + 1
+        assert np.array_equal(U1, U2)
+    s1, U1 = _split_solve(cfg, d, C, 1, 1e-6, 2 ** 20)
+    s2, U2 = _split_solve(cfg, d, C, 2, 1e-6, 2 ** 20)
+    _say(f"dims={dims} converged: sweeps {s1} / {s2}, U rel {_rel(U2, U1):.2e}")
+    assert s1 <= s2 <= s1 + 1
+    assert _rel(U2, U1) <= 1e-6
+
+
+def test_split_sweeps_per_launch_rejects_bad_value():
+    cfg = _cfg(8)
+    dom = host.Domain(cfg, max_particles=1024, precision=_lib.PICARD_FP32_SPLIT)
+    with pytest.raises(RuntimeError):
+        dom.split_sweeps_per_launch(3)
+    dom.close()
