@@ -780,24 +780,27 @@ __global__ void __launch_bounds__(kSwThreads, 1024 / kSwThreads) k_mx_sweep(Swee
 }
 
 // Two inner sweeps per launch, e_j -> e_{j+2}, for rows of one segment
-// (nx <= 256).  Each wave owns a row y and marches up a z chunk as in
-// k_mx_sweep; at every plane step p it computes the first sweep, e_{j+1}, on
-// rows y-1, y and y+1 at plane p (the neighbouring rows again, rather than
+// (nx <= 256).  Each wave owns R consecutive rows y0..y0+R-1 and marches up a
+// z chunk as in k_mx_sweep; at every plane step p it computes the first
+// sweep, e_{j+1}, on rows y0-1..y0+R at plane p (its rows and one more on
+// each side, which the neighbouring waves compute as well, rather than
 // sharing them through LDS with a barrier per plane), and the second sweep,
-// e_{j+2}, on row y at plane p-1 from the e_{j+1} planes p-2..p it holds.  The
-// unique traffic per launch is that of one sweep (e_j and r32 read, e_{j+2}
-// written); e_j of rows y-2..y+2 is read by five waves, mostly from cache.
-// Every cell's arithmetic is k_mx_sweep's, so e_{j+2} is bit for bit what two
-// launches of it give.  The stop-test residual is that of e_j (taken from the
-// first sweep's centre row, with the stale halo of e_{j-1}), so iterates are
-// tested every second sweep; the boundary copies of e_{j+1} go to `eo_out`,
-// for the residual of e_{j+2} in the next launch.  x-neighbours come from DPP
-// wave shifts and the periodic image of x from readlane; no load depends on
-// the lane beyond its address, so the wave stays convergent where it
-// exchanges values.  For j = 0 (e_0 = 0) the first sweep is -af r and every
-// e_j read goes to the zero row of `e`, which then only has to be a buffer of
-// the solve.
-template <bool PX, bool PAD>
+// e_{j+2}, on its rows at plane p-1 from the e_{j+1} planes p-2..p it holds.
+// The unique traffic per launch is that of one sweep (e_j and r32 read,
+// e_{j+2} written); rows of e_j shared by neighbouring waves come mostly from
+// cache.  Every cell's arithmetic is k_mx_sweep's, so e_{j+2} is bit for bit
+// what two launches of it give.  The stop-test residual is that of e_j (taken
+// from the first sweep of the wave's rows, with the stale halo of e_{j-1}), so
+// iterates are tested every second sweep; the boundary copies of e_{j+1} go to
+// `eo_out`, for the residual of e_{j+2} in the next launch.  x-neighbours
+// come from DPP wave shifts and the periodic image of x from readlane; no
+// load depends on the lane beyond its address, so the wave stays convergent
+// where it exchanges values.  For j = 0 (e_0 = 0) the first sweep is -af r
+// and every e_j read goes to the zero row of `e`, which then only has to be a
+// buffer of the solve.
+constexpr int kPairRows = 2;   // rows per wave of the two-sweep kernel
+
+template <bool PX, bool PAD, int R>
 __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoef cf,
                                                   const float *__restrict__ r32,
                                                   const float *__restrict__ e,
@@ -846,24 +849,24 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
   for (int t = w; t < sg.TP; t += sg.W) {
     const int zi = t % sg.nzc, ty = t / sg.nzc;            // one segment per row
     const int ka = z0 + ((z1 - z0) * zi) / sg.nzc, kb = z0 + ((z1 - z0) * (zi + 1)) / sg.nzc;
-    const int y = ty * kSwRows + (int)threadIdx.y;
-    if (y >= ny || kb <= ka) continue;                      // the whole wave
+    const int y0 = (ty * kSwRows + (int)threadIdx.y) * R;
+    if (y0 >= ny || kb <= ka) continue;                     // the whole wave
     const int xs = 4 * lane;                                // the lane's first cell
     const bool live = xs < nx;                              // lanes past the row end store nothing
     const uint32_t x0 = live ? (uint32_t)xs : 0u;
     const int ql = nx - 1 - xs;                             // x = nx - 1 at cell ql
     const bool hasf = PX && xs == 0;                        // the lane holds x = 0 (periodic x)
     const bool hasl = PX && live && ql < 4;                 // ... x = nx - 1
-    const bool yN = sg.py && y == ny - 1, yS = sg.py && y == 0;
-    const bool edge_row = hasf || hasl || yN || yS;
     auto inrow = [&](int q) { return live && (!PAD || q <= ql); };
-    // rows y-2..y+2: offset and whether the row is in the grid (or its
-    // periodic image); a flow guard row reads the zero row
-    uint32_t ro[5];
-    bool rv[5];
+    // rows y0-2..y0+R+1 (index d: row y0+d-2): the lane's address of its
+    // cells, whether the row is in the grid (or its periodic image), and
+    // whether e_j is read there (else the zero row); flags are combined
+    // without short-circuit so that every load is one select, not a branch
+    uint32_t ra[R + 4];
+    bool rv[R + 4], rok[R + 4];
 #pragma unroll
-    for (int d = 0; d < 5; ++d) {
-      int yy = y + d - 2;
+    for (int d = 0; d < R + 4; ++d) {
+      int yy = y0 + d - 2;
       bool ok = true;
       if (yy < 0 || yy >= ny) {
         if (sg.py) {
@@ -871,107 +874,90 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
           if (yy < 0) yy += ny;
         } else {
           ok = false;
-          yy = y;
+          yy = y0;
         }
       }
-      ro[d] = NX * (uint32_t)yy;
+      ra[d] = x0 + NX * (uint32_t)yy;
       rv[d] = ok;
+      rok[d] = (!first) & live & ok;
     }
-    auto pvalid = [&](int pp) { return sg.pz | ((pp >= 0) & (pp < nz)); };
-    // per row: the lane's address of its cells, and whether e_j is read there
-    // (else the zero row); the flags are combined without short-circuit so
-    // that every load is one select, not a branch
-    uint32_t ra[5];
-    bool rok[5];
+    // the wave's rows: in the grid (the last wave may hold fewer than R)
+    bool own[R], yN[R], yS[R], edge[R];
 #pragma unroll
-    for (int d = 0; d < 5; ++d) {
-      ra[d] = x0 + ro[d];
-      rok[d] = (!first) & live & rv[d];
+    for (int r = 0; r < R; ++r) {
+      own[r] = y0 + r < ny;
+      yN[r] = sg.py && y0 + r == ny - 1;
+      yS[r] = sg.py && y0 + r == 0;
+      edge[r] = hasf || hasl || yN[r] || yS[r];
     }
+    const bool anyN = sg.py && y0 <= ny - 1 && ny - 1 < y0 + R;
+    const bool anyS = sg.py && y0 == 0;
     const uint32_t zr = zrow + x0;
-    // e_j on row y+d-2 at plane pp (plane offset o, read when v)
+    auto pvalid = [&](int pp) { return sg.pz | ((pp >= 0) & (pp < nz)); };
+    auto poff = [&](int pp) { return P * (uint32_t)wrapz(pp); };
     auto e0at = [&](int d, uint32_t o, bool v) -> float4 {
       return ld4(e + ((v & rok[d]) ? ra[d] + o : zr));
     };
-    auto poff = [&](int pp) { return P * (uint32_t)wrapz(pp); };
-    // what plane step p reads: e_j on rows y-1..y+1 at plane p+1 and on rows
-    // y-2, y+2 at plane p, r32 on rows y-1..y+1 at plane p, and e_{j-1} at
-    // the neighbours of row y across periodic boundaries (the stop test)
+    // what plane step p reads: e_j on rows y0-1..y0+R at plane p+1 and on
+    // rows y0-2, y0+R+1 at plane p, r32 on rows y0-1..y0+R at plane p, and
+    // e_{j-1} at the neighbours of the wave's rows across periodic
+    // boundaries (the stop test)
     struct StepIn {
-      float4 c0, c1, c2, f0, f1, r0, r1, r2, py, pz;
-      float pw, pe;
+      float4 c[R + 2], f[2], r[R + 2], pyn, pys, pz[R];
+      float pw[R], pe[R];
     };
     auto step_in = [&](int p) -> StepIn {
       StepIn a;
-      a.py = zero4;
-      a.pz = zero4;
-      a.pw = 0.0f;
-      a.pe = 0.0f;
+      a.pyn = zero4;
+      a.pys = zero4;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        a.pz[r] = zero4;
+        a.pw[r] = 0.0f;
+        a.pe[r] = 0.0f;
+      }
       if (res && p >= ka && p < kb) {
         const uint32_t pk = P * (uint32_t)p;
-        if (yN) a.py = ld4(eo_prev + x0 + pk);                                   // row 0
-        else if (yS) a.py = ld4(eo_prev + x0 + NX * (uint32_t)(ny - 1) + pk);    // row ny - 1
-        if (sg.pz && p == nz - 1) a.pz = ld4(eo_prev + x0 + ro[2]);              // plane 0
-        else if (sg.pz && p == 0) a.pz = ld4(eo_prev + x0 + ro[2] + P * (uint32_t)(nz - 1));
+        if (anyN) a.pyn = ld4(eo_prev + x0 + pk);                               // row 0
+        if (anyS) a.pys = ld4(eo_prev + x0 + NX * (uint32_t)(ny - 1) + pk);     // row ny - 1
+        if (sg.pz && (p == nz - 1 || p == 0)) {
+          const uint32_t pi = p == 0 ? P * (uint32_t)(nz - 1) : 0u;           // plane nz-1 or 0
+#pragma unroll
+          for (int r = 0; r < R; ++r) a.pz[r] = ld4(eo_prev + ra[r + 2] + pi);
+        }
         if (PX) {
-          a.pw = eo_prev[ro[2] + pk + (uint32_t)(nx - 1)];
-          a.pe = eo_prev[ro[2] + pk];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const uint32_t rowb = ra[r + 2] - x0 + pk;
+            a.pw[r] = eo_prev[rowb + (uint32_t)(nx - 1)];
+            a.pe[r] = eo_prev[rowb];
+          }
         }
       }
       const uint32_t o0 = poff(p), o1 = poff(p + 1);
       const bool v0 = pvalid(p), v1 = pvalid(p + 1);
-      a.c0 = e0at(1, o1, v1);
-      a.c1 = e0at(2, o1, v1);
-      a.c2 = e0at(3, o1, v1);
-      a.f0 = e0at(0, o0, v0);
-      a.f1 = e0at(4, o0, v0);
+#pragma unroll
+      for (int i = 0; i < R + 2; ++i) a.c[i] = e0at(i + 1, o1, v1);
+      a.f[0] = e0at(0, o0, v0);
+      a.f[1] = e0at(R + 3, o0, v0);
       // r32 at any cell where the sweep is not used (guard rows and planes)
       const uint32_t orr = v0 ? o0 : P * (uint32_t)ka;
-      a.r0 = ld4(r32 + ra[1] + orr);
-      a.r1 = ld4(r32 + ra[2] + orr);
-      a.r2 = ld4(r32 + ra[3] + orr);
+#pragma unroll
+      for (int i = 0; i < R + 2; ++i) a.r[i] = ld4(r32 + ra[i + 1] + orr);
       return a;
     };
     // one Jacobi sweep of a row at one plane (k_mx_sweep's arithmetic):
-    // centre c, z-neighbours zb/zt, y-neighbours n/s, right-hand side r
-    auto sweep_row = [&](const float4 &c, const float4 &zb, const float4 &zt, const float4 &n,
-                         const float4 &s, const float4 &r, bool zero, bool first1) -> float4 {
+    // centre c, z-neighbours zb/zt, y-neighbours n/s, right-hand side r; and,
+    // when test, the stop-test residual of c there with the stale values
+    // (sn, ss, st, sb when the flags are set; pw/pe across x)
+    auto sweep = [&](const float4 &c, const float4 &zb, const float4 &zt, const float4 &n,
+                     const float4 &s, const float4 &r, bool zero, bool first1, bool test,
+                     bool fN, const float4 &qn, bool fS, const float4 &qs, bool zT,
+                     bool zB, const float4 &qz, float pw, float pe) -> float4 {
       const float ce[4] = {c.x, c.y, c.z, c.w};
       const float cn[4] = {n.x, n.y, n.z, n.w}, cs[4] = {s.x, s.y, s.z, s.w};
       const float ct[4] = {zt.x, zt.y, zt.z, zt.w}, cb[4] = {zb.x, zb.y, zb.z, zb.w};
       const float cr[4] = {r.x, r.y, r.z, r.w};
-      const float wv = shr(c.w), ev = shl(c.x);
-      float wrapW = 0.0f, wrapE = 0.0f;
-      if (PX) {
-        const float lastc = qw == 0 ? c.x : (qw == 1 ? c.y : (qw == 2 ? c.z : c.w));
-        wrapW = bcast(lastc, Lw);
-        wrapE = bcast(c.x, 0);
-      }
-      float v[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        float ew = q == 0 ? wv : ce[q - 1];
-        if (hasf && q == 0) ew = wrapW;
-        float ee = q == 3 ? ev : ce[q + 1];
-        if (hasl && q == ql) ee = wrapE;
-        v[q] = cf.wx * (ee + ew) + cf.wy * (cn[q] + cs[q]) + cf.wz * (ct[q] + cb[q]) +
-               cf.kappa * ce[q] - cf.af * cr[q];
-        if (first1) v[q] = -cf.af * cr[q];     // from e_0 = 0, as k_mx_sweep writes e_1
-        if (!inrow(q) || zero) v[q] = 0.0f;
-      }
-      return make_float4(v[0], v[1], v[2], v[3]);
-    };
-    // the first sweep of row y at plane p, with the stop-test residual of e_j there
-    auto sweep_centre = [&](const float4 &c, const float4 &zb, const float4 &zt, const float4 &n,
-                            const float4 &s, const float4 &r, const StepIn &a, int p,
-                            bool zero, bool test) -> float4 {
-      const float ce[4] = {c.x, c.y, c.z, c.w};
-      const float cn[4] = {n.x, n.y, n.z, n.w}, cs[4] = {s.x, s.y, s.z, s.w};
-      const float ct[4] = {zt.x, zt.y, zt.z, zt.w}, cb[4] = {zb.x, zb.y, zb.z, zb.w};
-      const float cr[4] = {r.x, r.y, r.z, r.w};
-      const float qy[4] = {a.py.x, a.py.y, a.py.z, a.py.w};
-      const float qz[4] = {a.pz.x, a.pz.y, a.pz.z, a.pz.w};
-      const bool zT = sg.pz && p == nz - 1, zB = sg.pz && p == 0;
       const float wv = shr(c.w), ev = shl(c.x);
       float wrapW = 0.0f, wrapE = 0.0f;
       if (PX) {
@@ -988,56 +974,94 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
         if (hasl && q == ql) ee = wrapE;
         v[q] = cf.wx * (ee + ew) + cf.wy * (cn[q] + cs[q]) + cf.wz * (ct[q] + cb[q]) +
                cf.kappa * ce[q] - cf.af * cr[q];
-        if (first) v[q] = -cf.af * cr[q];      // from e_0 = 0, as k_mx_sweep writes e_1
+        if (first1) v[q] = -cf.af * cr[q];     // from e_0 = 0, as k_mx_sweep writes e_1
         if (!inrow(q) || zero) v[q] = 0.0f;
-        // stop-test residual: e_{j-1} across the periodic boundaries
-        const float se = (hasl && q == ql) ? a.pe : ee;
-        const float sw = (hasf && q == 0) ? a.pw : ew;
-        const float sn = yN ? qy[q] : cn[q];
-        const float ss = yS ? qy[q] : cs[q];
-        const float st_ = zT ? qz[q] : ct[q];
-        const float sb = zB ? qz[q] : cb[q];
-        const float e2 = 2.0f * ce[q];
-        const float rr = cr[q] - ((se + sw - e2) * cf.idx2 + (sn + ss - e2) * cf.idy2 +
-                                  (st_ + sb - e2) * cf.idz2);
-        if (inrow(q)) sq = sq + rr * rr;       // the lane's 4 squares in FP32
+        if (test) {
+          // stop-test residual: e_{j-1} across the periodic boundaries
+          const float qN[4] = {qn.x, qn.y, qn.z, qn.w}, qS[4] = {qs.x, qs.y, qs.z, qs.w};
+          const float qZ[4] = {qz.x, qz.y, qz.z, qz.w};
+          const float se = (hasl && q == ql) ? pe : ee;
+          const float sw = (hasf && q == 0) ? pw : ew;
+          const float sn = fN ? qN[q] : cn[q];
+          const float ss = fS ? qS[q] : cs[q];
+          const float st_ = zT ? qZ[q] : ct[q];
+          const float sb = zB ? qZ[q] : cb[q];
+          const float e2 = 2.0f * ce[q];
+          const float rr = cr[q] - ((se + sw - e2) * cf.idx2 + (sn + ss - e2) * cf.idy2 +
+                                    (st_ + sb - e2) * cf.idz2);
+          if (inrow(q)) sq = sq + rr * rr;     // the lane's 4 squares in FP32
+        }
       }
       if (test) r2 = r2 + (double)sq;
       return make_float4(v[0], v[1], v[2], v[3]);
     };
-    // windows: e_j on rows y-1..y+1 at planes p-1 (m*) and p (c*); e_{j+1} on
-    // row y at planes p-2 (u0) and p-1 (u1), on rows y+-1 at plane p-1 (un, us)
+    // windows: e_j on rows y0-1..y0+R at planes p-1 (wm) and p (wc); e_{j+1}
+    // on the wave's rows at planes p-2 (u0) and p-1 (u1), on rows y0-1 and
+    // y0+R at plane p-1 (hs, hn); r32 of the wave's rows at plane p-1 (rp)
     const int pa = ka - 1, pb = kb;                         // plane steps pa..pb
-    const uint32_t oam = poff(pa - 1), oa = poff(pa);
-    const bool vam = pvalid(pa - 1), va = pvalid(pa);
-    float4 m0 = e0at(1, oam, vam), m1 = e0at(2, oam, vam), m2 = e0at(3, oam, vam);
-    float4 c0 = e0at(1, oa, va), c1 = e0at(2, oa, va), c2 = e0at(3, oa, va);
-    float4 u0 = zero4, u1 = zero4, un = zero4, us = zero4, rprev = zero4;
+    float4 wm[R + 2], wc[R + 2], u0[R], u1[R], rp[R];
+    float4 hs = zero4, hn = zero4;
+    {
+      const uint32_t oam = poff(pa - 1), oa = poff(pa);
+      const bool vam = pvalid(pa - 1), va = pvalid(pa);
+#pragma unroll
+      for (int i = 0; i < R + 2; ++i) {
+        wm[i] = e0at(i + 1, oam, vam);
+        wc[i] = e0at(i + 1, oa, va);
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        u0[r] = zero4;
+        u1[r] = zero4;
+        rp[r] = zero4;
+      }
+    }
     auto do_step = [&](int p, const StepIn &a) {
       const bool pv = pvalid(p);
       const bool in = p >= ka && p < kb;
-      // first sweep at plane p: rows y-1 (south), y, y+1 (north)
-      const float4 vs = sweep_row(c0, m0, a.c0, c1, a.f0, a.r0, !pv || !rv[1], first);
-      const float4 vc = sweep_centre(c1, m1, a.c1, c2, c0, a.r1, a, p, !pv, res && in);
-      const float4 vn = sweep_row(c2, m2, a.c2, a.f1, c1, a.r2, !pv || !rv[3], first);
-      if (in && live && (edge_row || (sg.pz && (p == 0 || p == nz - 1))))
-        st4(eo_out + x0 + ro[2] + P * (uint32_t)p, vc);
-      // second sweep at plane p-1
-      if (p - 1 >= ka) {
-        const float4 v2 = sweep_row(u1, u0, vc, un, us, rprev, false, false);
-        if (live) st4(out + x0 + ro[2] + P * (uint32_t)(p - 1), v2);
+      const bool zT = sg.pz && p == nz - 1, zB = sg.pz && p == 0;
+      // first sweep at plane p, rows y0-1..y0+R (index i: row y0-1+i)
+      float4 v1[R + 2];
+#pragma unroll
+      for (int i = 0; i < R + 2; ++i) {
+        const float4 &n = i + 1 < R + 2 ? wc[i + 1] : a.f[1];
+        const float4 &s = i > 0 ? wc[i - 1] : a.f[0];
+        if (i >= 1 && i <= R) {
+          const int r = i - 1;
+          v1[i] = sweep(wc[i], wm[i], a.c[i], n, s, a.r[i], !pv || !rv[i + 1], first,
+                        res && in && own[r], yN[r], a.pyn, yS[r], a.pys, zT, zB, a.pz[r],
+                        a.pw[r], a.pe[r]);
+          if (in && live && own[r] && (edge[r] || zT || zB))
+            st4(eo_out + ra[i + 1] + P * (uint32_t)p, v1[i]);
+        } else {
+          v1[i] = sweep(wc[i], wm[i], a.c[i], n, s, a.r[i], !pv || !rv[i + 1], first, false,
+                        false, zero4, false, zero4, false, false, zero4, 0.0f, 0.0f);
+        }
       }
-      m0 = c0;
-      m1 = c1;
-      m2 = c2;
-      c0 = a.c0;
-      c1 = a.c1;
-      c2 = a.c2;
-      u0 = u1;
-      u1 = vc;
-      un = vn;
-      us = vs;
-      rprev = a.r1;
+      // second sweep at plane p-1 on the wave's rows
+      if (p - 1 >= ka) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const float4 &n = r + 1 < R ? u1[r + 1] : hn;
+          const float4 &s = r > 0 ? u1[r - 1] : hs;
+          const float4 v2 = sweep(u1[r], u0[r], v1[r + 1], n, s, rp[r], false, false, false,
+                                  false, zero4, false, zero4, false, false, zero4, 0.0f, 0.0f);
+          if (live && own[r]) st4(out + ra[r + 2] + P * (uint32_t)(p - 1), v2);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < R + 2; ++i) {
+        wm[i] = wc[i];
+        wc[i] = a.c[i];
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        u0[r] = u1[r];
+        u1[r] = v1[r + 1];
+        rp[r] = a.r[r + 1];
+      }
+      hs = v1[0];
+      hn = v1[R + 1];
     };
     auto si = [&](int p) { return step_in(p < pb ? p : pb); };
     StepIn A = si(pa);
@@ -1362,7 +1386,9 @@ TileSched stencil_sched(picard_ctx *c) {
 // the resident blocks per XCD (at 256^3 and 4 blocks per CU: one chunk, 512
 // blocks of 4 waves marching 32 planes, measured 31 us against 38 us for two
 // chunks and 1024 blocks).
-SweepGeo sweep_geo(picard_ctx *c, int wmax) {
+// rows: rows per wave; fill: z chunks until the tiles fill the resident
+// blocks (else about half of them)
+SweepGeo sweep_geo(picard_ctx *c, int wmax, int rows = 1, bool fill = false) {
   const GridDims &g = c->gd;
   SweepGeo sg;
   sg.nx = g.nx;
@@ -1374,9 +1400,10 @@ SweepGeo sweep_geo(picard_ctx *c, int wmax) {
   sg.py = !g.flow[1];
   sg.pz = !g.flow[2];
   sg.ntx = (g.nx + kSwSeg - 1) / kSwSeg;
-  sg.nty = (g.ny + kSwRows - 1) / kSwRows;
+  sg.nty = (g.ny + kSwRows * rows - 1) / (kSwRows * rows);
   const int per_chunk = sg.ntx * sg.nty;
-  int nzc = (wmax + 2 * per_chunk - 1) / (2 * per_chunk);
+  const int span = fill ? per_chunk : 2 * per_chunk;
+  int nzc = (wmax + span - 1) / span;
   const int zmax = g.nz / 64 > 1 ? g.nz / 64 : 1;         // chunks of at least 8 planes
   if (nzc > zmax) nzc = zmax;
   if (nzc < 1) nzc = 1;
@@ -1397,7 +1424,7 @@ SweepGeo sweep_sched(picard_ctx *c) { return sweep_geo(c, sweep_wmax(c)); }
 SweepGeo pair_sched(picard_ctx *c) {
   if (c->pair_wmax <= 0) {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_pair<false, false>, kSwThreads, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_pair<false, false, kPairRows>, kSwThreads, 0) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
         per_cu <= 0 || cus < 8) {
       c->pair_wmax = 64;
@@ -1405,7 +1432,7 @@ SweepGeo pair_sched(picard_ctx *c) {
       c->pair_wmax = (cus / 8) * per_cu;
     }
   }
-  return sweep_geo(c, c->pair_wmax);
+  return sweep_geo(c, c->pair_wmax, kPairRows, true);
 }
 
 // One two-sweep launch (timed with events t0/t1 when given).
@@ -1414,7 +1441,7 @@ void launch_pair(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t 
                  const SweepGeo &sg, const SweepCoef &cf, const float *r32, const float *e,
                  float *out, float *eo_out, const float *eo_prev, const double *pin, double *pout,
                  JacobiState *js, const MxLaunch &L) {
-  auto fn = pad ? k_mx_pair<PX, true> : k_mx_pair<PX, false>;
+  auto fn = pad ? k_mx_pair<PX, true, kPairRows> : k_mx_pair<PX, false, kPairRows>;
   hipExtLaunchKernelGGL(fn, dim3(blocks), dim3(64, kSwRows), 0, s, t0, t1, 0, sg, cf, r32, e,
                         out, eo_out, eo_prev, pin, pout, js, L);
 }

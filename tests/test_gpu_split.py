@@ -245,7 +245,10 @@ def _split_solve(cfg, d, C, sweeps_per_launch, maxerr, maxit, inner=128):
                                        ((30, 17, 9), (False, False, False)),
                                        ((64, 5, 3), (False, True, False)),
                                        ((256, 8, 12), (True, False, True)),
-                                       ((13, 6, 40), (False, True, True))])
+                                       ((13, 6, 40), (False, True, True)),
+                                       ((24, 3, 10), (True, False, False)),
+                                       ((20, 2, 6), (False, False, False)),
+                                       ((40, 9, 2), (True, False, False))])
 def test_split_two_sweeps_per_launch_same_iterates(dims, flow):
     """Two sweeps per launch (rows of at most 256 cells) give the iterates of
     two single-sweep launches bit for bit: with every stop test failing, both
