@@ -800,7 +800,7 @@ __global__ void __launch_bounds__(kSwThreads, 1024 / kSwThreads) k_mx_sweep(Swee
 // buffer of the solve.
 constexpr int kPairRows = 2;   // rows per wave of the two-sweep kernel
 
-template <bool PX, bool PAD, int R>
+template <bool PX, bool PAD, int R, bool FIRST>
 __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoef cf,
                                                   const float *__restrict__ r32,
                                                   const float *__restrict__ e,
@@ -819,8 +819,8 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   if (cur.done) return;
-  const bool res = L.j >= 1;
-  const bool first = L.j == 0;
+  const bool res = !FIRST;                                  // FIRST: j = 0
+  constexpr bool first = FIRST;
   const uint32_t NX = sg.NX, P = sg.P;
   const int nx = sg.nx, ny = sg.ny, nz = sg.nz;
   const int lane = (int)threadIdx.x;
@@ -849,7 +849,8 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
   for (int t = w; t < sg.TP; t += sg.W) {
     const int zi = t % sg.nzc, ty = t / sg.nzc;            // one segment per row
     const int ka = z0 + ((z1 - z0) * zi) / sg.nzc, kb = z0 + ((z1 - z0) * (zi + 1)) / sg.nzc;
-    const int y0 = (ty * kSwRows + (int)threadIdx.y) * R;
+    // the wave's first row (the same in every lane: scalar flags below)
+    const int y0 = __builtin_amdgcn_readfirstlane((ty * kSwRows + (int)threadIdx.y) * R);
     if (y0 >= ny || kb <= ka) continue;                     // the whole wave
     const int xs = 4 * lane;                                // the lane's first cell
     const bool live = xs < nx;                              // lanes past the row end store nothing
@@ -906,16 +907,9 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
       float4 c[R + 2], f[2], r[R + 2], pyn, pys, pz[R];
       float pw[R], pe[R];
     };
-    auto step_in = [&](int p) -> StepIn {
-      StepIn a;
-      a.pyn = zero4;
-      a.pys = zero4;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        a.pz[r] = zero4;
-        a.pw[r] = 0.0f;
-        a.pe[r] = 0.0f;
-      }
+    // loads into a in place: the stale values are read only at the steps
+    // and rows whose flags select them, so they need no clearing per step
+    auto step_in = [&](StepIn &a, int p) {
       if (res && p >= ka && p < kb) {
         const uint32_t pk = P * (uint32_t)p;
         if (anyN) a.pyn = ld4(eo_prev + x0 + pk);                               // row 0
@@ -944,7 +938,6 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
       const uint32_t orr = v0 ? o0 : P * (uint32_t)ka;
 #pragma unroll
       for (int i = 0; i < R + 2; ++i) a.r[i] = ld4(r32 + ra[i + 1] + orr);
-      return a;
     };
     // one Jacobi sweep of a row at one plane (k_mx_sweep's arithmetic):
     // centre c, z-neighbours zb/zt, y-neighbours n/s, right-hand side r; and,
@@ -965,6 +958,7 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
         wrapW = bcast(lastc, Lw);
         wrapE = bcast(c.x, 0);
       }
+      const bool stale = PX || fN || fS || zT || zB;
       float v[4], sq = 0.0f;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -972,23 +966,30 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
         if (hasf && q == 0) ew = wrapW;
         float ee = q == 3 ? ev : ce[q + 1];
         if (hasl && q == ql) ee = wrapE;
-        v[q] = cf.wx * (ee + ew) + cf.wy * (cn[q] + cs[q]) + cf.wz * (ct[q] + cb[q]) +
-               cf.kappa * ce[q] - cf.af * cr[q];
+        const float sxx = ee + ew, syy = cn[q] + cs[q], szz = ct[q] + cb[q];
+        v[q] = cf.wx * sxx + cf.wy * syy + cf.wz * szz + cf.kappa * ce[q] - cf.af * cr[q];
         if (first1) v[q] = -cf.af * cr[q];     // from e_0 = 0, as k_mx_sweep writes e_1
         if (!inrow(q) || zero) v[q] = 0.0f;
         if (test) {
-          // stop-test residual: e_{j-1} across the periodic boundaries
-          const float qN[4] = {qn.x, qn.y, qn.z, qn.w}, qS[4] = {qs.x, qs.y, qs.z, qs.w};
-          const float qZ[4] = {qz.x, qz.y, qz.z, qz.w};
-          const float se = (hasl && q == ql) ? pe : ee;
-          const float sw = (hasf && q == 0) ? pw : ew;
-          const float sn = fN ? qN[q] : cn[q];
-          const float ss = fS ? qS[q] : cs[q];
-          const float st_ = zT ? qZ[q] : ct[q];
-          const float sb = zB ? qZ[q] : cb[q];
+          // stop-test residual; across the periodic boundaries the
+          // neighbours are e_{j-1}, elsewhere the sums are the update's
+          float sx = sxx, sy = syy, sz = szz;
+          if (stale) {
+            const float qN[4] = {qn.x, qn.y, qn.z, qn.w}, qS[4] = {qs.x, qs.y, qs.z, qs.w};
+            const float qZ[4] = {qz.x, qz.y, qz.z, qz.w};
+            const float se = (hasl && q == ql) ? pe : ee;
+            const float sw = (hasf && q == 0) ? pw : ew;
+            const float sn = fN ? qN[q] : cn[q];
+            const float ss = fS ? qS[q] : cs[q];
+            const float st_ = zT ? qZ[q] : ct[q];
+            const float sb = zB ? qZ[q] : cb[q];
+            sx = se + sw;
+            sy = sn + ss;
+            sz = st_ + sb;
+          }
           const float e2 = 2.0f * ce[q];
-          const float rr = cr[q] - ((se + sw - e2) * cf.idx2 + (sn + ss - e2) * cf.idy2 +
-                                    (st_ + sb - e2) * cf.idz2);
+          const float rr = cr[q] - ((sx - e2) * cf.idx2 + (sy - e2) * cf.idy2 +
+                                    (sz - e2) * cf.idz2);
           if (inrow(q)) sq = sq + rr * rr;     // the lane's 4 squares in FP32
         }
       }
@@ -1063,13 +1064,20 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
       hs = v1[0];
       hn = v1[R + 1];
     };
-    auto si = [&](int p) { return step_in(p < pb ? p : pb); };
-    StepIn A = si(pa);
+    auto si = [&](StepIn &a, int p) { step_in(a, p < pb ? p : pb); };
+    StepIn A, B;
+    A.pyn = A.pys = B.pyn = B.pys = zero4;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      A.pz[r] = B.pz[r] = zero4;
+      A.pw[r] = A.pe[r] = B.pw[r] = B.pe[r] = 0.0f;
+    }
+    si(A, pa);
     for (int p = pa;; p += 2) {
-      StepIn B = si(p + 1);
+      si(B, p + 1);
       do_step(p, A);
       if (p + 1 > pb) break;
-      A = si(p + 2);
+      si(A, p + 2);
       do_step(p + 1, B);
       if (p + 2 > pb) break;
     }
@@ -1424,7 +1432,7 @@ SweepGeo sweep_sched(picard_ctx *c) { return sweep_geo(c, sweep_wmax(c)); }
 SweepGeo pair_sched(picard_ctx *c) {
   if (c->pair_wmax <= 0) {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_pair<false, false, kPairRows>, kSwThreads, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_pair<false, false, kPairRows, false>, kSwThreads, 0) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
         per_cu <= 0 || cus < 8) {
       c->pair_wmax = 64;
@@ -1441,7 +1449,8 @@ void launch_pair(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t 
                  const SweepGeo &sg, const SweepCoef &cf, const float *r32, const float *e,
                  float *out, float *eo_out, const float *eo_prev, const double *pin, double *pout,
                  JacobiState *js, const MxLaunch &L) {
-  auto fn = pad ? k_mx_pair<PX, true, kPairRows> : k_mx_pair<PX, false, kPairRows>;
+  auto fn = L.j == 0 ? (pad ? k_mx_pair<PX, true, kPairRows, true> : k_mx_pair<PX, false, kPairRows, true>)
+                     : (pad ? k_mx_pair<PX, true, kPairRows, false> : k_mx_pair<PX, false, kPairRows, false>);
   hipExtLaunchKernelGGL(fn, dim3(blocks), dim3(64, kSwRows), 0, s, t0, t1, 0, sg, cf, r32, e,
                         out, eo_out, eo_prev, pin, pout, js, L);
 }
