@@ -102,7 +102,7 @@ def run_form(cfg, precision, steps, warmup, capture_after=1):
                                                 ("N", _lib.FIELD_N))}
     dom.close()
     log(f"{name}: {steps} steps in {elapsed:.2f} s, sweeps {sweeps}, "
-        f"sweep kernel {1e3 * jac_ms / max(jac_launches, 1):.2f} us avg over {jac_launches}")
+        f"solve kernel {1e3 * jac_ms / max(jac_launches, 1):.2f} us per launch, avg over {jac_launches}")
     return dict(pushed=pushed, elapsed=elapsed, sweeps=sweeps, jac_ms=jac_ms,
                 jac_launches=jac_launches, fields=fields, captured_at=min(capture_after, it))
 
@@ -215,18 +215,24 @@ def main():
     headname = "split" if "split" in runs else "fp64"
     value = head["pushed"] / head["elapsed"]
     ncell = cfg.nx * cfg.ny * cfg.nz
-    # algorithmic bytes per launch of the dominant kernel: FP32 sweep reads the
-    # correction e and the FP32 residual and writes e (12 B per interior
-    # cell); the FP64 pass reads U and b and writes U (24 B per cell)
-    bpc = 12.0 if headname == "split" else 24.0
+    # Sweeps per launch of each form's solve kernel: the split form runs two
+    # FP32 sweeps per launch on rows of at most 256 cells (the default inner
+    # length, 128, is even), the FP64 form one pass per launch.
+    spl = {"split": 2 if cfg.nx <= 256 else 1, "fp64": 1}
+    # Algorithmic bytes per launch of the dominant kernel, per cell-sweep
+    # times the cell-sweeps of a launch: an FP32 sweep reads the correction e
+    # and the FP32 residual and writes e (12 B per interior cell); the FP64
+    # pass reads U and b and writes U (24 B per cell).
+    bpc = (12.0 if headname == "split" else 24.0) * spl[headname]
     bytes_per_launch = bpc * ncell
     avg_s = (head["jac_ms"] / 1e3) / max(head["jac_launches"], 1)
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
     per_form = {}
     for k, r in runs.items():
+        launch_us = 1e3 * r["jac_ms"] / max(r["jac_launches"], 1)
         per_form[k] = dict(value=r["pushed"] / r["elapsed"], ms_per_step=1e3 * r["elapsed"] / args.steps,
-                           sweeps_per_step=r["sweeps"],
-                           sweep_kernel_us=1e3 * r["jac_ms"] / max(r["jac_launches"], 1))
+                           sweeps_per_step=r["sweeps"], sweeps_per_launch=spl[k],
+                           launch_kernel_us=launch_us, sweep_kernel_us=launch_us / spl[k])
     errors = None
     if "split" in runs and "fp64" in runs:
         a, b = runs["split"]["fields"], runs["fp64"]["fields"]
@@ -235,7 +241,7 @@ def main():
                   "N_rel": _rel(a["N"], b["N"]),
                   "phi_rel_inf": float(np.abs(a["U"] - b["U"]).max() / np.abs(b["U"]).max())}
     # the FP64 form runs the two-cells-per-lane pass when nx is even
-    kname = ("k_mx_sweep" if headname == "split"
+    kname = (("k_mx_pair" if spl["split"] == 2 else "k_mx_sweep") if headname == "split"
              else ("k_jacobi_pass2" if cfg.nx % 2 == 0 else "k_jacobi_pass"))
     out = {
         "metric": METRIC,
@@ -271,6 +277,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles(kname),
             "bytes_per_launch": bytes_per_launch,
+            "sweeps_per_launch": spl[headname],
             "avg_launch_us": avg_s * 1e6,
             "timed_launches": head["jac_launches"],
             "timed_every": 16,
