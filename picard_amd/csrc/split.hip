@@ -858,6 +858,7 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
     const int ql = nx - 1 - xs;                             // x = nx - 1 at cell ql
     const bool hasf = PX && xs == 0;                        // the lane holds x = 0 (periodic x)
     const bool hasl = PX && live && ql < 4;                 // ... x = nx - 1
+    const bool haslast = live && ql < 4;                    // the lane holds x = nx - 1
     auto inrow = [&](int q) { return live && (!PAD || q <= ql); };
     // rows y0-2..y0+R+1 (index d: row y0+d-2): the lane's address of its
     // cells, whether the row is in the grid (or its periodic image), and
@@ -943,10 +944,18 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
     // centre c, z-neighbours zb/zt, y-neighbours n/s, right-hand side r; and,
     // when test, the stop-test residual of c there with the stale values
     // (sn, ss, st, sb when the flags are set; pw/pe across x)
+    // one Jacobi sweep of a row at one plane (k_mx_sweep's arithmetic):
+    // centre c, z-neighbours zb/zt, y-neighbours n/s, right-hand side r; and,
+    // when test, the stop-test residual of c there.  zero: a flow guard row
+    // or plane, whose correction is 0 (a wave-uniform branch).  Lanes past the
+    // row end are not zeroed: their only reader is the east neighbour of the
+    // lane holding x = nx - 1, which takes the guard (or the periodic image)
+    // instead; padding cells are zeroed, as they are stored.
     auto sweep = [&](const float4 &c, const float4 &zb, const float4 &zt, const float4 &n,
                      const float4 &s, const float4 &r, bool zero, bool first1, bool test,
                      bool fN, const float4 &qn, bool fS, const float4 &qs, bool zT,
                      bool zB, const float4 &qz, float pw, float pe) -> float4 {
+      if (zero) return zero4;
       const float ce[4] = {c.x, c.y, c.z, c.w};
       const float cn[4] = {n.x, n.y, n.z, n.w}, cs[4] = {s.x, s.y, s.z, s.w};
       const float ct[4] = {zt.x, zt.y, zt.z, zt.w}, cb[4] = {zb.x, zb.y, zb.z, zb.w};
@@ -958,42 +967,52 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
         wrapW = bcast(lastc, Lw);
         wrapE = bcast(c.x, 0);
       }
-      const bool stale = PX || fN || fS || zT || zB;
-      float v[4], sq = 0.0f;
+      float ew[4], ee[4], v[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        float ew = q == 0 ? wv : ce[q - 1];
-        if (hasf && q == 0) ew = wrapW;
-        float ee = q == 3 ? ev : ce[q + 1];
-        if (hasl && q == ql) ee = wrapE;
-        const float sxx = ee + ew, syy = cn[q] + cs[q], szz = ct[q] + cb[q];
-        v[q] = cf.wx * sxx + cf.wy * syy + cf.wz * szz + cf.kappa * ce[q] - cf.af * cr[q];
+        ew[q] = q == 0 ? wv : ce[q - 1];
+        if (hasf && q == 0) ew[q] = wrapW;
+        ee[q] = q == 3 ? ev : ce[q + 1];
+        if (haslast && q == ql) ee[q] = PX ? wrapE : 0.0f;
+        v[q] = cf.wx * (ee[q] + ew[q]) + cf.wy * (cn[q] + cs[q]) + cf.wz * (ct[q] + cb[q]) +
+               cf.kappa * ce[q] - cf.af * cr[q];
         if (first1) v[q] = -cf.af * cr[q];     // from e_0 = 0, as k_mx_sweep writes e_1
-        if (!inrow(q) || zero) v[q] = 0.0f;
-        if (test) {
-          // stop-test residual; across the periodic boundaries the
-          // neighbours are e_{j-1}, elsewhere the sums are the update's
-          float sx = sxx, sy = syy, sz = szz;
-          if (stale) {
-            const float qN[4] = {qn.x, qn.y, qn.z, qn.w}, qS[4] = {qs.x, qs.y, qs.z, qs.w};
-            const float qZ[4] = {qz.x, qz.y, qz.z, qz.w};
-            const float se = (hasl && q == ql) ? pe : ee;
-            const float sw = (hasf && q == 0) ? pw : ew;
+        if (PAD && !inrow(q)) v[q] = 0.0f;
+      }
+      if (test) {
+        // stop-test residual; across the periodic boundaries the neighbours
+        // are e_{j-1} (a wave-uniform branch: boundary rows and planes only,
+        // or every row when x is periodic), elsewhere the sums are the update's
+        float sq = 0.0f;
+        if (PX || fN || fS || zT || zB) {
+          const float qN[4] = {qn.x, qn.y, qn.z, qn.w}, qS[4] = {qs.x, qs.y, qs.z, qs.w};
+          const float qZ[4] = {qz.x, qz.y, qz.z, qz.w};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float se = (hasl && q == ql) ? pe : ee[q];
+            const float sw = (hasf && q == 0) ? pw : ew[q];
             const float sn = fN ? qN[q] : cn[q];
             const float ss = fS ? qS[q] : cs[q];
             const float st_ = zT ? qZ[q] : ct[q];
             const float sb = zB ? qZ[q] : cb[q];
-            sx = se + sw;
-            sy = sn + ss;
-            sz = st_ + sb;
+            const float e2 = 2.0f * ce[q];
+            const float rr = cr[q] - ((se + sw - e2) * cf.idx2 + (sn + ss - e2) * cf.idy2 +
+                                      (st_ + sb - e2) * cf.idz2);
+            if (!PAD || q <= ql) sq = sq + rr * rr;
           }
-          const float e2 = 2.0f * ce[q];
-          const float rr = cr[q] - ((sx - e2) * cf.idx2 + (sy - e2) * cf.idy2 +
-                                    (sz - e2) * cf.idz2);
-          if (inrow(q)) sq = sq + rr * rr;     // the lane's 4 squares in FP32
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float e2 = 2.0f * ce[q];
+            const float rr = cr[q] - ((ee[q] + ew[q] - e2) * cf.idx2 +
+                                      (cn[q] + cs[q] - e2) * cf.idy2 +
+                                      (ct[q] + cb[q] - e2) * cf.idz2);
+            if (!PAD || q <= ql) sq = sq + rr * rr;
+          }
         }
+        if (!live) sq = 0.0f;                  // the lane's 4 squares in FP32
+        r2 = r2 + (double)sq;
       }
-      if (test) r2 = r2 + (double)sq;
       return make_float4(v[0], v[1], v[2], v[3]);
     };
     // windows: e_j on rows y0-1..y0+R at planes p-1 (wm) and p (wc); e_{j+1}
