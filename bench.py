@@ -194,7 +194,7 @@ def main():
                     help="profiling: the 1 GiB copies the byte counters are scaled by")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the domain "
+        print("bench.py measures one GPU: there is no multi-GPU path (the domain "
               "decomposition it would use is described with the design); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)

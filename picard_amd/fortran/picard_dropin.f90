@@ -151,7 +151,7 @@ subroutine ParticlesTransfer(real_particles, int_particles, Lx_min, Lx_max, Ly_m
   integer, intent(inout) :: int_particles(max_per_proc), num_local
   real*8, intent(inout) :: real_particles(8, max_per_proc)
   if (iprocs * jprocs * kprocs /= 1) then
-    write(*,*) 'picard: this build runs one domain per GPU (iprocs = jprocs = kprocs = 1)'
+    write(*,*) 'picard: one domain per GPU, one rank (iprocs = jprocs = kprocs = 1)'
     error stop 1
   end if
   call picard_dropin_sync(num_local)
