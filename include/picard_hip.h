@@ -205,9 +205,11 @@ int picard_field_write(picard_ctx *ctx, int32_t which, const double *host, int64
 int picard_timing_enable(picard_ctx *ctx, int32_t enable);
 int picard_timing_read(picard_ctx *ctx, double *jacobi_ms, int64_t *jacobi_launches);
 /* FP32-split solve: `inner` FP32 sweeps per FP64 refinement step, 2..4096
- * (default 128).  No reference counterpart; the stop rule is applied to every
- * iterate whatever the choice, and the solution agrees with the FP64 form's
- * to the split form's tolerance for each. */
+ * (default 512).  No reference counterpart; the solution agrees with the FP64
+ * form's to the split form's tolerance for each choice.  Near the FP32 floor
+ * of a tight tolerance (relative residual far below 1e-6) the solve may run
+ * up to `inner` sweeps past the FP64 form's stop before the next FP64
+ * refinement. */
 int picard_split_solver(picard_ctx *ctx, int32_t inner);
 /* FP32-split solve: inner sweeps per kernel launch, 1 or 2 (default 2).  With
  * 2, one launch performs two sweeps with the traffic of one, on grids whose

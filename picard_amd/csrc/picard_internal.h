@@ -124,7 +124,7 @@ struct picard_ctx {
   // side, ping-pong correction, inner iterates j % 3 on periodic boundary planes
   float *r32 = nullptr, *e32[2] = {nullptr, nullptr};
   float *eo32[3] = {nullptr, nullptr, nullptr};
-  int mx_inner = 128;        // split: FP32 sweeps per FP64 refinement
+  int mx_inner = 512;        // split: FP32 sweeps per FP64 refinement
   bool mx_pairs = true;      // split: two sweeps per launch where the grid allows
   int pair_wmax = 0;         // split: resident blocks per XCD of the two-sweep kernel
   int outer_wmax = 0;        // split: resident blocks per XCD of the outer-step kernel

@@ -664,7 +664,7 @@ class Domain:
     def timing(self, enable: bool):
         check("picard_timing_enable", lib().picard_timing_enable(self.h, 1 if enable else 0))
 
-    def split_solver(self, inner: int = 128):
+    def split_solver(self, inner: int = 512):
         """FP32 sweeps per FP64 refinement step of the FP32-split solve."""
         check("picard_split_solver", lib().picard_split_solver(self.h, inner))
 

@@ -201,7 +201,7 @@ def test_split_inner_length_keeps_solution(dims, flow):
     out = {}
     for prec, inner in ((_lib.PICARD_FP64, None), (_lib.PICARD_FP32_SPLIT, 2),
                         (_lib.PICARD_FP32_SPLIT, 7), (_lib.PICARD_FP32_SPLIT, 64),
-                        (_lib.PICARD_FP32_SPLIT, 128)):
+                        (_lib.PICARD_FP32_SPLIT, 128), (_lib.PICARD_FP32_SPLIT, 512)):
         dom = host.Domain(cfg, max_particles=1024, d=d, precision=prec)
         if inner:
             dom.split_solver(inner=inner)
@@ -211,7 +211,7 @@ def test_split_inner_length_keeps_solution(dims, flow):
         out[inner] = (dom.calc_epotential(), dom.field(_lib.FIELD_U))
         dom.close()
     s0, U0 = out[None]
-    for inner in (2, 7, 64, 128):
+    for inner in (2, 7, 64, 128, 512):
         s1, U1 = out[inner]
         _say(f"dims={dims} inner={inner}: sweeps {s1} (fp64 {s0}) U rel {_rel(U1, U0):.2e}")
         assert abs(s1 - s0) <= 2
