@@ -215,7 +215,9 @@ int picard_split_solver(picard_ctx *ctx, int32_t inner);
  * 2, one launch performs two sweeps with the traffic of one, on grids whose
  * rows are at most 256 cells and with an even inner length (otherwise one
  * sweep per launch); the iterates are the same bit for bit, and the stop rule
- * is then applied to every second iterate.  No reference counterpart. */
+ * is then applied to every eighth iterate (one launch in four evaluates it),
+ * so the solve stops up to seven sweeps after the first iterate that passes.
+ * No reference counterpart. */
 int picard_split_sweeps_per_launch(picard_ctx *ctx, int32_t sweeps);
 
 /* One whole main-loop body after BCinflow (picard.f90:564-631, no output

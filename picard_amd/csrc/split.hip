@@ -336,6 +336,7 @@ struct MxLaunch {
   int32_t m;             // inner sweeps of this outer step (update launch)
   int32_t nin;           // partials in Pin
   int32_t pairs;         // two sweeps per launch: e_k lives in E[((k + 1) / 2) % 2]
+  int32_t test;          // two-sweep launch: evaluate the stop-test residual of e_j
 };
 
 // Compact FP32 layout of the correction, its right-hand side and the
@@ -799,6 +800,7 @@ __global__ void __launch_bounds__(kSwThreads, 1024 / kSwThreads) k_mx_sweep(Swee
 // and every e_j read goes to the zero row of `e`, which then only has to be a
 // buffer of the solve.
 constexpr int kPairRows = 2;   // rows per wave of the two-sweep kernel
+constexpr int kPairTestEvery = 4;   // two-sweep launches per stop test
 
 template <bool PX, bool PAD, int R, bool FIRST>
 __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoef cf,
@@ -819,7 +821,7 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   if (cur.done) return;
-  const bool res = !FIRST;                                  // FIRST: j = 0
+  const bool res = !FIRST && L.test;                        // FIRST: j = 0
   constexpr bool first = FIRST;
   const uint32_t NX = sg.NX, P = sg.P;
   const int nx = sg.nx, ny = sg.ny, nz = sg.nz;
@@ -898,6 +900,7 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
     auto pvalid = [&](int pp) { return sg.pz | ((pp >= 0) & (pp < nz)); };
     auto poff = [&](int pp) { return P * (uint32_t)wrapz(pp); };
     auto e0at = [&](int d, uint32_t o, bool v) -> float4 {
+      if constexpr (FIRST) return zero4;       // e_0 = 0: nothing to read
       return ld4(e + ((v & rok[d]) ? ra[d] + o : zr));
     };
     // what plane step p reads: e_j on rows y0-1..y0+R at plane p+1 and on
@@ -1586,10 +1589,9 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
   int32_t nin = 0;                          // partials the last writer stored
   StopSlot fin{};
   while (true) {
-    // timing: events around one sweep in kTimeEvery, mid-way between outer
-    // steps (event records between every pair of sweeps would slow the timed
-    // solve itself)
-    const int timed = (m + kTimeEvery / 2 - 1) / kTimeEvery;     // sweeps j = 8, 24, ...
+    // timing: events around one launch in kTimeEvery (event records around
+    // every launch would slow the timed solve itself)
+    const int timed = (m + kTimeEvery - 1) / kTimeEvery;         // one launch position in 16
     if (c->timing) {
       while ((int64_t)c->tev.size() < 2 * timed) {
         hipEvent_t e;
@@ -1601,7 +1603,7 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
     // launch); the stale-halo norm needs e_m and e_{m-1} of the previous outer
     double *usrc = Ub[outer & 1], *udst = Ub[(outer + 1) & 1];
     {
-      MxLaunch L{seq, base, 0, -1, m, 0, pairs ? 1 : 0};
+      MxLaunch L{seq, base, 0, -1, m, 0, pairs ? 1 : 0, 1};
       // e_m and the boundary copies of e_{m-1} of the previous outer step
       const float *em = ebuf(m);
       const float *eo = EO[(m - 1) % 3];
@@ -1622,19 +1624,25 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
     for (int j = 0; j < m;) {
       // the iterate-0 test used b itself (launch 0); later ones use residuals
       const int32_t chk = (check >= 1) ? check : -1;
-      MxLaunch L{seq, base, j, chk, m, nin, pairs ? 1 : 0};
-      const double *pin = P[pw ^ 1];
-      double *pout = P[pw];
       // two sweeps, unless the first of them reaches iterate 2*maxit+1 (the
       // solve's last, which must exist)
       const bool two = pairs && j + 2 <= m && base + j + 1 != last_it;
+      // the stop test of iterate base + j: every single sweep, every
+      // kPairTestEvery-th two-sweep launch (every 2 kPairTestEvery iterates)
+      const bool test = j >= 1 && (!two || (j / 2) % kPairTestEvery == 0);
+      MxLaunch L{seq, base, j, chk, m, nin, pairs ? 1 : 0, test ? 1 : 0};
+      const double *pin = P[pw ^ 1];
+      double *pout = P[pw];
       // e_j (none for j = 0) -> e_{j+1} or e_{j+2}; boundary copies of
       // e_{j+1} to slot (j+1) % 3, e_{j-1} from slot (j-1) % 3
       const float *e = j >= 1 || two ? ebuf(j) : nullptr;
       float *out = ebuf(j + (two ? 2 : 1));
       float *eo_out = EO[(j + 1) % 3];
       const float *eo_prev = j >= 2 ? EO[(j - 1) % 3] : nullptr;
-      const bool tj = c->timing && j % kTimeEvery == kTimeEvery / 2 && nrec < timed;
+      // the timed position moves by two launches per refinement step, so that
+      // launches with and without the stop test are sampled in proportion
+      const int tpos = (kTimeEvery / 2 + 2 * outer) % kTimeEvery;
+      const bool tj = c->timing && j >= 1 && j % kTimeEvery == tpos && nrec < timed;
       hipEvent_t t0 = tj ? c->tev[2 * nrec] : nullptr;
       hipEvent_t t1 = tj ? c->tev[2 * nrec + 1] : nullptr;
       if (tj) ++nrec;
@@ -1651,18 +1659,18 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
         launch_sweep<false>(pad, swB, c->stream, t0, t1, sg, cf, c->r32, e, out, eo_out, eo_prev,
                             pin, pout, c->js, L);
       ++seq;
-      if (j >= 1) {               // wrote the residual of iterate base + j
+      if (test) {                 // wrote the residual of iterate base + j
         check = base + j;
         nin = two ? prB : swB;
         pw ^= 1;
-      } else {
+      } else {                    // the launch took the previous test's decision
         check = -1;
       }
       j += two ? 2 : 1;
     }
     {
       const int32_t chk = (check >= 1) ? check : -1;
-      MxLaunch L{seq, base, 0, chk, m, nin, pairs ? 1 : 0};
+      MxLaunch L{seq, base, 0, chk, m, nin, pairs ? 1 : 0, 1};
       k_mx_update<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, cd, ts, udst, E0, E1, P[pw ^ 1],
                                                               c->js, L);
       ++seq;

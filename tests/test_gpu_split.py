@@ -6,8 +6,10 @@ relative, densities within 1e-4 relative (normwise) after one step; the
 fixed-point deposit of the same particles within 2e-7 relative (FP32 weights);
 the mixed-precision solve of the same charge within 1e-5 of the FP64 solve.
 The split form applies the reference's stop test (stale periodic halo,
-CalcEpotential.f90:83-92) to FP32-evaluated residuals, so it stops within a
-sweep or two of the FP64 form."""
+CalcEpotential.f90:83-92) to FP32-evaluated residuals of every eighth iterate
+(two sweeps per launch, a test every fourth launch; every iterate with one
+sweep per launch), so it stops within a sweep or two of the FP64 form, or up
+to seven sweeps after it."""
 import numpy as np
 import pytest
 
@@ -36,6 +38,13 @@ def _pair(cfg, cap=None):
 
 def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _stops_together(s_fp64, s_split):
+    """The split stop against the FP64 one: FP32 rounding moves it by a sweep
+    or two either way, and with two sweeps per launch the next tested iterate
+    may be up to seven sweeps later."""
+    return -2 <= s_split - s_fp64 <= 9
 
 
 def test_split_deposit_matches_fp64():
@@ -78,7 +87,7 @@ def test_split_solve_matches_fp64(flow):
         sw.append(dom.calc_epotential())
     Ua, Ub = a.field(_lib.FIELD_U), b.field(_lib.FIELD_U)
     _say(f"sweeps fp64={sw[0]} split={sw[1]}  U rel {_rel(Ub, Ua):.3e}")
-    assert abs(sw[0] - sw[1]) <= 2
+    assert _stops_together(sw[0], sw[1])
     assert _rel(Ub, Ua) <= 1e-5
     Ea = a.field(_lib.FIELD_E)
     b.calc_efield()
@@ -112,7 +121,7 @@ def test_split_solve_shapes(dims, flow):
         sw.append(dom.calc_epotential())
     Ua, Ub = a.field(_lib.FIELD_U), b.field(_lib.FIELD_U)
     _say(f"sweeps fp64={sw[0]} split={sw[1]}  U rel {_rel(Ub, Ua):.3e}")
-    assert abs(sw[0] - sw[1]) <= 2
+    assert _stops_together(sw[0], sw[1])
     assert _rel(Ub, Ua) <= 1e-5
     a.close()
     b.close()
@@ -136,7 +145,7 @@ def test_split_steps_error_report():
              f"phi rel {eU:.2e} N rel {eN:.2e}")
         if it == 1:
             assert eU <= 1e-5 and eN <= 1e-4
-        assert abs(sa.sweeps - sb.sweeps) <= 2
+        assert _stops_together(sa.sweeps, sb.sweeps)
         assert abs(sa.num_local - sb.num_local) <= 2
     # particle trajectories by id (slot 8 of the FP64 form, id of the split form)
     ra, _ = a.particles()
@@ -190,7 +199,9 @@ def test_split_bc_potentials_fft(dims, flow):
 def test_split_inner_length_keeps_solution(dims, flow):
     """The number of FP32 sweeps per FP64 refinement step changes where the
     refinement happens, not the iteration: every choice stops within two
-    sweeps of the FP64 form at the same solution (1e-5)."""
+    sweeps of the FP64 form at the same solution (1e-5), or, with two sweeps
+    per launch (even lengths above 2), at the next tested iterate: every
+    eighth is tested, so up to seven sweeps later."""
     nx, ny, nz = dims
     cfg = host.Config(nx=nx, ny=ny, nz=nz, flow=flow,
                       gmin=(-5.0 * nx, -5.0 * ny, -5.0 * nz), gmax=(5.0 * nx, 5.0 * ny, 5.0 * nz),
@@ -214,7 +225,7 @@ def test_split_inner_length_keeps_solution(dims, flow):
     for inner in (2, 7, 64, 128, 512):
         s1, U1 = out[inner]
         _say(f"dims={dims} inner={inner}: sweeps {s1} (fp64 {s0}) U rel {_rel(U1, U0):.2e}")
-        assert abs(s1 - s0) <= 2
+        assert _stops_together(s0, s1)
         assert _rel(U1, U0) <= 1e-5
 
 
@@ -248,14 +259,16 @@ def _split_solve(cfg, d, C, sweeps_per_launch, maxerr, maxit, inner=128):
                                        ((13, 6, 40), (False, True, True)),
                                        ((24, 3, 10), (True, False, False)),
                                        ((20, 2, 6), (False, False, False)),
-                                       ((40, 9, 2), (True, False, False))])
+                                       ((40, 9, 2), (True, False, False)),
+                                       ((256, 6, 10), (False, True, True))])
 def test_split_two_sweeps_per_launch_same_iterates(dims, flow):
     """Two sweeps per launch (rows of at most 256 cells) give the iterates of
     two single-sweep launches bit for bit: with every stop test failing, both
     end at iterate 2*maxit+1 with the same potential, also when that iterate
     ends an odd number of sweeps into a refinement step (a single sweep then
     finishes it).  With the stop rule live, the two-sweep solve tests every
-    second iterate and stops at most one sweep later."""
+    eighth iterate (every fourth launch) and stops at most seven sweeps later,
+    at a potential within 1e-6 of the single-sweep one."""
     nx, ny, nz = dims
     cfg = host.Config(nx=nx, ny=ny, nz=nz, flow=flow,
                       gmin=(-5.0 * nx, -5.0 * ny, -5.0 * nz), gmax=(5.0 * nx, 5.0 * ny, 5.0 * nz),
@@ -272,7 +285,7 @@ def test_split_two_sweeps_per_launch_same_iterates(dims, flow):
     s1, U1 = _split_solve(cfg, d, C, 1, 1e-6, 2 ** 20)
     s2, U2 = _split_solve(cfg, d, C, 2, 1e-6, 2 ** 20)
     _say(f"dims={dims} converged: sweeps {s1} / {s2}, U rel {_rel(U2, U1):.2e}")
-    assert s1 <= s2 <= s1 + 1
+    assert s1 <= s2 <= s1 + 7
     assert _rel(U2, U1) <= 1e-6
 
 
