@@ -214,10 +214,12 @@ int picard_split_solver(picard_ctx *ctx, int32_t inner);
 /* FP32-split solve: inner sweeps per kernel launch, 1 or 2 (default 2).  With
  * 2, one launch performs two sweeps with the traffic of one, on grids whose
  * rows are at most 256 cells and with an even inner length (otherwise one
- * sweep per launch); the iterates are the same bit for bit, and the stop rule
- * is then applied to every eighth iterate (one launch in four evaluates it),
- * so the solve stops up to seven sweeps after the first iterate that passes.
- * No reference counterpart. */
+ * sweep per launch); the iterates are the same bit for bit.  The stop rule is
+ * then applied to every 32nd iterate while the residual, extrapolated from
+ * its decay, is far from the threshold, and to every second iterate near it,
+ * so the solve stops at most a sweep after the first iterate that passes
+ * unless the residual falls much faster than it did between tests.  No
+ * reference counterpart. */
 int picard_split_sweeps_per_launch(picard_ctx *ctx, int32_t sweeps);
 
 /* One whole main-loop body after BCinflow (picard.f90:564-631, no output

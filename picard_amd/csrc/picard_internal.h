@@ -62,8 +62,10 @@ struct ParticlesSplit {
 // one of its own blocks writes.
 struct StopSlot {
   int32_t done;
-  int32_t pad;
+  int32_t near;              // split form: the stop is expected before the next sparse test
   int64_t result;            // sweeps of the final iterate, valid when done
+  double rlast;              // split form: the last tested residual norm (0: none yet)
+  int64_t itlast;            // split form: the iterate it belongs to
 };
 
 // Device-side scalars of the Jacobi solve (one per context).

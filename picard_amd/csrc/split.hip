@@ -337,6 +337,7 @@ struct MxLaunch {
   int32_t nin;           // partials in Pin
   int32_t pairs;         // two sweeps per launch: e_k lives in E[((k + 1) / 2) % 2]
   int32_t test;          // two-sweep launch: evaluate the stop-test residual of e_j
+  int32_t prevtest;      // the previous launch wrote partials whatever its near flag
 };
 
 // Compact FP32 layout of the correction, its right-hand side and the
@@ -374,18 +375,46 @@ struct SweepCoef {
 __device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
 __device__ __forceinline__ void st4(float *p, float4 v) { *reinterpret_cast<float4 *>(p) = v; }
 
+// The stop test of iterate L.check (CalcEpotential.f90:60-61) on the
+// previous launch's partials, when that launch tested: the host marked it
+// (prevtest) or its near flag was set (it tested every launch then).  Each
+// test also sets `near`:
+// whether the residual, extrapolated at the decay per sweep seen since the
+// previous test to the next sparse test (kNearSpan sweeps on), is within
+// kNear of the threshold.  While near, the two-sweep kernel tests every
+// launch, so the solve stops at most a sweep after the first iterate that
+// passes; the flag is taken again at every test, so a fast early decay that
+// overstates the next one is corrected by the tests that follow.
+constexpr int kPairTestEvery = 16;  // two-sweep launches per stop test, far from the stop
+constexpr double kNear = 1.25;
+constexpr double kNearSpan = 2.0 * kPairTestEvery;   // sweeps between sparse tests
+
 __device__ __forceinline__ StopSlot mx_decide(JacobiState *st, const MxLaunch &L,
                                               const double *__restrict__ Pin, double *sh,
                                               int *decide) {
   const StopSlot prev = st->slot[(L.seq - 1) & 1];
   StopSlot cur = prev;
-  if (!prev.done && L.check >= 1) {
+  if (!prev.done && L.check >= 1 && (L.prevtest || prev.near)) {
     const double S = sum_partials(Pin, L.nin, sh);
-    if (lin_tid() == 0) *decide = (L.check + 1 <= st->maxit && sqrt(S) <= st->thresh) ? 1 : 0;
+    const bool tested = S >= 0.0;
+    const double rn = sqrt(tested ? S : 0.0);
+    if (lin_tid() == 0) {
+      double next = rn;
+      const int64_t gap = (int64_t)L.check - prev.itlast;
+      if (prev.rlast > 0.0 && rn > 0.0 && gap > 0)
+        next = rn * pow(rn / prev.rlast, kNearSpan / (double)gap);
+      *decide = ((tested && L.check + 1 <= st->maxit && rn <= st->thresh) ? 1 : 0) |
+                ((tested && fmin(rn, next) <= kNear * st->thresh) ? 2 : 0);
+    }
     __syncthreads();
-    if (*decide) {
+    if (*decide & 1) {
       cur.done = 1;
       cur.result = L.check;
+    }
+    if (tested) {
+      cur.near = (*decide & 2) ? 1 : 0;
+      cur.rlast = rn;
+      cur.itlast = L.check;
     }
   }
   return cur;
@@ -800,7 +829,6 @@ __global__ void __launch_bounds__(kSwThreads, 1024 / kSwThreads) k_mx_sweep(Swee
 // and every e_j read goes to the zero row of `e`, which then only has to be a
 // buffer of the solve.
 constexpr int kPairRows = 2;   // rows per wave of the two-sweep kernel
-constexpr int kPairTestEvery = 4;   // two-sweep launches per stop test
 
 template <bool PX, bool PAD, int R, bool FIRST>
 __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoef cf,
@@ -821,7 +849,9 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   if (cur.done) return;
-  const bool res = !FIRST && L.test;                        // FIRST: j = 0
+  // the stop-test residual of e_j: in the launches the host marks, and in
+  // every launch once a tested residual came near the threshold
+  const bool res = !FIRST && (L.test || cur.near);
   constexpr bool first = FIRST;
   const uint32_t NX = sg.NX, P = sg.P;
   const int nx = sg.nx, ny = sg.ny, nz = sg.nz;
@@ -1586,6 +1616,7 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
   int32_t seq = 1, base = 0;
   int pw = 0;                               // partials buffer the next writer uses
   int32_t check = -1;                       // iterate whose residual the last writer stored
+  bool prevtest = true;                     // the last launch tested whatever its near flag
   int32_t nin = 0;                          // partials the last writer stored
   StopSlot fin{};
   while (true) {
@@ -1603,7 +1634,7 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
     // launch); the stale-halo norm needs e_m and e_{m-1} of the previous outer
     double *usrc = Ub[outer & 1], *udst = Ub[(outer + 1) & 1];
     {
-      MxLaunch L{seq, base, 0, -1, m, 0, pairs ? 1 : 0, 1};
+      MxLaunch L{seq, base, 0, -1, m, 0, pairs ? 1 : 0, 1, 1};
       // e_m and the boundary copies of e_{m-1} of the previous outer step
       const float *em = ebuf(m);
       const float *eo = EO[(m - 1) % 3];
@@ -1619,6 +1650,7 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       check = base;
       nin = ouB;
       pw ^= 1;
+      prevtest = true;
     }
     int nrec = 0;                           // timed launches of this outer step
     for (int j = 0; j < m;) {
@@ -1629,8 +1661,9 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       const bool two = pairs && j + 2 <= m && base + j + 1 != last_it;
       // the stop test of iterate base + j: every single sweep, every
       // kPairTestEvery-th two-sweep launch (every 2 kPairTestEvery iterates)
+      // and, on the device, every launch while the stop is near
       const bool test = j >= 1 && (!two || (j / 2) % kPairTestEvery == 0);
-      MxLaunch L{seq, base, j, chk, m, nin, pairs ? 1 : 0, test ? 1 : 0};
+      MxLaunch L{seq, base, j, chk, m, nin, pairs ? 1 : 0, test ? 1 : 0, prevtest ? 1 : 0};
       const double *pin = P[pw ^ 1];
       double *pout = P[pw];
       // e_j (none for j = 0) -> e_{j+1} or e_{j+2}; boundary copies of
@@ -1659,18 +1692,19 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
         launch_sweep<false>(pad, swB, c->stream, t0, t1, sg, cf, c->r32, e, out, eo_out, eo_prev,
                             pin, pout, c->js, L);
       ++seq;
-      if (test) {                 // wrote the residual of iterate base + j
+      if (j >= 1) {               // may have written the residual of iterate base + j
         check = base + j;
         nin = two ? prB : swB;
         pw ^= 1;
-      } else {                    // the launch took the previous test's decision
+      } else {
         check = -1;
       }
+      prevtest = test;
       j += two ? 2 : 1;
     }
     {
       const int32_t chk = (check >= 1) ? check : -1;
-      MxLaunch L{seq, base, 0, chk, m, nin, pairs ? 1 : 0, 1};
+      MxLaunch L{seq, base, 0, chk, m, nin, pairs ? 1 : 0, 1, prevtest ? 1 : 0};
       k_mx_update<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, cd, ts, udst, E0, E1, P[pw ^ 1],
                                                               c->js, L);
       ++seq;
