@@ -133,14 +133,17 @@ def cpu_baseline(budget_s=25.0):
                        f"restatement in reference order, FP64, 1 thread, {el:.1f} s")
 
 
-def solve_only(cfg, form, sweeps):
+def solve_only(cfg, form, sweeps, inner=0):
     """Profiling mode: one field solve of the workload's grid on a seeded
     random charge, capped at `sweeps` sweeps (no particles), so that counter
-    passes over every dispatch stay short."""
+    passes over every dispatch stay short; `inner` > 0 sets the split form's
+    FP32 sweeps per FP64 refinement (else the library default)."""
     from picard_amd import _lib, host
     d = host.derive(cfg)
     prec = _lib.PICARD_FP32_SPLIT if form == "split" else _lib.PICARD_FP64
     dom = host.Domain(cfg, max_particles=1024, d=d, precision=prec)
+    if inner > 0 and form == "split":
+        dom.split_solver(inner)
     dom.calc_distance()
     rng = np.random.default_rng(SEED)
     dom.set_field(_lib.FIELD_C, rng.normal(size=(cfg.nz + 2, cfg.ny + 2, cfg.nx + 2)) * 1e-16)
@@ -190,6 +193,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--solve-only", type=int, default=0, metavar="SWEEPS",
                     help="profiling: one capped field solve of --form (split or fp64) only")
+    ap.add_argument("--inner", type=int, default=0, metavar="M",
+                    help="with --solve-only: FP32 sweeps per FP64 refinement of the split form")
     ap.add_argument("--copy-cal", action="store_true",
                     help="profiling: the 1 GiB copies the byte counters are scaled by")
     args = ap.parse_args()
@@ -204,7 +209,7 @@ def main():
         copy_calibration()
         return
     if args.solve_only:
-        solve_only(cfg, "fp64" if args.form == "fp64" else "split", args.solve_only)
+        solve_only(cfg, "fp64" if args.form == "fp64" else "split", args.solve_only, args.inner)
         return
     runs = {}
     if args.form in ("both", "split"):
