@@ -820,14 +820,14 @@ __global__ void __launch_bounds__(kSwThreads, 1024 / kSwThreads) k_mx_sweep(Swee
 // e_{j+2} written); rows of e_j shared by neighbouring waves come mostly from
 // cache.  Every cell's arithmetic is k_mx_sweep's, so e_{j+2} is bit for bit
 // what two launches of it give.  The stop-test residual is that of e_j (taken
-// from the first sweep of the wave's rows, with the stale halo of e_{j-1}), so
-// iterates are tested every second sweep; the boundary copies of e_{j+1} go to
+// from the first sweep of the wave's rows, with the stale halo of e_{j-1}),
+// evaluated in the launches the host marks and in every launch while the last
+// test found the stop near (mx_decide); the boundary copies of e_{j+1} go to
 // `eo_out`, for the residual of e_{j+2} in the next launch.  x-neighbours
 // come from DPP wave shifts and the periodic image of x from readlane; no
 // load depends on the lane beyond its address, so the wave stays convergent
-// where it exchanges values.  For j = 0 (e_0 = 0) the first sweep is -af r
-// and every e_j read goes to the zero row of `e`, which then only has to be a
-// buffer of the solve.
+// where it exchanges values.  FIRST: j = 0 (e_0 = 0), the first sweep is
+// -af r and nothing of e is read.
 constexpr int kPairRows = 2;   // rows per wave of the two-sweep kernel
 
 template <bool PX, bool PAD, int R, bool FIRST>
@@ -850,7 +850,7 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   if (cur.done) return;
   // the stop-test residual of e_j: in the launches the host marks, and in
-  // every launch once a tested residual came near the threshold
+  // every launch while the last test found the stop near
   const bool res = !FIRST && (L.test || cur.near);
   constexpr bool first = FIRST;
   const uint32_t NX = sg.NX, P = sg.P;
