@@ -196,10 +196,13 @@ int picard_advance_velocity_exp(picard_ctx *ctx, double deltat, double dt);
 int picard_field_read(picard_ctx *ctx, int32_t which, double *host, int64_t count);
 int picard_field_write(picard_ctx *ctx, int32_t which, const double *host, int64_t count);
 
-/* Kernel timing of the Jacobi sweep kernel, for roofline measurement; no
- * reference counterpart (the reference has no timers).  While enabled one
- * sweep launch in 16 carries start/stop events stamped by its dispatch
- * packet (events around every launch would slow the timed solve);
+/* Kernel timing of the Jacobi solve kernel (the FP64 pass, or the split
+ * form's sweep launches, two sweeps each where they apply), for roofline
+ * measurement; no reference counterpart (the reference has no timers).  While
+ * enabled one launch in 16 carries start/stop events stamped by its dispatch
+ * packet (events around every launch would slow the timed solve; in the split
+ * form the timed position moves from one refinement step to the next, so
+ * launches with and without the stop test are sampled in proportion);
  * picard_timing_read returns the summed kernel time and the count of timed
  * launches since the last enable. */
 int picard_timing_enable(picard_ctx *ctx, int32_t enable);
