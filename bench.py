@@ -221,9 +221,10 @@ def main():
     value = head["pushed"] / head["elapsed"]
     ncell = cfg.nx * cfg.ny * cfg.nz
     # Sweeps per launch of each form's solve kernel: the split form runs two
-    # FP32 sweeps per launch on rows of at most 256 cells (the default inner
-    # length, 512, is even), the FP64 form one pass per launch.
-    spl = {"split": 2 if cfg.nx <= 256 else 1, "fp64": 1}
+    # FP32 sweeps per launch on rows of at most 256 cells, or wider rows with
+    # x not periodic (the default inner length, 512, is even), the FP64 form
+    # one pass per launch.
+    spl = {"split": 2 if cfg.nx <= 256 or cfg.flow[0] else 1, "fp64": 1}
     # Algorithmic bytes per launch of the dominant kernel, per cell-sweep
     # times the cell-sweeps of a launch: an FP32 sweep reads the correction e
     # and the FP32 residual and writes e (12 B per interior cell); the FP64

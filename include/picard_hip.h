@@ -216,8 +216,9 @@ int picard_timing_read(picard_ctx *ctx, double *jacobi_ms, int64_t *jacobi_launc
 int picard_split_solver(picard_ctx *ctx, int32_t inner);
 /* FP32-split solve: inner sweeps per kernel launch, 1 or 2 (default 2).  With
  * 2, one launch performs two sweeps with the traffic of one, on grids whose
- * rows are at most 256 cells and with an even inner length (otherwise one
- * sweep per launch); the iterates are the same bit for bit.  The stop rule is
+ * rows are at most 256 cells or whose x boundaries are not periodic, and
+ * with an even inner length (otherwise one sweep per launch); the iterates
+ * are the same bit for bit.  The stop rule is
  * then applied to every 32nd iterate while the residual, extrapolated from
  * its decay, is far from the threshold, and to every second iterate near it,
  * so the solve stops at most a sweep after the first iterate that passes

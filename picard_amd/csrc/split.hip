@@ -843,9 +843,17 @@ __global__ void __launch_bounds__(kSwThreads, 1024 / kSwThreads) k_mx_sweep(Swee
 // load depends on the lane beyond its address, so the wave stays convergent
 // where it exchanges values.  FIRST: j = 0 (e_0 = 0), the first sweep is
 // -af r and nothing of e is read.
-constexpr int kPairRows = 2;   // rows per wave of the two-sweep kernel
+//
+// MULTI: rows wider than one segment (x not periodic).  Segment k keeps the
+// cells [kPairKeep*k, kPairKeep*k + kPairKeep) and its wave holds the 256
+// cells from four before them: the first sweep of the cells on either side
+// of the kept ones is then the wave's own (lanes 0 and 63, whose outer cells
+// lack a neighbour, feed only cells that are not kept), at the price of
+// sweeping 256 cells for 248.
+constexpr int kPairRows = 2;     // rows per wave of the two-sweep kernel
+constexpr int kPairKeep = 248;   // cells a wave keeps of a segment (MULTI)
 
-template <bool PX, bool PAD, int R, bool FIRST>
+template <bool PX, bool PAD, int R, bool FIRST, bool MULTI>
 __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoef cf,
                                                   const float *__restrict__ r32,
                                                   const float *__restrict__ e,
@@ -855,6 +863,7 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
                                                   const double *__restrict__ Pin,
                                                   double *__restrict__ Pout, JacobiState *st,
                                                   MxLaunch L) {
+  static_assert(!(MULTI && PX), "overlapping segments need x not periodic");
   __shared__ double sh[kSwThreads / 64];
   __shared__ int decide;
   StopSlot cur = mx_decide(st, L, Pin, sh, &decide);
@@ -894,13 +903,17 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
     return kk >= nz ? kk - nz : kk;
   };
   for (int t = w; t < sg.TP; t += sg.W) {
-    const int zi = t % sg.nzc, ty = t / sg.nzc;            // one segment per row
+    const int zi = t % sg.nzc, tt = t / sg.nzc;
+    const int tx = MULTI ? tt % sg.ntx : 0, ty = MULTI ? tt / sg.ntx : tt;
     const int ka = z0 + ((z1 - z0) * zi) / sg.nzc, kb = z0 + ((z1 - z0) * (zi + 1)) / sg.nzc;
     // the wave's first row (the same in every lane: scalar flags below)
     const int y0 = __builtin_amdgcn_readfirstlane((ty * kSwRows + (int)threadIdx.y) * R);
     if (y0 >= ny || kb <= ka) continue;                     // the whole wave
-    const int xs = 4 * lane;                                // the lane's first cell
-    const bool live = xs < nx;                              // lanes past the row end store nothing
+    const int xs = MULTI ? kPairKeep * tx - 4 + 4 * lane : 4 * lane;   // the lane's first cell
+    const bool live = xs >= 0 && xs < nx;                   // lanes outside the row store nothing
+    // the lane's cells are the wave's to store (MULTI: lanes 1..62)
+    const bool keep = live && (!MULTI || (lane >= 1 && lane <= 62));
+    const bool has0 = MULTI && xs == 0;                     // the lane holds x = 0 (flow guard west)
     const uint32_t x0 = live ? (uint32_t)xs : 0u;
     const int ql = nx - 1 - xs;                             // x = nx - 1 at cell ql
     const bool hasf = PX && xs == 0;                        // the lane holds x = 0 (periodic x)
@@ -1021,6 +1034,7 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
       for (int q = 0; q < 4; ++q) {
         ew[q] = q == 0 ? wv : ce[q - 1];
         if (hasf && q == 0) ew[q] = wrapW;
+        if (has0 && q == 0) ew[q] = 0.0f;      // the lane below holds x = -1 (MULTI)
         ee[q] = q == 3 ? ev : ce[q + 1];
         // without padding, x = nx - 1 is the last cell of its lane
         if ((PAD || q == 3) && haslast && q == ql) ee[q] = PX ? wrapE : 0.0f;
@@ -1059,7 +1073,7 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
             if (!PAD || q <= ql) sq = sq + rr * rr;
           }
         }
-        if (!live) sq = 0.0f;                  // the lane's 4 squares in FP32
+        if (!keep) sq = 0.0f;                  // the lane's 4 squares in FP32
         r2 = r2 + (double)sq;
       }
       return make_float4(v[0], v[1], v[2], v[3]);
@@ -1103,7 +1117,7 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
           v1[i] = sweep(wc[i], wm[i], cn[i], n, s, a.r[i], !pv || !rv[i + 1], first,
                         res && in && own[r], yN[r], a.pyn, yS[r], a.pys, zT, zB, a.pz[r],
                         a.pw[r], a.pe[r]);
-          if (in && live && own[r] && (edge[r] || zT || zB))
+          if (in && keep && own[r] && (edge[r] || zT || zB))
             st4(eo_out + ra[i + 1] + P * (uint32_t)p, v1[i]);
         } else {
           v1[i] = sweep(wc[i], wm[i], cn[i], n, s, a.r[i], !pv || !rv[i + 1], first, false,
@@ -1118,7 +1132,7 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
           const float4 &s = r > 0 ? u1[r - 1] : hs;
           const float4 v2 = sweep(u1[r], u0[r], v1[r + 1], n, s, rp[r], false, false, false,
                                   false, zero4, false, zero4, false, false, zero4, 0.0f, 0.0f);
-          if (live && own[r]) st4(out + ra[r + 2] + P * (uint32_t)(p - 1), v2);
+          if (keep && own[r]) st4(out + ra[r + 2] + P * (uint32_t)(p - 1), v2);
         }
       }
 #pragma unroll
@@ -1469,8 +1483,8 @@ TileSched stencil_sched(picard_ctx *c) {
 // blocks of 4 waves marching 32 planes, measured 31 us against 38 us for two
 // chunks and 1024 blocks).
 // rows: rows per wave; fill: z chunks until the tiles fill the resident
-// blocks (else about half of them)
-SweepGeo sweep_geo(picard_ctx *c, int wmax, int rows = 1, bool fill = false) {
+// blocks (else about half of them); seg: cells of a row a wave keeps
+SweepGeo sweep_geo(picard_ctx *c, int wmax, int rows = 1, bool fill = false, int seg = kSwSeg) {
   const GridDims &g = c->gd;
   SweepGeo sg;
   sg.nx = g.nx;
@@ -1481,7 +1495,7 @@ SweepGeo sweep_geo(picard_ctx *c, int wmax, int rows = 1, bool fill = false) {
   sg.px = !g.flow[0];
   sg.py = !g.flow[1];
   sg.pz = !g.flow[2];
-  sg.ntx = (g.nx + kSwSeg - 1) / kSwSeg;
+  sg.ntx = (g.nx + seg - 1) / seg;
   sg.nty = (g.ny + kSwRows * rows - 1) / (kSwRows * rows);
   const int per_chunk = sg.ntx * sg.nty;
   const int span = fill ? per_chunk : 2 * per_chunk;
@@ -1502,11 +1516,12 @@ SweepGeo sweep_geo(picard_ctx *c, int wmax, int rows = 1, bool fill = false) {
 SweepGeo sweep_sched(picard_ctx *c) { return sweep_geo(c, sweep_wmax(c)); }
 
 // The two-sweep kernel's schedule: the same tiles, sized by its own
-// occupancy (it holds more registers per wave).
+// occupancy (it holds more registers per wave); rows wider than one segment
+// in overlapping segments that keep kPairKeep cells each.
 SweepGeo pair_sched(picard_ctx *c) {
   if (c->pair_wmax <= 0) {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_pair<false, false, kPairRows, false>, kSwThreads, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mx_pair<false, false, kPairRows, false, false>, kSwThreads, 0) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
         per_cu <= 0 || cus < 8) {
       c->pair_wmax = 64;
@@ -1514,17 +1529,18 @@ SweepGeo pair_sched(picard_ctx *c) {
       c->pair_wmax = (cus / 8) * per_cu;
     }
   }
-  return sweep_geo(c, c->pair_wmax, kPairRows, true);
+  return sweep_geo(c, c->pair_wmax, kPairRows, true, c->gd.nx > kSwSeg ? kPairKeep : kSwSeg);
 }
 
 // One two-sweep launch (timed with events t0/t1 when given).
-template <bool PX>
+template <bool PX, bool MULTI>
 void launch_pair(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t t1,
                  const SweepGeo &sg, const SweepCoef &cf, const float *r32, const float *e,
                  float *out, float *eo_out, const float *eo_prev, const double *pin, double *pout,
                  JacobiState *js, const MxLaunch &L) {
-  auto fn = L.j == 0 ? (pad ? k_mx_pair<PX, true, kPairRows, true> : k_mx_pair<PX, false, kPairRows, true>)
-                     : (pad ? k_mx_pair<PX, true, kPairRows, false> : k_mx_pair<PX, false, kPairRows, false>);
+  constexpr bool M = MULTI;
+  auto fn = L.j == 0 ? (pad ? k_mx_pair<PX, true, kPairRows, true, M> : k_mx_pair<PX, false, kPairRows, true, M>)
+                     : (pad ? k_mx_pair<PX, true, kPairRows, false, M> : k_mx_pair<PX, false, kPairRows, false, M>);
   hipExtLaunchKernelGGL(fn, dim3(blocks), dim3(64, kSwRows), 0, s, t0, t1, 0, sg, cf, r32, e,
                         out, eo_out, eo_prev, pin, pout, js, L);
 }
@@ -1622,7 +1638,9 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
   }
   const int m = c->mx_inner;
   // two sweeps per launch where rows are one segment and m is even
-  const bool pairs = c->mx_pairs && sg.ntx == 1 && m % 2 == 0;
+  // two sweeps per launch: rows of one segment, or wider rows with x not
+  // periodic (overlapping segments)
+  const bool pairs = c->mx_pairs && (sg.ntx == 1 || g.flow[0]) && m % 2 == 0;
   const SweepGeo pgeo = pairs ? pair_sched(c) : sg;
   const int prB = 8 * pgeo.W;
   double *P[2] = {c->s.partials, c->s.partials + c->s.partials_len};
@@ -1702,11 +1720,14 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       hipEvent_t t1 = tj ? c->tev[2 * nrec + 1] : nullptr;
       if (tj) ++nrec;
       if (two && sg.px)
-        launch_pair<true>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out, eo_prev,
-                          pin, pout, c->js, L);
+        launch_pair<true, false>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out,
+                                 eo_prev, pin, pout, c->js, L);
+      else if (two && pgeo.ntx > 1)
+        launch_pair<false, true>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out,
+                                 eo_prev, pin, pout, c->js, L);
       else if (two)
-        launch_pair<false>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out, eo_prev,
-                           pin, pout, c->js, L);
+        launch_pair<false, false>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out,
+                                  eo_prev, pin, pout, c->js, L);
       else if (sg.px)
         launch_sweep<true>(pad, swB, c->stream, t0, t1, sg, cf, c->r32, e, out, eo_out, eo_prev,
                            pin, pout, c->js, L);

@@ -260,9 +260,16 @@ def _split_solve(cfg, d, C, sweeps_per_launch, maxerr, maxit, inner=128):
                                        ((24, 3, 10), (True, False, False)),
                                        ((20, 2, 6), (False, False, False)),
                                        ((40, 9, 2), (True, False, False)),
-                                       ((256, 6, 10), (False, True, True))])
+                                       ((256, 6, 10), (False, True, True)),
+                                       ((300, 6, 8), (True, True, False)),
+                                       ((508, 5, 6), (True, False, True)),
+                                       ((301, 4, 6), (True, True, False)),
+                                       ((260, 4, 5), (False, True, True))])
 def test_split_two_sweeps_per_launch_same_iterates(dims, flow):
-    """Two sweeps per launch (rows of at most 256 cells) give the iterates of
+    """Two sweeps per launch (rows of at most 256 cells; wider rows with x
+    not periodic in overlapping segments of 256 cells that keep 248, here two
+    and three of them, with and without padding; wider periodic rows take
+    single sweeps) give the iterates of
     two single-sweep launches bit for bit: with every stop test failing, both
     end at iterate 2*maxit+1 with the same potential, also when that iterate
     ends an odd number of sweeps into a refinement step (a single sweep then
