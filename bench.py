@@ -5,6 +5,7 @@
                     [--form both|split|fp64] [--no-cpu-baseline]
     python bench.py --solve-only SWEEPS [--form split|fp64]   (profiling)
     python bench.py --copy-cal                                (profiling)
+    python bench.py --error-steps N [--grid G]               (split vs FP64 errors)
 
 Workload: 256^3 cells of 10 m, H+ and e- at 32 per cell (1.07e9 particles),
 drifting Maxwellian 400 km/s at 10 eV, B = (0, 0, 5 nT), x in/outflow, y/z
@@ -158,6 +159,62 @@ def solve_only(cfg, form, sweeps, inner=0):
                       "sweeps": sw, "seconds": el}), flush=True)
 
 
+def error_report(cfg, steps):
+    """Error mode: N steps of both forms side by side from the same
+    device-generated plasma (the workload at --grid, its Haser source
+    included).  Per step, the normwise relative error of the split form's
+    phi, E and densities against the FP64 form's; after the last step, the
+    particle trajectories matched by id (slot 8 of the FP64 particle, the id
+    of the split one): position and velocity differences over the particles
+    both forms still hold."""
+    from picard_amd import _lib, host
+    d = host.derive(cfg)
+    ncell = cfg.nx * cfg.ny * cfg.nz
+    per_cell = int(d.ppc[:cfg.species].sum())
+    cap = int(1.12 * per_cell * ncell) + 64 * per_cell * (cfg.ny + 2) * (cfg.nz + 2) + (1 << 20)
+    doms = {}
+    for name, prec in (("fp64", _lib.PICARD_FP64), ("split", _lib.PICARD_FP32_SPLIT)):
+        dom = host.Domain(cfg, max_particles=cap, d=d, precision=prec)
+        dom.generate_initial(SEED)
+        dom.calc_distance()
+        dom.set_production(SEED)
+        doms[name] = dom
+    per_step = []
+    for it in range(1, steps + 1):
+        st = {}
+        for name, dom in doms.items():
+            dom.generate_inflow(SEED, it)
+            st[name] = dom.step()
+        f = {name: {k: dom.field(w) for k, w in (("U", _lib.FIELD_U), ("E", _lib.FIELD_E),
+                                                  ("N", _lib.FIELD_N))}
+             for name, dom in doms.items()}
+        a, b = f["split"], f["fp64"]
+        row = dict(step=it, n=[st["split"].num_local, st["fp64"].num_local],
+                   sweeps=[st["split"].sweeps, st["fp64"].sweeps],
+                   phi_rel=_rel(a["U"], b["U"]), E_rel=_rel(a["E"], b["E"]), N_rel=_rel(a["N"], b["N"]))
+        per_step.append(row)
+        log(f"error report: step {it}: " + ", ".join(f"{k} {v}" for k, v in row.items() if k != "step"))
+    ra, _ = doms["fp64"].particles()
+    rb, _ = doms["split"].particles()
+    for dom in doms.values():
+        dom.close()
+    ia, ib = np.argsort(ra[:, 7], kind="stable"), np.argsort(rb[:, 7], kind="stable")
+    ra, rb = ra[ia], rb[ib]
+    common, ka, kb = np.intersect1d(ra[:, 7], rb[:, 7], return_indices=True)
+    pa, pb = ra[ka], rb[kb]
+    dxv = np.linalg.norm(pa[:, :3] - pb[:, :3], axis=1)
+    dvv = np.linalg.norm(pa[:, 3:6] - pb[:, 3:6], axis=1)
+    vrms = float(np.sqrt(np.mean(np.sum(pa[:, 3:6] ** 2, axis=1))))
+    dx = float(d.dxyz[0])
+    traj = dict(matched=int(len(common)), only_fp64=int(len(ra) - len(common)),
+                only_split=int(len(rb) - len(common)),
+                dx_max_m=float(dxv.max()), dx_rms_m=float(np.sqrt(np.mean(dxv ** 2))),
+                dx_max_cells=float(dxv.max() / dx),
+                dv_max_rel=float(dvv.max() / vrms), dv_rms_rel=float(np.sqrt(np.mean(dvv ** 2)) / vrms))
+    print(json.dumps({"mode": "error-report", "grid": [cfg.nx, cfg.ny, cfg.nz], "steps": steps,
+                      "per_step": per_step, "trajectories": traj}), flush=True)
+
+
 def copy_calibration():
     """Counter calibration: three device copies of 1 GiB (read 1 GiB, write
     1 GiB each), the plain copy kernel the HBM byte counters are scaled by."""
@@ -195,6 +252,9 @@ def main():
                     help="profiling: one capped field solve of --form (split or fp64) only")
     ap.add_argument("--inner", type=int, default=0, metavar="M",
                     help="with --solve-only: FP32 sweeps per FP64 refinement of the split form")
+    ap.add_argument("--error-steps", type=int, default=0, metavar="N",
+                    help="error mode: N steps of both forms from the same plasma, split vs FP64 "
+                         "errors per step and particle trajectories after the last")
     ap.add_argument("--copy-cal", action="store_true",
                     help="profiling: the 1 GiB copies the byte counters are scaled by")
     args = ap.parse_args()
@@ -210,6 +270,9 @@ def main():
         return
     if args.solve_only:
         solve_only(cfg, "fp64" if args.form == "fp64" else "split", args.solve_only, args.inner)
+        return
+    if args.error_steps:
+        error_report(cfg, args.error_steps)
         return
     runs = {}
     if args.form in ("both", "split"):
