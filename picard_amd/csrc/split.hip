@@ -94,31 +94,63 @@ __device__ __forceinline__ void sp_wrap(const GridDims &g, int c[3], const float
   }
 }
 
+// counter[k] += 1 for every lane of the wave with k >= 0, one atomic per
+// distinct key: the lanes sharing the leader's key (the lowest pending lane)
+// take ranks by lane order after the value the leader's add of their count
+// returned.  Returns what the lane's own add of 1 would have returned in one
+// order of the adds (-1 for k < 0).  Particles arrive nearly sorted by key,
+// so a wave holds two or three keys and the adds do not meet on one address
+// lane after lane.  Every lane stays in the loop (its condition is the
+// wave-uniform mask of pending lanes) and values cross lanes only through
+// readlane, which reads the lane whatever the exec mask; the leader's bit
+// leaves the mask every pass, so the loop ends within 64 passes.
+__device__ __forceinline__ int32_t wave_add1(int32_t *counter, int32_t k) {
+  const int lane = (int)(threadIdx.x & 63);
+  bool pending = k >= 0;
+  int32_t ret = -1;
+  uint64_t todo = __ballot(pending);
+  while (todo) {
+    const int leader = (int)__builtin_ctzll(todo);
+    const int32_t kl = __builtin_amdgcn_readlane(k, leader);
+    const bool mine = pending && k == kl;
+    const uint64_t m = __ballot(mine) | (1ull << leader);
+    int32_t base = 0;
+    if (lane == leader) base = atomicAdd(&counter[kl], (int32_t)__popcll(m));
+    base = __builtin_amdgcn_readlane(base, leader);
+    if (mine) {
+      ret = base + (int32_t)__popcll(m & ((1ull << lane) - 1ull));
+      pending = false;
+    }
+    todo &= ~m;
+  }
+  return ret;
+}
+
 // Sort key of a kept particle (its lower node and species), -1 for a deleted one.
 __global__ void __launch_bounds__(kT) k_sp_flag(ParticlesSplit p, int64_t n, GridDims g,
                                                  int wrap, int32_t *__restrict__ key,
                                                  int32_t *__restrict__ hist) {
   const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (q >= n) return;
-  int c[3] = {p.c[0][q], p.c[1][q], p.c[2][q]};
-  const float f[3] = {p.f[0][q], p.f[1][q], p.f[2][q]};
-  if (sp_delete(g, c, f)) {
-    key[q] = -1;
-    return;
+  int32_t k = -1;                          // lanes past n stay for the wave's adds
+  if (q < n) {
+    int c[3] = {p.c[0][q], p.c[1][q], p.c[2][q]};
+    const float f[3] = {p.f[0][q], p.f[1][q], p.f[2][q]};
+    if (!sp_delete(g, c, f)) {
+      if (wrap) {
+        sp_wrap(g, c, f);
+        p.c[0][q] = (int16_t)c[0];
+        p.c[1][q] = (int16_t)c[1];
+        p.c[2][q] = (int16_t)c[2];
+      }
+      const bool in = c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] <= g.nx && c[1] <= g.ny &&
+                      c[2] <= g.nz;
+      // key (cell, species): the deposit then sees runs of one cell and species
+      const int32_t cells = (int32_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+      k = in ? (int32_t)lin_key(g, c[0], c[1], c[2]) * g.S + ((int32_t)p.sp[q] - 1) : cells * g.S;
+    }
+    key[q] = k;
   }
-  if (wrap) {
-    sp_wrap(g, c, f);
-    p.c[0][q] = (int16_t)c[0];
-    p.c[1][q] = (int16_t)c[1];
-    p.c[2][q] = (int16_t)c[2];
-  }
-  const bool in = c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] <= g.nx && c[1] <= g.ny && c[2] <= g.nz;
-  // key (cell, species): the deposit then sees runs of one cell and species
-  const int32_t cells = (int32_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
-  const int32_t k = in ? (int32_t)lin_key(g, c[0], c[1], c[2]) * g.S + ((int32_t)p.sp[q] - 1)
-                       : cells * g.S;
-  key[q] = k;
-  atomicAdd(&hist[k], 1);
+  wave_add1(hist, k);
 }
 
 __global__ void __launch_bounds__(kT) k_sp_scatter(ParticlesSplit src, ParticlesSplit dst, int64_t n,
@@ -126,10 +158,10 @@ __global__ void __launch_bounds__(kT) k_sp_scatter(ParticlesSplit src, Particles
                                                     const int32_t *__restrict__ off,
                                                     int32_t *__restrict__ cursor) {
   const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (q >= n) return;
-  const int32_t k = key[q];
-  if (k < 0) return;
-  const int64_t at = off[k] + atomicAdd(&cursor[k], 1);
+  const int32_t k = q < n ? key[q] : -1;   // lanes past n stay for the wave's adds
+  const int32_t r = wave_add1(cursor, k);
+  if (k < 0 || r < 0) return;
+  const int64_t at = (int64_t)off[k] + r;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     dst.c[d][at] = src.c[d][q];
