@@ -2,6 +2,7 @@
 """Benchmark of the PIC time step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 256]
+                    [--fp64-steps K64] [--fp64-warmup W64]
                     [--form both|split|fp64] [--no-cpu-baseline]
     python bench.py --solve-only SWEEPS [--form split|fp64]   (profiling)
     python bench.py --copy-cal                                (profiling)
@@ -20,10 +21,14 @@ AdvanceVelocity.
 
 metric: particle-steps/s = sum over the timed steps of num_local at the
 velocity push / wall time of the timed steps (state resident in HBM).  The
-FP32-split form gives `value`; the FP64 form (reference order, bit-exact
-against the oracle) runs the same steps after it, and the line reports both,
-their ratio, and the split form's error against the FP64 form after the first
-step (phi, E, densities: normwise relative).
+FP32-split form gives `value` (K timed steps after W warmup steps); the FP64
+form (reference order, bit-exact against the oracle) runs after it from the
+same plasma, W64 warmup and K64 timed steps (an FP64 step costs about six
+split steps, so its count is its own and the whole run stays within a few
+minutes at K = 20).  The line reports both forms, their ratio over the same
+step indices (whole step, and the solve phase alone per sweep), and the split
+form's error against the FP64 form after the first step (phi, E, densities:
+normwise relative).
 
 Prints one JSON line with the roofline of the dominant kernel (the split
 form's FP32 Jacobi sweep, timed live with per-launch HIP events) and a CPU
@@ -56,8 +61,9 @@ def _rel(a, b):
 
 
 def run_form(cfg, precision, steps, warmup, capture_after=1):
-    """W warmup + K timed steps of one form; returns timings and the fields
-    after step `capture_after` (read outside the timed region)."""
+    """W warmup + K timed steps of one form; returns timings (whole and per
+    step, with the solve phase's wall time per step from HIP events) and the
+    fields after step `capture_after` (read outside the timed region)."""
     from picard_amd import _lib, host
     d = host.derive(cfg)
     ncell = cfg.nx * cfg.ny * cfg.nz
@@ -73,65 +79,116 @@ def run_form(cfg, precision, steps, warmup, capture_after=1):
     log(f"{name}: setup {time.perf_counter() - t0:.1f} s, {dom.num_local} particles, cap {cap}")
     fields = None
     it = 0
+    warm_steps = []
     for _ in range(warmup):
         it += 1
+        ts = time.perf_counter()
         dom.generate_inflow(SEED, it)
         st = dom.step()
         dom.synchronize()
-        log(f"{name}: warmup step {it}: n={st.num_local} sweeps={st.sweeps}")
+        te = time.perf_counter()
+        warm_steps.append(dict(step=it, seconds=te - ts, pushed=st.pushed, sweeps=st.sweeps))
+        log(f"{name}: warmup step {it}: n={st.num_local} sweeps={st.sweeps} {te - ts:.2f} s")
         if it == capture_after:
             fields = {k: dom.field(w) for k, w in (("U", _lib.FIELD_U), ("E", _lib.FIELD_E),
                                                     ("N", _lib.FIELD_N))}
     dom.synchronize()
     dom.timing(True)
-    pushed, sweeps = 0, []
+    pushed, sweeps, per_step = 0, [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         it += 1
+        ts = time.perf_counter()
         dom.generate_inflow(SEED, it)
         st = dom.step()
+        dom.synchronize()
+        te = time.perf_counter()
         pushed += st.pushed
         sweeps.append(st.sweeps)
+        per_step.append(dict(step=it, seconds=te - ts, pushed=st.pushed, sweeps=st.sweeps))
         log(f"{name}: timed step {it}: n={st.num_local} sweeps={st.sweeps} "
-            f"({time.perf_counter() - t0:.1f} s)")
+            f"{te - ts:.2f} s ({te - t0:.1f} s)")
     dom.synchronize()
     elapsed = time.perf_counter() - t0
     jac_ms, jac_launches = dom.timing_read()
+    solve_ms, solves = dom.timing_solve()
     dom.timing(False)
     if fields is None:
         fields = {k: dom.field(w) for k, w in (("U", _lib.FIELD_U), ("E", _lib.FIELD_E),
                                                 ("N", _lib.FIELD_N))}
     dom.close()
-    log(f"{name}: {steps} steps in {elapsed:.2f} s, sweeps {sweeps}, "
-        f"solve kernel {1e3 * jac_ms / max(jac_launches, 1):.2f} us per launch, avg over {jac_launches}")
+    log(f"{name}: {steps} steps in {elapsed:.2f} s, sweeps {sweeps}, solve phase "
+        f"{solve_ms / max(solves, 1):.1f} ms per step, solve kernel "
+        f"{1e3 * jac_ms / max(jac_launches, 1):.2f} us per launch, avg over {jac_launches}")
     return dict(pushed=pushed, elapsed=elapsed, sweeps=sweeps, jac_ms=jac_ms,
-                jac_launches=jac_launches, fields=fields, captured_at=min(capture_after, it))
+                jac_launches=jac_launches, solve_ms=solve_ms, solves=solves, per_step=per_step,
+                warm_steps=warm_steps, fields=fields, captured_at=min(capture_after, it))
 
 
-def cpu_baseline(budget_s=25.0):
+def _host_cores():
+    """Logical CPUs the process may run on, and the machine's count."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return usable, os.cpu_count()
+
+
+def cpu_baseline(budget_s=20.0, workload=None):
     """The oracle (C restatement of the reference order, FP64, 1 thread) on a
     bounded sample of the workload: whole steps of a 48^3 block of it (same
-    cell size, plasma and 32 particles per cell per species, no source)."""
+    cell size, plasma and 32 particles per cell per species, no source), each
+    phase timed.  `workload` (cells, particles, sweeps per step of the full
+    grid, from the GPU run) scales the sample's phase times to the whole
+    workload: particle phases by the particle count, the solve by cells x
+    sweeps, the face sum by face nodes x cells; that estimate is reported
+    beside the measured rate and labelled as such."""
     from picard_amd import host
     from tests.pic_driver import OracleSim
-    cfg = host.config2(48, species=2, Qn=(0.0,) * 8)
+    n = 48
+    cfg = host.config2(n, species=2, Qn=(0.0,) * 8)
     d = host.derive(cfg)
     rng = np.random.default_rng(SEED)
     rp, sp = host.particles_initialize(cfg, d, rng)
     sim = OracleSim(cfg, d, rp, sp)
-    pushed, steps = 0, 0
+    sim.times = {}
+    pushed, steps, sweeps = 0, 0, 0
     t0 = time.perf_counter()
     while steps < 1 or (time.perf_counter() - t0 < budget_s * 0.5 and steps < 20):
         sim.append(*host.bc_inflow(cfg, d, rng))
-        sim.step()
+        st = sim.step()
         pushed += sim.n
+        sweeps += st["sweeps"]
         steps += 1
-        log(f"cpu baseline: step {steps} n={sim.n} ({time.perf_counter() - t0:.1f} s)")
+        log(f"cpu baseline: step {steps} n={sim.n} sweeps={st['sweeps']} "
+            f"({time.perf_counter() - t0:.1f} s)")
     el = time.perf_counter() - t0
-    return dict(value=pushed / el, unit="particle-steps/s", cores=1, kind="port",
-                sample=f"{steps} step(s) of a 48^3 block of the workload (10 m cells, H+ and e- "
-                       f"at 32 per cell, {sim.n} particles, no Haser source); oracle C "
-                       f"restatement in reference order, FP64, 1 thread, {el:.1f} s")
+    usable, total = _host_cores()
+    out = dict(value=pushed / el, unit="particle-steps/s", cores=1, kind="port",
+               host_cores=usable, host_cores_total=total,
+               sample=f"{steps} whole step(s) of a 48^3 block of the workload (10 m cells, H+ and "
+                      f"e- at 32 per cell, {sim.n} particles, x in/outflow, no Haser source; "
+                      f"{sweeps // steps} Jacobi sweeps per step); oracle C restatement in "
+                      f"reference order, FP64, 1 thread of the {usable} usable host cores, "
+                      f"{el:.1f} s")
+    if workload:
+        t = {k: v / steps for k, v in sim.times.items()}
+        cells_s = n ** 3
+        face_s = 2 * n * n * cells_s
+        part = sum(v for k, v in t.items() if k not in ("solve", "bcpot", "efield"))
+        P = workload["particles"]
+        est = (part * P / (pushed / steps) + t.get("efield", 0.0) * workload["cells"] / cells_s +
+               t.get("solve", 0.0) * (workload["cells"] * workload["sweeps"]) /
+               (cells_s * sweeps / steps) +
+               t.get("bcpot", 0.0) * (2 * workload["nx"] ** 2 * workload["cells"]) / face_s)
+        out["workload_estimate"] = dict(
+            value=P / est, seconds_per_step=est,
+            method="the sample's phase times per step scaled to the full workload: particle "
+                   "phases x particles, the E field x cells, the solve x cells x sweeps (sweeps "
+                   "per step from the "
+                   "GPU run), the face sum x face nodes x cells; one thread; an estimate, "
+                   "not a run")
+    return out
 
 
 def solve_only(cfg, form, sweeps, inner=0):
@@ -243,8 +300,13 @@ def traffic_from_profiles(kernel):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=2,
+                    help="timed steps of the FP32-split form (the value)")
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=2,
+                    help="timed steps of the FP64 form, after --fp64-warmup untimed ones (an "
+                         "FP64 step at 256^3 takes ~23 s, a split one ~4 s)")
+    ap.add_argument("--fp64-warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--form", choices=("both", "split", "fp64"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -278,9 +340,11 @@ def main():
     if args.form in ("both", "split"):
         runs["split"] = run_form(cfg, _lib.PICARD_FP32_SPLIT, args.steps, args.warmup)
     if args.form in ("both", "fp64"):
-        runs["fp64"] = run_form(cfg, _lib.PICARD_FP64, args.steps, args.warmup)
+        runs["fp64"] = run_form(cfg, _lib.PICARD_FP64, args.fp64_steps if "split" in runs else
+                                args.steps, args.fp64_warmup if "split" in runs else args.warmup)
     head = runs.get("split") or runs["fp64"]
     headname = "split" if "split" in runs else "fp64"
+    head_steps = len(head["per_step"])
     value = head["pushed"] / head["elapsed"]
     ncell = cfg.nx * cfg.ny * cfg.nz
     # Sweeps per launch of each form's solve kernel: the split form runs two
@@ -288,20 +352,48 @@ def main():
     # x not periodic (the default inner length, 512, is even), the FP64 form
     # one pass per launch.
     spl = {"split": 2 if cfg.nx <= 256 or cfg.flow[0] else 1, "fp64": 1}
-    # Algorithmic bytes per launch of the dominant kernel, per cell-sweep
-    # times the cell-sweeps of a launch: an FP32 sweep reads the correction e
-    # and the FP32 residual and writes e (12 B per interior cell); the FP64
-    # pass reads U and b and writes U (24 B per cell).
-    bpc = (12.0 if headname == "split" else 24.0) * spl[headname]
-    bytes_per_launch = bpc * ncell
+    # Bytes a launch of the dominant kernel must move: an FP32 sweep reads the
+    # correction e and the FP32 residual and writes e (12 B per interior
+    # cell), and a two-sweep launch keeps its first sweep's output on chip, so
+    # it moves those 12 B once for two sweeps; the FP64 pass reads U and b and
+    # writes U (24 B per cell).  The sweep-byte rate (12 B per cell and sweep)
+    # is reported beside it.
+    bytes_per_launch = (12.0 if headname == "split" else 24.0) * ncell
+    sweep_bytes_per_launch = (12.0 if headname == "split" else 24.0) * spl[headname] * ncell
     avg_s = (head["jac_ms"] / 1e3) / max(head["jac_launches"], 1)
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
     per_form = {}
     for k, r in runs.items():
         launch_us = 1e3 * r["jac_ms"] / max(r["jac_launches"], 1)
-        per_form[k] = dict(value=r["pushed"] / r["elapsed"], ms_per_step=1e3 * r["elapsed"] / args.steps,
+        ns = len(r["per_step"])
+        per_form[k] = dict(value=r["pushed"] / r["elapsed"], timed_steps=ns,
+                           ms_per_step=1e3 * r["elapsed"] / ns,
                            sweeps_per_step=r["sweeps"], sweeps_per_launch=spl[k],
-                           launch_kernel_us=launch_us, sweep_kernel_us=launch_us / spl[k])
+                           solve_ms_per_step=r["solve_ms"] / max(r["solves"], 1),
+                           solve_us_per_sweep=1e3 * r["solve_ms"] / max(sum(r["sweeps"]), 1),
+                           launch_kernel_us=launch_us, sweep_kernel_us=launch_us / spl[k],
+                           per_step=r["per_step"], warmup_steps=r["warm_steps"])
+    ratio = None
+    if "split" in runs and "fp64" in runs:
+        # like for like: the split form's rate over the same step indices the
+        # FP64 form timed (both run the same plasma, inflow and source from
+        # the same seeds; the sweep count per step changes from step to step,
+        # 1.8e5 to 2.5e5 over the first 25 steps, so other steps are not
+        # comparable), taken from its warmup steps where the indices fall
+        # there; and the solve phase alone, per sweep
+        idx = {p["step"] for p in runs["fp64"]["per_step"]}
+        sp_same = [p for p in runs["split"]["warm_steps"] + runs["split"]["per_step"]
+                   if p["step"] in idx]
+        fp = runs["fp64"]["per_step"]
+        ratio = {"whole_step_timed": per_form["split"]["value"] / per_form["fp64"]["value"],
+                 "solve_per_sweep": (per_form["fp64"]["solve_us_per_sweep"] /
+                                     max(per_form["split"]["solve_us_per_sweep"], 1e-30))}
+        if len(sp_same) == len(fp):
+            ratio["whole_step_same_steps"] = ((sum(p["pushed"] for p in sp_same) /
+                                               sum(p["seconds"] for p in sp_same)) /
+                                              (sum(p["pushed"] for p in fp) /
+                                               sum(p["seconds"] for p in fp)))
+            ratio["same_steps"] = sorted(idx)
     errors = None
     if "split" in runs and "fp64" in runs:
         a, b = runs["split"]["fields"], runs["fp64"]["fields"]
@@ -317,9 +409,9 @@ def main():
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": head_steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * head["elapsed"] / args.steps,
+        "ms_per_step": 1e3 * head["elapsed"] / head_steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -333,8 +425,7 @@ def main():
                         f"value = {headname} form",
             "grid": [cfg.nx, cfg.ny, cfg.nz],
             "forms": per_form,
-            "split_over_fp64": (per_form["split"]["value"] / per_form["fp64"]["value"]
-                                if len(per_form) == 2 else None),
+            "split_over_fp64": ratio,
             "split_error_vs_fp64": errors,
         },
         "roofline": {
@@ -347,6 +438,8 @@ def main():
             "traffic": traffic_from_profiles(kname),
             "bytes_per_launch": bytes_per_launch,
             "sweeps_per_launch": spl[headname],
+            "sweep_bytes_per_launch": sweep_bytes_per_launch,
+            "sweep_bytes_GBs": sweep_bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0,
             "avg_launch_us": avg_s * 1e6,
             "timed_launches": head["jac_launches"],
             "timed_every": 16,
@@ -354,7 +447,10 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        P = head["pushed"] / head_steps
+        out["cpu_baseline"] = cpu_baseline(workload=dict(
+            particles=P, cells=ncell, nx=cfg.nx,
+            sweeps=float(np.mean(head["sweeps"]))))
     print(json.dumps(out), flush=True)
 
 

@@ -207,12 +207,17 @@ int picard_field_write(picard_ctx *ctx, int32_t which, const double *host, int64
  * launches since the last enable. */
 int picard_timing_enable(picard_ctx *ctx, int32_t enable);
 int picard_timing_read(picard_ctx *ctx, double *jacobi_ms, int64_t *jacobi_launches);
-/* FP32-split solve: `inner` FP32 sweeps per FP64 refinement step, 2..4096
- * (default 512).  No reference counterpart; the solution agrees with the FP64
- * form's to the split form's tolerance for each choice.  Near the FP32 floor
- * of a tight tolerance (relative residual far below 1e-6) the solve may run
- * up to `inner` sweeps past the FP64 form's stop before the next FP64
- * refinement. */
+/* Wall time of the CalcEpotential phase of picard_step (HIP events before and
+ * after the solve on the context's stream), summed over the steps run since
+ * the last picard_timing_enable, and the number of solves timed.  No
+ * reference counterpart (the reference calls no timer). */
+int picard_timing_solve(picard_ctx *ctx, double *solve_ms, int64_t *solves);
+/* FP32-split solve: `inner` FP32 sweeps per FP64 refinement step, 2..512
+ * (default 512); other values return PICARD_ERR_ARG.  No reference
+ * counterpart; the solution agrees with the FP64 form's to the split form's
+ * tolerance for each choice.  Near the FP32 floor of a tight tolerance
+ * (relative residual far below 1e-6) the solve may run up to `inner` sweeps
+ * past the FP64 form's stop before the next FP64 refinement. */
 int picard_split_solver(picard_ctx *ctx, int32_t inner);
 /* FP32-split solve: inner sweeps per kernel launch, 1 or 2 (default 2).  With
  * 2, one launch performs two sweeps with the traffic of one, on grids whose

@@ -34,7 +34,7 @@ EXPORTS = (
     "picard_particles_transfer", "picard_particles_gridify", "picard_calc_distance",
     "picard_bc_potentials", "picard_calc_epotential", "picard_calc_efield", "picard_bc_fields",
     "picard_advance_velocity", "picard_field_read", "picard_field_write", "picard_step",
-    "picard_timing_enable", "picard_timing_read", "picard_split_solver",
+    "picard_timing_enable", "picard_timing_read", "picard_timing_solve", "picard_split_solver",
     "picard_split_sweeps_per_launch", "picard_generate_initial",
     "picard_generate_inflow", "picard_particles_produce", "picard_set_production",
     "picard_diagnostics", "picard_advance_velocity_imp", "picard_advance_velocity_exp",
@@ -132,6 +132,7 @@ def lib():
                          ct.POINTER(StepStats)], ct.c_int),
         "picard_timing_enable": ([vp, ct.c_int32], ct.c_int),
         "picard_timing_read": ([vp, dp, lp], ct.c_int),
+        "picard_timing_solve": ([vp, dp, lp], ct.c_int),
         "picard_split_solver": ([vp, ct.c_int32], ct.c_int),
         "picard_split_sweeps_per_launch": ([vp, ct.c_int32], ct.c_int),
         "picard_generate_initial": ([vp, ct.POINTER(Plasma), ct.c_uint64], ct.c_int),

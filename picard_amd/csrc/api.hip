@@ -151,6 +151,8 @@ void release(picard_ctx *c) {
   dfree(c->s.partials);
   for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
   c->tev.clear();
+  for (hipEvent_t &e : c->solve_ev)
+    if (e) (void)hipEventDestroy(e), e = nullptr;
   if (c->js_host) (void)hipHostFree(c->js_host);
   if (c->host_counters) (void)hipHostFree(c->host_counters);
   c->js_host = nullptr;
@@ -596,6 +598,15 @@ int picard_timing_enable(picard_ctx *c, int32_t enable) {
   c->timing = enable != 0;
   c->t_ms = 0.0;
   c->t_launches = 0;
+  c->solve_ms = 0.0;
+  c->solves = 0;
+  return PICARD_OK;
+}
+
+int picard_timing_solve(picard_ctx *c, double *solve_ms, int64_t *solves) {
+  CTX_CHECK(c);
+  if (solve_ms) *solve_ms = c->solve_ms;
+  if (solves) *solves = c->solves;
   return PICARD_OK;
 }
 
@@ -608,8 +619,9 @@ int picard_timing_read(picard_ctx *c, double *jacobi_ms, int64_t *jacobi_launche
 
 int picard_split_solver(picard_ctx *c, int32_t inner) {
   CTX_CHECK(c);
-  if (inner < 2 || inner > 4096) {
-    set_error("picard_split_solver: inner sweeps per refinement must be in [2, 4096], got " +
+  if (inner < 2 || inner > kMaxSplitInner) {
+    set_error("picard_split_solver: inner sweeps per refinement must be in [2, " +
+              std::to_string(kMaxSplitInner) + "], got " +
               std::to_string(inner));
     return PICARD_ERR_ARG;
   }
@@ -625,6 +637,24 @@ int picard_split_sweeps_per_launch(picard_ctx *c, int32_t sweeps) {
     return PICARD_ERR_ARG;
   }
   c->mx_pairs = sweeps == 2;
+  return PICARD_OK;
+}
+
+// While timing is on, events around the solve phase of picard_step: at
+// `end`, waits for the closing event (the solve has already read its sweep
+// count back, so the stream is idle there) and adds the elapsed time.
+static int solve_timer(picard_ctx *c, int end) {
+  if (!c->timing) return PICARD_OK;
+  for (hipEvent_t &e : c->solve_ev)
+    if (!e) PICARD_HIP(hipEventCreate(&e));
+  PICARD_HIP(hipEventRecord(c->solve_ev[end], c->stream));
+  if (end) {
+    float ms = 0.0f;
+    PICARD_HIP(hipEventSynchronize(c->solve_ev[1]));
+    PICARD_HIP(hipEventElapsedTime(&ms, c->solve_ev[0], c->solve_ev[1]));
+    c->solve_ms += ms;
+    ++c->solves;
+  }
   return PICARD_OK;
 }
 
@@ -645,8 +675,10 @@ int picard_step(picard_ctx *c, double dt, double maxerr, int32_t maxit, const do
     st.removed = before - c->n;
     PICARD_TRY(sp_gridify(c));                                           // :577
     PICARD_TRY(launch_bc_potentials_fft(c));                             // :581
+    PICARD_TRY(solve_timer(c, 0));
     rc = sp_epotential(c, maxerr, maxit, &st.sweeps);                    // :584
     if (rc < 0) return rc;
+    PICARD_TRY(solve_timer(c, 1));
     PICARD_TRY(sp_efield(c));                                            // :587-589
     st.pushed = c->n;
     PICARD_TRY(sp_boris(c, dt));                                         // :630
@@ -660,8 +692,10 @@ int picard_step(picard_ctx *c, double dt, double maxerr, int32_t maxit, const do
     PICARD_TRY(launch_wrap(c));                                         // :572
     PICARD_TRY(launch_gridify(c));                                      // :577
     PICARD_TRY(launch_bc_potentials(c));                                // :581
+    PICARD_TRY(solve_timer(c, 0));
     rc = launch_calc_epotential(c, maxerr, maxit, &st.sweeps);          // :584
     if (rc < 0) return rc;
+    PICARD_TRY(solve_timer(c, 1));
     PICARD_TRY(launch_calc_efield(c));                                  // :587
     PICARD_TRY(launch_bc_fields(c));                                    // :589
     st.pushed = c->n;

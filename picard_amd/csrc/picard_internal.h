@@ -184,11 +184,21 @@ struct picard_ctx {
   std::vector<hipEvent_t> tev;
   double t_ms = 0.0;
   int64_t t_launches = 0;
+  // wall time of the whole CalcEpotential phase inside picard_step while
+  // timing is on (events before and after the solve on the context's stream)
+  hipEvent_t solve_ev[2] = {nullptr, nullptr};
+  double solve_ms = 0.0;
+  int64_t solves = 0;
 };
 
 namespace picard {
 
 constexpr int kTimeEvery = 16;   // timed sweeps: one launch in this many
+// Longest FP32-split refinement step (FP32 sweeps per FP64 refinement) the
+// API accepts: with one refinement covering a whole small solve (4096 sweeps
+// at 16^3, a 1,832-sweep solve) the FP32 iterate stopped 26 sweeps after the
+// FP64 form, outside the agreement the split form keeps for 2..512.
+constexpr int kMaxSplitInner = 512;
 
 // ---- particles.hip ----
 int launch_push_x(picard_ctx *c, int64_t first, int64_t count, double dt);

@@ -354,13 +354,19 @@ def particles_initialize(cfg: Config, d: Derived, rng):
 
 
 def bc_inflow(cfg: Config, d: Derived, rng):
-    """BCinflow.f90: fresh plasma in the guard cells of every flow face."""
+    """BCinflow.f90: fresh plasma in the guard cells of every flow face, in the
+    reference's face order and cell ranges (+x, -x over the y/z-widened face;
+    +y with z up to Nz+1+zflow, -y from 2-zflow; +z with y from 2-yflow, -z up
+    to Ny+1+yflow; species from 2 on the y and z faces)."""
     xf, yf, zf = cfg.flow
     yneg = ypos = 1 if yf else 0
     zneg = zpos = 1 if zf else 0
     parts_r, parts_s = [], []
 
-    def add(corners, first_species):
+    def add(corners, first_species, upper=None):
+        if upper is not None:             # the + faces start at L_max itself (:59, :201, :342)
+            corners = corners.copy()
+            corners[:, upper] = cfg.gmax[upper]
         r, s = _emit_cells(rng, d, cfg, corners, first_species)
         parts_r.append(r)
         parts_s.append(s)
@@ -369,16 +375,18 @@ def bc_inflow(cfg: Config, d: Derived, rng):
     kk = np.arange(2 - zneg, cfg.nz + 2 + zpos)
     if xf:
         c = _cell_corners(cfg, d, np.array([cfg.nx + 2]), jj, kk)    # x0 = Lx_max (:59)
-        add(c, 1)
+        add(c, 1, upper=0)
         c = _cell_corners(cfg, d, np.array([1]), jj, kk)             # x0 = Lx_min - dx (:130)
         add(c, 1)
     ii = np.arange(2, cfg.nx + 2)
     if yf:
-        add(_cell_corners(cfg, d, ii, np.array([cfg.ny + 2]), np.arange(2, cfg.nz + 2 + zpos)), 2)
+        add(_cell_corners(cfg, d, ii, np.array([cfg.ny + 2]), np.arange(2, cfg.nz + 2 + zpos)), 2,
+            upper=1)
         add(_cell_corners(cfg, d, ii, np.array([1]), np.arange(2 - zneg, cfg.nz + 2)), 2)
-    if zf:
-        add(_cell_corners(cfg, d, ii, np.arange(2, cfg.ny + 2), np.array([cfg.nz + 2])), 2)
-        add(_cell_corners(cfg, d, ii, np.arange(2, cfg.ny + 2), np.array([1])), 2)
+    if zf:   # the z faces take one y guard row on opposite sides (BCinflow.f90:337, :407)
+        add(_cell_corners(cfg, d, ii, np.arange(2 - yneg, cfg.ny + 2), np.array([cfg.nz + 2])), 2,
+            upper=2)
+        add(_cell_corners(cfg, d, ii, np.arange(2, cfg.ny + 2 + ypos), np.array([1])), 2)
     if not parts_r:
         return np.zeros((0, 8)), np.zeros(0, np.int32)
     return np.concatenate(parts_r), np.concatenate(parts_s).astype(np.int32)
@@ -676,6 +684,13 @@ class Domain:
     def timing_read(self):
         ms, n = ct.c_double(), ct.c_int64()
         check("picard_timing_read", lib().picard_timing_read(self.h, ct.byref(ms), ct.byref(n)))
+        return ms.value, n.value
+
+    def timing_solve(self):
+        """Wall time [ms] of the solve phase of step() since timing(True), and
+        the number of solves timed."""
+        ms, n = ct.c_double(), ct.c_int64()
+        check("picard_timing_solve", lib().picard_timing_solve(self.h, ct.byref(ms), ct.byref(n)))
         return ms.value, n.value
 
     def synchronize(self):

@@ -3,6 +3,8 @@ oracle and on the GPU Domain from identical host-generated particles, so
 tests can compare them step by step."""
 from __future__ import annotations
 
+import time
+
 import numpy as np
 
 import oracle
@@ -28,6 +30,13 @@ class OracleSim:
         self.E = np.zeros(self.g.shape + (3,))
         self.N = None
         self.C = None
+        self.times = None      # a dict here accumulates the seconds of each phase
+
+    def _tick(self, phase, t0):
+        t1 = time.perf_counter()
+        if self.times is not None:
+            self.times[phase] = self.times.get(phase, 0.0) + (t1 - t0)
+        return t1
 
     def append(self, rp, sp):
         self.rp = np.ascontiguousarray(np.concatenate([self.rp[: self.n], rp]))
@@ -36,6 +45,7 @@ class OracleSim:
 
     def step(self, produced=None):
         d, cfg = self.d, self.cfg
+        t = time.perf_counter()
         oracle.advance_position(self.rp, d.dt, self.n)                      # :564
         if produced is not None and produced[0].shape[0]:
             self.append(*produced)                                          # :566
@@ -47,14 +57,37 @@ class OracleSim:
         self.N, self.C, skipped = oracle.particles_gridify(self.g, self.rp, self.sp, d.charge,
                                                            d.n2p, self.n)   # :577
         assert skipped == 0
+        t = self._tick("particles", t)
         self.U = oracle.bc_potentials(self.g, self.C, self.D)               # :581
+        t = self._tick("bcpot", t)
         sweeps, _ = oracle.calc_epotential(self.g, self.U, self.C, cfg.maxerr, cfg.maxit)  # :584
+        t = self._tick("solve", t)
         self.E = oracle.calc_efield(self.g, self.U)                         # :587
         oracle.bc_fields(self.g, self.E)                                    # :589
+        t = self._tick("efield", t)
         oracle.advance_velocity(self.g, self.rp, self.sp, self.E, d.B0, d.dt, d.ve, d.eta,
                                 d.tB, d.sB, [int(f) for f in cfg.fluid],
                                 [int(f) for f in cfg.gyration], self.n)     # :630
+        self._tick("push", t)
         return dict(num_local=self.n, removed=len(removed), sweeps=sweeps)
+
+
+def sha(a) -> str:
+    """SHA-256 of an array's bytes (little-endian, C order), the digest the
+    multi-step fixtures store."""
+    import hashlib
+    a = np.ascontiguousarray(a)
+    return hashlib.sha256(a.astype(a.dtype.newbyteorder("<"), copy=False).tobytes()).hexdigest()
+
+
+def step_digest(it, st, rp, sp, N, C, U, E) -> dict:
+    """Per-step digest of the main loop's state after AdvanceVelocity: counts,
+    sweep count, SHA-256 of the particle and field byte images, and the
+    max-norm and 2-norm of U (readable values beside the hashes)."""
+    return dict(step=it, num_local=int(st["num_local"]), removed=int(st["removed"]),
+                sweeps=int(st["sweeps"]), particles=sha(rp), species=sha(sp), N=sha(N),
+                C=sha(C), U=sha(U), E=sha(E), U_inf=float(np.abs(U).max()),
+                U_2=float(np.linalg.norm(U)))
 
 
 def small_config(n=8, ppc=4, flow=(True, False, False), steps_seed=5, **kw) -> host.Config:

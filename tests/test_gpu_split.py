@@ -44,8 +44,8 @@ def _stops_together(s_fp64, s_split):
     """The split stop against the FP64 one: FP32 rounding moves it by a sweep
     or two either way; with two sweeps per launch the tested iterates are
     every second one near the threshold, and a residual that falls much faster
-    than its extrapolation may pass between sparse tests (bounded here by
-    seven sweeps)."""
+    than its extrapolation may pass between sparse tests: the split stop may
+    come up to two sweeps before and nine after the FP64 one."""
     return -2 <= s_split - s_fp64 <= 9
 
 
