@@ -376,6 +376,54 @@ def ref_particles_produce(g: Grid, rp, sp, n, seed, *, kelvin, mass, nu_d, nu_i,
     return nl.value
 
 
+def _plasma_args(plasma):
+    v0 = np.ascontiguousarray(np.asarray(plasma["v0"], np.float64).reshape(8, 4))
+    tB = np.ascontiguousarray(np.asarray(plasma["tB4"], np.float64).reshape(8, 4))
+    return (_d(_f8(plasma["kelvin"])), _d(_f8(plasma["mass"])), _d(v0), _d(_f8(plasma["ve4"], 4)),
+            tB, (_i(_i8(plasma["ppc"])), _i(_i8(plasma["epp"])), _i(_i8(plasma["hpp"])),
+                 _i(_i8(plasma["ipp"]))))
+
+
+def ref_particles_initialize(g: Grid, rp, sp, n, seed, plasma):
+    """Seeds the runtime generator with `seed`, then calls the reference's
+    ParticlesInitialize(real_particles, int_particles, Lx_min, Ly_min, Lz_min,
+    dxyz, kelvin, mass, v0, ve, tB, max_per_proc, num_local, ppc, epp, hpp,
+    ipp, species, Nx_local, Ny_local, Nz_local) (ParticlesInitialize.f90:1-2).
+    `plasma` holds kelvin, mass, v0 (8x4), ve4, tB4 (8x4), ppc, epp, hpp, ipp.
+    Returns the new num_local."""
+    R = ref()
+    R.picard_ref_seed(ct.c_int32(seed))
+    nl = ct.c_int32(n)
+    kel, mass, v0, ve, tB, ints = _plasma_args(plasma)
+    R.particlesinitialize_(_d(rp), _i(sp), _byref_d(g.gmin[0]), _byref_d(g.gmin[1]),
+                           _byref_d(g.gmin[2]), _d(np.asarray(g.dxyz, np.float64)), kel, mass, v0,
+                           ve, _d(tB), _byref_i(rp.shape[0]), ct.byref(nl), *ints,
+                           _byref_i(g.species), _byref_i(g.nx), _byref_i(g.ny), _byref_i(g.nz))
+    return nl.value
+
+
+def ref_bc_inflow(g: Grid, rp, sp, n, seed, plasma):
+    """Seeds the runtime generator with `seed`, then calls the reference's
+    BCinflow(real_particles, int_particles, Lx_min, Ly_min, Lz_min, Lx_max,
+    Ly_max, Lz_max, dxyz, kelvin, mass, v0, ve, max_per_proc, num_local, ppc,
+    epp, hpp, ipp, species, Nx_local, Ny_local, Nz_local, xmin, xmax, ymin,
+    ymax, zmin, zmax, tB, xflow, yflow, zflow) (BCinflow.f90:2-4) on one rank
+    (local box = global box).  Returns the new num_local."""
+    R = ref()
+    R.picard_ref_seed(ct.c_int32(seed))
+    nl = ct.c_int32(n)
+    kel, mass, v0, ve, tB, ints = _plasma_args(plasma)
+    fl = [ct.c_int32(1 if f else 0) for f in g.flow]
+    R.bcinflow_(_d(rp), _i(sp), _byref_d(g.gmin[0]), _byref_d(g.gmin[1]), _byref_d(g.gmin[2]),
+                _byref_d(g.gmax[0]), _byref_d(g.gmax[1]), _byref_d(g.gmax[2]),
+                _d(np.asarray(g.dxyz, np.float64)), kel, mass, v0, ve, _byref_i(rp.shape[0]),
+                ct.byref(nl), *ints, _byref_i(g.species), _byref_i(g.nx), _byref_i(g.ny),
+                _byref_i(g.nz), _byref_d(g.gmin[0]), _byref_d(g.gmax[0]), _byref_d(g.gmin[1]),
+                _byref_d(g.gmax[1]), _byref_d(g.gmin[2]), _byref_d(g.gmax[2]), _d(tB),
+                ct.byref(fl[0]), ct.byref(fl[1]), ct.byref(fl[2]))
+    return nl.value
+
+
 def ref_uniforms(seed, count):
     """The first `count` draws of the runtime's random_number after seeding
     with `seed` (the stream ref_particles_produce consumes)."""

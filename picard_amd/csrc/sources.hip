@@ -125,6 +125,10 @@ __device__ void half_rotate(const PlasmaConst &pc, int s, double v[3]) {
 // the slots the reference's loop order gives them (cell-major, species, ion).
 struct CellBox {
   int lo[3], hi[3];
+  // the dimension whose lower cell edge is L_max itself (the +x, +y, +z
+  // inflow faces: x0 = Lx_max, BCinflow.f90:59, y0 = Ly_max, :201, z0 =
+  // Lz_max, :342), or -1: elsewhere the edge is Lmin + dxyz*(i-2)
+  int upper = -1;
 };
 
 // Output of the sources: FP64 structure of arrays (reference form) ...
@@ -190,9 +194,9 @@ __global__ void __launch_bounds__(kT) k_emit(Sink sink, int64_t base, uint32_t i
   // counter = index within this source call: identical draws for any num_local
   const uint64_t local = (uint64_t)(slot - base);
   Stream st{key, (uint32_t)local, (uint32_t)(local >> 32), tag};
-  const double x0 = g.lmin[0] + g.dxyz[0] * (ci - 2);
-  const double y0 = g.lmin[1] + g.dxyz[1] * (cj - 2);
-  const double z0 = g.lmin[2] + g.dxyz[2] * (ck - 2);
+  const double x0 = box.upper == 0 ? g.lmax[0] : g.lmin[0] + g.dxyz[0] * (ci - 2);
+  const double y0 = box.upper == 1 ? g.lmax[1] : g.lmin[1] + g.dxyz[1] * (cj - 2);
+  const double z0 = box.upper == 2 ? g.lmax[2] : g.lmin[2] + g.dxyz[2] * (ck - 2);
   const double px = x0 + g.dxyz[0] * st.uniform();
   const double py = y0 + g.dxyz[1] * st.uniform();
   const double pz = z0 + g.dxyz[2] * st.uniform();
@@ -466,27 +470,31 @@ int generate_initial(picard_ctx *c, const picard_plasma *pl, uint64_t seed) {
   return emit_box(c, pl, box, 1, seed, 0xA11u);
 }
 
-// BCinflow.f90: +x face (x0 = Lx_max), -x face (x0 = Lx_min - dx), then the y
-// and z faces (species from 2), over the same cell ranges as the reference.
+// BCinflow.f90: +x face (x0 = Lx_max, :53-121), -x face (x0 = Lx_min - dx,
+// :124-191), then the y faces (:194-333) and z faces (:334-470), species from
+// 2, over the same cell ranges as the reference.
 int generate_inflow(picard_ctx *c, const picard_plasma *pl, uint64_t seed, int64_t step) {
   const GridDims &g = c->gd;
   const int yn = g.flow[1] ? 1 : 0, zn = g.flow[2] ? 1 : 0;
   const uint32_t base_tag = 0x1000u + 8u * (uint32_t)step;
   if (g.flow[0]) {
-    PICARD_TRY(emit_box(c, pl, CellBox{{g.nx + 2, 2 - yn, 2 - zn}, {g.nx + 2, g.ny + 1 + yn, g.nz + 1 + zn}},
+    PICARD_TRY(emit_box(c, pl,
+                        CellBox{{g.nx + 2, 2 - yn, 2 - zn}, {g.nx + 2, g.ny + 1 + yn, g.nz + 1 + zn}, 0},
                         1, seed, base_tag + 0));
-    PICARD_TRY(emit_box(c, pl, CellBox{{1, 2 - yn, 2 - zn}, {1, g.ny + 1 + yn, g.nz + 1 + zn}}, 1, seed,
-                        base_tag + 1));
+    PICARD_TRY(emit_box(c, pl, CellBox{{1, 2 - yn, 2 - zn}, {1, g.ny + 1 + yn, g.nz + 1 + zn}}, 1,
+                        seed, base_tag + 1));
   }
   if (g.flow[1]) {
-    PICARD_TRY(emit_box(c, pl, CellBox{{2, g.ny + 2, 2}, {g.nx + 1, g.ny + 2, g.nz + 1 + zn}}, 2, seed,
-                        base_tag + 2));
-    PICARD_TRY(emit_box(c, pl, CellBox{{2, 1, 2 - zn}, {g.nx + 1, 1, g.nz + 1}}, 2, seed, base_tag + 3));
+    PICARD_TRY(emit_box(c, pl, CellBox{{2, g.ny + 2, 2}, {g.nx + 1, g.ny + 2, g.nz + 1 + zn}, 1}, 2,
+                        seed, base_tag + 2));
+    PICARD_TRY(emit_box(c, pl, CellBox{{2, 1, 2 - zn}, {g.nx + 1, 1, g.nz + 1}}, 2, seed,
+                        base_tag + 3));
   }
-  if (g.flow[2]) {
-    PICARD_TRY(emit_box(c, pl, CellBox{{2, 2, g.nz + 2}, {g.nx + 1, g.ny + 1, g.nz + 2}}, 2, seed,
-                        base_tag + 4));
-    PICARD_TRY(emit_box(c, pl, CellBox{{2, 2, 1}, {g.nx + 1, g.ny + 1, 1}}, 2, seed, base_tag + 5));
+  if (g.flow[2]) {   // +z takes the -y guard row, -z the +y one (BCinflow.f90:337, :407)
+    PICARD_TRY(emit_box(c, pl, CellBox{{2, 2 - yn, g.nz + 2}, {g.nx + 1, g.ny + 1, g.nz + 2}, 2}, 2,
+                        seed, base_tag + 4));
+    PICARD_TRY(emit_box(c, pl, CellBox{{2, 2, 1}, {g.nx + 1, g.ny + 1 + yn, 1}}, 2, seed,
+                        base_tag + 5));
   }
   return PICARD_OK;
 }

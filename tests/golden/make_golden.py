@@ -3,7 +3,7 @@ subroutines (AdvancePosition, BCoutflow, CalcDistance, AdvanceVelocity),
 computed by the reference's own Fortran compiled into oracle/_ref.
 
 Run in the container where /root/reference is mounted:
-    make -C oracle && python tests/golden/make_golden.py
+    make -C oracle && python tests/golden/make_golden.py [parameters|sources]
 The .npz files are data only (inputs and the reference's outputs)."""
 import itertools
 import os
@@ -96,6 +96,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "ref_advance_velocity_ie.npz"), **d)
     produce_cases()
     dump_cases()
+    source_cases()
     print("golden vectors written to", HERE)
 
 
@@ -207,6 +208,71 @@ def produce_cases():
     np.savez_compressed(os.path.join(HERE, "ref_particles_produce.npz"), **d_out)
 
 
+def source_plasma(cfg, d):
+    """The plasma tables the sources read (picard.f90:371-466 derived)."""
+    return dict(species=cfg.species, ipp=d.ipp, epp=d.epp, hpp=d.hpp, ppc=d.ppc,
+                kelvin=d.kelvin, mass=d.mass, v0=d.v0, ve4=d.ve, tB4=d.tB)
+
+
+def source_cases():
+    """ParticlesInitialize and BCinflow of the reference on small grids (two
+    plasmas: H+ and e-; and a 4-species plasma with a +2e and a -e ion) at
+    several flow-flag combinations: the inputs, the runtime uniform stream
+    each call consumed, and the reference's particles."""
+    from oracle import sources
+    from picard_amd import host
+    cases = [dict(dims=(5, 4, 3), species=2, flows=[(True, False, False), (True, True, True),
+                                                    (False, True, True), (False, False, True)]),
+             dict(dims=(4, 3, 3), species=4, flows=[(True, True, False), (True, False, True)])]
+    out = {}
+    t = 0
+    for k, cs in enumerate(cases):
+        nx, ny, nz = cs["dims"]
+        kw = dict(nx=nx, ny=ny, nz=nz, species=cs["species"],
+                  gmin=(-5.0 * nx, -6.0 * ny, -4.0 * nz), gmax=(5.0 * nx, 6.0 * ny, 4.0 * nz))
+        if cs["species"] == 4:
+            kw.update(ppc=(0, 4, 2, 1, 0, 0, 0, 0), density=(0.0, 5.0e6, 1.0e6, 5.0e5, 0, 0, 0, 0),
+                      charge3=(2 * host.EC, -host.EC) + (host.EC,) * 4,
+                      kelvin=(116045.0, 58022.0, 2.0e5, 3.0e4, 0, 0, 0, 0))
+        else:
+            kw.update(ppc=(0, 3, 0, 0, 0, 0, 0, 0))
+        for q, flow in enumerate([None] + cs["flows"]):
+            cfg = host.Config(flow=flow or (False, False, False), **kw)
+            d = host.derive(cfg)
+            g = oracle.Grid(nx, ny, nz, cfg.species, flow=cfg.flow, gmin=cfg.gmin,
+                            gmax=cfg.gmax)
+            pl = source_plasma(cfg, d)
+            cap = 20000
+            rp = np.zeros((cap, 8))
+            sp = np.zeros(cap, np.int32)
+            seed = 4321 + 17 * t
+            fn = oracle.ref_particles_initialize if flow is None else oracle.ref_bc_inflow
+            n = fn(g, rp, sp, 0, seed, pl)
+            u = oracle.ref_uniforms(seed, 400000)
+            draws = sources.SequentialDraws(u)
+            p2 = dict(pl, v0=d.v0[:, :3], ve=d.ve[:3], tB=d.tB[:, :3])
+            r2, s2 = (sources.particles_initialize if flow is None else sources.bc_inflow)(
+                g, p2, draws)
+            assert np.array_equal(s2, sp[:n]) and np.array_equal(r2[:, :6], rp[:n, :6])
+            out[f"kind{t}"] = np.array("initialize" if flow is None else "inflow")
+            out[f"dims{t}"] = np.array([nx, ny, nz, cfg.species], np.int32)
+            out[f"gmin{t}"] = np.array(cfg.gmin)
+            out[f"gmax{t}"] = np.array(cfg.gmax)
+            out[f"flow{t}"] = np.array(cfg.flow, np.int32)
+            for key in ("ipp", "epp", "hpp", "ppc"):
+                out[f"{key}{t}"] = np.asarray(pl[key], np.int32)
+            for key in ("kelvin", "mass", "v0", "ve4", "tB4"):
+                out[f"{key}{t}"] = np.asarray(pl[key], np.float64)
+            out[f"u{t}"] = u[: draws.pos]
+            out[f"rp_out{t}"] = rp[:n, :6].copy()
+            out[f"sp_out{t}"] = sp[:n].copy()
+            print(f"source case {t} ({out[f'kind{t}']}, flow {cfg.flow}): {n} particles, "
+                  f"{draws.pos} uniforms")
+            t += 1
+    out["ncase"] = np.int64(t)
+    np.savez_compressed(os.path.join(HERE, "ref_sources.npz"), **out)
+
+
 def parameters_cases():
     """WriteWarnings' save/parameters.dat (WriteWarnings.f90:130-256) for
     three setups: the 32^3 case with picard.f90's own Haser parameters
@@ -257,10 +323,10 @@ def parameters_cases():
 
 
 if __name__ == "__main__":
-    if sys.argv[1:] == ["parameters"]:
+    if sys.argv[1:] in (["parameters"], ["sources"]):
         if not oracle.ref_available():
             raise SystemExit("oracle/_ref is not built; run `make -C oracle` where the reference is")
-        parameters_cases()
+        parameters_cases() if sys.argv[1] == "parameters" else source_cases()
     else:
         main()
         parameters_cases()

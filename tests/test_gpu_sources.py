@@ -164,3 +164,86 @@ def test_haser_in_step():
     assert st2.num_local + st2.removed == st1.num_local + n2
     a.close()
     b.close()
+
+
+def _philox_draws(seed, tags):
+    """The device's per-ion streams: key = seed, counter (slot within the
+    source block, 0, tag of the block, block index); `tags` maps the
+    restatement's face number to the block's tag."""
+    from oracle import sources
+    from tests.philox import uniforms
+
+    def fn(face, local):
+        return uniforms(seed, np.uint64(local & 0xFFFFFFFF), np.uint64(local >> 32), tags[face],
+                        48)
+    return sources.PerIonDraws(fn)
+
+
+def _plasma(cfg, d):
+    return dict(species=cfg.species, ipp=d.ipp, epp=d.epp, hpp=d.hpp, kelvin=d.kelvin,
+                mass=d.mass, v0=d.v0[:, :3], ve=d.ve[:3], tB=d.tB[:, :3])
+
+
+def _four_species(flow):
+    return host.Config(nx=6, ny=5, nz=4, species=4, flow=flow, gmin=(-30.0, -27.5, -18.0),
+                       gmax=(30.0, 27.5, 18.0), ppc=(0, 4, 2, 1, 0, 0, 0, 0),
+                       density=(0.0, 5.0e6, 1.0e6, 5.0e5, 0, 0, 0, 0),
+                       charge3=(2 * host.EC, -host.EC) + (host.EC,) * 4,
+                       kelvin=(116045.0, 58022.0, 2.0e5, 3.0e4, 0, 0, 0, 0))
+
+
+def _compare_sources(rp, sp, ro, so, d, label):
+    assert rp.shape[0] == ro.shape[0], label
+    assert np.array_equal(sp, so), label
+    # positions: lower edge + dxyz * u in IEEE arithmetic on both sides
+    assert np.array_equal(rp[:, :3], ro[:, :3]), label
+    # velocities: the device's log/sin/sqrt against the C library's
+    err = np.abs(rp[:, 3:6] - ro[:, 3:6]) / np.maximum(np.abs(ro[:, 3:6]).max(axis=1,
+                                                                           keepdims=True), 1.0)
+    assert err.max() <= 1e-13, (label, err.max())
+    assert np.array_equal(rp[:, 7], np.arange(rp.shape[0], dtype=np.float64)), label
+
+
+@pytest.mark.parametrize("flow", [(True, False, False), (True, True, True), (False, True, True),
+                                  (True, False, True)])
+def test_device_sources_match_oracle(flow):
+    """Device ParticlesInitialize and BCinflow against the oracle's
+    restatement (bit-exact against the reference build on the runtime's
+    stream) driven by the device's Philox streams: counts, slot order and
+    species exactly, positions exactly, velocities to 1e-13 of the
+    particle's speed scale; the split form stores the same particles."""
+    from oracle import sources
+    print(f"[gpu-sources] oracle pin flow={flow}", flush=True)
+    cfg = _four_species(flow)
+    d = host.derive(cfg)
+    g = _oracle_grid(cfg)
+    pl = _plasma(cfg, d)
+    seed, step = 20261207, 3
+    ro, so = sources.particles_initialize(g, pl, _philox_draws(seed, {0: 0xA11}))
+    dom = host.Domain(cfg, max_particles=200000, d=d)
+    dom.generate_initial(seed)
+    rp, sp = dom.particles()
+    _compare_sources(rp, sp, ro, so, d, "initialize")
+    n0 = rp.shape[0]
+    tags = {f: 0x1000 + 8 * step + f for f in range(6)}
+    ri, si = sources.bc_inflow(g, pl, _philox_draws(seed, tags))
+    dom.generate_inflow(seed, step)
+    rp2, sp2 = dom.particles(n0)
+    rp2[:, 7] -= n0                      # ids continue from the initial load
+    _compare_sources(rp2, sp2, ri, si, d, "inflow")
+    sd = host.Domain(cfg, max_particles=200000, d=d, precision=_lib.PICARD_FP32_SPLIT)
+    sd.generate_initial(seed)
+    sd.generate_inflow(seed, step)
+    r3, s3 = sd.particles()
+    assert np.array_equal(s3, np.concatenate([so, si]))
+    ref = np.concatenate([ro, ri])
+    assert np.abs(r3[:, :3] - ref[:, :3]).max() <= 1e-5 * float(d.dxyz.max())
+    assert np.abs(r3[:, 3:6] - ref[:, 3:6]).max() <= 1e-6 * np.abs(ref[:, 3:6]).max()
+    dom.close()
+    sd.close()
+
+
+def _oracle_grid(cfg):
+    import oracle
+    return oracle.Grid(cfg.nx, cfg.ny, cfg.nz, cfg.species, flow=cfg.flow, gmin=cfg.gmin,
+                       gmax=cfg.gmax)

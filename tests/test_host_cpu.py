@@ -216,3 +216,39 @@ def test_poisson_charge_haser_term_matches_oracle_distance():
     amp = 1.0e-3 * host.EC * cfg.density[1] * d.dV
     assert np.all(np.abs(Cb - C) <= 16 * amp * (1 + 1e-12))
     assert np.any(Cb != C)
+
+
+class _HalfRng:
+    """A stand-in generator whose every uniform is 1/2."""
+
+    def random(self, shape=None):
+        return np.full(shape, 0.5) if shape is not None else 0.5
+
+
+@pytest.mark.parametrize("flow", [(True, False, False), (True, True, True), (False, True, True),
+                                  (True, False, True)])
+def test_host_sources_follow_the_reference_layout(flow):
+    """The host sources (seeded numpy draws) put their particles in the slots,
+    species and cells of the oracle's restatement of BCinflow and
+    ParticlesInitialize (pinned bit for bit against the reference build):
+    with every uniform 1/2 both place each particle at its cell's centre, so
+    positions agree exactly, face by face, including the z faces' y guard
+    rows (BCinflow.f90:337, :407) and the + faces' L_max edges."""
+    import oracle
+    from oracle import sources
+    cfg = host.Config(nx=5, ny=4, nz=3, species=4, flow=flow, gmin=(-25.0, -24.0, -12.0),
+                      gmax=(25.0, 24.0, 12.0), ppc=(0, 4, 2, 1, 0, 0, 0, 0),
+                      density=(0.0, 5.0e6, 1.0e6, 5.0e5, 0, 0, 0, 0),
+                      charge3=(2 * host.EC, -host.EC) + (host.EC,) * 4)
+    d = host.derive(cfg)
+    g = oracle.Grid(5, 4, 3, 4, flow=flow, gmin=cfg.gmin, gmax=cfg.gmax)
+    pl = dict(species=4, ipp=d.ipp, epp=d.epp, hpp=d.hpp, kelvin=d.kelvin, mass=d.mass,
+              v0=d.v0[:, :3], ve=d.ve[:3], tB=d.tB[:, :3])
+    half = sources.PerIonDraws(lambda face, local: np.full(64, 0.5))
+    for hfn, ofn in ((host.particles_initialize, sources.particles_initialize),
+                     (host.bc_inflow, sources.bc_inflow)):
+        rh, sh = hfn(cfg, d, _HalfRng())
+        ro, so = ofn(g, pl, half)
+        assert np.array_equal(sh, so)
+        assert np.array_equal(rh[:, :3], ro[:, :3])
+        np.testing.assert_allclose(rh[:, 3:6], ro[:, 3:6], rtol=1e-13, atol=1e-9)

@@ -285,3 +285,91 @@ def test_golden_advance_velocity_imp_exp(orc):
         oracle.advance_velocity_ie(mode, g, rp, z["sp"], z["E"], z["B0"], 0.5 * dt, dt, z["ve"],
                                    z["eta"], z["tB"], z["fluid"], z["gyration"], int(z["n"]))
         assert np.array_equal(rp, z[f"rp_out{mode}"])
+
+
+def _sources_from_golden(z, t):
+    from oracle import sources
+    nx, ny, nz, S = [int(v) for v in z[f"dims{t}"]]
+    g = oracle.Grid(nx, ny, nz, S, flow=tuple(bool(f) for f in z[f"flow{t}"]),
+                    gmin=tuple(z[f"gmin{t}"]), gmax=tuple(z[f"gmax{t}"]))
+    pl = dict(species=S, ipp=z[f"ipp{t}"], epp=z[f"epp{t}"], hpp=z[f"hpp{t}"],
+              kelvin=z[f"kelvin{t}"], mass=z[f"mass{t}"], v0=z[f"v0{t}"][:, :3],
+              ve=z[f"ve4{t}"][:3], tB=z[f"tB4{t}"][:, :3])
+    draws = sources.SequentialDraws(z[f"u{t}"])
+    fn = sources.particles_initialize if str(z[f"kind{t}"]) == "initialize" else sources.bc_inflow
+    rp, sp = fn(g, pl, draws)
+    return rp, sp, draws
+
+
+def test_golden_sources():
+    """ParticlesInitialize and BCinflow: the restatement, fed the runtime
+    uniform stream the reference consumed, reproduces the reference's
+    particles bit for bit (slot order, species, positions, velocities) and
+    consumes exactly that stream, at every flow combination stored (the
+    z faces' cell ranges depend on the y flow, BCinflow.f90:337, :407)."""
+    z = _golden("ref_sources.npz")
+    kinds = set()
+    for t in range(int(z["ncase"])):
+        rp, sp, draws = _sources_from_golden(z, t)
+        kinds.add(str(z[f"kind{t}"]))
+        assert draws.pos == z[f"u{t}"].size, t
+        assert np.array_equal(sp, z[f"sp_out{t}"]), t
+        assert np.array_equal(rp[:, :6], z[f"rp_out{t}"]), t
+    assert kinds == {"initialize", "inflow"}
+
+
+def test_golden_sources_invariants():
+    """Closed-form checks of the same outputs: companions sit at their ion's
+    position; every particle lies in its guard or interior cell; with T = 0
+    and tB = 0 the velocity reduces to v0 (Tools.f90:24-26), checked by
+    rerunning the restatement with those tables."""
+    from oracle import sources
+    z = _golden("ref_sources.npz")
+    for t in range(int(z["ncase"])):
+        rp, sp = z[f"rp_out{t}"], z[f"sp_out{t}"]
+        epp, hpp = z[f"epp{t}"], z[f"hpp{t}"]
+        q = 0
+        while q < len(sp):                   # groups: ion, epp electrons, hpp protons
+            s = int(sp[q]) - 1
+            grp = [1] * int(epp[s]) + [2] * int(hpp[s])
+            assert list(sp[q + 1:q + 1 + len(grp)]) == grp
+            assert np.all(rp[q + 1:q + 1 + len(grp), :3] == rp[q, :3])
+            q += 1 + len(grp)
+        nx, ny, nz, S = [int(v) for v in z[f"dims{t}"]]
+        lo, hi = z[f"gmin{t}"], z[f"gmax{t}"]
+        dx = (hi - lo) / np.array([nx, ny, nz])
+        assert np.all(rp[:, :3] >= lo - dx) and np.all(rp[:, :3] <= hi + dx)
+        g = oracle.Grid(nx, ny, nz, S, flow=tuple(bool(f) for f in z[f"flow{t}"]),
+                        gmin=tuple(lo), gmax=tuple(hi))
+        pl = dict(species=S, ipp=z[f"ipp{t}"], epp=z[f"epp{t}"], hpp=z[f"hpp{t}"],
+                  kelvin=np.zeros(8), mass=z[f"mass{t}"], v0=z[f"v0{t}"][:, :3],
+                  ve=z[f"ve4{t}"][:3], tB=np.zeros((8, 3)))
+        fn = sources.particles_initialize if str(z[f"kind{t}"]) == "initialize" else sources.bc_inflow
+        rc, sc = fn(g, pl, sources.SequentialDraws(z[f"u{t}"]))
+        assert np.array_equal(sc, sp)
+        assert np.array_equal(rc[:, 3:6], np.tile(z[f"v0{t}"][0, :3], (len(sp), 1)))
+
+
+@needs_ref
+@pytest.mark.parametrize("flow", [(True, False, False), (True, True, True), (False, True, True),
+                                  (False, False, True)])
+def test_sources_vs_ref(orc, flow):
+    """Live: BCinflow and ParticlesInitialize of the reference build against
+    the restatement on the same runtime stream, bit for bit."""
+    from oracle import sources
+    from picard_amd import host
+    cfg = host.Config(nx=4, ny=5, nz=3, flow=flow, gmin=(-20.0, -25.0, -15.0),
+                      gmax=(20.0, 25.0, 15.0), ppc=(0, 2, 0, 0, 0, 0, 0, 0))
+    d = host.derive(cfg)
+    g = oracle.Grid(4, 5, 3, 2, flow=flow, gmin=cfg.gmin, gmax=cfg.gmax)
+    pl = dict(species=2, ipp=d.ipp, epp=d.epp, hpp=d.hpp, ppc=d.ppc, kelvin=d.kelvin,
+              mass=d.mass, v0=d.v0, ve4=d.ve, tB4=d.tB)
+    p2 = dict(pl, v0=d.v0[:, :3], ve=d.ve[:3], tB=d.tB[:, :3])
+    for ref_fn, fn in ((oracle.ref_particles_initialize, sources.particles_initialize),
+                       (oracle.ref_bc_inflow, sources.bc_inflow)):
+        rp = np.zeros((5000, 8))
+        sp = np.zeros(5000, np.int32)
+        n = ref_fn(g, rp, sp, 0, 99, pl)
+        draws = sources.SequentialDraws(oracle.ref_uniforms(99, 100000))
+        r2, s2 = fn(g, p2, draws)
+        assert np.array_equal(s2, sp[:n]) and np.array_equal(r2[:, :6], rp[:n, :6])
