@@ -13,6 +13,17 @@
 ! after num_local (BCinflow, ParticlesProduce) are uploaded by the next call;
 ! host copies of device state are refreshed only by picard_dropin_download,
 ! or after every call when eager_download is set.
+!
+! Arguments: the domain (cell counts, flow flags, local and global bounds,
+! cell sizes, species count, max_per_proc) and the species tables (charge,
+! n2p, eta, ve, tB, sB, B0, E0, fluid, gyration) are fixed by
+! picard_dropin_init.  Every subroutine compares the ones it is passed with
+! them, bit for bit, and stops with an error naming the argument on a
+! mismatch: a host that changes them between calls would otherwise get the
+! initial values silently.  dt, deltat, maxerr and maxit are forwarded on
+! every call.  The field arrays (N, C, U, E, D) are device-resident: host
+! writes to them between calls are not uploaded, and their host copies are
+! refreshed only as described above.
 module picard_dropin_state
   use iso_c_binding
   use picard_hip_iface
@@ -26,8 +37,119 @@ module picard_dropin_state
   logical, save :: eager_download = .false.
   integer(c_int64_t), save :: last_sweeps = 0
   integer(c_int64_t), save :: last_removed = 0
+  type(picard_geometry), save :: geo0              ! the domain given to picard_dropin_init
+  type(picard_species), save :: spc0
 
 contains
+
+  subroutine arg_mismatch(where, what)
+    character(len=*), intent(in) :: where, what
+    write(*,'(a,a,a,a,a)') 'picard: ', where, ': argument ', what, &
+        ' differs from the domain given to picard_dropin_init'
+    error stop 2
+  end subroutine arg_mismatch
+
+  logical function differs(a, b, n)
+    integer, intent(in) :: n
+    real(c_double), intent(in) :: a(n), b(n)
+    integer :: i
+    differs = .false.
+    do i = 1, n
+      if (transfer(a(i), 0_c_int64_t) /= transfer(b(i), 0_c_int64_t)) differs = .true.
+    end do
+  end function differs
+
+  subroutine check_flows(where, xflow, yflow, zflow)
+    character(len=*), intent(in) :: where
+    logical, intent(in) :: xflow, yflow, zflow
+    if ((xflow .neqv. (geo0%flow(1) /= 0)) .or. (yflow .neqv. (geo0%flow(2) /= 0)) .or. &
+        (zflow .neqv. (geo0%flow(3) /= 0))) call arg_mismatch(where, 'xflow/yflow/zflow')
+  end subroutine check_flows
+
+  subroutine check_cells(where, nx, ny, nz)
+    character(len=*), intent(in) :: where
+    integer, intent(in) :: nx, ny, nz
+    if (nx /= geo0%nx .or. ny /= geo0%ny .or. nz /= geo0%nz) &
+      call arg_mismatch(where, 'Nx_local/Ny_local/Nz_local')
+  end subroutine check_cells
+
+  subroutine check_local_box(where, lo, hi)
+    character(len=*), intent(in) :: where
+    real(c_double), intent(in) :: lo(3), hi(3)
+    if (differs(lo, geo0%lmin, 3) .or. differs(hi, geo0%lmax, 3)) &
+      call arg_mismatch(where, 'Lx_min..Lz_max')
+  end subroutine check_local_box
+
+  subroutine check_global_box(where, lo, hi)
+    character(len=*), intent(in) :: where
+    real(c_double), intent(in) :: lo(3), hi(3)
+    if (differs(lo, geo0%gmin, 3) .or. differs(hi, geo0%gmax, 3)) &
+      call arg_mismatch(where, 'xmin..zmax')
+  end subroutine check_global_box
+
+  subroutine check_dxyz(where, dxyz)
+    character(len=*), intent(in) :: where
+    real(c_double), intent(in) :: dxyz(3)
+    if (differs(dxyz, geo0%dxyz, 3)) call arg_mismatch(where, 'dxyz')
+  end subroutine check_dxyz
+
+  subroutine check_species(where, species)
+    character(len=*), intent(in) :: where
+    integer, intent(in) :: species
+    if (species /= geo0%species) call arg_mismatch(where, 'species')
+  end subroutine check_species
+
+  subroutine check_capacity(where, max_per_proc)
+    character(len=*), intent(in) :: where
+    integer, intent(in) :: max_per_proc
+    if (max_per_proc /= host_max) call arg_mismatch(where, 'max_per_proc')
+  end subroutine check_capacity
+
+  subroutine check_charges(where, charge, n2p)
+    character(len=*), intent(in) :: where
+    real(c_double), intent(in) :: charge(8), n2p(8)
+    if (differs(charge, spc0%charge, 8)) call arg_mismatch(where, 'charge')
+    if (differs(n2p, spc0%n2p, 8)) call arg_mismatch(where, 'n2p')
+  end subroutine check_charges
+
+  subroutine check_one_rank(where, iprocs, jprocs, kprocs, myid)
+    character(len=*), intent(in) :: where
+    integer, intent(in) :: iprocs, jprocs, kprocs, myid
+    if (iprocs /= 1 .or. jprocs /= 1 .or. kprocs /= 1 .or. myid /= 0) &
+      call arg_mismatch(where, 'iprocs/jprocs/kprocs/myid (one domain, one rank)')
+  end subroutine check_one_rank
+
+  ! The arguments AdvanceVelocity, AdvanceVelocityImp and AdvanceVelocityExp
+  ! share (AdvanceVelocity.f90:6-23; Exp :128-146; Imp :239-257); v0 is in their
+  ! argument lists but none of them reads it.
+  subroutine check_velocity(where, Lx_min, Ly_min, Lz_min, dxyz, B0, ve, eta, tB, fluid, gyration, &
+                            max_per_proc, species, nx, ny, nz)
+    character(len=*), intent(in) :: where
+    real(c_double), intent(in) :: Lx_min, Ly_min, Lz_min, dxyz(3), B0(4), ve(4), eta(8), tB(32)
+    logical, intent(in) :: fluid(8), gyration(8)
+    integer, intent(in) :: max_per_proc, species, nx, ny, nz
+    if (differs([Lx_min, Ly_min, Lz_min], geo0%lmin, 3)) call arg_mismatch(where, 'Lx_min/Ly_min/Lz_min')
+    call check_dxyz(where, dxyz)
+    call check_push(where, B0, ve, eta, tB, fluid, gyration)
+    call check_capacity(where, max_per_proc)
+    call check_species(where, species)
+    call check_cells(where, nx, ny, nz)
+  end subroutine check_velocity
+
+  subroutine check_push(where, B0, ve, eta, tB, fluid, gyration)
+    character(len=*), intent(in) :: where
+    real(c_double), intent(in) :: B0(4), ve(4), eta(8), tB(32)
+    logical, intent(in) :: fluid(8), gyration(8)
+    integer :: s
+    if (differs(B0, spc0%B0, 4)) call arg_mismatch(where, 'B0')
+    if (differs(ve, spc0%ve, 4)) call arg_mismatch(where, 've')
+    if (differs(eta, spc0%eta, 8)) call arg_mismatch(where, 'eta')
+    if (differs(tB, spc0%tB, 32)) call arg_mismatch(where, 'tB')
+    do s = 1, 8
+      if (fluid(s) .neqv. (spc0%fluid(s) /= 0)) call arg_mismatch(where, 'fluid')
+      if (gyration(s) .neqv. (spc0%gyration(s) /= 0)) call arg_mismatch(where, 'gyration')
+    end do
+  end subroutine check_push
 
   subroutine picard_check(rc, where)
     integer(c_int), intent(in) :: rc
@@ -47,6 +169,8 @@ contains
     integer, intent(in), target :: int_particles(max_per_proc)
     if (c_associated(ctx)) call picard_check(picard_destroy(ctx), 'picard_destroy')
     call picard_check(picard_create(geo, spc, ctx), 'picard_create')
+    geo0 = geo
+    spc0 = spc
     host_rp = c_loc(real_particles)
     host_sp = c_loc(int_particles)
     host_max = max_per_proc
@@ -114,6 +238,7 @@ subroutine AdvancePosition(real_particles, dt, max_per_proc, num_local)
   integer, intent(in) :: max_per_proc, num_local
   real*8, intent(inout) :: real_particles(8, max_per_proc)
   real*8, intent(in) :: dt
+  call check_capacity('AdvancePosition', max_per_proc)
   call picard_dropin_sync(num_local)
   call picard_check(picard_advance_position(ctx, dt), 'AdvancePosition')
   if (eager_download) call picard_dropin_download_particles()
@@ -131,6 +256,10 @@ subroutine BCoutflow(real_particles, int_particles, Lx_min, Lx_max, Ly_min, Ly_m
   real*8, intent(inout) :: real_particles(8, max_per_proc)
   integer, intent(inout) :: int_particles(max_per_proc), num_local
   integer(c_int64_t) :: n_new, removed
+  call check_capacity('BCoutflow', max_per_proc)
+  call check_local_box('BCoutflow', [Lx_min, Ly_min, Lz_min], [Lx_max, Ly_max, Lz_max])
+  call check_global_box('BCoutflow', [xmin, ymin, zmin], [xmax, ymax, zmax])
+  call check_flows('BCoutflow', xflow, yflow, zflow)
   call picard_dropin_sync(num_local)
   call picard_check(picard_bc_outflow(ctx, n_new, removed), 'BCoutflow')
   num_local = int(n_new)
@@ -150,10 +279,13 @@ subroutine ParticlesTransfer(real_particles, int_particles, Lx_min, Lx_max, Ly_m
   real*8, intent(in) :: Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, Lz_max, xmin, xmax, ymin, ymax, zmin, zmax
   integer, intent(inout) :: int_particles(max_per_proc), num_local
   real*8, intent(inout) :: real_particles(8, max_per_proc)
-  if (iprocs * jprocs * kprocs /= 1) then
-    write(*,*) 'picard: one domain per GPU, one rank (iprocs = jprocs = kprocs = 1)'
+  if (iprocs /= 1 .or. jprocs /= 1 .or. kprocs /= 1 .or. myid /= 0) then
+    write(*,*) 'picard: one domain per GPU, one rank (iprocs = jprocs = kprocs = 1, myid = 0)'
     error stop 1
   end if
+  call check_capacity('ParticlesTransfer', max_per_proc)
+  call check_local_box('ParticlesTransfer', [Lx_min, Ly_min, Lz_min], [Lx_max, Ly_max, Lz_max])
+  call check_global_box('ParticlesTransfer', [xmin, ymin, zmin], [xmax, ymax, zmax])
   call picard_dropin_sync(num_local)
   call picard_check(picard_particles_transfer(ctx), 'ParticlesTransfer')
   if (eager_download) call picard_dropin_download_particles()
@@ -177,6 +309,7 @@ subroutine ParticlesGridify(N, C, real_particles, int_particles, Lx_min, Lx_max,
   real*8, intent(out) :: N(species, Nx_local+2, Ny_local+2, Nz_local+2)
   real*8, intent(out) :: C(Nx_local+2, Ny_local+2, Nz_local+2)
   integer(c_int64_t) :: G
+  call check_gridify('ParticlesGridify')
   call picard_dropin_sync(num_local)
   call picard_check(picard_particles_gridify(ctx), 'ParticlesGridify')
   if (eager_download) then
@@ -184,6 +317,19 @@ subroutine ParticlesGridify(N, C, real_particles, int_particles, Lx_min, Lx_max,
     call picard_dropin_download_field(PICARD_FIELD_N, N, species * G)
     call picard_dropin_download_field(PICARD_FIELD_C, C, G)
   end if
+contains
+  subroutine check_gridify(where)
+    character(len=*), intent(in) :: where
+    call check_one_rank(where, iprocs, jprocs, kprocs, myid)
+    call check_capacity(where, max_per_proc)
+    call check_cells(where, Nx_local, Ny_local, Nz_local)
+    call check_species(where, species)
+    call check_local_box(where, [Lx_min, Ly_min, Lz_min], [Lx_max, Ly_max, Lz_max])
+    call check_global_box(where, [xmin, ymin, zmin], [xmax, ymax, zmax])
+    call check_dxyz(where, dxyz)
+    call check_flows(where, xflow, yflow, zflow)
+    call check_charges(where, charge, n2p)
+  end subroutine check_gridify
 end subroutine ParticlesGridify
 
 
@@ -195,6 +341,9 @@ subroutine CalcDistance(D, Nx, Ny, Nz, dxyz, xflow, yflow, zflow)
   integer, intent(in) :: Nx, Ny, Nz
   real*8, intent(in) :: dxyz(3)
   real*8, intent(out) :: D(2*Nx+1, 2*Ny+1, 2*Nz+1)
+  call check_cells('CalcDistance', Nx, Ny, Nz)
+  call check_dxyz('CalcDistance', dxyz)
+  call check_flows('CalcDistance', xflow, yflow, zflow)
   call picard_check(picard_calc_distance(ctx), 'CalcDistance')
   if (eager_download) &
     call picard_dropin_download_field(PICARD_FIELD_D, D, int(2*Nx+1, c_int64_t) * (2*Ny+1) * (2*Nz+1))
@@ -212,6 +361,12 @@ subroutine BCpotentials(U, C, D, xmin, xmax, ymin, ymax, zmin, zmax, Nx_local, N
   real*8, intent(in) :: D(2*Nx+1, 2*Ny+1, 2*Nz+1)
   real*8, intent(in) :: C(Nx_local+2, Ny_local+2, Nz_local+2)
   real*8, intent(inout) :: U(Nx_local+2, Ny_local+2, Nz_local+2)
+  call check_one_rank('BCpotentials', iprocs, jprocs, kprocs, myid)
+  call check_cells('BCpotentials', Nx_local, Ny_local, Nz_local)
+  call check_cells('BCpotentials', Nx, Ny, Nz)
+  call check_global_box('BCpotentials', [xmin, ymin, zmin], [xmax, ymax, zmax])
+  call check_flows('BCpotentials', xflow, yflow, zflow)
+  call check_species('BCpotentials', species)
   call picard_check(picard_bc_potentials(ctx), 'BCpotentials')
   if (eager_download) &
     call picard_dropin_download_field(PICARD_FIELD_U, U, int(Nx_local+2, c_int64_t) * (Ny_local+2) * (Nz_local+2))
@@ -231,6 +386,13 @@ subroutine CalcEpotential(U, C, Nx_local, Ny_local, Nz_local, Lx_min, Lx_max, Ly
   real*8, intent(in) :: C(Nx_local+2, Ny_local+2, Nz_local+2)
   real*8, intent(inout) :: U(Nx_local+2, Ny_local+2, Nz_local+2)
   integer(c_int) :: rc
+  call check_one_rank('CalcEpotential', iprocs, jprocs, kprocs, myid)
+  call check_cells('CalcEpotential', Nx_local, Ny_local, Nz_local)
+  call check_local_box('CalcEpotential', [Lx_min, Ly_min, Lz_min], [Lx_max, Ly_max, Lz_max])
+  call check_global_box('CalcEpotential', [xmin, ymin, zmin], [xmax, ymax, zmax])
+  call check_flows('CalcEpotential', xflow, yflow, zflow)
+  call check_dxyz('CalcEpotential', dxyz)
+  call check_species('CalcEpotential', species)
   rc = picard_calc_epotential(ctx, maxerr, int(maxit, c_int32_t), last_sweeps)
   call picard_check(rc, 'CalcEpotential')
   if (rc == PICARD_WARN_NOCONV) write(*,*) 'Maximum iteration reached!'
@@ -247,6 +409,10 @@ subroutine CalcEfield(E, U, Nx_local, Ny_local, Nz_local, dxyz, xflow, yflow, zf
   integer, intent(in) :: Nx_local, Ny_local, Nz_local, iprocs, jprocs, kprocs, myid
   real*8, intent(in) :: U(Nx_local+2, Ny_local+2, Nz_local+2), dxyz(3)
   real*8, intent(out) :: E(3, Nx_local+2, Ny_local+2, Nz_local+2)
+  call check_one_rank('CalcEfield', iprocs, jprocs, kprocs, myid)
+  call check_cells('CalcEfield', Nx_local, Ny_local, Nz_local)
+  call check_dxyz('CalcEfield', dxyz)
+  call check_flows('CalcEfield', xflow, yflow, zflow)
   call picard_check(picard_calc_efield(ctx), 'CalcEfield')
   if (eager_download) &
     call picard_dropin_download_field(PICARD_FIELD_E, E, 3_c_int64_t * (Nx_local+2) * (Ny_local+2) * (Nz_local+2))
@@ -262,6 +428,11 @@ subroutine BCfields(E, U, B0, E0, Nx_local, Ny_local, Nz_local, xflow, yflow, zf
   real*8 :: B0(4), E0(4)
   real*8, intent(in) :: U(Nx_local+2, Ny_local+2, Nz_local+2)
   real*8, intent(inout) :: E(3, Nx_local+2, Ny_local+2, Nz_local+2)
+  call check_one_rank('BCfields', iprocs, jprocs, kprocs, myid)
+  call check_cells('BCfields', Nx_local, Ny_local, Nz_local)
+  call check_flows('BCfields', xflow, yflow, zflow)
+  if (differs(B0, spc0%B0, 4)) call arg_mismatch('BCfields', 'B0')
+  if (differs(E0, spc0%E0, 4)) call arg_mismatch('BCfields', 'E0')
   call picard_check(picard_bc_fields(ctx), 'BCfields')
   if (eager_download) &
     call picard_dropin_download_field(PICARD_FIELD_E, E, 3_c_int64_t * (Nx_local+2) * (Ny_local+2) * (Nz_local+2))
@@ -281,13 +452,16 @@ subroutine AdvanceVelocity(real_particles, int_particles, E, B0, Lx_min, Ly_min,
   integer, intent(in) :: int_particles(max_per_proc)
   real*8, intent(in) :: B0(4)
   real*8, intent(in) :: E(3, Nx_local+2, Ny_local+2, Nz_local+2)
+  call check_velocity('AdvanceVelocity', Lx_min, Ly_min, Lz_min, dxyz, B0, ve, eta, tB, fluid, gyration, &
+                      max_per_proc, species, Nx_local, Ny_local, Nz_local)
+  if (differs(sB, spc0%sB, 32)) call arg_mismatch('AdvanceVelocity', 'sB')
   call picard_dropin_sync(num_local)
   call picard_check(picard_advance_velocity(ctx, dt), 'AdvanceVelocity')
   if (eager_download) call picard_dropin_download_particles()
 end subroutine AdvanceVelocity
 
 
-! AdvanceVelocity.f90:128 (output steps, before the dumps)
+! AdvanceVelocity.f90:239 (output steps, before the dumps)
 subroutine AdvanceVelocityImp(real_particles, int_particles, E, B0, Lx_min, Ly_min, Lz_min, dxyz, deltat, dt, &
                               v0, ve, eta, tB, num_local, max_per_proc, species, Nx_local, Ny_local, Nz_local, &
                               fluid, gyration)
@@ -301,13 +475,15 @@ subroutine AdvanceVelocityImp(real_particles, int_particles, E, B0, Lx_min, Ly_m
   integer, intent(in) :: int_particles(max_per_proc)
   real*8, intent(in) :: B0(4)
   real*8, intent(in) :: E(3, Nx_local+2, Ny_local+2, Nz_local+2)
+  call check_velocity('AdvanceVelocityImp', Lx_min, Ly_min, Lz_min, dxyz, B0, ve, eta, tB, fluid, gyration, &
+                      max_per_proc, species, Nx_local, Ny_local, Nz_local)
   call picard_dropin_sync(num_local)
   call picard_check(picard_advance_velocity_imp(ctx, deltat, dt), 'AdvanceVelocityImp')
   if (eager_download) call picard_dropin_download_particles()
 end subroutine AdvanceVelocityImp
 
 
-! AdvanceVelocity.f90:240 (output steps, after the dumps)
+! AdvanceVelocity.f90:128 (output steps, after the dumps)
 subroutine AdvanceVelocityExp(real_particles, int_particles, E, B0, Lx_min, Ly_min, Lz_min, dxyz, deltat, dt, &
                               v0, ve, eta, tB, num_local, max_per_proc, species, Nx_local, Ny_local, Nz_local, &
                               fluid, gyration)
@@ -321,6 +497,8 @@ subroutine AdvanceVelocityExp(real_particles, int_particles, E, B0, Lx_min, Ly_m
   integer, intent(in) :: int_particles(max_per_proc)
   real*8, intent(in) :: B0(4)
   real*8, intent(in) :: E(3, Nx_local+2, Ny_local+2, Nz_local+2)
+  call check_velocity('AdvanceVelocityExp', Lx_min, Ly_min, Lz_min, dxyz, B0, ve, eta, tB, fluid, gyration, &
+                      max_per_proc, species, Nx_local, Ny_local, Nz_local)
   call picard_dropin_sync(num_local)
   call picard_check(picard_advance_velocity_exp(ctx, deltat, dt), 'AdvanceVelocityExp')
   if (eager_download) call picard_dropin_download_particles()
@@ -346,6 +524,15 @@ subroutine ParticlesGridifyAll(F, N, C, real_particles, int_particles, Lx_min, L
   real*8, intent(out) :: N(species, Nx_local+2, Ny_local+2, Nz_local+2)
   real*8, intent(out) :: C(Nx_local+2, Ny_local+2, Nz_local+2)
   integer(c_int64_t) :: G
+  call check_one_rank('ParticlesGridifyAll', iprocs, jprocs, kprocs, myid)
+  call check_capacity('ParticlesGridifyAll', max_per_proc)
+  call check_cells('ParticlesGridifyAll', Nx_local, Ny_local, Nz_local)
+  call check_species('ParticlesGridifyAll', species)
+  call check_local_box('ParticlesGridifyAll', [Lx_min, Ly_min, Lz_min], [Lx_max, Ly_max, Lz_max])
+  call check_global_box('ParticlesGridifyAll', [xmin, ymin, zmin], [xmax, ymax, zmax])
+  call check_dxyz('ParticlesGridifyAll', dxyz)
+  call check_flows('ParticlesGridifyAll', xflow, yflow, zflow)
+  call check_charges('ParticlesGridifyAll', charge, n2p)
   call picard_dropin_sync(num_local)
   call picard_check(picard_particles_gridify_all(ctx), 'ParticlesGridifyAll')
   G = int(Nx_local+2, c_int64_t) * (Ny_local+2) * (Nz_local+2)

@@ -1,6 +1,10 @@
 ! dropin_check.f90 -- the Fortran drop-in subroutines (GPU) against the C
 ! oracle on the same host-generated particles, bit for bit, step by step.
 !   dropin_check [steps]      prints 'DROPIN OK' and exits 0 on parity
+!   dropin_check mismatch K   calls one drop-in with argument K changed from
+!                             the domain given at init (xflow, dxyz, tB,
+!                             max_per_proc, n2p, E0, myid); the call must stop
+!                             with an error before any device work
 program dropin_check
   use iso_c_binding
   use picard_hip_iface
@@ -94,14 +98,19 @@ program dropin_check
   integer, allocatable :: sp(:), sp_o(:)
   integer :: maxp, nl, it, steps, first, nargs, bad
   integer(c_int64_t) :: cnt_o, sw_o, dummy
-  character(len=32) :: arg
+  character(len=32) :: arg, mism
   real*8 :: Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, Lz_max
 
   steps = 4
+  mism = ''
   nargs = command_argument_count()
   if (nargs >= 1) then
     call get_command_argument(1, arg)
-    read(arg, *) steps
+    if (arg == 'mismatch') then
+      call get_command_argument(2, mism)
+    else
+      read(arg, *) steps
+    end if
   end if
   cfg%nx = 10; cfg%ny = 9; cfg%nz = 8; cfg%ppc(2) = 4
   cfg%xmin = -50d0; cfg%xmax = 50d0; cfg%ymin = -45d0; cfg%ymax = 45d0; cfg%zmin = -40d0; cfg%zmax = 40d0
@@ -133,6 +142,7 @@ program dropin_check
   call orc_calc_distance(D_o, cfg%nx, cfg%ny, cfg%nz, dc%dxyz, og%flow)
   bad = 0
   call cmp('D', Dt, D_o, size(Dt))
+  if (mism /= '') call mismatched_call(trim(mism))
 
   do it = 1, steps
     first = nl + 1
@@ -188,6 +198,49 @@ program dropin_check
   write(*,'(a)') 'DROPIN OK'
 
 contains
+
+  ! One drop-in call with one argument changed; returns only if the call
+  ! accepted it (then the program stops with a failure of its own).
+  subroutine mismatched_call(which)
+    character(len=*), intent(in) :: which
+    real*8 :: dx2(3), tB2(4, 8), n2p2(8), E02(4)
+    write(*,'(a,a)') 'mismatch case ', which
+    dx2 = dc%dxyz
+    tB2 = dc%tB
+    n2p2 = dc%n2p
+    E02 = dc%E0
+    select case (which)
+    case ('xflow')
+      call BCoutflow(rp, sp, Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, Lz_max, cfg%xmin, cfg%xmax, cfg%ymin, &
+                     cfg%ymax, cfg%zmin, cfg%zmax, maxp, nl, .not. cfg%xflow, cfg%yflow, cfg%zflow)
+    case ('dxyz')
+      dx2(2) = 2d0 * dx2(2)
+      call CalcEfield(E, U, cfg%nx, cfg%ny, cfg%nz, dx2, cfg%xflow, cfg%yflow, cfg%zflow, 1, 1, 1, 0)
+    case ('tB')
+      tB2(3, 1) = tB2(3, 1) * (1d0 + epsilon(1d0))
+      call AdvanceVelocity(rp, sp, E, dc%B0, Lx_min, Ly_min, Lz_min, dc%dxyz, dc%dt, dc%v0, dc%ve, dc%eta, &
+                           tB2, dc%sB, nl, maxp, cfg%species, cfg%nx, cfg%ny, cfg%nz, cfg%fluid, cfg%gyration)
+    case ('max_per_proc')
+      call AdvancePosition(rp, dc%dt, maxp - 1, nl)
+    case ('n2p')
+      n2p2(2) = 0.5d0 * n2p2(2)
+      call ParticlesGridify(N, C, rp, sp, Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, Lz_max, dc%dxyz, cfg%xmin, &
+                            cfg%xmax, cfg%ymin, cfg%ymax, cfg%zmin, cfg%zmax, cfg%species, cfg%nx, cfg%ny, &
+                            cfg%nz, maxp, nl, cfg%xflow, cfg%yflow, cfg%zflow, 1, 1, 1, dc%charge, n2p2, 0)
+    case ('E0')
+      E02(1) = E02(1) + 1d0
+      call BCfields(E, U, dc%B0, E02, cfg%nx, cfg%ny, cfg%nz, cfg%xflow, cfg%yflow, cfg%zflow, 1, 1, 1, 0)
+    case ('myid')
+      call CalcEpotential(U, C, cfg%nx, cfg%ny, cfg%nz, Lx_min, Lx_max, Ly_min, Ly_max, Lz_min, Lz_max, &
+                          cfg%xflow, cfg%yflow, cfg%zflow, cfg%xmin, cfg%xmax, cfg%ymin, cfg%ymax, cfg%zmin, &
+                          cfg%zmax, dc%dxyz, cfg%maxerr, cfg%maxit, 1, 1, 1, cfg%species, 1)
+    case default
+      write(*,'(a,a)') 'unknown mismatch case ', which
+      error stop 3
+    end select
+    write(*,'(a)') 'MISMATCH ACCEPTED'
+    error stop 4
+  end subroutine mismatched_call
 
   subroutine cmp(what, a, b, n)
     character(len=*), intent(in) :: what

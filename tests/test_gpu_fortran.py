@@ -25,6 +25,22 @@ def test_fortran_dropin_matches_oracle():
     assert "DROPIN OK" in out.stdout
 
 
+@pytest.mark.parametrize("case,arg", [("xflow", "xflow/yflow/zflow"), ("dxyz", "dxyz"),
+                                      ("tB", "tB"), ("max_per_proc", "max_per_proc"),
+                                      ("n2p", "n2p"), ("E0", "E0"),
+                                      ("myid", "iprocs/jprocs/kprocs/myid")])
+def test_fortran_dropin_rejects_changed_arguments(case, arg):
+    """A drop-in passed an argument that differs from the domain registered
+    at picard_dropin_init (one flag, one cell size, one bit of tB, ...)
+    stops with an error naming it, before any device work, instead of
+    silently using the registered value."""
+    exe = os.path.join(ROOT, "tests", "fortran", "dropin_check")
+    out = _run([exe, "mismatch", case], 120)
+    assert out.returncode == 2, out.returncode
+    assert f"argument {arg}" in out.stdout and "differs from the domain" in out.stdout
+    assert "MISMATCH ACCEPTED" not in out.stdout
+
+
 def test_fortran_host_runs(tmp_path):
     exe = os.path.join(ROOT, "picard_amd", "picard_host")
     assert os.path.exists(exe), "build() compiles picard_amd/picard_host"
