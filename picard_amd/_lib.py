@@ -1,4 +1,4 @@
-"""ctypes binding of libpicard_hip.so (include/picard_hip.h).
+"""ctypes binding of libpicard_hip.so (include/picard_hip.h, include/picard_ensemble.h).
 
 The library is built in-tree by ``__graft_entry__.build()`` (or
 ``make -C picard_amd/csrc``).  There is no fallback: if the library is
@@ -39,6 +39,15 @@ EXPORTS = (
     "picard_generate_inflow", "picard_particles_produce", "picard_set_production",
     "picard_diagnostics", "picard_advance_velocity_imp", "picard_advance_velocity_exp",
     "picard_particles_gridify_all",
+)
+
+# every symbol include/picard_ensemble.h declares
+ENSEMBLE_EXPORTS = (
+    "picard_sim_derive", "picard_sim_footprint", "picard_ensemble_create",
+    "picard_ensemble_destroy", "picard_ensemble_add", "picard_ensemble_remove",
+    "picard_ensemble_step", "picard_ensemble_status", "picard_ensemble_field_read",
+    "picard_ensemble_particles_read", "picard_ensemble_tune", "picard_ensemble_set_tile",
+    "picard_ensemble_get_tile",
 )
 
 
@@ -85,6 +94,44 @@ class Diag(ct.Structure):
 class StepStats(ct.Structure):
     _fields_ = [("num_local", ct.c_int64), ("removed", ct.c_int64), ("sweeps", ct.c_int64),
                 ("pushed", ct.c_int64)]
+
+
+class SimParams(ct.Structure):
+    _fields_ = [("nx", ct.c_int32), ("ny", ct.c_int32), ("nz", ct.c_int32), ("ppc", ct.c_int32),
+                ("dx", ct.c_double), ("density", ct.c_double), ("kelvin", ct.c_double),
+                ("drift", ct.c_double), ("B", ct.c_double * 3), ("dt_wpe", ct.c_double),
+                ("maxerr", ct.c_double), ("maxit", ct.c_int32), ("reserved", ct.c_int32),
+                ("seed", ct.c_uint64)]
+
+
+class SimDerived(ct.Structure):
+    _fields_ = [("dxyz", ct.c_double), ("dV", ct.c_double), ("lmin", ct.c_double * 3),
+                ("lmax", ct.c_double * 3), ("charge", ct.c_double * 2), ("n2p", ct.c_double * 2),
+                ("eta", ct.c_double * 2), ("ve", ct.c_double * 3), ("dt", ct.c_double),
+                ("omega_pe", ct.c_double), ("tB", (ct.c_double * 3) * 2), ("tB4", ct.c_double * 2),
+                ("sB", (ct.c_double * 3) * 2)]
+
+
+class EnsembleConfig(ct.Structure):
+    _fields_ = [("device", ct.c_int32), ("max_sims", ct.c_int32), ("max_cells", ct.c_int64),
+                ("max_particles", ct.c_int64), ("inner", ct.c_int32), ("use_graphs", ct.c_int32)]
+
+
+class EnsembleTile(ct.Structure):
+    _fields_ = [("cells_per_thread", ct.c_int32), ("threads", ct.c_int32),
+                ("blocks_per_cu", ct.c_int32), ("reserved", ct.c_int32)]
+
+
+class EnsembleStats(ct.Structure):
+    _fields_ = [("sims", ct.c_int32), ("noconv", ct.c_int32), ("particles", ct.c_int64),
+                ("pushed", ct.c_int64), ("max_sweeps", ct.c_int64), ("sum_sweeps", ct.c_int64),
+                ("solve_chunks", ct.c_int64), ("cell_sweeps", ct.c_int64), ("step_ms", ct.c_double),
+                ("solve_ms", ct.c_double)]
+
+
+class SimStatus(ct.Structure):
+    _fields_ = [("live", ct.c_int32), ("noconv", ct.c_int32), ("steps", ct.c_int64),
+                ("num_local", ct.c_int64), ("pushed", ct.c_int64), ("sweeps", ct.c_int64)]
 
 
 class PicardError(RuntimeError):
@@ -145,6 +192,21 @@ def lib():
         "picard_advance_velocity_imp": ([vp, ct.c_double, ct.c_double], ct.c_int),
         "picard_particles_gridify_all": ([vp], ct.c_int),
         "picard_advance_velocity_exp": ([vp, ct.c_double, ct.c_double], ct.c_int),
+        # ensemble (include/picard_ensemble.h)
+        "picard_sim_derive": ([ct.POINTER(SimParams), ct.POINTER(SimDerived)], ct.c_int),
+        "picard_sim_footprint": ([ct.POINTER(SimParams), lp, lp], ct.c_int),
+        "picard_ensemble_create": ([ct.POINTER(EnsembleConfig), ct.POINTER(vp)], ct.c_int),
+        "picard_ensemble_destroy": ([vp], ct.c_int),
+        "picard_ensemble_add": ([vp, ct.POINTER(SimParams), ip], ct.c_int),
+        "picard_ensemble_remove": ([vp, ct.c_int32], ct.c_int),
+        "picard_ensemble_step": ([vp, ct.POINTER(EnsembleStats)], ct.c_int),
+        "picard_ensemble_status": ([vp, ct.c_int32, ct.POINTER(SimStatus)], ct.c_int),
+        "picard_ensemble_field_read": ([vp, ct.c_int32, ct.c_int32, dp, ct.c_int64], ct.c_int),
+        "picard_ensemble_particles_read": ([vp, ct.c_int32, dp, ip, ct.c_int64], ct.c_int),
+        "picard_ensemble_tune": ([vp, ct.c_int32, ct.POINTER(EnsembleTile), dp, ct.c_int32, ip],
+                                 ct.c_int),
+        "picard_ensemble_set_tile": ([vp, ct.POINTER(EnsembleTile)], ct.c_int),
+        "picard_ensemble_get_tile": ([vp, ct.POINTER(EnsembleTile)], ct.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

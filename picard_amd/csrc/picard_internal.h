@@ -251,6 +251,10 @@ int sp_read(picard_ctx *c, int64_t first, int64_t count, double *host_rp, int32_
 // ---- scan.hip ----
 // Exclusive prefix sum of n int32 values (in may alias out); out[n] = total.
 int scan_exclusive_i32(picard_ctx *c, const int32_t *in, int32_t *out, int64_t n);
+// The same on a stream with a caller's temporary arena (n >= 1, captured into
+// graphs: no memset).
+int scan_exclusive_stream(hipStream_t st, const int32_t *in, int32_t *out, int64_t n,
+                          int32_t *tmp, int64_t tmp_len);
 int64_t scan_tmp_required(int64_t n);
 
 // 3D launch shape over the interior cells: x fastest, 256 threads per block.

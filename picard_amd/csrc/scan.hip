@@ -95,6 +95,15 @@ static int scan_level(hipStream_t st, const int32_t *in, int32_t *out, int64_t n
   return PICARD_OK;
 }
 
+int scan_exclusive_stream(hipStream_t st, const int32_t *in, int32_t *out, int64_t n,
+                          int32_t *tmp, int64_t tmp_len) {
+  if (n <= 0 || scan_tmp_required(n) > tmp_len) {
+    set_error("scan: empty input or temporary arena too small");
+    return PICARD_ERR_STATE;
+  }
+  return scan_level(st, in, out, n, tmp);
+}
+
 int scan_exclusive_i32(picard_ctx *c, const int32_t *in, int32_t *out, int64_t n) {
   if (n <= 0) {
     PICARD_HIP(hipMemsetAsync(out, 0, sizeof(int32_t), c->stream));

@@ -16,8 +16,8 @@ from tests.helpers import random_particles
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared_symbols():
-    txt = open(os.path.join(ROOT, "include", "picard_hip.h")).read()
+def _declared_symbols(header="picard_hip.h"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     return sorted(set(re.findall(r"\b(picard_[a-z0-9_]+)\s*\(", txt)))
 
 
@@ -28,6 +28,20 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(L, name), name
     assert sorted(_lib.EXPORTS) == declared
+
+
+def test_every_header_is_covered():
+    """Every header under include/ is one of the two bound ones, and every
+    symbol each declares is exported and bound."""
+    headers = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
+    assert headers == ["picard_ensemble.h", "picard_hip.h"]
+    declared = _declared_symbols("picard_ensemble.h")
+    assert len(declared) >= 12
+    L = ct.CDLL(_lib.LIB_PATH)
+    for name in declared:
+        assert hasattr(L, name), name
+    assert sorted(_lib.ENSEMBLE_EXPORTS) == declared
+    _lib.lib()                                          # every binding resolves
 
 
 def test_library_missing_fails_loudly(monkeypatch):
