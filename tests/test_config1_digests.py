@@ -15,5 +15,8 @@ def test_oracle_reproduces_first_digests(orc):
     with open(os.path.join(HERE, "golden", "config1_digests.json")) as f:
         want = json.load(f)["steps"]
     _, got = make_config1_digests.run(3)
-    assert got == want[:3]
+    for g, w in zip(got, want[:3]):
+        # U's 2-norm is informational: its last bit follows the BLAS summation order
+        assert abs(g.pop("U_2") - w["U_2"]) <= 1e-12 * abs(w["U_2"])
+        assert g == {k: v for k, v in w.items() if k != "U_2"}
     assert [w["step"] for w in want] == list(range(1, 101))
