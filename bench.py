@@ -60,7 +60,7 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
 
 
-def run_form(cfg, precision, steps, warmup, capture_after=1):
+def run_form(cfg, precision, steps, warmup, capture_after=1, order_free=False):
     """W warmup + K timed steps of one form; returns timings (whole and per
     step, with the solve phase's wall time per step from HIP events) and the
     fields after step `capture_after` (read outside the timed region)."""
@@ -69,9 +69,10 @@ def run_form(cfg, precision, steps, warmup, capture_after=1):
     ncell = cfg.nx * cfg.ny * cfg.nz
     per_cell = int(d.ppc[:cfg.species].sum())
     cap = int(1.12 * per_cell * ncell) + 64 * per_cell * (cfg.ny + 2) * (cfg.nz + 2) + (1 << 20)
-    name = "split" if precision == _lib.PICARD_FP32_SPLIT else "fp64"
+    name = ("split" if precision == _lib.PICARD_FP32_SPLIT else
+            "fp64_order_free" if order_free else "fp64")
     t0 = time.perf_counter()
-    dom = host.Domain(cfg, max_particles=cap, d=d, precision=precision)
+    dom = host.Domain(cfg, max_particles=cap, d=d, precision=precision, order_free=order_free)
     dom.generate_initial(SEED)
     dom.calc_distance()
     dom.set_production(SEED)
@@ -272,6 +273,99 @@ def error_report(cfg, steps):
                       "per_step": per_step, "trajectories": traj}), flush=True)
 
 
+def ensemble_bench(count, steps, warmup, tune_sweeps=32, baseline=True, seed=20261210,
+                   graphs=True):
+    """The 64-simulation ensemble: grids log-uniform in 16^3-96^3, 4-32
+    particles per cell per species, plasma drawn per simulation, stepped
+    together in one set of launches per phase (FP32-split form), the solve
+    tile shape from the autotuner; W warmup and K timed steps.  Baseline: the
+    same simulations one at a time in the single-domain FP32-split form (each
+    one's second step timed after its first), summed."""
+    from picard_amd import _lib, ensemble, host
+    specs = ensemble.draw_ensemble(count, seed=seed)
+    t0 = time.perf_counter()
+    ens = ensemble.Ensemble.sized_for(specs, steps_room=steps + warmup + 4, graphs=graphs)
+    slots = [ens.add(s) for s in specs]
+    log(f"ensemble: {count} simulations, {sum(s.cells for s in specs)} cells, "
+        f"{sum(ens.status(q).num_local for q in slots)} particles, setup "
+        f"{time.perf_counter() - t0:.1f} s")
+    tile, times = ens.tune(tune_sweeps)
+    log(f"ensemble: tuner per-sweep us {np.round(times, 2).tolist()} -> {tile.cells_per_thread} "
+        f"cells/thread, {tile.blocks_per_cu} blocks/CU")
+    for w in range(warmup):
+        st = ens.step()
+        log(f"ensemble: warmup step {w + 1}: {st.particles} particles, sweeps max {st.max_sweeps} "
+            f"sum {st.sum_sweeps}, {st.step_ms:.0f} ms (solve {st.solve_ms:.0f})")
+    pushed, cell_sweeps, solve_ms, per = 0, 0, 0.0, []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        st = ens.step()
+        pushed += st.pushed
+        cell_sweeps += st.cell_sweeps
+        solve_ms += st.solve_ms
+        per.append(dict(step_ms=st.step_ms, solve_ms=st.solve_ms, max_sweeps=st.max_sweeps,
+                        sum_sweeps=st.sum_sweeps, chunks=st.solve_chunks, noconv=st.noconv))
+        log(f"ensemble: timed step {warmup + k + 1}: {st.particles} particles, sweeps max "
+            f"{st.max_sweeps} sum {st.sum_sweeps}, {st.step_ms:.0f} ms (solve {st.solve_ms:.0f})")
+    el = time.perf_counter() - t0
+    ens.close()
+    out = dict(value=pushed / el, seconds=el, steps=steps, warmup=warmup, per_step=per,
+               tile=dict(cells_per_thread=tile.cells_per_thread, blocks_per_cu=tile.blocks_per_cu),
+               tuner_us_per_sweep=[float(v) for v in times],
+               solve_fraction=solve_ms / 1e3 / el,
+               solve_sweep_bytes_per_s=12.0 * cell_sweeps / (solve_ms / 1e3) if solve_ms else 0.0,
+               sims=count, cells=int(sum(s.cells for s in specs)))
+    if baseline:
+        tot_ps, tot_s = 0, 0.0
+        for q, s in enumerate(specs):
+            hx = [0.5 * s.dx * n for n in (s.nx, s.ny, s.nz)]
+            cfg = host.Config(nx=s.nx, ny=s.ny, nz=s.nz, species=2, flow=(True, False, False),
+                              gmin=tuple(-h for h in hx), gmax=tuple(hx),
+                              density=(0.0, s.density, 0, 0, 0, 0, 0, 0),
+                              ppc=(0, s.ppc, 0, 0, 0, 0, 0, 0),
+                              kelvin=(s.kelvin,) * 2 + (0.0,) * 6, v0=(s.drift, 0.0, 0.0),
+                              B0=tuple(s.B))
+            d = host.derive(cfg)
+            dom = host.Domain(cfg, d=d, precision=_lib.PICARD_FP32_SPLIT)
+            dom.generate_initial(s.seed)
+            dom.calc_distance()
+            dom.generate_inflow(s.seed, 1)
+            dom.step()
+            dom.synchronize()
+            ts = time.perf_counter()
+            dom.generate_inflow(s.seed, 2)
+            st = dom.step()
+            dom.synchronize()
+            tot_s += time.perf_counter() - ts
+            tot_ps += st.pushed
+            dom.close()
+            if q % 16 == 15:
+                log(f"ensemble baseline: {q + 1} simulations one at a time, {tot_s:.2f} s")
+        out["one_at_a_time"] = dict(value=tot_ps / tot_s, seconds=tot_s,
+                                    how="each simulation alone in the single-domain FP32-split "
+                                        "form, its step 2 timed after its step 1, summed")
+        out["batched_over_one_at_a_time"] = out["value"] / out["one_at_a_time"]["value"]
+    return out
+
+
+def server_bench(rate, jobs, scale_steps, nmin, nmax, time_limit, seed=20261211):
+    """The ensemble server on a seeded Poisson stream of jobs (rate jobs/s),
+    grids log-uniform in nmin^3-nmax^3, 8-32 particles per cell, 200-2000
+    steps scaled by `scale_steps`, admitted by a 250 GB HBM budget."""
+    from picard_amd import server
+    js = server.draw_jobs(rate, jobs, seed=seed, nmin=nmin, nmax=nmax,
+                          steps=(max(1, int(200 * scale_steps)), max(1, int(2000 * scale_steps))))
+    srv = server.Server(tune_sweeps=32)
+    rep = srv.run(js, time_limit=time_limit, log=log)
+    srv.close()
+    done = rep.pop("done")
+    rep["offered_gpu_s_per_s_estimate"] = server.offered_load(js, sweeps_per_n2=6.0)
+    rep["jobs"] = jobs
+    rep["rate_jobs_per_s"] = rate
+    rep["latencies_s"] = sorted(round(j.latency, 3) for j in done)
+    return rep
+
+
 def copy_calibration():
     """Counter calibration: three device copies of 1 GiB (read 1 GiB, write
     1 GiB each), the plain copy kernel the HBM byte counters are scaled by."""
@@ -307,6 +401,8 @@ def main():
                     help="timed steps of the FP64 form, after --fp64-warmup untimed ones (an "
                          "FP64 step at 256^3 takes ~23 s, a split one ~4 s)")
     ap.add_argument("--fp64-warmup", type=int, default=1)
+    ap.add_argument("--no-order-free", action="store_true",
+                    help="skip the order-free FP64 form (fixed-point deposit, FFT face sums)")
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--form", choices=("both", "split", "fp64"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -319,6 +415,20 @@ def main():
                          "errors per step and particle trajectories after the last")
     ap.add_argument("--copy-cal", action="store_true",
                     help="profiling: the 1 GiB copies the byte counters are scaled by")
+    ap.add_argument("--ensemble", type=int, default=0, metavar="SIMS",
+                    help="the batched ensemble of SIMS simulations (64 in the workload)")
+    ap.add_argument("--no-baseline", action="store_true",
+                    help="with --ensemble: skip the one-at-a-time baseline")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="with --ensemble: launch the phases directly instead of replaying graphs")
+    ap.add_argument("--server", action="store_true", help="the ensemble server")
+    ap.add_argument("--rate", type=float, default=2.0, help="--server: arrivals per second")
+    ap.add_argument("--jobs", type=int, default=40, help="--server: jobs in the stream")
+    ap.add_argument("--scale-steps", type=float, default=0.01,
+                    help="--server: scale of the 200-2000 step counts")
+    ap.add_argument("--nmax", type=int, default=160, help="--server: largest grid edge")
+    ap.add_argument("--nmin", type=int, default=32, help="--server: smallest grid edge")
+    ap.add_argument("--time-limit", type=float, default=240.0, help="--server: seconds")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the domain "
@@ -326,6 +436,33 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     from picard_amd import _lib, host
+    if args.ensemble:
+        r = ensemble_bench(args.ensemble, args.steps, args.warmup, baseline=not args.no_baseline,
+                           graphs=not args.no_graphs)
+        print(json.dumps({"metric": "particle-steps/s, batched FP32-split ensemble",
+                          "value": r.pop("value"), "unit": "particle-steps/s", "n_gpus": 1,
+                          "higher_is_better": True, "dtype": "f32+f64",
+                          "data": "synthetic (per-simulation Philox plasma and inflow)",
+                          "config": {"workload": f"{args.ensemble} simulations, grids log-uniform "
+                                                 "16^3-96^3, 4-32 ppc per species, n 1-20 cm^-3, "
+                                                 "T 1-50 eV, drift 300-700 km/s, |B| 1-10 nT "
+                                                 "random direction, dx = Debye length, x "
+                                                 "in/outflow, y/z periodic", **r}}), flush=True)
+        return
+    if args.server:
+        r = server_bench(args.rate, args.jobs, args.scale_steps, args.nmin, args.nmax,
+                         args.time_limit)
+        print(json.dumps({"metric": "p50/p99 job completion latency and particle-steps/s, "
+                                    "ensemble server",
+                          "value": r["particle_steps_per_s"], "unit": "particle-steps/s",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f32+f64",
+                          "data": "synthetic (seeded Poisson arrivals, per-job Philox plasma)",
+                          "config": {"workload": f"Poisson arrivals at {args.rate} jobs/s, "
+                                                 f"{args.jobs} jobs, grids log-uniform "
+                                                 f"{args.nmin}^3-{args.nmax}^3, 8-32 ppc, "
+                                                 f"200-2000 steps x {args.scale_steps}, 250 GB "
+                                                 "HBM admission budget", **r}}), flush=True)
+        return
     cfg = host.config2(args.grid)
     if args.copy_cal:
         copy_calibration()
@@ -342,6 +479,9 @@ def main():
     if args.form in ("both", "fp64"):
         runs["fp64"] = run_form(cfg, _lib.PICARD_FP64, args.fp64_steps if "split" in runs else
                                 args.steps, args.fp64_warmup if "split" in runs else args.warmup)
+    if args.form == "both" and not args.no_order_free:
+        runs["fp64_order_free"] = run_form(cfg, _lib.PICARD_FP64, args.fp64_steps,
+                                           args.fp64_warmup, order_free=True)
     head = runs.get("split") or runs["fp64"]
     headname = "split" if "split" in runs else "fp64"
     head_steps = len(head["per_step"])
@@ -351,7 +491,7 @@ def main():
     # FP32 sweeps per launch on rows of at most 256 cells, or wider rows with
     # x not periodic (the default inner length, 512, is even), the FP64 form
     # one pass per launch.
-    spl = {"split": 2 if cfg.nx <= 256 or cfg.flow[0] else 1, "fp64": 1}
+    spl = {"split": 2 if cfg.nx <= 256 or cfg.flow[0] else 1, "fp64": 1, "fp64_order_free": 1}
     # Bytes a launch of the dominant kernel must move: an FP32 sweep reads the
     # correction e and the FP32 residual and writes e (12 B per interior
     # cell), and a two-sweep launch keeps its first sweep's output on chip, so
@@ -381,19 +521,23 @@ def main():
         # 1.8e5 to 2.5e5 over the first 25 steps, so other steps are not
         # comparable), taken from its warmup steps where the indices fall
         # there; and the solve phase alone, per sweep
-        idx = {p["step"] for p in runs["fp64"]["per_step"]}
-        sp_same = [p for p in runs["split"]["warm_steps"] + runs["split"]["per_step"]
-                   if p["step"] in idx]
-        fp = runs["fp64"]["per_step"]
+        def same_steps(other):
+            idx = {p["step"] for p in runs[other]["per_step"]}
+            sp_same = [p for p in runs["split"]["warm_steps"] + runs["split"]["per_step"]
+                       if p["step"] in idx]
+            fp = runs[other]["per_step"]
+            if len(sp_same) != len(fp):
+                return None
+            return ((sum(p["pushed"] for p in sp_same) / sum(p["seconds"] for p in sp_same)) /
+                    (sum(p["pushed"] for p in fp) / sum(p["seconds"] for p in fp)))
+
         ratio = {"whole_step_timed": per_form["split"]["value"] / per_form["fp64"]["value"],
                  "solve_per_sweep": (per_form["fp64"]["solve_us_per_sweep"] /
-                                     max(per_form["split"]["solve_us_per_sweep"], 1e-30))}
-        if len(sp_same) == len(fp):
-            ratio["whole_step_same_steps"] = ((sum(p["pushed"] for p in sp_same) /
-                                               sum(p["seconds"] for p in sp_same)) /
-                                              (sum(p["pushed"] for p in fp) /
-                                               sum(p["seconds"] for p in fp)))
-            ratio["same_steps"] = sorted(idx)
+                                     max(per_form["split"]["solve_us_per_sweep"], 1e-30)),
+                 "whole_step_same_steps": same_steps("fp64"),
+                 "same_steps": sorted(p["step"] for p in runs["fp64"]["per_step"])}
+        if "fp64_order_free" in runs:
+            ratio["whole_step_same_steps_vs_order_free_fp64"] = same_steps("fp64_order_free")
     errors = None
     if "split" in runs and "fp64" in runs:
         a, b = runs["split"]["fields"], runs["fp64"]["fields"]

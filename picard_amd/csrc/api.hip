@@ -59,8 +59,8 @@ int check_geometry(const picard_geometry *g) {
     set_error("precision must be PICARD_FP64 or PICARD_FP32_SPLIT");
     return PICARD_ERR_ARG;
   }
-  if (g->precision == PICARD_FP64 && g->deposit != PICARD_DEPOSIT_REFERENCE_ORDER) {
-    set_error("the FP64 form deposits in reference order (PICARD_DEPOSIT_REFERENCE_ORDER)");
+  if (g->deposit != PICARD_DEPOSIT_REFERENCE_ORDER && g->deposit != PICARD_DEPOSIT_FIXED_POINT) {
+    set_error("deposit must be PICARD_DEPOSIT_REFERENCE_ORDER or PICARD_DEPOSIT_FIXED_POINT");
     return PICARD_ERR_ARG;
   }
   if (g->precision == PICARD_FP32_SPLIT && (g->nx > 16000 || g->ny > 16000 || g->nz > 16000)) {
@@ -210,6 +210,8 @@ int create_impl(const picard_geometry *geo, const picard_species *sp, picard_ctx
   } else {
     for (int q = 0; q < 8; ++q) PICARD_TRY(dalloc(&c->p.r[q], cap));
     PICARD_TRY(dalloc(&c->p.sp, cap));
+    c->order_free = geo->deposit == PICARD_DEPOSIT_FIXED_POINT;
+    if (c->order_free) PICARD_TRY(dalloc(&c->Nfix, g.S * g.G));
   }
   PICARD_TRY(dalloc(&c->U, g.G));
   PICARD_TRY(dalloc(&c->E, 3 * g.G));
@@ -530,7 +532,7 @@ int picard_calc_distance(picard_ctx *c) {
 
 int picard_bc_potentials(picard_ctx *c) {
   CTX_CHECK(c);
-  return c->split ? launch_bc_potentials_fft(c) : launch_bc_potentials(c);
+  return (c->split || c->order_free) ? launch_bc_potentials_fft(c) : launch_bc_potentials(c);
 }
 
 int picard_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) {
@@ -691,7 +693,7 @@ int picard_step(picard_ctx *c, double dt, double maxerr, int32_t maxit, const do
     PICARD_TRY(launch_outflow(c, &st.num_local, &st.removed));          // :569
     PICARD_TRY(launch_wrap(c));                                         // :572
     PICARD_TRY(launch_gridify(c));                                      // :577
-    PICARD_TRY(launch_bc_potentials(c));                                // :581
+    PICARD_TRY(c->order_free ? launch_bc_potentials_fft(c) : launch_bc_potentials(c));  // :581
     PICARD_TRY(solve_timer(c, 0));
     rc = launch_calc_epotential(c, maxerr, maxit, &st.sweeps);          // :584
     if (rc < 0) return rc;

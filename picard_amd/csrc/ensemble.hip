@@ -18,13 +18,15 @@
 // Solve.  Mixed precision per simulation: an FP64 refinement pass forms
 // u' = u + e and r = b - A u' (ping-pong FP64 buffers), then FP32 Jacobi
 // sweeps on A e = r.  Every sweep launch also evaluates the residual of the
-// iterate it reads, r - A e_j, converts each cell's square to fixed point
-// (units of the simulation's threshold^2 * 2^-40, clamped at 2^40 + 1) and
-// adds it with integer atomics, so the sum is the same for any tiling and any
-// mix of simulations; the block that completes a simulation's tiles applies
-// the stop rule of CalcEpotential.f90:60-62 (sum <= 2^40, i.e. |res| <=
-// maxerr |b|) and advances or stops that simulation.  Simulations stop
-// independently; tiles of stopped ones are skipped.
+// iterate it reads, r - A e_j, and converts each cell's square to fixed point
+// (units of the simulation's threshold^2 * 2^-40, clamped at 2^40 + 1); each
+// tile stores its integer sum, and a small launch after the sweep sums each
+// simulation's tiles (integers: the same for any tiling and any mix of
+// simulations) and applies the stop rule of CalcEpotential.f90:60-62 (sum <=
+// 2^40, i.e. |res| <= maxerr |b|), advancing or stopping that simulation.
+// Simulations stop independently; tiles of stopped ones are skipped.  (One
+// atomic per tile on a per-simulation sum and counter cost 0.76 ms per sweep
+// for 64 simulations: the tiles of one simulation met on the same address.)
 #include <hip/hip_ext.h>
 
 #include <algorithm>
@@ -67,6 +69,7 @@ struct SimDev {
   int64_t fbase;            // face-sum partials: 2 faces x fchunks x ny*nz
   int32_t kbase, nkeys;     // sort keys
   int32_t tiles, fchunks;   // solve tiles (current shape), x-plane chunks of the face sums
+  int64_t ubase;            // first entry of this slot's per-tile residual units
   int64_t pfirst;           // first particle of this slot in one step's inflow (live slots)
   int64_t inflow;           // particles one step's inflow appends for this slot
   double lmin[3], xmax;     // box lower corner, Lx_max
@@ -76,7 +79,12 @@ struct SimDev {
   double maxerr;
   int32_t maxit, pad0;
   float etadt2[2], tB[2][3], sB[2][3], ve[3], dtdx;
-  float wf, aff, i2f, pad1; // FP32 sweep: e' = wf * sum6 - aff * r;  res = r - i2f (sum6 - 6 e)
+  // FP32 sweep: e' = wf sum6 + kf e - aff r with 6 wf + kf = 1 exactly (wf =
+  // fl32(1/6), kf = 1 - 6 wf = -2^-25), so the iteration keeps the FP64
+  // Jacobi spectrum: with 6 fl32(1/6) alone both ends move by ~1e-7, which
+  // near the slowest modes' decay changes the stop by many sweeps;
+  // res = r - i2f (sum6 - 6 e)
+  float wf, kf, aff, i2f;
   uint64_t seed;
   PlasmaConst pc;
 };
@@ -86,9 +94,7 @@ struct SimState {
   int32_t ucur, stop_j, pad0, pad1;
   int64_t base, sweeps, step, next_id;
   int64_t start, count, pushed;
-  unsigned long long acc;
-  uint32_t done_tiles, pad2;
-  double thresh2, fixq;
+  double thresh2, ifixq;    // threshold^2 and 2^40 / threshold^2 (residual units per V^2/m^4)
 };
 
 struct Global {
@@ -114,6 +120,7 @@ struct PartArena {
 };
 
 struct Arenas {
+  unsigned long long *units;   // per solve tile: fixed-point residual units of the last sweep
   SimDev *sims;
   SimState *state;
   Global *g;
@@ -505,9 +512,9 @@ __global__ void __launch_bounds__(kT) k_bnorm(Arenas A) {
     const SimDev &sd = A.sims[s];
     double acc = 0.0;
     const int64_t X = sd.X, XY = (int64_t)sd.X * sd.Y;
-    for (int64_t c = threadIdx.x; c < sd.cells; c += kT) {
-      const int x = (int)(c % sd.nx), y = (int)((c / sd.nx) % sd.ny);
-      const int z = (int)(c / ((int64_t)sd.nx * sd.ny));
+    const uint32_t nx = (uint32_t)sd.nx, ny = (uint32_t)sd.ny, cells = (uint32_t)sd.cells;
+    for (uint32_t c = threadIdx.x; c < cells; c += kT) {
+      const uint32_t x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
       const double b = -A.C[sd.gbase + (x + 1) + X * (y + 1) + XY * (z + 1)] / sd.dVeps0;
       acc = acc + b * b;
     }
@@ -516,7 +523,7 @@ __global__ void __launch_bounds__(kT) k_bnorm(Arenas A) {
       SimState &st = A.state[s];
       const double th = sd.maxerr * sqrt(tot);
       st.thresh2 = th * th;
-      st.fixq = st.thresh2 > 0.0 ? st.thresh2 / (double)kStopUnits : 1e-300;
+      st.ifixq = st.thresh2 > 0.0 ? (double)kStopUnits / st.thresh2 : 1e300;
       st.j = 0;
       st.base = 0;
       st.solving = 1;
@@ -524,8 +531,6 @@ __global__ void __launch_bounds__(kT) k_bnorm(Arenas A) {
       st.have_e = 0;
       st.ucur = 0;
       st.stop_j = 0;
-      st.acc = 0;
-      st.done_tiles = 0;
     }
     __syncthreads();
   }
@@ -623,7 +628,6 @@ __global__ void __launch_bounds__(kT) k_facereduce(Arenas A) {
 // with fresh periodic images, r32 = r.  The block completing a slot's tiles
 // folds the inner count into the base and flips the buffers.
 __global__ void __launch_bounds__(kT) k_outer(Arenas A, int cpt) {
-  __shared__ int last;
   const int nt = A.g->n_cell_tiles;
   for (int t = blockIdx.x; t < nt; t += gridDim.x) {
     const Tile tl = A.cell_tiles[t];
@@ -635,22 +639,21 @@ __global__ void __launch_bounds__(kT) k_outer(Arenas A, int cpt) {
     double *un = A.u[st.ucur ^ 1] + sd.gbase;
     const float *e = A.e32[st.j & 1] + sd.cbase;
     const int64_t X = sd.X, XY = (int64_t)sd.X * sd.Y;
-    const int64_t c0 = (int64_t)tl.a + (int64_t)threadIdx.x * cpt;
+    const uint32_t nx = (uint32_t)sd.nx, ny = (uint32_t)sd.ny, nz = (uint32_t)sd.nz;
+    const uint32_t pl = nx * ny;
+    const uint32_t c0 = (uint32_t)tl.a + threadIdx.x * (uint32_t)cpt;
     for (int q = 0; q < cpt; ++q) {
-      const int64_t c = c0 + q;
-      if (c >= sd.cells) break;
-      const int x = (int)(c % sd.nx), y = (int)((c / sd.nx) % sd.ny);
-      const int z = (int)(c / ((int64_t)sd.nx * sd.ny));
+      const uint32_t c = c0 + q;
+      if (c >= (uint32_t)sd.cells) break;
+      const uint32_t x = c % nx, y = (c / nx) % ny, z = c / pl;
       const int64_t node = (x + 1) + X * (y + 1) + XY * (z + 1);
-      const int yn = y + 1 < sd.ny ? y + 1 : 0, ys = y > 0 ? y - 1 : sd.ny - 1;
-      const int zt = z + 1 < sd.nz ? z + 1 : 0, zb = z > 0 ? z - 1 : sd.nz - 1;
+      const uint32_t yn = y + 1 < ny ? y + 1 : 0, ys = y > 0 ? y - 1 : ny - 1;
+      const uint32_t zt = z + 1 < nz ? z + 1 : 0, zb = z > 0 ? z - 1 : nz - 1;
       const int64_t nN = (x + 1) + X * (yn + 1) + XY * (z + 1), nS = (x + 1) + X * (ys + 1) + XY * (z + 1);
       const int64_t nT = (x + 1) + X * (y + 1) + XY * (zt + 1), nB = (x + 1) + X * (y + 1) + XY * (zb + 1);
-      const int64_t cE = c + 1, cW = c - 1;
-      const int64_t cN = x + (int64_t)sd.nx * (yn + (int64_t)sd.ny * z);
-      const int64_t cS = x + (int64_t)sd.nx * (ys + (int64_t)sd.ny * z);
-      const int64_t cT = x + (int64_t)sd.nx * (y + (int64_t)sd.ny * zt);
-      const int64_t cB = x + (int64_t)sd.nx * (y + (int64_t)sd.ny * zb);
+      const uint32_t cE = c + 1, cW = c - 1;
+      const uint32_t cN = x + nx * yn + pl * z, cS = x + nx * ys + pl * z;
+      const uint32_t cT = x + nx * y + pl * zt, cB = x + nx * y + pl * zb;
       double uC = uc[node], uE = uc[node + 1], uW = uc[node - 1];
       double uN = uc[nN], uS = uc[nS], uT = uc[nT], uB = uc[nB];
       if (upd) {
@@ -667,24 +670,22 @@ __global__ void __launch_bounds__(kT) k_outer(Arenas A, int cpt) {
       const double lap = ((uE + uW - 2.0 * uC) + (uN + uS - 2.0 * uC) + (uT + uB - 2.0 * uC)) * sd.i2;
       A.r32[sd.cbase + c] = (float)(b - lap);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      __threadfence();
-      const uint32_t old = atomicAdd(&st.done_tiles, 1u);
-      last = old + 1 == (uint32_t)sd.tiles;
+  }
+}
+
+// After a refinement pass, per solving slot: the inner count folds into the
+// base and the FP64 buffers flip.
+__global__ void k_outer_done(Arenas A) {
+  const int nl = A.g->n_live;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nl; q += gridDim.x * blockDim.x) {
+    SimState &st = A.state[A.live[q]];
+    if (!st.solving) continue;
+    if (st.have_e) {
+      st.ucur ^= 1;
+      st.base += st.j;
     }
-    __syncthreads();
-    if (last && threadIdx.x == 0) {
-      __threadfence();
-      if (st.have_e) {
-        st.ucur ^= 1;
-        st.base += st.j;
-      }
-      st.have_e = 1;
-      st.j = 0;
-      st.done_tiles = 0;
-      __threadfence();
-    }
+    st.have_e = 1;
+    st.j = 0;
   }
 }
 
@@ -693,7 +694,6 @@ __global__ void __launch_bounds__(kT) k_outer(Arenas A, int cpt) {
 template <int CPT, bool TUNE>
 __global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner) {
   __shared__ unsigned long long wsum[kT / 64];
-  __shared__ int last;
   const int nt = A.g->n_cell_tiles;
   for (int t = blockIdx.x; t < nt; t += gridDim.x) {
     const Tile tl = A.cell_tiles[t];
@@ -704,55 +704,79 @@ __global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner) {
     const float *ein = A.e32[j & 1] + sd.cbase;
     float *eout = A.e32[(j + 1) & 1] + sd.cbase;
     const float *r = A.r32 + sd.cbase;
-    const double fixq = st.fixq;
+    const double ifixq = st.ifixq;
     unsigned long long units = 0;
-    const int64_t c0 = (int64_t)tl.a + (int64_t)threadIdx.x * CPT;
+    // the thread's first cell, then its neighbours along the row (32-bit
+    // index arithmetic: a simulation holds fewer than 2^31 cells)
+    const uint32_t nx = (uint32_t)sd.nx, ny = (uint32_t)sd.ny, nz = (uint32_t)sd.nz;
+    const uint32_t cells = (uint32_t)sd.cells, pl = nx * ny;
+    uint32_t c = (uint32_t)tl.a + threadIdx.x * CPT;
+    uint32_t x = c % nx, y = (c / nx) % ny, z = c / pl;
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
-      const int64_t c = c0 + q;
-      if (c < sd.cells) {
-        const int x = (int)(c % sd.nx), y = (int)((c / sd.nx) % sd.ny);
-        const int z = (int)(c / ((int64_t)sd.nx * sd.ny));
+      if (c < cells) {
         const float rr = r[c];
         float sum6 = 0.0f, ec = 0.0f;
         if (j > 0) {
-          const int yn = y + 1 < sd.ny ? y + 1 : 0, ys = y > 0 ? y - 1 : sd.ny - 1;
-          const int zt = z + 1 < sd.nz ? z + 1 : 0, zb = z > 0 ? z - 1 : sd.nz - 1;
-          const float eE = x + 1 < sd.nx ? ein[c + 1] : 0.0f;
+          const uint32_t cn = y + 1 < ny ? c + nx : c - (ny - 1) * nx;
+          const uint32_t cs = y > 0 ? c - nx : c + (ny - 1) * nx;
+          const uint32_t ct = z + 1 < nz ? c + pl : c - (nz - 1) * pl;
+          const uint32_t cb = z > 0 ? c - pl : c + (nz - 1) * pl;
+          const float eE = x + 1 < nx ? ein[c + 1] : 0.0f;
           const float eW = x > 0 ? ein[c - 1] : 0.0f;
-          const float eN = ein[x + (int64_t)sd.nx * (yn + (int64_t)sd.ny * z)];
-          const float eS = ein[x + (int64_t)sd.nx * (ys + (int64_t)sd.ny * z)];
-          const float eT = ein[x + (int64_t)sd.nx * (y + (int64_t)sd.ny * zt)];
-          const float eB = ein[x + (int64_t)sd.nx * (y + (int64_t)sd.ny * zb)];
+          const float eN = ein[cn], eS = ein[cs], eT = ein[ct], eB = ein[cb];
           ec = ein[c];
           sum6 = ((eE + eW) + (eN + eS)) + (eT + eB);
         }
-        eout[c] = __builtin_fmaf(sd.wf, sum6, -sd.aff * rr);
+        eout[c] = __builtin_fmaf(sd.wf, sum6, __builtin_fmaf(sd.kf, ec, -sd.aff * rr));
         const float res = rr - sd.i2f * (sum6 - 6.0f * ec);
-        const double u2 = ((double)res * (double)res) / fixq;
+        const double u2 = ((double)res * (double)res) * ifixq;
         units += u2 < (double)(kStopUnits + 1) ? (unsigned long long)u2 : kStopUnits + 1;
       }
+      ++c;
+      if (++x == nx) {
+        x = 0;
+        if (++y == ny) {
+          y = 0;
+          ++z;
+        }
+      }
     }
-    if (TUNE) continue;
-    // wave, then block sum of the integer units: one atomic per tile
+    // wave, then block sum of the integer units, stored for the tile
     for (int off = 32; off > 0; off >>= 1) units += __shfl_down(units, off, 64);
     if (lane_id() == 0) wsum[threadIdx.x >> 6] = units;
     __syncthreads();
     if (threadIdx.x == 0) {
-      unsigned long long s = 0;
-      for (int w = 0; w < kT / 64; ++w) s += wsum[w];
-      atomicAdd(&st.acc, s);
-      __threadfence();
-      const uint32_t old = atomicAdd(&st.done_tiles, 1u);
-      last = old + 1 == (uint32_t)sd.tiles;
+      unsigned long long sum = 0;
+      for (int w = 0; w < kT / 64; ++w) sum += wsum[w];
+      A.units[sd.ubase + tl.b] = sum;
     }
     __syncthreads();
-    if (last && threadIdx.x == 0) {
-      __threadfence();
-      const unsigned long long acc = atomicAdd(&st.acc, 0ull);
+  }
+}
+
+// After a sweep, per solving slot (one block each): the residual of iterate
+// base + j from its tiles' units, and the stop rule.
+__global__ void __launch_bounds__(kT) k_decide(Arenas A, int inner) {
+  __shared__ unsigned long long wsum[kT / 64];
+  const int nl = A.g->n_live;
+  for (int q = blockIdx.x; q < nl; q += gridDim.x) {
+    const int s = A.live[q];
+    const SimDev &sd = A.sims[s];
+    SimState &st = A.state[s];
+    if (!st.solving || st.j >= inner) continue;       // block-uniform
+    unsigned long long acc = 0;
+    for (int t = threadIdx.x; t < sd.tiles; t += kT) acc += A.units[sd.ubase + t];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane_id() == 0) wsum[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long tot = 0;
+      for (int w = 0; w < kT / 64; ++w) tot += wsum[w];
+      const int j = st.j;
       const int64_t it = st.base + j;
       const int64_t cap = 2 * (int64_t)sd.maxit + 1;
-      if (acc <= kStopUnits) {                   // |res| <= maxerr |b| (CalcEpotential.f90:60-62)
+      if (tot <= kStopUnits) {                   // |res| <= maxerr |b| (CalcEpotential.f90:60-62)
         st.solving = 0;
         st.sweeps = it;
         st.stop_j = j;
@@ -764,9 +788,6 @@ __global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner) {
       } else {
         st.j = j + 1;
       }
-      st.acc = 0;
-      st.done_tiles = 0;
-      __threadfence();
     }
     __syncthreads();
   }
@@ -1084,6 +1105,7 @@ static int derive_sim(const picard_sim_params *p, picard_sim_derived *d) {
     return PICARD_ERR_ARG;
   }
   if (p->nx < 8 || p->ny < 8 || p->nz < 8 || p->nx > 4096 || p->ny > 4096 || p->nz > 4096 ||
+      (int64_t)p->nx * p->ny * p->nz > ((int64_t)1 << 30) ||
       p->ppc < 1 || p->ppc > 1024 || !(p->dx > 0.0) || !(p->density > 0.0) ||
       !(p->kelvin >= 0.0) || !(p->dt_wpe > 0.0) || !(p->maxerr > 0.0) || p->maxit < 1) {
     set_error("picard_sim_derive: parameters out of range");
@@ -1182,6 +1204,7 @@ static SimDev make_simdev(const SlotHost &h, int slot) {
   s.i2 = 1.0 / (d.dxyz * d.dxyz);
   // cubic cells: ax = ay = az = 1/6, af = dx^2 / 6 (CalcEpotential.f90:44-47)
   s.wf = 1.0f / 6.0f;
+  s.kf = (float)(1.0 - 6.0 * (double)s.wf);
   s.aff = (float)(d.dxyz * d.dxyz / 6.0);
   s.i2f = (float)s.i2;
   s.maxerr = p.maxerr;
@@ -1216,7 +1239,7 @@ static int rebuild(picard_ensemble *e) {
   std::vector<Tile> node_t, cell_t, face_t, fred_t, imap;
   std::vector<int32_t> live;
   const int64_t tcells = (int64_t)e->tile.threads * e->tile.cells_per_thread;
-  int64_t pfirst = 0, ions = 0;
+  int64_t pfirst = 0, ions = 0, ubase = 0;
   for (int s = 0; s < S; ++s) {
     const SlotHost &h = e->slots[s];
     sims[s] = make_simdev(h, s);
@@ -1229,10 +1252,13 @@ static int rebuild(picard_ensemble *e) {
     // block b (on XCD b % 8) works on one share of every slot
     const int64_t nt = (sd.cells + tcells - 1) / tcells;
     sd.tiles = (int32_t)nt;
+    sd.ubase = ubase;
+    ubase += nt;
     std::vector<std::vector<Tile>> share(8);
     for (int x = 0; x < 8; ++x) {
       const int64_t t0 = nt * x / 8, t1 = nt * (x + 1) / 8;
-      for (int64_t t = t0; t < t1; ++t) share[x].push_back({s, (int32_t)(t * tcells), 0, 0});
+      for (int64_t t = t0; t < t1; ++t)
+        share[x].push_back({s, (int32_t)(t * tcells), (int32_t)t, 0});
     }
     size_t mx = 0;
     for (auto &v : share) mx = std::max(mx, v.size());
@@ -1315,8 +1341,12 @@ static int launch_pre(picard_ensemble *e, int cur) {
 template <int CPT>
 static void launch_sweep(picard_ensemble *e, bool tune) {
   const int blocks = 8 * ((e->tile.blocks_per_cu * e->cus + 7) / 8);
-  if (tune) k_sweep<CPT, true><<<blocks, kT, 0, e->stream>>>(e->A, e->inner);
-  else k_sweep<CPT, false><<<blocks, kT, 0, e->stream>>>(e->A, e->inner);
+  if (tune) {
+    k_sweep<CPT, true><<<blocks, kT, 0, e->stream>>>(e->A, e->inner);
+  } else {
+    k_sweep<CPT, false><<<blocks, kT, 0, e->stream>>>(e->A, e->inner);
+    k_decide<<<e->cus, kT, 0, e->stream>>>(e->A, e->inner);
+  }
 }
 
 static int launch_sweeps(picard_ensemble *e, int count, bool tune) {
@@ -1333,7 +1363,10 @@ static int launch_sweeps(picard_ensemble *e, int count, bool tune) {
 
 static int launch_chunk(picard_ensemble *e, bool with_outer) {
   const int blocks = 8 * ((e->tile.blocks_per_cu * e->cus + 7) / 8);
-  if (with_outer) k_outer<<<blocks, kT, 0, e->stream>>>(e->A, e->tile.cells_per_thread);
+  if (with_outer) {
+    k_outer<<<blocks, kT, 0, e->stream>>>(e->A, e->tile.cells_per_thread);
+    k_outer_done<<<(kMaxSlots + kT - 1) / kT, kT, 0, e->stream>>>(e->A);
+  }
   PICARD_TRY(launch_sweeps(e, kChunkSweeps, false));
   k_count_active<<<1, kT, 0, e->stream>>>(e->A, e->host_active);
   PICARD_HIP(hipGetLastError());
@@ -1485,6 +1518,7 @@ int picard_ensemble_create(const picard_ensemble_config *cfg, picard_ensemble **
   e->cap_fred_t = e->max_face / kT + 2 * S;
   PICARD_TRY(dalloc_n(e, &A.node_tiles, e->cap_node_t));
   PICARD_TRY(dalloc_n(e, &A.cell_tiles, e->cap_cell_t));
+  PICARD_TRY(dalloc_n(e, &A.units, e->cap_cell_t));
   PICARD_TRY(dalloc_n(e, &A.face_tiles, e->cap_face_t));
   PICARD_TRY(dalloc_n(e, &A.fred_tiles, e->cap_fred_t));
   PICARD_TRY(dalloc_n(e, &A.U, e->max_nodes));

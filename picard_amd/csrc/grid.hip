@@ -1004,11 +1004,99 @@ __global__ void __launch_bounds__(kT) k_bc_fields(GridDims g, double *__restrict
   }
 }
 
+// Order-free FP64 deposit (PICARD_DEPOSIT_FIXED_POINT): each particle's
+// weights F1..F8 of ParticlesGridify.f90:81-88 in FP64 as the reference forms
+// them, rounded to int64 fixed point (2^-48 units) and added with integer
+// atomics, so N does not depend on the order of the additions; within
+// 2^-49 per weight of the sequential sum.
+constexpr double kFix48 = 281474976710656.0;          // 2^48
+constexpr double kFix48Inv = 1.0 / 281474976710656.0;
+
+__global__ void __launch_bounds__(kT) k_dep_fix64(ParticlesF64 p, int64_t n, GridDims g,
+                                                  unsigned long long *__restrict__ Nfix,
+                                                  int32_t *__restrict__ errors) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= n) return;
+  const CicLoc L = cic_locate(g, p.r[0][q], p.r[1][q], p.r[2][q]);
+  const int s0 = p.sp[q] - 1;
+  if (!cic_in_bounds(g, L) || s0 < 0 || s0 >= g.S) {
+    atomicAdd(errors, 1);
+    return;
+  }
+  const int64_t base = node_index(g, L.idx[0], L.idx[1], L.idx[2]);
+#pragma unroll
+  for (int o = 0; o < 8; ++o) {
+    const double fx = (o & 1) ? L.rest[0] : L.drest[0];
+    const double fy = (o & 2) ? L.rest[1] : L.drest[1];
+    const double fz = (o & 4) ? L.rest[2] : L.drest[2];
+    const double F = fx * fy * fz / g.dV;
+    const int64_t node = base + (o & 1) + (int64_t)g.X * ((o >> 1) & 1) +
+                         (int64_t)g.X * g.Y * (o >> 2);
+    atomicAdd(&Nfix[node * g.S + s0], (unsigned long long)llrint(F * kFix48));
+  }
+}
+
+// Integer guard folds of the fixed-point sums, then N = W 2^-48 n2p and
+// C = sum_s N charge (ParticlesGridify.f90:129-293).
+__global__ void __launch_bounds__(kT) k_fold_fix64(GridDims g,
+                                                   const unsigned long long *__restrict__ Nfix,
+                                                   double *__restrict__ N, double *__restrict__ C,
+                                                   const double *consts) {
+  const int64_t node = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (node >= g.G) return;
+  const int i = (int)(node % g.X) + 1;
+  const int j = (int)((node / g.X) % g.Y) + 1;
+  const int k = (int)(node / ((int64_t)g.X * g.Y)) + 1;
+  int sx[2], sy[2], sz[2];
+  bool zx, zy, zz;
+  const int nxs = fold_src(i, g.nx, g.flow[0], sx, &zx);
+  const int nys = fold_src(j, g.ny, g.flow[1], sy, &zy);
+  const int nzs = fold_src(k, g.nz, g.flow[2], sz, &zz);
+  double cacc = 0.0;
+  for (int s = 0; s < g.S; ++s) {
+    unsigned long long w = 0;
+    if (!zx && !zy && !zz)
+      for (int a = 0; a < nxs; ++a)
+        for (int b = 0; b < nys; ++b)
+          for (int c = 0; c < nzs; ++c) w += Nfix[node_index(g, sx[a], sy[b], sz[c]) * g.S + s];
+    const double v = (double)w * kFix48Inv * consts[s];
+    N[node * g.S + s] = v;
+    cacc = cacc + v * consts[8 + s];
+  }
+  C[node] = cacc;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q < n) p[q] = 0ull;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 
+static int launch_gridify_fixed(picard_ctx *c) {
+  const GridDims &g = c->gd;
+  const int64_t n = c->n;
+  PICARD_HIP(hipMemsetAsync(c->s.counters, 0, sizeof(int32_t), c->stream));
+  k_zero_u64<<<grid_blocks((int64_t)g.S * g.G, kT), kT, 0, c->stream>>>(c->Nfix, (int64_t)g.S * g.G);
+  if (n > 0)
+    k_dep_fix64<<<grid_blocks(n, kT), kT, 0, c->stream>>>(c->p, n, g, c->Nfix, c->s.counters);
+  k_fold_fix64<<<grid_blocks(g.G, kT), kT, 0, c->stream>>>(g, c->Nfix, c->N, c->C, c->consts);
+  PICARD_HIP(hipGetLastError());
+  PICARD_HIP(hipMemcpyAsync(c->host_counters, c->s.counters, sizeof(int32_t),
+                            hipMemcpyDeviceToHost, c->stream));
+  PICARD_HIP(hipStreamSynchronize(c->stream));
+  if (c->host_counters[0] != 0) {
+    set_error("ParticlesGridify: " + std::to_string(c->host_counters[0]) +
+              " particle(s) outside the grid arrays were not deposited");
+    return PICARD_ERR_RANGE;
+  }
+  return PICARD_OK;
+}
+
 int launch_gridify(picard_ctx *c, bool flux) {
+  if (!flux && c->order_free) return launch_gridify_fixed(c);
   const GridDims &g = c->gd;
   const int64_t n = c->n;
   const int64_t nb = (int64_t)g.S * c->K;

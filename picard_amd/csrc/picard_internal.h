@@ -119,6 +119,9 @@ struct picard_ctx {
   int64_t n = 0;             // num_local (authoritative on the host)
   picard::ParticlesF64 p{};
   bool split = false;        // PICARD_FP32_SPLIT
+  // FP64 form with PICARD_DEPOSIT_FIXED_POINT: fixed-point deposit and FFT
+  // face sums (order-free, within 1e-12 of the reference order)
+  bool order_free = false;
   picard::ParticlesSplit ps{}, ps2{};
   unsigned long long *Nfix = nullptr;   // split: fixed-point densities
   float *E32 = nullptr;      // split: E as float4 per node

@@ -395,7 +395,7 @@ def bc_inflow(cfg: Config, d: Derived, rng):
 # ---------------------------------------------------------------------------
 
 def geometry(cfg: Config, d: Derived, max_particles: int, device: int = 0,
-             precision: int = _lib.PICARD_FP64) -> _lib.Geometry:
+             precision: int = _lib.PICARD_FP64, order_free: bool = False) -> _lib.Geometry:
     g = _lib.Geometry()
     g.nx, g.ny, g.nz, g.species = cfg.nx, cfg.ny, cfg.nz, cfg.species
     for c in range(3):
@@ -404,7 +404,8 @@ def geometry(cfg: Config, d: Derived, max_particles: int, device: int = 0,
         g.lmax[c] = g.gmax[c] = cfg.gmax[c]
         g.dxyz[c] = d.dxyz[c]
     g.precision = precision
-    g.deposit = _lib.PICARD_DEPOSIT_REFERENCE_ORDER
+    g.deposit = (_lib.PICARD_DEPOSIT_FIXED_POINT if order_free
+                 else _lib.PICARD_DEPOSIT_REFERENCE_ORDER)
     g.device = device
     g.max_particles = max_particles
     return g
@@ -465,13 +466,17 @@ class Domain:
     """One device-resident domain; methods follow the reference's subroutines."""
 
     def __init__(self, cfg: Config, max_particles: int | None = None, device: int = 0,
-                 d: Derived | None = None, precision: int = _lib.PICARD_FP64):
+                 d: Derived | None = None, precision: int = _lib.PICARD_FP64,
+                 order_free: bool = False):
+        """order_free: the FP64 form with the fixed-point deposit and the FFT
+        face sums (PICARD_DEPOSIT_FIXED_POINT) instead of the reference's
+        summation orders."""
         self.cfg = cfg
         self.d = derive(cfg) if d is None else d
         if max_particles is None:
             per_cell = int(self.d.ppc[: cfg.species].sum())
             max_particles = int(1.5 * per_cell * (cfg.nx + 2) * (cfg.ny + 2) * (cfg.nz + 2)) + 1024
-        self.geo = geometry(cfg, self.d, max_particles, device, precision)
+        self.geo = geometry(cfg, self.d, max_particles, device, precision, order_free)
         self.precision = precision
         self.spc = species_struct(cfg, self.d)
         h = ct.c_void_p()
