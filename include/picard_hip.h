@@ -99,6 +99,17 @@ int picard_particles_read(picard_ctx *ctx, int64_t first, int64_t count, double 
                           int32_t *int_particles);
 int picard_particles_count(picard_ctx *ctx, int64_t *num_local);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid hipMalloc):
+ * picard_particles_read staged through a fixed 16 MiB device array, one per
+ * GPU, instead of a staging buffer the library allocates.  At most
+ * PICARD_READ_NOALLOC_MAX particles per call (larger counts return
+ * PICARD_ERR_ARG; read in pieces with `first`); sp may be NULL.  Every call
+ * on a GPU shares that GPU's array: calls that overlap on one GPU race. */
+#define PICARD_READ_NOALLOC_MAX 246723
+int picard_particles_read_noalloc(picard_ctx *ctx, int64_t first, int64_t count, double *rp,
+                                  int32_t *sp);
+
 /* Device-side particle sources, appended after num_local.  They replace the
  * host loops of ParticlesInitialize (ParticlesInitialize.f90:2) and BCinflow
  * (BCinflow.f90:2) with the same cell ranges, loop order and distributions,

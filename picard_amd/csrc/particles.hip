@@ -432,6 +432,14 @@ int launch_append(picard_ctx *c, const double *host_rp, const int32_t *host_sp, 
   return PICARD_OK;
 }
 
+int stage_particles(picard_ctx *c, int64_t first, int64_t count, double *stage, int32_t *stage_sp) {
+  if (count <= 0) return PICARD_OK;
+  if (c->split) return sp_stage(c, first, count, stage, stage_sp);
+  k_soa_to_aos<<<grid_blocks(count, kT), kT, 0, c->stream>>>(c->p, first, count, stage, stage_sp);
+  PICARD_HIP(hipGetLastError());
+  return PICARD_OK;
+}
+
 int launch_read(picard_ctx *c, int64_t first, int64_t count, double *host_rp, int32_t *host_sp) {
   if (count <= 0) return PICARD_OK;
   if (first < 0 || first + count > c->n) {

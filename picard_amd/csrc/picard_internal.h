@@ -211,6 +211,9 @@ int launch_wrap(picard_ctx *c);
 int launch_boris(picard_ctx *c, double dt, int mode = 0, double deltat = 0.0);
 int launch_append(picard_ctx *c, const double *host_rp, const int32_t *host_sp, int64_t count);
 int launch_read(picard_ctx *c, int64_t first, int64_t count, double *host_rp, int32_t *host_sp);
+// particles [first, first + count) as real_particles(8, n) / int_particles(n)
+// into device buffers (either form)
+int stage_particles(picard_ctx *c, int64_t first, int64_t count, double *stage, int32_t *stage_sp);
 
 int ensure_stage(picard_ctx *c, int64_t want);
 
@@ -250,6 +253,7 @@ int sp_widen_e(picard_ctx *c);
 int sp_boris(picard_ctx *c, double dt, int mode = 0, double deltat = 0.0);
 int sp_append(picard_ctx *c, const double *host_rp, const int32_t *host_sp, int64_t count);
 int sp_read(picard_ctx *c, int64_t first, int64_t count, double *host_rp, int32_t *host_sp);
+int sp_stage(picard_ctx *c, int64_t first, int64_t count, double *stage, int32_t *stage_sp);
 
 // ---- scan.hip ----
 // Exclusive prefix sum of n int32 values (in may alias out); out[n] = total.

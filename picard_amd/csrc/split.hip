@@ -1866,6 +1866,14 @@ int sp_append(picard_ctx *c, const double *host_rp, const int32_t *host_sp, int6
   return PICARD_OK;
 }
 
+int sp_stage(picard_ctx *c, int64_t first, int64_t count, double *stage, int32_t *stage_sp) {
+  if (count <= 0) return PICARD_OK;
+  k_sp_to_aos<<<grid_blocks(count, kT), kT, 0, c->stream>>>(c->ps, first, count, c->gd, stage,
+                                                            stage_sp);
+  PICARD_HIP(hipGetLastError());
+  return PICARD_OK;
+}
+
 int sp_read(picard_ctx *c, int64_t first, int64_t count, double *host_rp, int32_t *host_sp) {
   if (count <= 0) return PICARD_OK;
   if (first < 0 || first + count > c->n) {
