@@ -31,8 +31,9 @@ form's error against the FP64 form after the first step (phi, E, densities:
 normwise relative).
 
 Prints one JSON line with the roofline of the dominant kernel (the split
-form's FP32 Jacobi sweep, timed live with per-launch HIP events) and a CPU
-baseline of the oracle on a bounded sample."""
+form's FP32 Jacobi sweep, timed live with per-launch HIP events), the rate a
+plain stream with the same byte mix and array size reaches beside it, and a
+CPU baseline of the oracle on a bounded sample."""
 from __future__ import annotations
 
 import argparse
@@ -368,15 +369,79 @@ def server_bench(rate, jobs, scale_steps, nmin, nmax, time_limit, seed=20261211)
 
 def copy_calibration():
     """Counter calibration: three device copies of 1 GiB (read 1 GiB, write
-    1 GiB each), the plain copy kernel the HBM byte counters are scaled by."""
+    1 GiB each), the plain copy kernel the HBM byte counters are scaled by.
+    Then the streaming rates the solve kernels are held against: torch's copy
+    (read 1, write 1) and add (read 2, write 1, the byte mix of an FP32 sweep)
+    on float32 arrays of the 256^3 solve's size (67 MB each, all three fit
+    the 256 MB last-level cache) and of 1 GiB each (they do not), timed with
+    events over 20 launches after 3 warmups."""
     import torch
     a = torch.zeros(1 << 28, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"mode": "copy-cal", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
-                      "copies": 3}), flush=True)
+    out = {"mode": "copy-cal", "bytes_read": 1 << 30, "bytes_written": 1 << 30, "copies": 3}
+    del a, b
+    rates = {}
+    for label, n in (("67MB", 1 << 24), ("1GiB", 1 << 28)):
+        x = torch.rand(n, device="cuda")
+        y = torch.rand(n, device="cuda")
+        z = torch.empty_like(x)
+        for op, nbytes, fn in (("copy", 8 * n, lambda: z.copy_(x)),
+                               ("add", 12 * n, lambda: torch.add(x, y, out=z))):
+            for _ in range(3):
+                fn()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(20):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) * 1e3 / 20
+            rates[f"{op}_{label}"] = {"us": us, "GBs": nbytes / us / 1e3}
+            log(f"copy-cal: {op} {label}: {us:.1f} us, {nbytes / us / 1e3:.0f} GB/s")
+        del x, y, z
+    out["stream_rates"] = rates
+    print(json.dumps(out), flush=True)
+
+
+def stream_reference(n):
+    """The rate a plain stream with an FP32 sweep's byte mix reaches on this
+    GPU: torch's add of two float32 arrays of n elements into a third (read
+    8 B, write 4 B per element, as the solve's arrays: they fit the 256 MB
+    last-level cache together), timed with events over 20 launches after 3
+    warmups.  It runs in a child process: torch brings its own HIP runtime,
+    which does not start in a process where the library's has."""
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--stream-ref", str(n)],
+                           capture_output=True, text=True, timeout=300)
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    except (subprocess.SubprocessError, ValueError, IndexError, OSError) as exc:
+        log(f"stream reference: not measured ({exc!r})")
+        return None
+
+
+def stream_reference_child(n):
+    """--stream-ref N: the measurement of stream_reference, in this process."""
+    import torch
+    x = torch.rand(n, device="cuda")
+    y = torch.rand(n, device="cuda")
+    z = torch.empty_like(x)
+    for _ in range(3):
+        torch.add(x, y, out=z)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        torch.add(x, y, out=z)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / 20
+    del x, y, z
+    torch.cuda.empty_cache()
+    print(json.dumps({"op": f"torch.add, float32, {n} elements (read 8 B, write 4 B each)",
+                      "us": us, "GBs": 12.0 * n / us / 1e3}), flush=True)
 
 
 def traffic_from_profiles(kernel):
@@ -413,6 +478,8 @@ def main():
     ap.add_argument("--error-steps", type=int, default=0, metavar="N",
                     help="error mode: N steps of both forms from the same plasma, split vs FP64 "
                          "errors per step and particle trajectories after the last")
+    ap.add_argument("--stream-ref", type=int, default=0, metavar="N",
+                    help="print the stream reference rate for N float32 elements and exit")
     ap.add_argument("--copy-cal", action="store_true",
                     help="profiling: the 1 GiB copies the byte counters are scaled by")
     ap.add_argument("--ensemble", type=int, default=0, metavar="SIMS",
@@ -464,6 +531,9 @@ def main():
                                                  "HBM admission budget", **r}}), flush=True)
         return
     cfg = host.config2(args.grid)
+    if args.stream_ref:
+        stream_reference_child(args.stream_ref)
+        return
     if args.copy_cal:
         copy_calibration()
         return
@@ -545,9 +615,15 @@ def main():
                   "phi_rel": _rel(a["U"], b["U"]), "E_rel": _rel(a["E"], b["E"]),
                   "N_rel": _rel(a["N"], b["N"]),
                   "phi_rel_inf": float(np.abs(a["U"] - b["U"]).max() / np.abs(b["U"]).max())}
-    # the FP64 form runs the two-cells-per-lane pass when nx is even
-    kname = (("k_mx_pair" if spl["split"] == 2 else "k_mx_sweep") if headname == "split"
+    # the FP64 form runs the two-cells-per-lane pass when nx is even; the
+    # split form's two-sweep launches share rows through LDS on rows of at
+    # most 256 cells with x not periodic, except the launches marked for the
+    # stop test (k_mx_pair; one in 16), and the timed launches sample both
+    lds = (cfg.nx <= 256 and cfg.flow[0] and os.environ.get("PICARD_PAIR_KERNEL") != "regs")
+    kname = ((("k_mx_pair_lds" if lds else "k_mx_pair") if spl["split"] == 2 else "k_mx_sweep")
+             if headname == "split"
              else ("k_jacobi_pass2" if cfg.nx % 2 == 0 else "k_jacobi_pass"))
+    stream = stream_reference(ncell) if headname == "split" else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -587,6 +663,10 @@ def main():
             "avg_launch_us": avg_s * 1e6,
             "timed_launches": head["jac_launches"],
             "timed_every": 16,
+            "launch_mix": ("k_mx_pair_lds, with k_mx_pair on the launches marked for the stop "
+                           "test (1 in 16)") if kname == "k_mx_pair_lds" else kname,
+            "stream_reference": stream,
+            "frac_of_stream": (achieved / stream["GBs"]) if stream else None,
         },
         "cpu_baseline": None,
     }

@@ -4,6 +4,7 @@
 // validates its arguments on the host, launches on the context's stream and
 // returns a status; kernels never see a shape that was not checked here.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -207,6 +208,8 @@ int create_impl(const picard_geometry *geo, const picard_species *sp, picard_ctx
       PICARD_HIP(hipMemsetAsync(c->e32[q], 0, sizeof(float) * (nc32 + NX32), c->stream));
     for (int q = 0; q < 3; ++q) PICARD_HIP(hipMemsetAsync(c->eo32[q], 0, sizeof(float) * nc32, c->stream));
     PICARD_HIP(hipMemsetAsync(c->E32, 0, sizeof(float) * 4 * g.G, c->stream));
+    const char *pk = std::getenv("PICARD_PAIR_KERNEL");
+    c->pair_lds = !(pk && std::strcmp(pk, "regs") == 0);
   } else {
     for (int q = 0; q < 8; ++q) PICARD_TRY(dalloc(&c->p.r[q], cap));
     PICARD_TRY(dalloc(&c->p.sp, cap));

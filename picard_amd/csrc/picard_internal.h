@@ -132,6 +132,11 @@ struct picard_ctx {
   int mx_inner = 512;        // split: FP32 sweeps per FP64 refinement
   bool mx_pairs = true;      // split: two sweeps per launch where the grid allows
   int pair_wmax = 0;         // split: resident blocks per XCD of the two-sweep kernel
+  // split: the two-sweep kernel that shares rows through LDS where the grid
+  // allows (x not periodic, nx <= 256); PICARD_PAIR_KERNEL=regs at create
+  // selects the register-window kernel everywhere
+  bool pair_lds = true;
+  int lds_wmax = 0;          // split: resident blocks per XCD of the LDS two-sweep kernel
   int outer_wmax = 0;        // split: resident blocks per XCD of the outer-step kernel
   double *Uw = nullptr;      // split: second FP64 potential buffer of the solve
   int sweep_wmax = 0;        // split: resident blocks per XCD of the sweep kernel (0: not yet known)

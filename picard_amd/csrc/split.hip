@@ -1206,6 +1206,356 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
   if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
 }
 
+// One cell group of the two-sweep kernels with x not periodic: the arithmetic
+// of k_mx_pair's sweep with no periodic image in x and rows of one segment.
+// Centre c, z-neighbours zb/zt, y-neighbours n/s, right-hand side r; zero: a
+// flow guard row or plane (or a row no reader needs), whose correction is 0;
+// first1: e_0 = 0 (the sweep is -af r); test: add the stop-test residual of c
+// to r2, with the stale values qn/qs/qz across periodic boundaries where the
+// flags fN/fS/zT/zB are set.
+template <bool PAD>
+__device__ __forceinline__ float4 mx_cell4(const SweepCoef &cf, const float4 &c, const float4 &zb,
+                                           const float4 &zt, const float4 &n, const float4 &s,
+                                           const float4 &r, bool zero, bool first1, bool test,
+                                           bool fN, const float *pn, bool fS, const float *ps,
+                                           bool zT, bool zB, const float *pz, bool live,
+                                           bool haslast, int ql, double &r2) {
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (zero) return zero4;
+  const float ce[4] = {c.x, c.y, c.z, c.w};
+  const float cn[4] = {n.x, n.y, n.z, n.w}, cs[4] = {s.x, s.y, s.z, s.w};
+  const float ct[4] = {zt.x, zt.y, zt.z, zt.w}, cb[4] = {zb.x, zb.y, zb.z, zb.w};
+  const float cr[4] = {r.x, r.y, r.z, r.w};
+  const float wv = __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(c.w), 0x138, 0xF, 0xF, true));
+  const float ev = __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(c.x), 0x130, 0xF, 0xF, true));
+  float ew[4], ee[4], v[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    ew[q] = q == 0 ? wv : ce[q - 1];
+    ee[q] = q == 3 ? ev : ce[q + 1];
+    if ((PAD || q == 3) && haslast && q == ql) ee[q] = 0.0f;
+    v[q] = jac_cell(cf, ee[q], ew[q], cn[q], cs[q], ct[q], cb[q], ce[q], cr[q]);
+    if (first1) v[q] = -cf.af * cr[q];
+    if (PAD && !(live && q <= ql)) v[q] = 0.0f;
+  }
+  if (test) {
+    float sq = 0.0f;
+    if (fN || fS || zT || zB) {
+      const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const float4 qn = fN ? ld4(pn) : zero4, qs = fS ? ld4(ps) : zero4;
+      const float4 qz = (zT || zB) ? ld4(pz) : zero4;
+      const float qN[4] = {qn.x, qn.y, qn.z, qn.w}, qS[4] = {qs.x, qs.y, qs.z, qs.w};
+      const float qZ[4] = {qz.x, qz.y, qz.z, qz.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float sn = fN ? qN[q] : cn[q];
+        const float ss = fS ? qS[q] : cs[q];
+        const float st_ = zT ? qZ[q] : ct[q];
+        const float sb = zB ? qZ[q] : cb[q];
+        const float e2 = 2.0f * ce[q];
+        const float rr = cr[q] - ((ee[q] + ew[q] - e2) * cf.idx2 + (sn + ss - e2) * cf.idy2 +
+                                  (st_ + sb - e2) * cf.idz2);
+        if (!PAD || q <= ql) sq = sq + rr * rr;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float e2 = 2.0f * ce[q];
+        const float rr = cr[q] - ((ee[q] + ew[q] - e2) * cf.idx2 + (cn[q] + cs[q] - e2) * cf.idy2 +
+                                  (ct[q] + cb[q] - e2) * cf.idz2);
+        if (!PAD || q <= ql) sq = sq + rr * rr;
+      }
+    }
+    if (!live) sq = 0.0f;
+    r2 = r2 + (double)sq;
+  }
+  return make_float4(v[0], v[1], v[2], v[3]);
+}
+
+// The two-sweep kernel with rows shared through LDS, for x not periodic and
+// rows of at most one segment (nx <= 256).  A block of kLdsWaves waves owns a
+// tile of at most kLdsRows consecutive rows ya..yb-1 and marches a z chunk.
+// Wave w computes the first sweep, e_{j+1}, on rows ya-1+2w and ya+2w (the
+// tile's rows and one more on each side), and the second sweep, e_{j+2}, on
+// those of them that are the tile's.  Each wave loads e_j and r32 of its two
+// rows only (wave 0 and the last wave also e_j of the row beyond the first or
+// last computed row), keeps them in registers for the centre and the z
+// neighbours, and hands them to the neighbouring waves through LDS: e_j of
+// plane p+1 and e_{j+1} of plane p at plane step p, one barrier per step.
+// Against k_mx_pair (each wave recomputes the rows next to its own and loads
+// them again), the first sweep runs on 16 rows for 14 and every row of e_j
+// and r32 is loaded once per tile, and a wave holds at most 128 VGPRs, so
+// that 16 waves per CU stay resident to cover HBM latency (k_mx_pair: 241,
+// 8 waves).  Every cell's arithmetic is k_mx_sweep's, so e_{j+2} is bit for
+// bit what two launches of it give, and the boundary copies of e_{j+1} are
+// those of k_mx_pair.  The host sends the launches it marks for the stop test
+// to k_mx_pair; this kernel evaluates the stop-test residual of e_j (cell for
+// cell k_mx_pair's) only while the stop is near, in a pass of its own ahead of
+// the sweeps, which keeps the sweeps' registers within 128.
+constexpr int kLdsWaves = 8;                  // waves per block
+constexpr int kLdsDepth = 1;                  // plane steps the loads run ahead
+constexpr int kLdsRows = 2 * kLdsWaves - 2;   // rows of a tile
+
+// NW waves per block (2 NW rows of the first sweep, 2 NW - 2 of a tile),
+// loads issued D plane steps ahead; at most 128 VGPRs for D = 1, so that
+// two blocks of 8 waves are resident per CU.  At 256^3: 8 waves, D = 1:
+// 36.3 us per launch (median); 6 waves 38.0; D = 2 (145 VGPRs, 3 waves per
+// SIMD) 46 us with 8 waves, 50 us with 6, as fewer blocks stay resident.
+
+template <bool PAD, bool FIRST, int NW, int D>
+__global__ void __launch_bounds__(64 * NW, D == 1 ? 4 : 3) k_mx_pair_lds(
+    SweepGeo sg, SweepCoef cf, const float *__restrict__ r32, const float *__restrict__ e,
+    float *__restrict__ out, float *__restrict__ eo_out, const float *__restrict__ eo_prev,
+    const double *__restrict__ Pin, double *__restrict__ Pout, JacobiState *st, MxLaunch L) {
+  constexpr int kLdsCRows = 2 * NW;
+  __shared__ double sh[NW];
+  __shared__ int decide;
+  // e_j of plane q in EJs[q & 1] (row i of the tile at index i + 1, i = -1..kLdsCRows);
+  // e_{j+1} of plane q in E1s[q & 1]
+  __shared__ float4 EJs[2][kLdsCRows + 2][64];
+  __shared__ float4 E1s[2][kLdsCRows][64];
+  StopSlot cur = mx_decide(st, L, Pin, sh, &decide);
+  if (!cur.done && (int64_t)L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
+    cur.done = 1;
+    cur.result = st->last;
+  }
+  if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
+  if (cur.done) return;
+  const bool res = !FIRST && (L.test || cur.near);
+  const uint32_t NX = sg.NX, P = sg.P;
+  const int nx = sg.nx, ny = sg.ny, nz = sg.nz;
+  const int lane = (int)threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const uint32_t zrow = P * (uint32_t)nz;                   // the zero row of E
+  const int xs = 4 * lane;
+  const bool live = xs < nx;
+  const uint32_t x0 = live ? (uint32_t)xs : 0u;
+  const uint32_t zr = zrow + x0;
+  const int ql = nx - 1 - xs;
+  const bool haslast = live && ql < 4;
+  double r2 = 0.0;
+  const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
+  const int z0 = (xcd * nz) >> 3, z1 = ((xcd + 1) * nz) >> 3;
+  auto wrapz = [&](int pp) {
+    int kk = pp < 0 ? pp + nz : pp;
+    kk = kk < 0 ? kk + nz : kk;
+    kk = kk >= nz ? kk - nz : kk;
+    return kk >= nz ? kk - nz : kk;
+  };
+  auto pvalid = [&](int pp) { return sg.pz | ((pp >= 0) & (pp < nz)); };
+  auto poff = [&](int pp) { return P * (uint32_t)wrapz(pp); };
+  const int i0 = 2 * wv, i1 = 2 * wv + 1;                   // the wave's rows of the tile
+  const int ix = wv == 0 ? -1 : kLdsCRows;                  // its extra row of e_j
+  for (int t = w; t < sg.TP; t += sg.W) {
+    // tile t: z chunk zi of the XCD slab, rows ya..yb-1 (block-uniform)
+    const int zi = t % sg.nzc, ty = t / sg.nzc;
+    const int ka = z0 + ((z1 - z0) * zi) / sg.nzc, kb = z0 + ((z1 - z0) * (zi + 1)) / sg.nzc;
+    const int ya = (ty * ny) / sg.nty, yb = ((ty + 1) * ny) / sg.nty;
+    const int Yo = yb - ya;
+    if (Yo <= 0 || kb <= ka) continue;
+    // row i of the tile is y = ya - 1 + i: its address (the periodic image,
+    // or row ya for a flow guard row) and whether it is in the grid
+    auto row_at = [&](int i, bool &ok) -> uint32_t {
+      int yy = ya - 1 + i;
+      ok = true;
+      if (yy < 0 || yy >= ny) {
+        if (sg.py) {
+          yy = yy % ny;
+          if (yy < 0) yy += ny;
+        } else {
+          ok = false;
+          yy = ya;
+        }
+      }
+      return x0 + NX * (uint32_t)yy;
+    };
+    bool ok0, ok1, okx;
+    const uint32_t ra0 = row_at(i0, ok0), ra1 = row_at(i1, ok1), rax = row_at(ix, okx);
+    // e_j is read at rows -1..Yo+2, the first sweep computed at rows 0..Yo+1
+    // (zero at flow guard rows and past Yo+1), the second at rows 1..Yo
+    const bool rok0 = !FIRST && live && ok0 && i0 <= Yo + 2;
+    const bool rok1 = !FIRST && live && ok1 && i1 <= Yo + 2;
+    const bool hasx = !FIRST && (wv == 0 || (wv == NW - 1 && Yo + 2 >= kLdsCRows));
+    const bool rokx = hasx && live && okx;
+    const bool cp0 = ok0 && i0 <= Yo + 1, cp1 = ok1 && i1 <= Yo + 1;
+    const bool op0 = i0 >= 1 && i0 <= Yo, op1 = i1 >= 1 && i1 <= Yo;
+    const int y0r = ya - 1 + i0, y1r = ya - 1 + i1;
+    const bool yN0 = sg.py && y0r == ny - 1, yS0 = sg.py && y0r == 0;
+    const bool yN1 = sg.py && y1r == ny - 1, yS1 = sg.py && y1r == 0;
+    auto eld = [&](uint32_t ra, bool rok, int pp) -> float4 {
+      if constexpr (FIRST) return zero4;      // e_0 = 0: nothing to read
+      return ld4(e + ((pvalid(pp) & rok) ? ra + poff(pp) : zr));
+    };
+    // the stop-test residual of e_j on the tile's rows (launches while the
+    // stop is near), in a pass of its own ahead of the sweeps, so that the
+    // sweeps hold no registers for it: across periodic boundaries the
+    // neighbours are e_{j-1} (rows 0 and ny-1, planes 0 and nz-1)
+    if (res) {
+      // row by row, the loads of plane p+1 issued before the residual of
+      // plane p
+#pragma unroll 1
+      for (int r = 0; r < 2; ++r) {
+        if (!(r ? op1 : op0)) continue;                      // wave-uniform
+        const int i = r ? i1 : i0;
+        const uint32_t rc = r ? ra1 : ra0;
+        bool okn, oks;
+        const uint32_t rn = row_at(i + 1, okn), rs = row_at(i - 1, oks);
+        const bool fN = r ? yN1 : yN0, fS = r ? yS1 : yS0;
+        struct RIn {
+          float4 t, n, s, q;
+        };
+        auto rissue = [&](RIn &a, int p) {
+          a.t = eld(rc, live, p + 1);
+          a.n = eld(rn, live && okn, p);
+          a.s = eld(rs, live && oks, p);
+          a.q = ld4(r32 + rc + poff(p));
+        };
+        float4 b = eld(rc, live, ka - 1), cc = eld(rc, live, ka);
+        auto rstep = [&](int p, const RIn &a) {
+          const uint32_t pk = P * (uint32_t)p;
+          const uint32_t pi = p == 0 ? P * (uint32_t)(nz - 1) : 0u;    // plane nz-1 or 0
+          (void)mx_cell4<PAD>(cf, cc, b, a.t, a.n, a.s, a.q, false, false, true, fN,
+                              eo_prev + x0 + pk, fS, eo_prev + x0 + NX * (uint32_t)(ny - 1) + pk,
+                              sg.pz && p == nz - 1, sg.pz && p == 0, eo_prev + rc + pi, live,
+                              haslast, ql, r2);
+          b = cc;
+          cc = a.t;
+        };
+        RIn RA, RB;
+        rissue(RA, ka);
+        for (int p = ka;; p += 2) {
+          rissue(RB, p + 1 < kb ? p + 1 : kb - 1);
+          rstep(p, RA);
+          if (p + 1 >= kb) break;
+          rissue(RA, p + 2 < kb ? p + 2 : kb - 1);
+          rstep(p + 1, RB);
+          if (p + 2 >= kb) break;
+        }
+      }
+    }
+    // the march: step k = 0..K has its front (first sweep) at plane f(k),
+    // up from ka-1 to kb in even chunks, down from kb to ka-1 in odd ones, so
+    // that neighbouring chunks of the slab read their shared planes at the
+    // same time (both at the start of their march or both at its end) and
+    // the second reads them from L2.  Going down swaps only the z-neighbours
+    // of each cell, whose sum (t + b) does not depend on their order.
+    const bool up = (zi & 1) == 0;
+    const int dz = up ? 1 : -1, K = kb - ka + 1;
+    auto front = [&](int k) { return up ? ka - 1 + k : kb - k; };
+    // registers: e_j of the wave's rows at f(k) (c*), e_{j+1} at f(k-1)
+    // (u1*), r32 at f(k-1) (rp*).  e_j at f(k-1) and e_{j+1} at f(k-2) of the
+    // wave's rows are read back from LDS, where only this wave writes them
+    // (before it overwrites them with f(k+1) and f(k)).  EJs[k & 1] holds e_j
+    // at f(k), E1s[k & 1] e_{j+1} at f(k).
+    float4 c0 = eld(ra0, rok0, front(0)), c1 = eld(ra1, rok1, front(0));
+    float4 u10 = zero4, u11 = zero4, rp0 = zero4, rp1 = zero4;
+    if (!FIRST) {
+      EJs[1][i0 + 1][lane] = eld(ra0, rok0, front(0) - dz);
+      EJs[1][i1 + 1][lane] = eld(ra1, rok1, front(0) - dz);
+      EJs[0][i0 + 1][lane] = c0;
+      EJs[0][i1 + 1][lane] = c1;
+      if (hasx) EJs[0][ix + 1][lane] = eld(rax, rokx, front(0));
+    }
+    // loads for step k: e_j of the wave's rows (and its extra row) at
+    // f(k+1), r32 of its rows at f(k) (plane ka where no sweep uses it)
+    struct In {
+      float4 e0, e1, ex, r0, r1;
+    };
+    auto issue = [&](In &a, int k) {
+      const int f = front(k);
+      a.e0 = eld(ra0, rok0, f + dz);
+      a.e1 = eld(ra1, rok1, f + dz);
+      if (hasx) a.ex = eld(rax, rokx, f + dz);
+      const uint32_t orr = pvalid(f) ? poff(f) : P * (uint32_t)ka;
+      a.r0 = ld4(r32 + ra0 + orr);
+      a.r1 = ld4(r32 + ra1 + orr);
+    };
+    In S[D + 1];
+#pragma unroll
+    for (int q = 0; q <= D; ++q) S[q].ex = zero4;
+#pragma unroll
+    for (int q = 0; q < D; ++q) issue(S[q], q < K ? q : K);
+    __syncthreads();
+    auto step = [&](int k, const In &a) {
+      const int p = front(k);
+      const bool pv = pvalid(p);
+      const bool in = p >= ka && p < kb;
+      const bool zT = sg.pz && p == nz - 1, zB = sg.pz && p == 0;
+      const int s = k & 1;
+      // e_j at f(k) of the rows next to the wave's (written by its
+      // neighbours), and of its own rows at f(k-1)
+      const float4 s0 = FIRST ? zero4 : EJs[s][i0][lane];
+      const float4 n1 = FIRST ? zero4 : EJs[s][i1 + 2][lane];
+      const float4 m0 = FIRST ? zero4 : EJs[s ^ 1][i0 + 1][lane];
+      const float4 m1 = FIRST ? zero4 : EJs[s ^ 1][i1 + 1][lane];
+      // first sweep at f(k)
+      const float4 v0 = mx_cell4<PAD>(cf, c0, m0, a.e0, c1, s0, a.r0, !pv || !cp0, FIRST, false,
+                                      false, nullptr, false, nullptr, false, false, nullptr, live,
+                                      haslast, ql, r2);
+      const float4 v1 = mx_cell4<PAD>(cf, c1, m1, a.e1, n1, c0, a.r1, !pv || !cp1, FIRST, false,
+                                      false, nullptr, false, nullptr, false, false, nullptr, live,
+                                      haslast, ql, r2);
+      if (in && live) {
+        if (op0 && (yN0 || yS0 || zT || zB)) st4(eo_out + ra0 + P * (uint32_t)p, v0);
+        if (op1 && (yN1 || yS1 || zT || zB)) st4(eo_out + ra1 + P * (uint32_t)p, v1);
+      }
+      // second sweep at f(k-1) on the tile's rows
+      if (k >= 2) {
+        const uint32_t po = P * (uint32_t)(p - dz);
+        if (op0) {
+          const float4 u00 = E1s[s][i0][lane];                 // f(k-2)
+          const float4 sv = E1s[s ^ 1][i0 - 1][lane];
+          const float4 v = mx_cell4<PAD>(cf, u10, u00, v0, u11, sv, rp0, false, false, false,
+                                         false, nullptr, false, nullptr, false, false, nullptr,
+                                         live, haslast, ql, r2);
+          if (live) st4(out + ra0 + po, v);
+        }
+        if (op1) {
+          const float4 u01 = E1s[s][i1][lane];                 // f(k-2)
+          const float4 n = E1s[s ^ 1][i1 + 1][lane];
+          const float4 v = mx_cell4<PAD>(cf, u11, u01, v1, n, u10, rp1, false, false, false,
+                                         false, nullptr, false, nullptr, false, false, nullptr,
+                                         live, haslast, ql, r2);
+          if (live) st4(out + ra1 + po, v);
+        }
+      }
+      // hand on e_j at f(k+1) and e_{j+1} at f(k)
+      if (!FIRST) {
+        EJs[s ^ 1][i0 + 1][lane] = a.e0;
+        EJs[s ^ 1][i1 + 1][lane] = a.e1;
+        if (hasx) EJs[s ^ 1][ix + 1][lane] = a.ex;
+      }
+      E1s[s][i0][lane] = v0;
+      E1s[s][i1][lane] = v1;
+      u10 = v0;
+      u11 = v1;
+      rp0 = a.r0;
+      rp1 = a.r1;
+      c0 = a.e0;
+      c1 = a.e1;
+      __syncthreads();
+    };
+    // the loads of step k+D go out before step k (into the set step k-1 used)
+    bool more = true;
+    for (int k0 = 0; more; k0 += D + 1) {
+#pragma unroll
+      for (int u = 0; u <= D; ++u) {
+        const int k = k0 + u;
+        if (more) {
+          issue(S[(u + D) % (D + 1)], k + D < K ? k + D : K);
+          step(k, S[u]);
+          more = k + 1 <= K;
+        }
+      }
+    }
+  }
+  const double sblk = block_sum(r2, sh);
+  if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
+}
+
 // End of an outer step: the stop test of its last inner iterate and, when
 // the solve ended inside this step (converged, or the last iterate reached),
 // u += e_k for that iterate's k; otherwise nothing (the next outer step
@@ -1564,6 +1914,60 @@ SweepGeo pair_sched(picard_ctx *c) {
   return sweep_geo(c, c->pair_wmax, kPairRows, true, c->gd.nx > kSwSeg ? kPairKeep : kSwSeg);
 }
 
+// The LDS two-sweep kernel's schedule: tiles of at most kLdsRows rows (the
+// rows split evenly over ceil(ny / kLdsRows) tiles) x z chunks of an XCD
+// slab, as many chunks as fill the resident blocks (chunks of at least 8
+// planes), blocks per XCD no more than are resident.
+SweepGeo lds_sched(picard_ctx *c) {
+  const GridDims &g = c->gd;
+  if (c->lds_wmax <= 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_mx_pair_lds<false, false, kLdsWaves, kLdsDepth>, 64 * kLdsWaves, 0) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        per_cu <= 0 || cus < 8) {
+      c->lds_wmax = 32;
+    } else {
+      c->lds_wmax = (cus / 8) * per_cu;
+    }
+  }
+  const int wmax = c->lds_wmax;
+  SweepGeo sg;
+  sg.nx = g.nx;
+  sg.ny = g.ny;
+  sg.nz = g.nz;
+  sg.NX = (uint32_t)(g.nx + 3) & ~3u;
+  sg.P = sg.NX * (uint32_t)g.ny;
+  sg.px = !g.flow[0];
+  sg.py = !g.flow[1];
+  sg.pz = !g.flow[2];
+  sg.ntx = 1;
+  sg.nty = (g.ny + kLdsRows - 1) / kLdsRows;
+  int nzc = wmax / sg.nty;
+  const int zmax = g.nz / 64 > 1 ? g.nz / 64 : 1;
+  if (nzc > zmax) nzc = zmax;
+  if (nzc < 1) nzc = 1;
+  sg.nzc = nzc;
+  sg.TP = sg.nty * nzc;
+  int W = sg.TP < wmax ? sg.TP : wmax;
+  W = (sg.TP + (sg.TP + W - 1) / W - 1) / ((sg.TP + W - 1) / W);
+  if (W > (int)(c->s.partials_len / 8)) W = (int)(c->s.partials_len / 8);
+  sg.W = W < 1 ? 1 : W;
+  return sg;
+}
+
+void launch_pair_lds(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t t1,
+                     const SweepGeo &sg, const SweepCoef &cf, const float *r32, const float *e,
+                     float *out, float *eo_out, const float *eo_prev, const double *pin,
+                     double *pout, JacobiState *js, const MxLaunch &L) {
+  constexpr int W = kLdsWaves, Dp = kLdsDepth;
+  auto fn = L.j == 0 ? (pad ? k_mx_pair_lds<true, true, W, Dp> : k_mx_pair_lds<false, true, W, Dp>)
+                     : (pad ? k_mx_pair_lds<true, false, W, Dp> : k_mx_pair_lds<false, false, W, Dp>);
+  hipExtLaunchKernelGGL(fn, dim3(blocks), dim3(64, kLdsWaves), 0, s, t0, t1, 0, sg, cf, r32, e,
+                        out, eo_out, eo_prev, pin, pout, js, L);
+}
+
 // One two-sweep launch (timed with events t0/t1 when given).
 template <bool PX, bool MULTI>
 void launch_pair(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t t1,
@@ -1673,8 +2077,17 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
   // two sweeps per launch: rows of one segment, or wider rows with x not
   // periodic (overlapping segments)
   const bool pairs = c->mx_pairs && (sg.ntx == 1 || g.flow[0]) && m % 2 == 0;
+  // rows of one segment with x not periodic: rows shared through LDS, except
+  // in the launches that test the stop, which take the register-window kernel
+  // (it evaluates the residual in its first sweep, 3 us more per launch, where
+  // the LDS kernel needs a pass of its own, 30 us): the launches the host
+  // marks, and every launch of a refinement step that begins with the stop
+  // near (the flag the last one left; near the end of a slowly converging
+  // solve it stays set for many refinement steps)
+  const bool lds = pairs && c->pair_lds && sg.ntx == 1 && g.flow[0];
   const SweepGeo pgeo = pairs ? pair_sched(c) : sg;
-  const int prB = 8 * pgeo.W;
+  const SweepGeo lgeo = lds ? lds_sched(c) : pgeo;
+  const int prB = 8 * pgeo.W, plB = 8 * lgeo.W;
   double *P[2] = {c->s.partials, c->s.partials + c->s.partials_len};
   float *E0 = c->e32[0], *E1 = c->e32[1];
   float *const E[2] = {E0, E1};
@@ -1691,6 +2104,7 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
   bool prevtest = true;                     // the last launch tested whatever its near flag
   int32_t nin = 0;                          // partials the last writer stored
   StopSlot fin{};
+  bool stop_near = false;                   // the stop test found the stop near
   while (true) {
     // timing: events around one launch in kTimeEvery (event records around
     // every launch would slow the timed solve itself)
@@ -1751,7 +2165,11 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       hipEvent_t t0 = tj ? c->tev[2 * nrec] : nullptr;
       hipEvent_t t1 = tj ? c->tev[2 * nrec + 1] : nullptr;
       if (tj) ++nrec;
-      if (two && sg.px)
+      const bool use_lds = two && lds && !test && !stop_near;
+      if (use_lds)
+        launch_pair_lds(pad, plB, c->stream, t0, t1, lgeo, cf, c->r32, e, out, eo_out, eo_prev,
+                        pin, pout, c->js, L);
+      else if (two && sg.px)
         launch_pair<true, false>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out,
                                  eo_prev, pin, pout, c->js, L);
       else if (two && pgeo.ntx > 1)
@@ -1769,7 +2187,7 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       ++seq;
       if (j >= 1) {               // may have written the residual of iterate base + j
         check = base + j;
-        nin = two ? prB : swB;
+        nin = two ? (use_lds ? plB : prB) : swB;
         pw ^= 1;
       } else {
         check = -1;
@@ -1798,6 +2216,7 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       c->t_launches += nrec;
     }
     fin = c->js_host->slot[(seq - 1) & 1];
+    stop_near = fin.near != 0;
     if (fin.done) {
       if (udst != c->U)
         PICARD_HIP(hipMemcpyAsync(c->U, udst, sizeof(double) * g.G, hipMemcpyDeviceToDevice,

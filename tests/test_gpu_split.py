@@ -296,6 +296,48 @@ def test_split_two_sweeps_per_launch_same_iterates(dims, flow):
     assert _rel(U2, U1) <= 1e-6
 
 
+@pytest.mark.parametrize("dims,flow", [((38, 29, 11), (True, False, True)),
+                                       ((64, 45, 17), (True, True, False)),
+                                       ((256, 30, 9), (True, False, False)),
+                                       ((100, 14, 6), (True, False, False)),
+                                       ((36, 2, 8), (True, False, False)),
+                                       ((29, 3, 2), (True, False, False)),
+                                       ((40, 20, 128), (True, False, False)),
+                                       ((36, 16, 130), (True, True, True))])
+def test_split_lds_pair_kernel_matches_register_kernel(dims, flow, monkeypatch):
+    """Rows of at most 256 cells with x not periodic take the two-sweep
+    kernel that shares rows through LDS (tiles of up to 14 rows, here one to
+    four of them, periodic and flow y and z, padded rows, two rows, two
+    planes; with 128 planes and more, the z chunks of an XCD's slab march up
+    and down in turn).  Its iterates are the register-window kernel's bit for bit (every
+    stop test failing, the capped solve ends on the same potential), and with
+    the stop rule live it stops on the same sweep: the stop-test residual is
+    the same cell for cell, summed in another order."""
+    nx, ny, nz = dims
+    cfg = host.Config(nx=nx, ny=ny, nz=nz, flow=flow,
+                      gmin=(-5.0 * nx, -5.0 * ny, -5.0 * nz), gmax=(5.0 * nx, 5.0 * ny, 5.0 * nz),
+                      ppc=(0, 8, 0, 0, 0, 0, 0, 0))
+    d = host.derive(cfg)
+    rng = np.random.default_rng(nx * 1000 + ny * 10 + nz + 7)
+    C = rng.normal(size=(nz + 2, ny + 2, nx + 2)) * 1e-16
+    out = {}
+    for kernel in ("lds", "regs"):
+        if kernel == "regs":
+            monkeypatch.setenv("PICARD_PAIR_KERNEL", "regs")
+        else:
+            monkeypatch.delenv("PICARD_PAIR_KERNEL", raising=False)
+        out[kernel] = (_split_solve(cfg, d, C, 2, 1e-30, 45, 16),
+                       _split_solve(cfg, d, C, 2, 1e-6, 2 ** 20))
+    (sa, Ua), (ca, Va) = out["lds"]
+    (sb, Ub), (cb, Vb) = out["regs"]
+    _say(f"dims={dims} flow={flow}: capped {sa}/{sb}, converged {ca}/{cb} sweeps, "
+         f"U rel {_rel(Va, Vb):.2e}")
+    assert sa == sb == 91
+    assert np.array_equal(Ua, Ub)
+    assert ca == cb
+    assert np.array_equal(Va, Vb)
+
+
 def test_split_sweeps_per_launch_rejects_bad_value():
     cfg = _cfg(8)
     dom = host.Domain(cfg, max_particles=1024, precision=_lib.PICARD_FP32_SPLIT)
