@@ -61,10 +61,13 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
 
 
-def run_form(cfg, precision, steps, warmup, capture_after=1, order_free=False):
+def run_form(cfg, precision, steps, warmup, capture_after=1, order_free=False,
+             sweeps_per_launch=0):
     """W warmup + K timed steps of one form; returns timings (whole and per
     step, with the solve phase's wall time per step from HIP events) and the
-    fields after step `capture_after` (read outside the timed region)."""
+    fields after step `capture_after` (read outside the timed region).
+    sweeps_per_launch > 0 caps the split solve's sweeps per launch (else the
+    library's default, 2)."""
     from picard_amd import _lib, host
     d = host.derive(cfg)
     ncell = cfg.nx * cfg.ny * cfg.nz
@@ -74,6 +77,8 @@ def run_form(cfg, precision, steps, warmup, capture_after=1, order_free=False):
             "fp64_order_free" if order_free else "fp64")
     t0 = time.perf_counter()
     dom = host.Domain(cfg, max_particles=cap, d=d, precision=precision, order_free=order_free)
+    if sweeps_per_launch and precision == _lib.PICARD_FP32_SPLIT:
+        dom.split_sweeps_per_launch(sweeps_per_launch)
     dom.generate_initial(SEED)
     dom.calc_distance()
     dom.set_production(SEED)
@@ -113,6 +118,7 @@ def run_form(cfg, precision, steps, warmup, capture_after=1, order_free=False):
     dom.synchronize()
     elapsed = time.perf_counter() - t0
     jac_ms, jac_launches = dom.timing_read()
+    jac_sweeps = dom.timing_sweeps()
     solve_ms, solves = dom.timing_solve()
     dom.timing(False)
     if fields is None:
@@ -123,7 +129,8 @@ def run_form(cfg, precision, steps, warmup, capture_after=1, order_free=False):
         f"{solve_ms / max(solves, 1):.1f} ms per step, solve kernel "
         f"{1e3 * jac_ms / max(jac_launches, 1):.2f} us per launch, avg over {jac_launches}")
     return dict(pushed=pushed, elapsed=elapsed, sweeps=sweeps, jac_ms=jac_ms,
-                jac_launches=jac_launches, solve_ms=solve_ms, solves=solves, per_step=per_step,
+                jac_launches=jac_launches, jac_sweeps=jac_sweeps, solve_ms=solve_ms,
+                solves=solves, per_step=per_step,
                 warm_steps=warm_steps, fields=fields, captured_at=min(capture_after, it))
 
 
@@ -478,6 +485,9 @@ def main():
     ap.add_argument("--error-steps", type=int, default=0, metavar="N",
                     help="error mode: N steps of both forms from the same plasma, split vs FP64 "
                          "errors per step and particle trajectories after the last")
+    ap.add_argument("--sweeps-per-launch", type=int, default=0, choices=(0, 1, 2, 3),
+                    help="at most this many FP32 sweeps per launch of the split solve "
+                         "(0: the library's default, 2)")
     ap.add_argument("--stream-ref", type=int, default=0, metavar="N",
                     help="print the stream reference rate for N float32 elements and exit")
     ap.add_argument("--copy-cal", action="store_true",
@@ -545,7 +555,8 @@ def main():
         return
     runs = {}
     if args.form in ("both", "split"):
-        runs["split"] = run_form(cfg, _lib.PICARD_FP32_SPLIT, args.steps, args.warmup)
+        runs["split"] = run_form(cfg, _lib.PICARD_FP32_SPLIT, args.steps, args.warmup,
+                                 sweeps_per_launch=args.sweeps_per_launch)
     if args.form in ("both", "fp64"):
         runs["fp64"] = run_form(cfg, _lib.PICARD_FP64, args.fp64_steps if "split" in runs else
                                 args.steps, args.fp64_warmup if "split" in runs else args.warmup)
@@ -557,15 +568,17 @@ def main():
     head_steps = len(head["per_step"])
     value = head["pushed"] / head["elapsed"]
     ncell = cfg.nx * cfg.ny * cfg.nz
-    # Sweeps per launch of each form's solve kernel: the split form runs two
-    # FP32 sweeps per launch on rows of at most 256 cells, or wider rows with
-    # x not periodic (the default inner length, 512, is even), the FP64 form
-    # one pass per launch.
-    spl = {"split": 2 if cfg.nx <= 256 or cfg.flow[0] else 1, "fp64": 1, "fp64_order_free": 1}
+    # Sweeps per launch of each form's solve kernel, over the timed launches:
+    # the split form runs two FP32 sweeps per launch on rows of at most 256
+    # cells or wider rows with x not periodic (three where asked for, on rows
+    # of at most 256 cells with x not periodic); the FP64 form one pass per
+    # launch.
+    spl = {k: (r["jac_sweeps"] / r["jac_launches"] if r["jac_launches"] and r["jac_sweeps"]
+               else 1.0) for k, r in runs.items()}
     # Bytes a launch of the dominant kernel must move: an FP32 sweep reads the
     # correction e and the FP32 residual and writes e (12 B per interior
-    # cell), and a two-sweep launch keeps its first sweep's output on chip, so
-    # it moves those 12 B once for two sweeps; the FP64 pass reads U and b and
+    # cell), and a launch of several sweeps keeps the intermediate iterates on
+    # chip, so it moves those 12 B once; the FP64 pass reads U and b and
     # writes U (24 B per cell).  The sweep-byte rate (12 B per cell and sweep)
     # is reported beside it.
     bytes_per_launch = (12.0 if headname == "split" else 24.0) * ncell
@@ -620,7 +633,8 @@ def main():
     # most 256 cells with x not periodic, except the launches marked for the
     # stop test (k_mx_pair; one in 16), and the timed launches sample both
     lds = (cfg.nx <= 256 and cfg.flow[0] and os.environ.get("PICARD_PAIR_KERNEL") != "regs")
-    kname = ((("k_mx_pair_lds" if lds else "k_mx_pair") if spl["split"] == 2 else "k_mx_sweep")
+    kname = (((("k_mx_triple_lds" if args.sweeps_per_launch == 3 else "k_mx_pair_lds") if lds
+               else "k_mx_pair") if spl["split"] > 1.5 else "k_mx_sweep")
              if headname == "split"
              else ("k_jacobi_pass2" if cfg.nx % 2 == 0 else "k_jacobi_pass"))
     stream = stream_reference(ncell) if headname == "split" else None
@@ -663,8 +677,12 @@ def main():
             "avg_launch_us": avg_s * 1e6,
             "timed_launches": head["jac_launches"],
             "timed_every": 16,
-            "launch_mix": ("k_mx_pair_lds, with k_mx_pair on the launches marked for the stop "
-                           "test (1 in 16)") if kname == "k_mx_pair_lds" else kname,
+            "launch_mix": {"k_mx_pair_lds": "k_mx_pair_lds, with k_mx_pair on the launches "
+                                            "that apply the stop test",
+                           "k_mx_triple_lds": "k_mx_triple_lds (three sweeps), with k_mx_pair "
+                                              "(two) on the launches that apply the stop test "
+                                              "and k_mx_pair_lds (two) where a refinement "
+                                              "step's end calls for it"}.get(kname, kname),
             "stream_reference": stream,
             "frac_of_stream": (achieved / stream["GBs"]) if stream else None,
         },

@@ -116,7 +116,7 @@ void release(picard_ctx *c) {
   dfree(c->r32);
   dfree(c->Uw);
   for (int q = 0; q < 2; ++q) dfree(c->e32[q]);
-  for (int q = 0; q < 3; ++q) dfree(c->eo32[q]);
+  for (int q = 0; q < 4; ++q) dfree(c->eo32[q]);
   for (int q = 0; q < 8; ++q) dfree(c->p.r[q]);
   dfree(c->p.sp);
   dfree(c->U);
@@ -202,11 +202,11 @@ int create_impl(const picard_geometry *geo, const picard_species *sp, picard_ctx
     PICARD_TRY(dalloc(&c->r32, nc32));
     PICARD_TRY(dalloc(&c->Uw, g.G));
     for (int q = 0; q < 2; ++q) PICARD_TRY(dalloc(&c->e32[q], nc32 + NX32));
-    for (int q = 0; q < 3; ++q) PICARD_TRY(dalloc(&c->eo32[q], nc32));
+    for (int q = 0; q < 4; ++q) PICARD_TRY(dalloc(&c->eo32[q], nc32));
     PICARD_HIP(hipMemsetAsync(c->r32, 0, sizeof(float) * nc32, c->stream));
     for (int q = 0; q < 2; ++q)
       PICARD_HIP(hipMemsetAsync(c->e32[q], 0, sizeof(float) * (nc32 + NX32), c->stream));
-    for (int q = 0; q < 3; ++q) PICARD_HIP(hipMemsetAsync(c->eo32[q], 0, sizeof(float) * nc32, c->stream));
+    for (int q = 0; q < 4; ++q) PICARD_HIP(hipMemsetAsync(c->eo32[q], 0, sizeof(float) * nc32, c->stream));
     PICARD_HIP(hipMemsetAsync(c->E32, 0, sizeof(float) * 4 * g.G, c->stream));
     const char *pk = std::getenv("PICARD_PAIR_KERNEL");
     c->pair_lds = !(pk && std::strcmp(pk, "regs") == 0);
@@ -603,6 +603,7 @@ int picard_timing_enable(picard_ctx *c, int32_t enable) {
   c->timing = enable != 0;
   c->t_ms = 0.0;
   c->t_launches = 0;
+  c->t_sweeps = 0;
   c->solve_ms = 0.0;
   c->solves = 0;
   return PICARD_OK;
@@ -622,6 +623,12 @@ int picard_timing_read(picard_ctx *c, double *jacobi_ms, int64_t *jacobi_launche
   return PICARD_OK;
 }
 
+int picard_timing_sweeps(picard_ctx *c, int64_t *jacobi_sweeps) {
+  CTX_CHECK(c);
+  if (jacobi_sweeps) *jacobi_sweeps = c->t_sweeps;
+  return PICARD_OK;
+}
+
 int picard_split_solver(picard_ctx *c, int32_t inner) {
   CTX_CHECK(c);
   if (inner < 2 || inner > kMaxSplitInner) {
@@ -636,12 +643,13 @@ int picard_split_solver(picard_ctx *c, int32_t inner) {
 
 int picard_split_sweeps_per_launch(picard_ctx *c, int32_t sweeps) {
   CTX_CHECK(c);
-  if (sweeps != 1 && sweeps != 2) {
-    set_error("picard_split_sweeps_per_launch: 1 or 2 sweeps per launch, got " +
+  if (sweeps < 1 || sweeps > 3) {
+    set_error("picard_split_sweeps_per_launch: 1, 2 or 3 sweeps per launch, got " +
               std::to_string(sweeps));
     return PICARD_ERR_ARG;
   }
-  c->mx_pairs = sweeps == 2;
+  c->mx_pairs = sweeps >= 2;
+  c->mx_triples = sweeps == 3;
   return PICARD_OK;
 }
 

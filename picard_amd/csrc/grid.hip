@@ -1329,6 +1329,7 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
         c->t_ms += ms;
       }
       c->t_launches += timed;
+      c->t_sweeps += timed;
     }
     fin = c->js_host->slot[(p - 1) & 1];
     if (fin.done) break;

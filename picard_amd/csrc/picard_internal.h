@@ -66,6 +66,8 @@ struct StopSlot {
   int64_t result;            // sweeps of the final iterate, valid when done
   double rlast;              // split form: the last tested residual norm (0: none yet)
   int64_t itlast;            // split form: the iterate it belongs to
+  int32_t buf;               // split form: which correction buffer holds iterate `result`
+  int32_t reserved;
 };
 
 // Device-side scalars of the Jacobi solve (one per context).
@@ -128,15 +130,17 @@ struct picard_ctx {
   // split solve, compact interior layout (rows padded to 4 cells): right-hand
   // side, ping-pong correction, inner iterates j % 3 on periodic boundary planes
   float *r32 = nullptr, *e32[2] = {nullptr, nullptr};
-  float *eo32[3] = {nullptr, nullptr, nullptr};
+  float *eo32[4] = {nullptr, nullptr, nullptr, nullptr};
   int mx_inner = 512;        // split: FP32 sweeps per FP64 refinement
   bool mx_pairs = true;      // split: two sweeps per launch where the grid allows
+  bool mx_triples = false;   // split: three sweeps per launch where the grid allows
   int pair_wmax = 0;         // split: resident blocks per XCD of the two-sweep kernel
   // split: the two-sweep kernel that shares rows through LDS where the grid
   // allows (x not periodic, nx <= 256); PICARD_PAIR_KERNEL=regs at create
   // selects the register-window kernel everywhere
   bool pair_lds = true;
   int lds_wmax = 0;          // split: resident blocks per XCD of the LDS two-sweep kernel
+  int tri_wmax = 0;          // split: resident blocks per XCD of the three-sweep kernel
   int outer_wmax = 0;        // split: resident blocks per XCD of the outer-step kernel
   double *Uw = nullptr;      // split: second FP64 potential buffer of the solve
   int sweep_wmax = 0;        // split: resident blocks per XCD of the sweep kernel (0: not yet known)
@@ -192,6 +196,7 @@ struct picard_ctx {
   std::vector<hipEvent_t> tev;
   double t_ms = 0.0;
   int64_t t_launches = 0;
+  int64_t t_sweeps = 0;      // Jacobi sweeps of the timed launches
   // wall time of the whole CalcEpotential phase inside picard_step while
   // timing is on (events before and after the solve on the context's stream)
   hipEvent_t solve_ev[2] = {nullptr, nullptr};

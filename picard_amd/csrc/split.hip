@@ -367,9 +367,11 @@ struct MxLaunch {
   int32_t check;         // iterate whose residual Pin holds, or -1
   int32_t m;             // inner sweeps of this outer step (update launch)
   int32_t nin;           // partials in Pin
-  int32_t pairs;         // two sweeps per launch: e_k lives in E[((k + 1) / 2) % 2]
+  int32_t pairs;         // the solve runs launches of more than one sweep
   int32_t test;          // two-sweep launch: evaluate the stop-test residual of e_j
   int32_t prevtest;      // the previous launch wrote partials whatever its near flag
+  int32_t ibuf;          // correction buffer holding e_j (update launch: e_m)
+  int32_t cbuf;          // correction buffer holding iterate `check`
 };
 
 // Compact FP32 layout of the correction, its right-hand side and the
@@ -462,6 +464,7 @@ __device__ __forceinline__ StopSlot mx_decide(JacobiState *st, const MxLaunch &L
     if (*decide & 1) {
       cur.done = 1;
       cur.result = L.check;
+      cur.buf = L.cbuf;
     }
     if (tested) {
       cur.near = (*decide & 2) ? 1 : 0;
@@ -690,6 +693,7 @@ __global__ void __launch_bounds__(kSwThreads, 1024 / kSwThreads) k_mx_sweep(Swee
   if (!cur.done && (int64_t)L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
     cur.done = 1;
     cur.result = st->last;
+    cur.buf = L.ibuf;
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   if (cur.done) return;
@@ -902,6 +906,7 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
   if (!cur.done && (int64_t)L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
     cur.done = 1;
     cur.result = st->last;
+    cur.buf = L.ibuf;
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   if (cur.done) return;
@@ -1320,6 +1325,7 @@ __global__ void __launch_bounds__(64 * NW, D == 1 ? 4 : 3) k_mx_pair_lds(
   if (!cur.done && (int64_t)L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
     cur.done = 1;
     cur.result = st->last;
+    cur.buf = L.ibuf;
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   if (cur.done) return;
@@ -1556,6 +1562,284 @@ __global__ void __launch_bounds__(64 * NW, D == 1 ? 4 : 3) k_mx_pair_lds(
   if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
 }
 
+// The three-sweep kernel: k_mx_pair_lds's scheme one level deeper, e_j ->
+// e_{j+3} with the HBM traffic of one sweep (read e_j and r32, write
+// e_{j+3}).  A block of NW waves owns a tile of at most 2 NW - 4 rows
+// ya..yb-1; wave w computes the first sweep on rows ya-2+2w and ya-1+2w at
+// the march front f(k), the second sweep on those of them in ya-1..yb at
+// f(k-1), the third on those in ya..yb-1 at f(k-2), and hands each level's
+// rows to its neighbours through LDS, one barrier per plane step.  The z
+// chunk's march covers planes ka-2..kb+1 for the first sweep.  Every cell's
+// arithmetic is k_mx_sweep's (e_{j+3} is bit for bit what three single
+// sweeps give); the boundary copies it leaves are those of e_{j+2}, which the
+// next launch's stop test (of e_{j+3}) takes as the stale halo.  It does not
+// evaluate the stop test except while the stop is near (the host sends the
+// launches it marks for the test to k_mx_pair), in a pass of its own as
+// k_mx_pair_lds.  With 6 waves per block, 76 KB of LDS and at most 168 VGPRs,
+// two blocks are resident per CU.  Measured at 256^3: 68 us per launch, 22.6
+// us per sweep against 18.9 with two sweeps per launch (k_mx_pair_lds: 36 us
+// per launch): the three levels' chain per plane step, the LDS traffic
+// (26 accesses of 1 KB per wave and step, about 1 us of a CU's LDS bandwidth)
+// and the tiles of 8 rows (14 rows of e_j read per 8) cost more than the
+// bytes saved, so it runs only when asked for (picard_split_sweeps_per_launch
+// with 3).
+constexpr int kTriWaves = 6;                   // waves per block
+constexpr int kTriRows = 2 * kTriWaves - 4;    // rows of a tile
+
+template <bool PAD, bool FIRST, int NW>
+__global__ void __launch_bounds__(64 * NW, 3) k_mx_triple_lds(
+    SweepGeo sg, SweepCoef cf, const float *__restrict__ r32, const float *__restrict__ e,
+    float *__restrict__ out, float *__restrict__ eo_out, const float *__restrict__ eo_prev,
+    const double *__restrict__ Pin, double *__restrict__ Pout, JacobiState *st, MxLaunch L) {
+  constexpr int CR = 2 * NW;                    // rows of the first sweep
+  __shared__ double sh[NW];
+  __shared__ int decide;
+  // e_j at f(k) in EJs[k & 1] (row i of the tile at index i + 1, i = -1..CR);
+  // e_{j+1} at f(k) in E1s[k & 1]; e_{j+2} at f(k-1) in E2s[k & 1]
+  __shared__ float4 EJs[2][CR + 2][64];
+  __shared__ float4 E1s[2][CR][64];
+  __shared__ float4 E2s[2][CR][64];
+  StopSlot cur = mx_decide(st, L, Pin, sh, &decide);
+  if (!cur.done && (int64_t)L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
+    cur.done = 1;
+    cur.result = st->last;
+    cur.buf = L.ibuf;
+  }
+  if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
+  if (cur.done) return;
+  const bool res = !FIRST && (L.test || cur.near);
+  const uint32_t NX = sg.NX, P = sg.P;
+  const int nx = sg.nx, ny = sg.ny, nz = sg.nz;
+  const int lane = (int)threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const uint32_t zrow = P * (uint32_t)nz;                   // the zero row of E
+  const int xs = 4 * lane;
+  const bool live = xs < nx;
+  const uint32_t x0 = live ? (uint32_t)xs : 0u;
+  const uint32_t zr = zrow + x0;
+  const int ql = nx - 1 - xs;
+  const bool haslast = live && ql < 4;
+  double r2 = 0.0;
+  const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3;
+  const int z0 = (xcd * nz) >> 3, z1 = ((xcd + 1) * nz) >> 3;
+  auto wrapz = [&](int pp) {
+    int kk = pp < 0 ? pp + nz : pp;
+    kk = kk < 0 ? kk + nz : kk;
+    kk = kk < 0 ? kk + nz : kk;
+    kk = kk >= nz ? kk - nz : kk;
+    kk = kk >= nz ? kk - nz : kk;
+    return kk >= nz ? kk - nz : kk;
+  };
+  auto pvalid = [&](int pp) { return sg.pz | ((pp >= 0) & (pp < nz)); };
+  auto poff = [&](int pp) { return P * (uint32_t)wrapz(pp); };
+  const int i0 = 2 * wv, i1 = 2 * wv + 1;                   // the wave's rows of the tile
+  const int ix = wv == 0 ? -1 : CR;                         // its extra row of e_j
+  for (int t = w; t < sg.TP; t += sg.W) {
+    const int zi = t % sg.nzc, ty = t / sg.nzc;
+    const int ka = z0 + ((z1 - z0) * zi) / sg.nzc, kb = z0 + ((z1 - z0) * (zi + 1)) / sg.nzc;
+    const int ya = (ty * ny) / sg.nty, yb = ((ty + 1) * ny) / sg.nty;
+    const int Yo = yb - ya;
+    if (Yo <= 0 || kb <= ka) continue;
+    // row i of the tile is y = ya - 2 + i
+    auto row_at = [&](int i, bool &ok) -> uint32_t {
+      int yy = ya - 2 + i;
+      ok = true;
+      if (yy < 0 || yy >= ny) {
+        if (sg.py) {
+          yy = yy % ny;
+          if (yy < 0) yy += ny;
+        } else {
+          ok = false;
+          yy = ya;
+        }
+      }
+      return x0 + NX * (uint32_t)yy;
+    };
+    bool ok0, ok1, okx;
+    const uint32_t ra0 = row_at(i0, ok0), ra1 = row_at(i1, ok1), rax = row_at(ix, okx);
+    // e_j is read at rows -1..Yo+4; the first sweep at rows 0..Yo+3, the
+    // second at 1..Yo+2, the third (the tile's rows) at 2..Yo+1
+    const bool rok0 = !FIRST && live && ok0 && i0 <= Yo + 4;
+    const bool rok1 = !FIRST && live && ok1 && i1 <= Yo + 4;
+    const bool hasx = !FIRST && (wv == 0 || (wv == NW - 1 && Yo + 4 >= CR));
+    const bool rokx = hasx && live && okx;
+    const bool a0 = ok0 && i0 <= Yo + 3, a1 = ok1 && i1 <= Yo + 3;
+    const bool b0 = ok0 && i0 >= 1 && i0 <= Yo + 2, b1 = ok1 && i1 >= 1 && i1 <= Yo + 2;
+    const bool op0 = i0 >= 2 && i0 <= Yo + 1, op1 = i1 >= 2 && i1 <= Yo + 1;
+    const int y0r = ya - 2 + i0, y1r = ya - 2 + i1;
+    const bool yN0 = sg.py && y0r == ny - 1, yS0 = sg.py && y0r == 0;
+    const bool yN1 = sg.py && y1r == ny - 1, yS1 = sg.py && y1r == 0;
+    auto eld = [&](uint32_t ra, bool rok, int pp) -> float4 {
+      if constexpr (FIRST) return zero4;      // e_0 = 0: nothing to read
+      return ld4(e + ((pvalid(pp) & rok) ? ra + poff(pp) : zr));
+    };
+    if (res) {
+      // the stop-test residual of e_j on the tile's rows (launches while the
+      // stop is near), in a pass of its own, row by row, the loads of plane
+      // p+1 issued before the residual of plane p
+#pragma unroll 1
+      for (int r = 0; r < 2; ++r) {
+        if (!(r ? op1 : op0)) continue;                      // wave-uniform
+        const int i = r ? i1 : i0;
+        const uint32_t rc = r ? ra1 : ra0;
+        bool okn, oks;
+        const uint32_t rn = row_at(i + 1, okn), rs = row_at(i - 1, oks);
+        const bool fN = r ? yN1 : yN0, fS = r ? yS1 : yS0;
+        struct RIn {
+          float4 t, n, s, q;
+        };
+        auto rissue = [&](RIn &a, int p) {
+          a.t = eld(rc, live, p + 1);
+          a.n = eld(rn, live && okn, p);
+          a.s = eld(rs, live && oks, p);
+          a.q = ld4(r32 + rc + poff(p));
+        };
+        float4 b = eld(rc, live, ka - 1), cc = eld(rc, live, ka);
+        auto rstep = [&](int p, const RIn &a) {
+          const uint32_t pk = P * (uint32_t)p;
+          const uint32_t pi = p == 0 ? P * (uint32_t)(nz - 1) : 0u;    // plane nz-1 or 0
+          (void)mx_cell4<PAD>(cf, cc, b, a.t, a.n, a.s, a.q, false, false, true, fN,
+                              eo_prev + x0 + pk, fS, eo_prev + x0 + NX * (uint32_t)(ny - 1) + pk,
+                              sg.pz && p == nz - 1, sg.pz && p == 0, eo_prev + rc + pi, live,
+                              haslast, ql, r2);
+          b = cc;
+          cc = a.t;
+        };
+        RIn RA, RB;
+        rissue(RA, ka);
+        for (int p = ka;; p += 2) {
+          rissue(RB, p + 1 < kb ? p + 1 : kb - 1);
+          rstep(p, RA);
+          if (p + 1 >= kb) break;
+          rissue(RA, p + 2 < kb ? p + 2 : kb - 1);
+          rstep(p + 1, RB);
+          if (p + 2 >= kb) break;
+        }
+      }
+    }
+    // the march: fronts f(0..K), up from ka-2 to kb+1 in even chunks, down
+    // in odd ones
+    const bool up = (zi & 1) == 0;
+    const int dz = up ? 1 : -1, K = kb - ka + 3;
+    auto front = [&](int k) { return up ? ka - 2 + k : kb + 1 - k; };
+    // registers: e_j of the wave's rows at f(k) (c*), e_{j+1} at f(k-1)
+    // (u*), e_{j+2} at f(k-2) (x*), r32 at f(k-1) (rp*) and f(k-2) (rq*);
+    // the level below each one's centre plane comes back from LDS, where
+    // only this wave writes its rows
+    float4 c0 = eld(ra0, rok0, front(0)), c1 = eld(ra1, rok1, front(0));
+    float4 u0 = zero4, u1 = zero4, x0v = zero4, x1v = zero4;
+    float4 rp0 = zero4, rp1 = zero4, rq0 = zero4, rq1 = zero4;
+    if (!FIRST) {
+      EJs[1][i0 + 1][lane] = eld(ra0, rok0, front(0) - dz);
+      EJs[1][i1 + 1][lane] = eld(ra1, rok1, front(0) - dz);
+      EJs[0][i0 + 1][lane] = c0;
+      EJs[0][i1 + 1][lane] = c1;
+      if (hasx) EJs[0][ix + 1][lane] = eld(rax, rokx, front(0));
+    }
+    struct In {
+      float4 e0, e1, ex, r0, r1;
+    };
+    auto issue = [&](In &a, int k) {
+      const int f = front(k);
+      a.e0 = eld(ra0, rok0, f + dz);
+      a.e1 = eld(ra1, rok1, f + dz);
+      if (hasx) a.ex = eld(rax, rokx, f + dz);
+      const uint32_t orr = pvalid(f) ? poff(f) : P * (uint32_t)ka;
+      a.r0 = ld4(r32 + ra0 + orr);
+      a.r1 = ld4(r32 + ra1 + orr);
+    };
+    In A, B;
+    A.ex = B.ex = zero4;
+    issue(A, 0);
+    __syncthreads();
+    auto step = [&](int k, const In &a) {
+      const int p = front(k);
+      const int s = k & 1;
+      // first sweep at f(k)
+      const bool pv = pvalid(p);
+      const float4 s0 = FIRST ? zero4 : EJs[s][i0][lane];
+      const float4 n1 = FIRST ? zero4 : EJs[s][i1 + 2][lane];
+      const float4 m0 = FIRST ? zero4 : EJs[s ^ 1][i0 + 1][lane];
+      const float4 m1 = FIRST ? zero4 : EJs[s ^ 1][i1 + 1][lane];
+      const float4 v0 = mx_cell4<PAD>(cf, c0, m0, a.e0, c1, s0, a.r0, !pv || !a0, FIRST, false,
+                                      false, nullptr, false, nullptr, false, false, nullptr, live,
+                                      haslast, ql, r2);
+      const float4 v1 = mx_cell4<PAD>(cf, c1, m1, a.e1, n1, c0, a.r1, !pv || !a1, FIRST, false,
+                                      false, nullptr, false, nullptr, false, false, nullptr, live,
+                                      haslast, ql, r2);
+      // second sweep at f(k-1)
+      float4 w0 = zero4, w1 = zero4;
+      if (k >= 2) {
+        const int q = p - dz;
+        const bool qv = pvalid(q);
+        const float4 ub0 = E1s[s][i0][lane], ub1 = E1s[s][i1][lane];        // f(k-2)
+        const float4 us0 = wv > 0 ? E1s[s ^ 1][i0 - 1][lane] : zero4;
+        const float4 un1 = wv < NW - 1 ? E1s[s ^ 1][i1 + 1][lane] : zero4;
+        w0 = mx_cell4<PAD>(cf, u0, ub0, v0, u1, us0, rp0, !qv || !b0, false, false, false,
+                           nullptr, false, nullptr, false, false, nullptr, live, haslast, ql, r2);
+        w1 = mx_cell4<PAD>(cf, u1, ub1, v1, un1, u0, rp1, !qv || !b1, false, false, false,
+                           nullptr, false, nullptr, false, false, nullptr, live, haslast, ql, r2);
+        if (live && q >= ka && q < kb) {
+          const bool zT = sg.pz && q == nz - 1, zB = sg.pz && q == 0;
+          if (op0 && (yN0 || yS0 || zT || zB)) st4(eo_out + ra0 + P * (uint32_t)q, w0);
+          if (op1 && (yN1 || yS1 || zT || zB)) st4(eo_out + ra1 + P * (uint32_t)q, w1);
+        }
+      }
+      // third sweep at f(k-2) on the tile's rows
+      if (k >= 4) {
+        const uint32_t po = P * (uint32_t)(p - 2 * dz);
+        const float4 xb0 = E2s[s][i0][lane], xb1 = E2s[s][i1][lane];        // f(k-3)
+        if (op0) {
+          const float4 xs0 = E2s[s ^ 1][i0 - 1][lane];
+          const float4 y = mx_cell4<PAD>(cf, x0v, xb0, w0, x1v, xs0, rq0, false, false, false,
+                                         false, nullptr, false, nullptr, false, false, nullptr,
+                                         live, haslast, ql, r2);
+          if (live) st4(out + ra0 + po, y);
+        }
+        if (op1) {
+          const float4 xn1 = E2s[s ^ 1][i1 + 1][lane];
+          const float4 y = mx_cell4<PAD>(cf, x1v, xb1, w1, xn1, x0v, rq1, false, false, false,
+                                         false, nullptr, false, nullptr, false, false, nullptr,
+                                         live, haslast, ql, r2);
+          if (live) st4(out + ra1 + po, y);
+        }
+      }
+      // hand on e_j at f(k+1), e_{j+1} at f(k), e_{j+2} at f(k-1)
+      if (!FIRST) {
+        EJs[s ^ 1][i0 + 1][lane] = a.e0;
+        EJs[s ^ 1][i1 + 1][lane] = a.e1;
+        if (hasx) EJs[s ^ 1][ix + 1][lane] = a.ex;
+      }
+      E1s[s][i0][lane] = v0;
+      E1s[s][i1][lane] = v1;
+      E2s[s][i0][lane] = w0;
+      E2s[s][i1][lane] = w1;
+      x0v = w0;
+      x1v = w1;
+      u0 = v0;
+      u1 = v1;
+      rq0 = rp0;
+      rq1 = rp1;
+      rp0 = a.r0;
+      rp1 = a.r1;
+      c0 = a.e0;
+      c1 = a.e1;
+      __syncthreads();
+    };
+    for (int k = 0;; k += 2) {
+      issue(B, k + 1 < K ? k + 1 : K);
+      step(k, A);
+      if (k + 1 > K) break;
+      issue(A, k + 2 < K ? k + 2 : K);
+      step(k + 1, B);
+      if (k + 2 > K) break;
+    }
+  }
+  const double sblk = block_sum(r2, sh);
+  if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
+}
+
 // End of an outer step: the stop test of its last inner iterate and, when
 // the solve ended inside this step (converged, or the last iterate reached),
 // u += e_k for that iterate's k; otherwise nothing (the next outer step
@@ -1571,6 +1855,7 @@ __global__ void __launch_bounds__(kStencilT) k_mx_update(GridDims g, CompactDims
   if (!cur.done && L.base + L.m >= st->last) {
     cur.done = 1;
     cur.result = st->last;
+    cur.buf = L.ibuf;
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
   // solves that ended before this outer step have nothing left to apply
@@ -1578,8 +1863,7 @@ __global__ void __launch_bounds__(kStencilT) k_mx_update(GridDims g, CompactDims
   if (!cur.done) return;
   const int64_t k = cur.result - L.base;
   if (k <= 0) return;
-  const int64_t bi = L.pairs ? ((k + 1) >> 1) & 1 : k & 1;
-  const float *e = bi ? E1 : E0;
+  const float *e = cur.buf ? E1 : E0;
   for_my_cells(ts, g.nx, g.ny, g.nz, [&](int i, int j, int kk, bool valid) {
     if (!valid) return;
     const int64_t c = node_index(g, i, j, kk);
@@ -1968,6 +2252,57 @@ void launch_pair_lds(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEven
                         out, eo_out, eo_prev, pin, pout, js, L);
 }
 
+// The three-sweep LDS kernel's schedule: tiles of at most kTriRows rows x z
+// chunks of an XCD slab, as lds_sched.
+SweepGeo tri_sched(picard_ctx *c) {
+  const GridDims &g = c->gd;
+  if (c->tri_wmax <= 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_mx_triple_lds<false, false, kTriWaves>, 64 * kTriWaves, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        per_cu <= 0 || cus < 8) {
+      c->tri_wmax = 32;
+    } else {
+      c->tri_wmax = (cus / 8) * per_cu;
+    }
+  }
+  const int wmax = c->tri_wmax;
+  SweepGeo sg;
+  sg.nx = g.nx;
+  sg.ny = g.ny;
+  sg.nz = g.nz;
+  sg.NX = (uint32_t)(g.nx + 3) & ~3u;
+  sg.P = sg.NX * (uint32_t)g.ny;
+  sg.px = !g.flow[0];
+  sg.py = !g.flow[1];
+  sg.pz = !g.flow[2];
+  sg.ntx = 1;
+  sg.nty = (g.ny + kTriRows - 1) / kTriRows;
+  int nzc = wmax / sg.nty;
+  const int zmax = g.nz / 64 > 1 ? g.nz / 64 : 1;
+  if (nzc > zmax) nzc = zmax;
+  if (nzc < 1) nzc = 1;
+  sg.nzc = nzc;
+  sg.TP = sg.nty * nzc;
+  int W = sg.TP < wmax ? sg.TP : wmax;
+  W = (sg.TP + (sg.TP + W - 1) / W - 1) / ((sg.TP + W - 1) / W);
+  if (W > (int)(c->s.partials_len / 8)) W = (int)(c->s.partials_len / 8);
+  sg.W = W < 1 ? 1 : W;
+  return sg;
+}
+
+void launch_triple_lds(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t t1,
+                       const SweepGeo &sg, const SweepCoef &cf, const float *r32, const float *e,
+                       float *out, float *eo_out, const float *eo_prev, const double *pin,
+                       double *pout, JacobiState *js, const MxLaunch &L) {
+  constexpr int W = kTriWaves;
+  auto fn = L.j == 0 ? (pad ? k_mx_triple_lds<true, true, W> : k_mx_triple_lds<false, true, W>)
+                     : (pad ? k_mx_triple_lds<true, false, W> : k_mx_triple_lds<false, false, W>);
+  hipExtLaunchKernelGGL(fn, dim3(blocks), dim3(64, kTriWaves), 0, s, t0, t1, 0, sg, cf, r32, e,
+                        out, eo_out, eo_prev, pin, pout, js, L);
+}
+
 // One two-sweep launch (timed with events t0/t1 when given).
 template <bool PX, bool MULTI>
 void launch_pair(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t t1,
@@ -2073,34 +2408,39 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
     cf.idz2 = mc.idz2f;
   }
   const int m = c->mx_inner;
-  // two sweeps per launch where rows are one segment and m is even
   // two sweeps per launch: rows of one segment, or wider rows with x not
-  // periodic (overlapping segments)
+  // periodic (overlapping segments), and an even inner length
   const bool pairs = c->mx_pairs && (sg.ntx == 1 || g.flow[0]) && m % 2 == 0;
-  // rows of one segment with x not periodic: rows shared through LDS, except
-  // in the launches that test the stop, which take the register-window kernel
-  // (it evaluates the residual in its first sweep, 3 us more per launch, where
-  // the LDS kernel needs a pass of its own, 30 us): the launches the host
-  // marks, and every launch of a refinement step that begins with the stop
-  // near (the flag the last one left; near the end of a slowly converging
-  // solve it stays set for many refinement steps)
+  // rows of one segment with x not periodic: rows shared through LDS, three
+  // sweeps per launch (or two where the sweeps left call for it), except in
+  // the launches that test the stop, which take the register-window
+  // two-sweep kernel (it evaluates the residual in its first sweep, 3 us
+  // more per launch, where the LDS kernels need a pass of their own, 30 us):
+  // the launches the host marks, and every launch of a refinement step that
+  // begins with the stop near (the flag the last one left; near the end of a
+  // slowly converging solve it stays set for many refinement steps)
   const bool lds = pairs && c->pair_lds && sg.ntx == 1 && g.flow[0];
+  const bool tri = lds && c->mx_triples;
   const SweepGeo pgeo = pairs ? pair_sched(c) : sg;
   const SweepGeo lgeo = lds ? lds_sched(c) : pgeo;
-  const int prB = 8 * pgeo.W, plB = 8 * lgeo.W;
+  const SweepGeo tgeo = tri ? tri_sched(c) : pgeo;
+  const int prB = 8 * pgeo.W, plB = 8 * lgeo.W, ptB = 8 * tgeo.W;
   double *P[2] = {c->s.partials, c->s.partials + c->s.partials_len};
   float *E0 = c->e32[0], *E1 = c->e32[1];
   float *const E[2] = {E0, E1};
-  // the buffer of e_k: E[k % 2] one sweep per launch; with two, a launch
-  // reads E[(k/2) % 2] and writes the other, so e_k (k even) is in
-  // E[(k/2) % 2] and the odd iterate a single sweep leaves at the end of
-  // the solve is in the buffer after it
-  auto ebuf = [&](int k) -> float * { return pairs ? E[((k + 1) / 2) & 1] : E[k & 1]; };
+  // a launch reads e_j from E[cur] and writes its last iterate to the other
+  // buffer; the launch input the stop test picks, and e_m, are found by the
+  // buffer index the host passes (ibuf, cbuf) and the stop slot keeps
+  int cur = 0;
   const int64_t last_it = init.last;
+  // boundary copies of e_k in slot k % 4: a launch of s sweeps with input j
+  // reads those of e_{j-1} (its stop test) and writes those of e_{j+s-1}
+  // (e_{j+1} for a single sweep), which the next launch's test takes
   float *const *EO = c->eo32;
   int32_t seq = 1, base = 0;
   int pw = 0;                               // partials buffer the next writer uses
   int32_t check = -1;                       // iterate whose residual the last writer stored
+  int32_t cbuf = 0;                         // the buffer holding iterate `check`
   bool prevtest = true;                     // the last launch tested whatever its near flag
   int32_t nin = 0;                          // partials the last writer stored
   StopSlot fin{};
@@ -2120,10 +2460,10 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
     // launch); the stale-halo norm needs e_m and e_{m-1} of the previous outer
     double *usrc = Ub[outer & 1], *udst = Ub[(outer + 1) & 1];
     {
-      MxLaunch L{seq, base, 0, -1, m, 0, pairs ? 1 : 0, 1, 1};
+      MxLaunch L{seq, base, 0, -1, m, 0, pairs ? 1 : 0, 1, 1, cur, cur};
       // e_m and the boundary copies of e_{m-1} of the previous outer step
-      const float *em = ebuf(m);
-      const float *eo = EO[(m - 1) % 3];
+      const float *em = E[cur];
+      const float *eo = EO[(m - 1) % 4];
       if (base > 0)
         k_mx_outer<true><<<ouB, dim3(64, kOutRows), 0, c->stream>>>(
             og, mc.dx2, mc.dy2, mc.dz2, usrc, udst, c->b, c->r32, em, eo, P[pw ^ 1], P[pw],
@@ -2134,70 +2474,97 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
             P[pw], c->js, L);
       ++seq;
       check = base;
+      cbuf = cur;
       nin = ouB;
       pw ^= 1;
       prevtest = true;
     }
-    int nrec = 0;                           // timed launches of this outer step
-    for (int j = 0; j < m;) {
+    int nrec = 0, tsw = 0;                  // timed launches of this outer step, their sweeps
+    int li = 0;                             // launches of this outer step
+    int jt = 2 * kPairTestEvery;            // the next iterate position due a stop test
+    for (int j = 0; j < m; ++li) {
       // the iterate-0 test used b itself (launch 0); later ones use residuals
       const int32_t chk = (check >= 1) ? check : -1;
-      // two sweeps, unless the first of them reaches iterate 2*maxit+1 (the
-      // solve's last, which must exist)
-      const bool two = pairs && j + 2 <= m && base + j + 1 != last_it;
-      // the stop test of iterate base + j: every single sweep, every
-      // kPairTestEvery-th two-sweep launch (every 2 kPairTestEvery iterates)
-      // and, on the device, every launch while the stop is near
-      const bool test = j >= 1 && (!two || (j / 2) % kPairTestEvery == 0);
-      MxLaunch L{seq, base, j, chk, m, nin, pairs ? 1 : 0, test ? 1 : 0, prevtest ? 1 : 0};
+      const int r = m - j;                  // sweeps left in this refinement step
+      const int64_t d = last_it - ((int64_t)base + j);   // sweeps to iterate 2*maxit+1
+      // a launch of s sweeps may not pass over iterate 2*maxit+1 (the
+      // solve's last, which must exist) and leaves no single sweep before the
+      // end of the refinement step (the next one's residual needs e_{m-1})
+      auto fits = [&](int sz) { return sz <= r && (d >= sz || d <= 0); };
+      // the stop test of iterate base + j: every single sweep, about every
+      // 2 kPairTestEvery iterates with more per launch and, on the device,
+      // every launch while the stop is near
+      const bool due = j >= 1 && j >= jt;
+      int size = 1;
+      if (pairs) {
+        if (tri && fits(3) && r != 4 && (!due || r == 3) && !stop_near) size = 3;
+        else if (fits(2)) size = 2;
+      }
+      const bool test = j >= 1 && (size == 1 || due);
+      if (test && size > 1) jt = j + 2 * kPairTestEvery;
+      MxLaunch L{seq, base, j, chk, m, nin, pairs ? 1 : 0, test ? 1 : 0, prevtest ? 1 : 0,
+                 cur, cbuf};
       const double *pin = P[pw ^ 1];
       double *pout = P[pw];
-      // e_j (none for j = 0) -> e_{j+1} or e_{j+2}; boundary copies of
-      // e_{j+1} to slot (j+1) % 3, e_{j-1} from slot (j-1) % 3
-      const float *e = j >= 1 || two ? ebuf(j) : nullptr;
-      float *out = ebuf(j + (two ? 2 : 1));
-      float *eo_out = EO[(j + 1) % 3];
-      const float *eo_prev = j >= 2 ? EO[(j - 1) % 3] : nullptr;
+      // e_j (none for a single first sweep) -> e_{j+size}
+      const float *e = j >= 1 || size > 1 ? E[cur] : nullptr;
+      float *out = E[cur ^ 1];
+      float *eo_out = EO[(size == 1 ? j + 1 : j + size - 1) % 4];
+      const float *eo_prev = j >= 2 ? EO[(j - 1) % 4] : nullptr;
       // the timed position moves by two launches per refinement step, so that
       // launches with and without the stop test are sampled in proportion
       const int tpos = (kTimeEvery / 2 + 2 * outer) % kTimeEvery;
-      const bool tj = c->timing && j >= 1 && j % kTimeEvery == tpos && nrec < timed;
+      const bool tj = c->timing && j >= 1 && li % kTimeEvery == tpos && nrec < timed;
       hipEvent_t t0 = tj ? c->tev[2 * nrec] : nullptr;
       hipEvent_t t1 = tj ? c->tev[2 * nrec + 1] : nullptr;
-      if (tj) ++nrec;
-      const bool use_lds = two && lds && !test && !stop_near;
-      if (use_lds)
+      if (tj) {
+        ++nrec;
+        tsw += size;
+      }
+      const bool use_lds = size == 2 && lds && !test && !stop_near;
+      int blocks = swB;
+      if (size == 3) {
+        blocks = ptB;
+        launch_triple_lds(pad, ptB, c->stream, t0, t1, tgeo, cf, c->r32, e, out, eo_out, eo_prev,
+                          pin, pout, c->js, L);
+      } else if (use_lds) {
+        blocks = plB;
         launch_pair_lds(pad, plB, c->stream, t0, t1, lgeo, cf, c->r32, e, out, eo_out, eo_prev,
                         pin, pout, c->js, L);
-      else if (two && sg.px)
-        launch_pair<true, false>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out,
-                                 eo_prev, pin, pout, c->js, L);
-      else if (two && pgeo.ntx > 1)
-        launch_pair<false, true>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out,
-                                 eo_prev, pin, pout, c->js, L);
-      else if (two)
-        launch_pair<false, false>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out,
-                                  eo_prev, pin, pout, c->js, L);
-      else if (sg.px)
+      } else if (size == 2) {
+        blocks = prB;
+        if (sg.px)
+          launch_pair<true, false>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out,
+                                   eo_prev, pin, pout, c->js, L);
+        else if (pgeo.ntx > 1)
+          launch_pair<false, true>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out,
+                                   eo_prev, pin, pout, c->js, L);
+        else
+          launch_pair<false, false>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out,
+                                    eo_out, eo_prev, pin, pout, c->js, L);
+      } else if (sg.px) {
         launch_sweep<true>(pad, swB, c->stream, t0, t1, sg, cf, c->r32, e, out, eo_out, eo_prev,
                            pin, pout, c->js, L);
-      else
+      } else {
         launch_sweep<false>(pad, swB, c->stream, t0, t1, sg, cf, c->r32, e, out, eo_out, eo_prev,
                             pin, pout, c->js, L);
+      }
       ++seq;
       if (j >= 1) {               // may have written the residual of iterate base + j
         check = base + j;
-        nin = two ? (use_lds ? plB : prB) : swB;
+        cbuf = cur;
+        nin = blocks;
         pw ^= 1;
       } else {
         check = -1;
       }
       prevtest = test;
-      j += two ? 2 : 1;
+      cur ^= 1;
+      j += size;
     }
     {
       const int32_t chk = (check >= 1) ? check : -1;
-      MxLaunch L{seq, base, 0, chk, m, nin, pairs ? 1 : 0, 1, prevtest ? 1 : 0};
+      MxLaunch L{seq, base, 0, chk, m, nin, pairs ? 1 : 0, 1, prevtest ? 1 : 0, cur, cbuf};
       k_mx_update<<<ts.B, dim3(ts.bx, ts.by), 0, c->stream>>>(g, cd, ts, udst, E0, E1, P[pw ^ 1],
                                                               c->js, L);
       ++seq;
@@ -2214,6 +2581,7 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
         c->t_ms += ms;
       }
       c->t_launches += nrec;
+      c->t_sweeps += tsw;
     }
     fin = c->js_host->slot[(seq - 1) & 1];
     stop_near = fin.near != 0;

@@ -682,7 +682,8 @@ class Domain:
         check("picard_split_solver", lib().picard_split_solver(self.h, inner))
 
     def split_sweeps_per_launch(self, sweeps: int = 2):
-        """Inner sweeps per launch of the FP32-split solve (1 or 2)."""
+        """At most this many inner sweeps per launch of the FP32-split solve
+        (1, 2 or 3; the library's default is 2)."""
         check("picard_split_sweeps_per_launch",
               lib().picard_split_sweeps_per_launch(self.h, sweeps))
 
@@ -690,6 +691,12 @@ class Domain:
         ms, n = ct.c_double(), ct.c_int64()
         check("picard_timing_read", lib().picard_timing_read(self.h, ct.byref(ms), ct.byref(n)))
         return ms.value, n.value
+
+    def timing_sweeps(self):
+        """Jacobi sweeps of the launches timing_read timed."""
+        n = ct.c_int64()
+        check("picard_timing_sweeps", lib().picard_timing_sweeps(self.h, ct.byref(n)))
+        return n.value
 
     def timing_solve(self):
         """Wall time [ms] of the solve phase of step() since timing(True), and

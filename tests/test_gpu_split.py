@@ -338,9 +338,49 @@ def test_split_lds_pair_kernel_matches_register_kernel(dims, flow, monkeypatch):
     assert np.array_equal(Va, Vb)
 
 
+@pytest.mark.parametrize("dims,flow", [((32, 24, 20), (True, False, False)),
+                                       ((38, 29, 11), (True, False, True)),
+                                       ((64, 45, 17), (True, True, False)),
+                                       ((256, 30, 9), (True, False, False)),
+                                       ((36, 2, 8), (True, False, False)),
+                                       ((29, 3, 2), (True, False, False)),
+                                       ((40, 20, 128), (True, False, False)),
+                                       ((36, 16, 130), (True, True, True)),
+                                       ((30, 17, 9), (False, False, False))])
+def test_split_three_sweeps_per_launch_same_iterates(dims, flow):
+    """Three sweeps per launch (rows of at most 256 cells with x not
+    periodic: tiles of up to 8 rows, periodic and flow y and z, padded rows,
+    chunks marching up and down; x periodic falls back to two) give the
+    iterates of single sweeps bit for bit: with every stop test failing, both
+    end at iterate 2*maxit+1 with the same potential, also where that iterate
+    or the end of a refinement step falls inside a launch of three (a launch
+    of two or one then lands on it).  With the stop rule live, the solve
+    stops on the same iterate or a few sweeps later."""
+    nx, ny, nz = dims
+    cfg = host.Config(nx=nx, ny=ny, nz=nz, flow=flow,
+                      gmin=(-5.0 * nx, -5.0 * ny, -5.0 * nz), gmax=(5.0 * nx, 5.0 * ny, 5.0 * nz),
+                      ppc=(0, 8, 0, 0, 0, 0, 0, 0))
+    d = host.derive(cfg)
+    rng = np.random.default_rng(nx * 1000 + ny * 10 + nz + 3)
+    C = rng.normal(size=(nz + 2, ny + 2, nx + 2)) * 1e-16
+    for maxit, inner in ((7, 128), (70, 128), (45, 16), (20, 2), (34, 22)):
+        s1, U1 = _split_solve(cfg, d, C, 1, 1e-30, maxit, inner)
+        s3, U3 = _split_solve(cfg, d, C, 3, 1e-30, maxit, inner)
+        _say(f"dims={dims} flow={flow} maxit={maxit} inner={inner}: sweeps {s1} / {s3}")
+        assert s1 == s3 == 2 * maxit + 1
+        assert np.array_equal(U1, U3)
+    s1, U1 = _split_solve(cfg, d, C, 1, 1e-6, 2 ** 20)
+    s3, U3 = _split_solve(cfg, d, C, 3, 1e-6, 2 ** 20)
+    _say(f"dims={dims} three per launch, converged: sweeps {s1} / {s3}, U rel {_rel(U3, U1):.2e}")
+    assert s1 <= s3 <= s1 + 7
+    assert _rel(U3, U1) <= 1e-6
+
+
 def test_split_sweeps_per_launch_rejects_bad_value():
     cfg = _cfg(8)
     dom = host.Domain(cfg, max_particles=1024, precision=_lib.PICARD_FP32_SPLIT)
     with pytest.raises(RuntimeError):
-        dom.split_sweeps_per_launch(3)
+        dom.split_sweeps_per_launch(4)
+    with pytest.raises(RuntimeError):
+        dom.split_sweeps_per_launch(0)
     dom.close()
