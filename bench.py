@@ -6,6 +6,7 @@
                     [--form both|split|fp64] [--no-cpu-baseline]
     python bench.py --solve-only SWEEPS [--form split|fp64]   (profiling)
     python bench.py --copy-cal                                (profiling)
+    python bench.py --stream-rates                            (stream reference)
     python bench.py --error-steps N [--grid G]               (split vs FP64 errors)
 
 Workload: 256^3 cells of 10 m, H+ and e- at 32 per cell (1.07e9 particles),
@@ -376,20 +377,25 @@ def server_bench(rate, jobs, scale_steps, nmin, nmax, time_limit, seed=20261211)
 
 def copy_calibration():
     """Counter calibration: three device copies of 1 GiB (read 1 GiB, write
-    1 GiB each), the plain copy kernel the HBM byte counters are scaled by.
-    Then the streaming rates the solve kernels are held against: torch's copy
-    (read 1, write 1) and add (read 2, write 1, the byte mix of an FP32 sweep)
-    on float32 arrays of the 256^3 solve's size (67 MB each, all three fit
-    the 256 MB last-level cache) and of 1 GiB each (they do not), timed with
-    events over 20 launches after 3 warmups."""
+    1 GiB each), the plain copy kernel the HBM byte counters are scaled by."""
     import torch
     a = torch.zeros(1 << 28, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    out = {"mode": "copy-cal", "bytes_read": 1 << 30, "bytes_written": 1 << 30, "copies": 3}
-    del a, b
+    print(json.dumps({"mode": "copy-cal", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
+                      "copies": 3}), flush=True)
+
+
+def stream_rates():
+    """The streaming rates the solve kernels are held against: torch's copy
+    (read 1, write 1) and add (read 2, write 1, the byte mix of an FP32 sweep)
+    on float32 arrays of the 256^3 solve's size (67 MB each, all three fit
+    the 256 MB last-level cache) and of 1 GiB each (they do not), timed with
+    events over 20 launches after 3 warmups."""
+    import torch
+    out = {"mode": "stream-rates"}
     rates = {}
     for label, n in (("67MB", 1 << 24), ("1GiB", 1 << 28)):
         x = torch.rand(n, device="cuda")
@@ -407,7 +413,7 @@ def copy_calibration():
             torch.cuda.synchronize()
             us = e0.elapsed_time(e1) * 1e3 / 20
             rates[f"{op}_{label}"] = {"us": us, "GBs": nbytes / us / 1e3}
-            log(f"copy-cal: {op} {label}: {us:.1f} us, {nbytes / us / 1e3:.0f} GB/s")
+            log(f"stream rates: {op} {label}: {us:.1f} us, {nbytes / us / 1e3:.0f} GB/s")
         del x, y, z
     out["stream_rates"] = rates
     print(json.dumps(out), flush=True)
@@ -490,6 +496,8 @@ def main():
                          "(0: the library's default, 2)")
     ap.add_argument("--stream-ref", type=int, default=0, metavar="N",
                     help="print the stream reference rate for N float32 elements and exit")
+    ap.add_argument("--stream-rates", action="store_true",
+                    help="time torch copy and add at the solve's array size and at 1 GiB")
     ap.add_argument("--copy-cal", action="store_true",
                     help="profiling: the 1 GiB copies the byte counters are scaled by")
     ap.add_argument("--ensemble", type=int, default=0, metavar="SIMS",
@@ -546,6 +554,9 @@ def main():
         return
     if args.copy_cal:
         copy_calibration()
+        return
+    if args.stream_rates:
+        stream_rates()
         return
     if args.solve_only:
         solve_only(cfg, "fp64" if args.form == "fp64" else "split", args.solve_only, args.inner)
