@@ -261,6 +261,16 @@ def error_report(cfg, steps):
                    phi_rel=_rel(a["U"], b["U"]), E_rel=_rel(a["E"], b["E"]), N_rel=_rel(a["N"], b["N"]))
         per_step.append(row)
         log(f"error report: step {it}: " + ", ".join(f"{k} {v}" for k, v in row.items() if k != "step"))
+    if max(doms["fp64"].num_local, doms["split"].num_local) > 300_000_000:
+        # matching 1e9 particles by id on the host takes minutes: per-step
+        # fields only at that size
+        for dom in doms.values():
+            dom.close()
+        print(json.dumps({"mode": "error-report", "grid": [cfg.nx, cfg.ny, cfg.nz],
+                          "steps": steps, "per_step": per_step,
+                          "trajectories": "not matched (more than 3e8 particles)"}), flush=True)
+        return
+    log("error report: matching particles by id")
     ra, _ = doms["fp64"].particles()
     rb, _ = doms["split"].particles()
     for dom in doms.values():
@@ -357,14 +367,17 @@ def ensemble_bench(count, steps, warmup, tune_sweeps=32, baseline=True, seed=202
     return out
 
 
-def server_bench(rate, jobs, scale_steps, nmin, nmax, time_limit, seed=20261211):
+def server_bench(rate, jobs, scale_steps, nmin, nmax, time_limit, seed=20261211,
+                 lanes_split=80):
     """The ensemble server on a seeded Poisson stream of jobs (rate jobs/s),
     grids log-uniform in nmin^3-nmax^3, 8-32 particles per cell, 200-2000
-    steps scaled by `scale_steps`, admitted by a 250 GB HBM budget."""
+    steps scaled by `scale_steps`, admitted by a 250 GB HBM budget; jobs up
+    to lanes_split^3 in one lane and the rest in another (0: one lane)."""
     from picard_amd import server
     js = server.draw_jobs(rate, jobs, seed=seed, nmin=nmin, nmax=nmax,
                           steps=(max(1, int(200 * scale_steps)), max(1, int(2000 * scale_steps))))
-    srv = server.Server(tune_sweeps=32)
+    srv = server.Server(tune_sweeps=32,
+                        lanes=server.two_lanes(lanes_split) if lanes_split else None)
     rep = srv.run(js, time_limit=time_limit, log=log)
     srv.close()
     done = rep.pop("done")
@@ -514,6 +527,9 @@ def main():
     ap.add_argument("--nmax", type=int, default=160, help="--server: largest grid edge")
     ap.add_argument("--nmin", type=int, default=32, help="--server: smallest grid edge")
     ap.add_argument("--time-limit", type=float, default=240.0, help="--server: seconds")
+    ap.add_argument("--lanes-split", type=int, default=80,
+                    help="--server: jobs with grid edges up to this in one lane, the rest in "
+                         "another, each stepped by its own host thread (0: one lane)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the domain "
@@ -536,7 +552,7 @@ def main():
         return
     if args.server:
         r = server_bench(args.rate, args.jobs, args.scale_steps, args.nmin, args.nmax,
-                         args.time_limit)
+                         args.time_limit, lanes_split=args.lanes_split)
         print(json.dumps({"metric": "p50/p99 job completion latency and particle-steps/s, "
                                     "ensemble server",
                           "value": r["particle_steps_per_s"], "unit": "particle-steps/s",
@@ -546,7 +562,10 @@ def main():
                                                  f"{args.jobs} jobs, grids log-uniform "
                                                  f"{args.nmin}^3-{args.nmax}^3, 8-32 ppc, "
                                                  f"200-2000 steps x {args.scale_steps}, 250 GB "
-                                                 "HBM admission budget", **r}}), flush=True)
+                                                 "HBM admission budget, "
+                                                 + (f"lanes: edges up to {args.lanes_split} and "
+                                                    "above" if args.lanes_split else "one lane"),
+                                     **r}}), flush=True)
         return
     cfg = host.config2(args.grid)
     if args.stream_ref:

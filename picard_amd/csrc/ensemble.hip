@@ -20,13 +20,19 @@
 // sweeps on A e = r.  Every sweep launch also evaluates the residual of the
 // iterate it reads, r - A e_j, and converts each cell's square to fixed point
 // (units of the simulation's threshold^2 * 2^-40, clamped at 2^40 + 1); each
-// tile stores its integer sum, and a small launch after the sweep sums each
-// simulation's tiles (integers: the same for any tiling and any mix of
-// simulations) and applies the stop rule of CalcEpotential.f90:60-62 (sum <=
-// 2^40, i.e. |res| <= maxerr |b|), advancing or stopping that simulation.
-// Simulations stop independently; tiles of stopped ones are skipped.  (One
-// atomic per tile on a per-simulation sum and counter cost 0.76 ms per sweep
-// for 64 simulations: the tiles of one simulation met on the same address.)
+// tile stores its integer sum, and a small launch after every kDecideEvery
+// sweeps sums each simulation's tiles (integers: the same for any tiling and
+// any mix of simulations) for each of those iterates in turn and applies the
+// stop rule of CalcEpotential.f90:60-62 (sum <= 2^40, i.e. |res| <= maxerr
+// |b|) to the first that passes, advancing or stopping that simulation.  The
+// corrections rotate over kDecideEvery + 1 buffers, so the iterate the rule
+// picks is still held when it is picked: a simulation stops on the same
+// iterate as with a decision after every sweep, and the sweeps it ran past it
+// are discarded.  Simulations stop independently; tiles of stopped ones are
+// skipped.  (One atomic per tile on a per-simulation sum and counter cost
+// 0.76 ms per sweep for 64 simulations: the tiles of one simulation met on the
+// same address.  A decision launch after every sweep cost 4 us of kernel and a
+// launch gap per sweep.)
 #include <hip/hip_ext.h>
 
 #include <algorithm>
@@ -54,6 +60,8 @@ constexpr double kTau = 6.2831853071795864769252867665590;
 constexpr double kEC = 1.60217646e-19, kMP = 1.6726216e-27, kME = 9.1093819e-31;
 constexpr double kEPS0 = 8.854187817e-12;
 constexpr int kChunkSweeps = 64;     // sweeps per replay of the solve graph
+constexpr int kDecideEvery = 4;      // sweeps per stop decision (divides kChunkSweeps)
+constexpr int kEBufs = kDecideEvery + 1;   // correction buffers
 constexpr int kFacePlanes = 8;       // x-planes per face-sum tile
 constexpr int kMaxSlots = 4096;
 
@@ -104,6 +112,8 @@ struct Global {
   int32_t n_live, n_inflow_map;
   int32_t active;           // simulations still solving
   int32_t err;              // bit 0: particle capacity; bit 1: particle outside its grid
+  int32_t chunk_j0;         // inner sweep index of the solve chunk's first sweep
+  int32_t pad;
 };
 
 struct Tile {               // work-list item
@@ -120,7 +130,9 @@ struct PartArena {
 };
 
 struct Arenas {
-  unsigned long long *units;   // per solve tile: fixed-point residual units of the last sweep
+  unsigned long long *units;   // per solve tile and iterate of the decision group:
+                               // fixed-point residual units, [j % kDecideEvery][tile]
+  int64_t ucap;             // tiles per units slot
   SimDev *sims;
   SimState *state;
   Global *g;
@@ -130,7 +142,7 @@ struct Arenas {
   double *U, *u[2], *C, *N, *D, *fscr;
   unsigned long long *Nfix;
   float4 *E32;
-  float *r32, *e32[2];
+  float *r32, *e32[kEBufs];
   int32_t *hist, *cursor, *key;
   int64_t kcap;             // key capacity
 };
@@ -637,7 +649,7 @@ __global__ void __launch_bounds__(kT) k_outer(Arenas A, int cpt) {
     const bool upd = st.have_e != 0;
     const double *uc = A.u[st.ucur] + sd.gbase;
     double *un = A.u[st.ucur ^ 1] + sd.gbase;
-    const float *e = A.e32[st.j & 1] + sd.cbase;
+    const float *e = A.e32[st.j % kEBufs] + sd.cbase;
     const int64_t X = sd.X, XY = (int64_t)sd.X * sd.Y;
     const uint32_t nx = (uint32_t)sd.nx, ny = (uint32_t)sd.ny, nz = (uint32_t)sd.nz;
     const uint32_t pl = nx * ny;
@@ -677,6 +689,7 @@ __global__ void __launch_bounds__(kT) k_outer(Arenas A, int cpt) {
 // base and the FP64 buffers flip.
 __global__ void k_outer_done(Arenas A) {
   const int nl = A.g->n_live;
+  if (blockIdx.x == 0 && threadIdx.x == 0) A.g->chunk_j0 = 0;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nl; q += gridDim.x * blockDim.x) {
     SimState &st = A.state[A.live[q]];
     if (!st.solving) continue;
@@ -692,17 +705,18 @@ __global__ void k_outer_done(Arenas A) {
 // One FP32 Jacobi sweep e_{j+1} = J(e_j) on every solving slot's cell tiles,
 // with the stop test of iterate base + j from the same neighbour sums.
 template <int CPT, bool TUNE>
-__global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner) {
+__global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner, int jj) {
   __shared__ unsigned long long wsum[kT / 64];
   const int nt = A.g->n_cell_tiles;
+  // every solving simulation is at the same inner sweep: jj-th of the chunk
+  const int j = TUNE ? 1 : A.g->chunk_j0 + jj;
   for (int t = blockIdx.x; t < nt; t += gridDim.x) {
     const Tile tl = A.cell_tiles[t];
     const SimDev &sd = A.sims[tl.slot];
     SimState &st = A.state[tl.slot];
-    if (!TUNE && (!st.solving || st.j >= inner)) continue;
-    const int j = TUNE ? 1 : st.j;
-    const float *ein = A.e32[j & 1] + sd.cbase;
-    float *eout = A.e32[(j + 1) & 1] + sd.cbase;
+    if (!TUNE && (!st.solving || j >= inner)) continue;
+    const float *ein = A.e32[j % kEBufs] + sd.cbase;
+    float *eout = A.e32[(j + 1) % kEBufs] + sd.cbase;
     const float *r = A.r32 + sd.cbase;
     const double ifixq = st.ifixq;
     unsigned long long units = 0;
@@ -749,47 +763,57 @@ __global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner) {
     if (threadIdx.x == 0) {
       unsigned long long sum = 0;
       for (int w = 0; w < kT / 64; ++w) sum += wsum[w];
-      A.units[sd.ubase + tl.b] = sum;
+      A.units[A.ucap * (j % kDecideEvery) + sd.ubase + tl.b] = sum;
     }
     __syncthreads();
   }
 }
 
-// After a sweep, per solving slot (one block each): the residual of iterate
-// base + j from its tiles' units, and the stop rule.
-__global__ void __launch_bounds__(kT) k_decide(Arenas A, int inner) {
+// After kDecideEvery sweeps (the first of them the jd-th of the chunk), per
+// solving slot (one block each): the residuals of iterates base + j .. base +
+// j + kDecideEvery - 1 from their tiles' units, and the stop rule applied to
+// each in turn, as after every sweep.
+__global__ void __launch_bounds__(kT) k_decide(Arenas A, int inner, int jd) {
   __shared__ unsigned long long wsum[kT / 64];
+  __shared__ int go;
   const int nl = A.g->n_live;
+  const int j0 = A.g->chunk_j0 + jd;
   for (int q = blockIdx.x; q < nl; q += gridDim.x) {
     const int s = A.live[q];
     const SimDev &sd = A.sims[s];
     SimState &st = A.state[s];
-    if (!st.solving || st.j >= inner) continue;       // block-uniform
-    unsigned long long acc = 0;
-    for (int t = threadIdx.x; t < sd.tiles; t += kT) acc += A.units[sd.ubase + t];
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane_id() == 0) wsum[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long tot = 0;
-      for (int w = 0; w < kT / 64; ++w) tot += wsum[w];
-      const int j = st.j;
-      const int64_t it = st.base + j;
-      const int64_t cap = 2 * (int64_t)sd.maxit + 1;
-      if (tot <= kStopUnits) {                   // |res| <= maxerr |b| (CalcEpotential.f90:60-62)
-        st.solving = 0;
-        st.sweeps = it;
-        st.stop_j = j;
-      } else if (it + 1 >= cap) {                // 2*maxit+1 sweeps (:67-69)
-        st.solving = 0;
-        st.sweeps = cap;
-        st.stop_j = j + 1;
-        st.noconv = 1;
-      } else {
-        st.j = j + 1;
+    if (!st.solving || j0 >= inner) continue;         // block-uniform
+    for (int k = 0; k < kDecideEvery; ++k) {
+      const int j = j0 + k;
+      const unsigned long long *u = A.units + A.ucap * (j % kDecideEvery) + sd.ubase;
+      unsigned long long acc = 0;
+      for (int t = threadIdx.x; t < sd.tiles; t += kT) acc += u[t];
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+      if (lane_id() == 0) wsum[threadIdx.x >> 6] = acc;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int w = 0; w < kT / 64; ++w) tot += wsum[w];
+        const int64_t it = st.base + j;
+        const int64_t cap = 2 * (int64_t)sd.maxit + 1;
+        go = 0;
+        if (tot <= kStopUnits) {                 // |res| <= maxerr |b| (CalcEpotential.f90:60-62)
+          st.solving = 0;
+          st.sweeps = it;
+          st.stop_j = j;
+        } else if (it + 1 >= cap) {              // 2*maxit+1 sweeps (:67-69)
+          st.solving = 0;
+          st.sweeps = cap;
+          st.stop_j = j + 1;
+          st.noconv = 1;
+        } else {
+          st.j = j + 1;
+          go = 1;
+        }
       }
+      __syncthreads();
+      if (!go) break;                            // block-uniform
     }
-    __syncthreads();
   }
 }
 
@@ -804,6 +828,7 @@ __global__ void k_count_active(Arenas A, int32_t *host_active) {
   __syncthreads();
   if (threadIdx.x == 0) {
     A.g->active = cnt;
+    A.g->chunk_j0 += kChunkSweeps;
     *reinterpret_cast<volatile int32_t *>(host_active) = cnt;
   }
 }
@@ -827,7 +852,7 @@ __global__ void __launch_bounds__(kT) k_finish(Arenas A) {
     const int64_t src = (i - 1) + (int64_t)sd.X * ((sj - 1) + (int64_t)sd.Y * (sk - 1));
     double v = A.u[st.ucur][sd.gbase + src];
     if (i >= 2 && i <= sd.nx + 1 && st.stop_j > 0)
-      v += (double)A.e32[st.stop_j & 1][sd.cbase + (i - 2) +
+      v += (double)A.e32[st.stop_j % kEBufs][sd.cbase + (i - 2) +
                                         (int64_t)sd.nx * ((sj - 2) + (int64_t)sd.ny * (sk - 2))];
     A.U[sd.gbase + ln] = v;
   }
@@ -1338,23 +1363,26 @@ static int launch_pre(picard_ensemble *e, int cur) {
   return PICARD_OK;
 }
 
+// Sweep jj of the chunk; after every kDecideEvery-th, the stop decision over
+// the group's iterates.
 template <int CPT>
-static void launch_sweep(picard_ensemble *e, bool tune) {
+static void launch_sweep(picard_ensemble *e, bool tune, int jj) {
   const int blocks = 8 * ((e->tile.blocks_per_cu * e->cus + 7) / 8);
   if (tune) {
-    k_sweep<CPT, true><<<blocks, kT, 0, e->stream>>>(e->A, e->inner);
+    k_sweep<CPT, true><<<blocks, kT, 0, e->stream>>>(e->A, e->inner, jj);
   } else {
-    k_sweep<CPT, false><<<blocks, kT, 0, e->stream>>>(e->A, e->inner);
-    k_decide<<<e->cus, kT, 0, e->stream>>>(e->A, e->inner);
+    k_sweep<CPT, false><<<blocks, kT, 0, e->stream>>>(e->A, e->inner, jj);
+    if ((jj + 1) % kDecideEvery == 0)
+      k_decide<<<e->cus, kT, 0, e->stream>>>(e->A, e->inner, jj + 1 - kDecideEvery);
   }
 }
 
 static int launch_sweeps(picard_ensemble *e, int count, bool tune) {
   for (int q = 0; q < count; ++q) {
     switch (e->tile.cells_per_thread) {
-      case 1: launch_sweep<1>(e, tune); break;
-      case 2: launch_sweep<2>(e, tune); break;
-      default: launch_sweep<4>(e, tune); break;
+      case 1: launch_sweep<1>(e, tune, q); break;
+      case 2: launch_sweep<2>(e, tune, q); break;
+      default: launch_sweep<4>(e, tune, q); break;
     }
   }
   PICARD_HIP(hipGetLastError());
@@ -1441,9 +1469,10 @@ int picard_sim_footprint(const picard_sim_params *p, int64_t *grid_bytes, int64_
   int64_t nodes, cells, dist, face, keys;
   int32_t fch;
   slot_sizes(p, &nodes, &cells, &dist, &face, &keys, &fch);
-  // U, u0, u1, C (FP64), N, Nfix (2 species), E32 (16 B), r32, e32 x2,
-  // distance table, face partials, keys (hist + cursor)
-  const int64_t b = nodes * (8 * 4 + 16 + 16 + 16) + cells * 12 + dist * 8 + face * 8 + keys * 8;
+  // U, u0, u1, C (FP64), N, Nfix (2 species), E32 (16 B), r32, the
+  // correction buffers, distance table, face partials, keys (hist + cursor)
+  const int64_t b = nodes * (8 * 4 + 16 + 16 + 16) + cells * (4 + 4 * kEBufs) + dist * 8 +
+                    face * 8 + keys * 8;
   if (grid_bytes) *grid_bytes = b;
   if (particles)
     *particles = cells * 2 * p->ppc + 2 * (int64_t)p->ny * p->nz * 2 * p->ppc;
@@ -1518,7 +1547,8 @@ int picard_ensemble_create(const picard_ensemble_config *cfg, picard_ensemble **
   e->cap_fred_t = e->max_face / kT + 2 * S;
   PICARD_TRY(dalloc_n(e, &A.node_tiles, e->cap_node_t));
   PICARD_TRY(dalloc_n(e, &A.cell_tiles, e->cap_cell_t));
-  PICARD_TRY(dalloc_n(e, &A.units, e->cap_cell_t));
+  PICARD_TRY(dalloc_n(e, &A.units, kDecideEvery * e->cap_cell_t));
+  A.ucap = e->cap_cell_t;
   PICARD_TRY(dalloc_n(e, &A.face_tiles, e->cap_face_t));
   PICARD_TRY(dalloc_n(e, &A.fred_tiles, e->cap_fred_t));
   PICARD_TRY(dalloc_n(e, &A.U, e->max_nodes));
@@ -1531,8 +1561,7 @@ int picard_ensemble_create(const picard_ensemble_config *cfg, picard_ensemble **
   PICARD_TRY(dalloc_n(e, &A.D, e->max_dist));
   PICARD_TRY(dalloc_n(e, &A.fscr, e->max_face));
   PICARD_TRY(dalloc_n(e, &A.r32, e->max_cells));
-  PICARD_TRY(dalloc_n(e, &A.e32[0], e->max_cells));
-  PICARD_TRY(dalloc_n(e, &A.e32[1], e->max_cells));
+  for (int q = 0; q < kEBufs; ++q) PICARD_TRY(dalloc_n(e, &A.e32[q], e->max_cells));
   PICARD_TRY(dalloc_n(e, &A.hist, kcap + 1));
   PICARD_TRY(dalloc_n(e, &A.cursor, kcap + 1));
   PICARD_TRY(dalloc_n(e, &A.key, e->cap));
@@ -1569,8 +1598,7 @@ int picard_ensemble_create(const picard_ensemble_config *cfg, picard_ensemble **
   Global g0{};
   PICARD_HIP(hipMemcpyAsync(A.g, &g0, sizeof(Global), hipMemcpyHostToDevice, e->stream));
   k_fill_f32<<<1024, kT, 0, e->stream>>>(A.r32, e->max_cells);
-  k_fill_f32<<<1024, kT, 0, e->stream>>>(A.e32[0], e->max_cells);
-  k_fill_f32<<<1024, kT, 0, e->stream>>>(A.e32[1], e->max_cells);
+  for (int q = 0; q < kEBufs; ++q) k_fill_f32<<<1024, kT, 0, e->stream>>>(A.e32[q], e->max_cells);
   k_fill_u64<<<1024, kT, 0, e->stream>>>(reinterpret_cast<unsigned long long *>(A.U),
                                          e->max_nodes);
   k_fill_u64<<<1024, kT, 0, e->stream>>>(reinterpret_cast<unsigned long long *>(A.E32),

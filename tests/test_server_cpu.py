@@ -34,3 +34,17 @@ def test_offered_load_estimate():
     est = server.offered_load(jobs, sweeps_per_n2=6.0)
     # the stated stream asks for far more than one GPU second per second
     assert est > 10.0
+
+
+def test_lane_routing():
+    lanes = server.two_lanes(80)
+    assert [ls.max_edge for ls in lanes] == sorted(ls.max_edge for ls in lanes)
+    jobs = server.draw_jobs(rate=2.0, count=200, seed=5)
+    q = [server.lane_index(lanes, j) for j in jobs]
+    for j, k in zip(jobs, q):
+        assert (k == 0) == (max(j.spec.nx, j.spec.ny, j.spec.nz) <= 80)
+    # both lanes get jobs from the stated stream (edges log-uniform in 32..160)
+    assert 0.3 < np.mean(np.array(q) == 0) < 0.8
+    # the lanes' arenas fit in the budget's share of HBM
+    total = sum(ls.max_particles * ensemble.PARTICLE_BYTES for ls in lanes)
+    assert total < 160e9
