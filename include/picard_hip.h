@@ -221,6 +221,12 @@ int picard_timing_read(picard_ctx *ctx, double *jacobi_ms, int64_t *jacobi_launc
 /* Jacobi sweeps the timed launches of picard_timing_read performed (a split
  * launch runs one, two or three).  No reference counterpart. */
 int picard_timing_sweeps(picard_ctx *ctx, int64_t *jacobi_sweeps);
+/* The same timed launches measured by the kernels themselves: the GPU's
+ * 100 MHz real-time clock read by each block when it starts and when it has
+ * finished, from the earliest start to the latest end (the dispatch-packet
+ * events of picard_timing_read add the packet processing around the kernel).
+ * No reference counterpart. */
+int picard_timing_kernel_clock(picard_ctx *ctx, double *jacobi_ms, int64_t *jacobi_launches);
 /* Wall time of the CalcEpotential phase of picard_step (HIP events before and
  * after the solve on the context's stream), summed over the steps run since
  * the last picard_timing_enable, and the number of solves timed.  No

@@ -39,6 +39,7 @@ EXPORTS = (
     "picard_generate_inflow", "picard_particles_produce", "picard_set_production",
     "picard_diagnostics", "picard_advance_velocity_imp", "picard_advance_velocity_exp",
     "picard_particles_gridify_all", "picard_particles_read_noalloc", "picard_timing_sweeps",
+    "picard_timing_kernel_clock",
 )
 
 READ_NOALLOC_MAX = 246723
@@ -167,6 +168,7 @@ def lib():
         "picard_particles_count": ([vp, lp], ct.c_int),
         "picard_particles_read_noalloc": ([vp, ct.c_int64, ct.c_int64, dp, ip], ct.c_int),
         "picard_timing_sweeps": ([vp, lp], ct.c_int),
+        "picard_timing_kernel_clock": ([vp, dp, lp], ct.c_int),
         "picard_advance_position": ([vp, ct.c_double], ct.c_int),
         "picard_bc_outflow": ([vp, lp, lp], ct.c_int),
         "picard_particles_transfer": ([vp], ct.c_int),

@@ -120,6 +120,7 @@ void release(picard_ctx *c) {
   for (int q = 0; q < 8; ++q) dfree(c->p.r[q]);
   dfree(c->p.sp);
   dfree(c->U);
+  dfree(c->kst);
   dfree(c->E);
   dfree(c->N);
   dfree(c->Nraw);
@@ -217,6 +218,7 @@ int create_impl(const picard_geometry *geo, const picard_species *sp, picard_ctx
     if (c->order_free) PICARD_TRY(dalloc(&c->Nfix, g.S * g.G));
   }
   PICARD_TRY(dalloc(&c->U, g.G));
+  PICARD_TRY(dalloc(&c->kst, (int64_t)picard_ctx::kStampWords * picard_ctx::kStamps));
   PICARD_TRY(dalloc(&c->E, 3 * g.G));
   PICARD_TRY(dalloc(&c->N, g.S * g.G));
   if (!c->split) PICARD_TRY(dalloc(&c->Nraw, g.S * g.G));
@@ -604,6 +606,8 @@ int picard_timing_enable(picard_ctx *c, int32_t enable) {
   c->t_ms = 0.0;
   c->t_launches = 0;
   c->t_sweeps = 0;
+  c->tk_ms = 0.0;
+  c->tk_launches = 0;
   c->solve_ms = 0.0;
   c->solves = 0;
   return PICARD_OK;
@@ -626,6 +630,13 @@ int picard_timing_read(picard_ctx *c, double *jacobi_ms, int64_t *jacobi_launche
 int picard_timing_sweeps(picard_ctx *c, int64_t *jacobi_sweeps) {
   CTX_CHECK(c);
   if (jacobi_sweeps) *jacobi_sweeps = c->t_sweeps;
+  return PICARD_OK;
+}
+
+int picard_timing_kernel_clock(picard_ctx *c, double *jacobi_ms, int64_t *jacobi_launches) {
+  CTX_CHECK(c);
+  if (jacobi_ms) *jacobi_ms = c->tk_ms;
+  if (jacobi_launches) *jacobi_launches = c->tk_launches;
   return PICARD_OK;
 }
 

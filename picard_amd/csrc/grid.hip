@@ -634,9 +634,10 @@ __global__ void __launch_bounds__(kStencilT, 8) k_jacobi_pass(PassGeo pg, Jacobi
                                                      double *b2, const double *__restrict__ b,
                                                      const double *__restrict__ Pin,
                                                      double *__restrict__ Pout, JacobiState *st,
-                                                     int32_t p) {
+                                                     int32_t p, unsigned long long *ks) {
   __shared__ double sh[kStencilT / 64];
   __shared__ int decide;
+  kstamp_begin(ks);
   const StopSlot prev = st->slot[(p - 1) & 1];
   const int t0 = lin_tid();
   // decision of this launch: carried over, or the stop test of loop
@@ -658,7 +659,10 @@ __global__ void __launch_bounds__(kStencilT, 8) k_jacobi_pass(PassGeo pg, Jacobi
     }
   }
   if (blockIdx.x == 0 && t0 == 0) st->slot[p & 1] = cur;
-  if (cur.done) return;
+  if (cur.done) {
+    kstamp_end(ks);
+    return;
+  }
 
   const int r = p % 3;
   const double *curb = r == 1 ? b0 : (r == 2 ? b1 : b2);           // U_{p-1} in bufs[(p-1)%3]
@@ -730,6 +734,7 @@ __global__ void __launch_bounds__(kStencilT, 8) k_jacobi_pass(PassGeo pg, Jacobi
   }
   const double sblk = block_sum(r2, sh);
   if (t0 == 0) Pout[blockIdx.x] = sblk;
+  kstamp_end(ks);
 }
 
 // The same pass with two cells per lane, for even nx: a wave owns 128 cells
@@ -761,9 +766,10 @@ __global__ void __launch_bounds__(kStencilT, 4) k_jacobi_pass2(PassGeo pg, Jacob
                                                        const double *__restrict__ b,
                                                        const double *__restrict__ Pin,
                                                        double *__restrict__ Pout, JacobiState *st,
-                                                       int32_t p) {
+                                                       int32_t p, unsigned long long *ks) {
   __shared__ double sh[kStencilT / 64];
   __shared__ int decide;
+  kstamp_begin(ks);
   const StopSlot prev = st->slot[(p - 1) & 1];
   const int t0 = lin_tid();
   StopSlot cur = prev;
@@ -783,7 +789,10 @@ __global__ void __launch_bounds__(kStencilT, 4) k_jacobi_pass2(PassGeo pg, Jacob
     }
   }
   if (blockIdx.x == 0 && t0 == 0) st->slot[p & 1] = cur;
-  if (cur.done) return;
+  if (cur.done) {
+    kstamp_end(ks);
+    return;
+  }
 
   const int r = p % 3;
   const double *curb = r == 1 ? b0 : (r == 2 ? b1 : b2);           // U_{p-1} in bufs[(p-1)%3]
@@ -934,6 +943,7 @@ __global__ void __launch_bounds__(kStencilT, 4) k_jacobi_pass2(PassGeo pg, Jacob
   }
   const double sblk = block_sum(r2, sh);
   if (t0 == 0) Pout[blockIdx.x] = sblk;
+  kstamp_end(ks);
 }
 
 // After q >= 1 sweeps: U(node) = U_q at the composite periodic image of the
@@ -1071,7 +1081,43 @@ __global__ void k_zero_u64(unsigned long long *p, int64_t n) {
   if (q < n) p[q] = 0ull;
 }
 
+__global__ void k_kstamp_reset(unsigned long long *ks, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;        // word
+  if (q < n * picard_ctx::kStampWords)
+    ks[q] = (q % picard_ctx::kStampWords) < kStampWays ? ~0ull : 0ull;
+}
+
 }  // namespace
+
+int kstamp_reset(picard_ctx *c, int n) {
+  if (n <= 0) return PICARD_OK;
+  static_assert(picard_ctx::kStampWords == 2 * kStampWays, "stamp layout");
+  k_kstamp_reset<<<(n * picard_ctx::kStampWords + 255) / 256, 256, 0, c->stream>>>(c->kst, n);
+  PICARD_HIP(hipGetLastError());
+  return PICARD_OK;
+}
+
+// The stream has been synchronized: the stamps of the n sampled launches are
+// final (a launch that did not run leaves its pair reset and counts nothing).
+int kstamp_collect(picard_ctx *c, int n) {
+  if (n <= 0) return PICARD_OK;
+  const int W = picard_ctx::kStampWords;
+  c->kst_host.resize((size_t)W * n);
+  PICARD_HIP(hipMemcpy(c->kst_host.data(), c->kst, sizeof(unsigned long long) * W * n,
+                       hipMemcpyDeviceToHost));
+  for (int q = 0; q < n; ++q) {
+    unsigned long long a = ~0ull, b = 0ull;
+    for (int w = 0; w < kStampWays; ++w) {
+      a = std::min(a, c->kst_host[(size_t)W * q + w]);
+      b = std::max(b, c->kst_host[(size_t)W * q + kStampWays + w]);
+    }
+    if (b > a) {
+      c->tk_ms += (double)(b - a) * 1e-5;                // 100 MHz ticks
+      c->tk_launches += 1;
+    }
+  }
+  return PICARD_OK;
+}
 
 // ---------------------------------------------------------------------------
 
@@ -1290,33 +1336,37 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
   StopSlot fin{};
   for (;;) {
     const int64_t todo = (last + 1 - (p - 1)) < batch ? (last + 1 - (p - 1)) : batch;
-    // timing: events around one pass in kTimeEvery (event records between
-    // every pair of passes would slow the timed solve itself)
-    const int64_t timed = (todo + kTimeEvery / 2 - 1) / kTimeEvery;   // passes q = 8, 24, ...
+    // timing: one pass in kTimeEvery, with dispatch-packet events around it
+    // and the kernel's own clock stamps (events around every pass would slow
+    // the timed solve itself)
+    int64_t timed = (todo + kTimeEvery / 2 - 1) / kTimeEvery;         // passes q = 8, 24, ...
+    if (timed > picard_ctx::kStamps) timed = picard_ctx::kStamps;
     if (c->timing) {
       while ((int64_t)c->tev.size() < 2 * timed) {
         hipEvent_t e;
         PICARD_HIP(hipEventCreate(&e));
         c->tev.push_back(e);
       }
+      PICARD_TRY(kstamp_reset(c, (int)timed));
     }
     for (int64_t q = 0; q < todo; ++q, ++p) {
-      const bool stamp = c->timing && q % kTimeEvery == kTimeEvery / 2;
+      const bool stamp = c->timing && q % kTimeEvery == kTimeEvery / 2 && q / kTimeEvery < timed;
+      unsigned long long *ks = stamp ? c->kst + picard_ctx::kStampWords * (q / kTimeEvery) : nullptr;
       hipEvent_t e0 = stamp ? c->tev[2 * (q / kTimeEvery)] : nullptr;
       hipEvent_t e1 = stamp ? c->tev[2 * (q / kTimeEvery) + 1] : nullptr;
       const double *pin = P[(p - 1) & 1];
       if (two && g.flow[0])
         hipExtLaunchKernelGGL(k_jacobi_pass2<false>, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
                               e0, e1, 0, pg, jc, c->jb[0], c->jb[1], c->jb[2],
-                              (const double *)c->b, pin, P[p & 1], c->js, (int32_t)p);
+                              (const double *)c->b, pin, P[p & 1], c->js, (int32_t)p, ks);
       else if (two)
         hipExtLaunchKernelGGL(k_jacobi_pass2<true>, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
                               e0, e1, 0, pg, jc, c->jb[0], c->jb[1], c->jb[2],
-                              (const double *)c->b, pin, P[p & 1], c->js, (int32_t)p);
+                              (const double *)c->b, pin, P[p & 1], c->js, (int32_t)p, ks);
       else
         hipExtLaunchKernelGGL(k_jacobi_pass, dim3(ts.B), dim3(ts.bx, ts.by), 0, c->stream,
                               e0, e1, 0, pg, jc, c->jb[0], c->jb[1], c->jb[2],
-                              (const double *)c->b, pin, P[p & 1], c->js, (int32_t)p);
+                              (const double *)c->b, pin, P[p & 1], c->js, (int32_t)p, ks);
     }
     PICARD_HIP(hipGetLastError());
     PICARD_HIP(hipMemcpyAsync(c->js_host, c->js, sizeof(JacobiState), hipMemcpyDeviceToHost,
@@ -1330,6 +1380,7 @@ int launch_calc_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t 
       }
       c->t_launches += timed;
       c->t_sweeps += timed;
+      PICARD_TRY(kstamp_collect(c, (int)timed));
     }
     fin = c->js_host->slot[(p - 1) & 1];
     if (fin.done) break;

@@ -194,9 +194,17 @@ struct picard_ctx {
   // picard::kTimeEvery
   bool timing = false;
   std::vector<hipEvent_t> tev;
+  // kernel-duration stamps of the sampled solve launches (2 x 16 words per
+  // launch, in ticks of the 100 MHz real-time clock) and their host copy
+  static constexpr int kStamps = 8192;
+  static constexpr int kStampWords = 32;
+  unsigned long long *kst = nullptr;
+  std::vector<unsigned long long> kst_host;
   double t_ms = 0.0;
   int64_t t_launches = 0;
   int64_t t_sweeps = 0;      // Jacobi sweeps of the timed launches
+  double tk_ms = 0.0;        // the same launches by the kernels' own clock stamps
+  int64_t tk_launches = 0;
   // wall time of the whole CalcEpotential phase inside picard_step while
   // timing is on (events before and after the solve on the context's stream)
   hipEvent_t solve_ev[2] = {nullptr, nullptr};
@@ -221,6 +229,10 @@ int launch_wrap(picard_ctx *c);
 int launch_boris(picard_ctx *c, double dt, int mode = 0, double deltat = 0.0);
 int launch_append(picard_ctx *c, const double *host_rp, const int32_t *host_sp, int64_t count);
 int launch_read(picard_ctx *c, int64_t first, int64_t count, double *host_rp, int32_t *host_sp);
+// Reset the first n kernel-duration stamp pairs; read back n and add their
+// durations to the timing totals.
+int kstamp_reset(picard_ctx *c, int n);
+int kstamp_collect(picard_ctx *c, int n);
 // particles [first, first + count) as real_particles(8, n) / int_particles(n)
 // into device buffers (either form)
 int stage_particles(picard_ctx *c, int64_t first, int64_t count, double *stage, int32_t *stage_sp);

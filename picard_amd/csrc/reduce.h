@@ -40,4 +40,26 @@ __device__ __forceinline__ double final_sum(const double *partials, unsigned n, 
   return block_sum(s, sh);
 }
 
+// Kernel duration of a sampled launch: each block's first thread takes the
+// GPU's constant 100 MHz real-time clock when the block starts and when it
+// has finished.  The earliest start and the latest end are kept in
+// kStampWays words each, a block using word blockIdx.x % kStampWays, so that
+// few blocks meet on one address (one word per launch held its blocks up by
+// about 2 us); ks[0 .. kStampWays) keep starts (reset to ~0), ks[kStampWays
+// .. 2 kStampWays) ends (reset to 0), and the host reduces them.  ks is null
+// in launches not sampled.
+constexpr int kStampWays = 16;
+__device__ __forceinline__ void kstamp_begin(unsigned long long *ks) {
+  if (ks && lin_tid() == 0)
+    atomicMin(ks + (blockIdx.x % kStampWays), (unsigned long long)__builtin_amdgcn_s_memrealtime());
+}
+// Every thread of the block calls it (block-uniform), after the block's work.
+__device__ __forceinline__ void kstamp_end(unsigned long long *ks) {
+  if (!ks) return;
+  __syncthreads();
+  if (lin_tid() == 0)
+    atomicMax(ks + kStampWays + (blockIdx.x % kStampWays),
+              (unsigned long long)__builtin_amdgcn_s_memrealtime());
+}
+
 }  // namespace picard

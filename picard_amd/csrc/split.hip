@@ -686,9 +686,11 @@ __global__ void __launch_bounds__(kSwThreads, 1024 / kSwThreads) k_mx_sweep(Swee
                                                    const float *__restrict__ eo_prev,
                                                    const double *__restrict__ Pin,
                                                    double *__restrict__ Pout, JacobiState *st,
-                                                   MxLaunch L) {
+                                                   MxLaunch L,
+    unsigned long long *ks) {
   __shared__ double sh[kSwThreads / 64];
   __shared__ int decide;
+  kstamp_begin(ks);
   StopSlot cur = mx_decide(st, L, Pin, sh, &decide);
   if (!cur.done && (int64_t)L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
     cur.done = 1;
@@ -696,7 +698,10 @@ __global__ void __launch_bounds__(kSwThreads, 1024 / kSwThreads) k_mx_sweep(Swee
     cur.buf = L.ibuf;
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
-  if (cur.done) return;
+  if (cur.done) {
+    kstamp_end(ks);
+    return;
+  }
   const bool res = L.j >= 1;
   const uint32_t NX = sg.NX, P = sg.P;
   const int nx = sg.nx, ny = sg.ny, nz = sg.nz;
@@ -858,6 +863,7 @@ __global__ void __launch_bounds__(kSwThreads, 1024 / kSwThreads) k_mx_sweep(Swee
   }
   const double sblk = block_sum(r2, sh);
   if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
+  kstamp_end(ks);
 }
 
 // Two inner sweeps per launch, e_j -> e_{j+2}, for rows of one segment
@@ -898,10 +904,12 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
                                                   const float *__restrict__ eo_prev,
                                                   const double *__restrict__ Pin,
                                                   double *__restrict__ Pout, JacobiState *st,
-                                                  MxLaunch L) {
+                                                  MxLaunch L,
+    unsigned long long *ks) {
   static_assert(!(MULTI && PX), "overlapping segments need x not periodic");
   __shared__ double sh[kSwThreads / 64];
   __shared__ int decide;
+  kstamp_begin(ks);
   StopSlot cur = mx_decide(st, L, Pin, sh, &decide);
   if (!cur.done && (int64_t)L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
     cur.done = 1;
@@ -909,7 +917,10 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
     cur.buf = L.ibuf;
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
-  if (cur.done) return;
+  if (cur.done) {
+    kstamp_end(ks);
+    return;
+  }
   // the stop-test residual of e_j: in the launches the host marks, and in
   // every launch while the last test found the stop near
   const bool res = !FIRST && (L.test || cur.near);
@@ -1209,6 +1220,7 @@ __global__ void __launch_bounds__(kSwThreads, 2) k_mx_pair(SweepGeo sg, SweepCoe
   }
   const double sblk = block_sum(r2, sh);
   if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
+  kstamp_end(ks);
 }
 
 // One cell group of the two-sweep kernels with x not periodic: the arithmetic
@@ -1313,7 +1325,8 @@ template <bool PAD, bool FIRST, int NW, int D>
 __global__ void __launch_bounds__(64 * NW, D == 1 ? 4 : 3) k_mx_pair_lds(
     SweepGeo sg, SweepCoef cf, const float *__restrict__ r32, const float *__restrict__ e,
     float *__restrict__ out, float *__restrict__ eo_out, const float *__restrict__ eo_prev,
-    const double *__restrict__ Pin, double *__restrict__ Pout, JacobiState *st, MxLaunch L) {
+    const double *__restrict__ Pin, double *__restrict__ Pout, JacobiState *st, MxLaunch L,
+    unsigned long long *ks) {
   constexpr int kLdsCRows = 2 * NW;
   __shared__ double sh[NW];
   __shared__ int decide;
@@ -1321,6 +1334,7 @@ __global__ void __launch_bounds__(64 * NW, D == 1 ? 4 : 3) k_mx_pair_lds(
   // e_{j+1} of plane q in E1s[q & 1]
   __shared__ float4 EJs[2][kLdsCRows + 2][64];
   __shared__ float4 E1s[2][kLdsCRows][64];
+  kstamp_begin(ks);
   StopSlot cur = mx_decide(st, L, Pin, sh, &decide);
   if (!cur.done && (int64_t)L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
     cur.done = 1;
@@ -1328,7 +1342,10 @@ __global__ void __launch_bounds__(64 * NW, D == 1 ? 4 : 3) k_mx_pair_lds(
     cur.buf = L.ibuf;
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
-  if (cur.done) return;
+  if (cur.done) {
+    kstamp_end(ks);
+    return;
+  }
   const bool res = !FIRST && (L.test || cur.near);
   const uint32_t NX = sg.NX, P = sg.P;
   const int nx = sg.nx, ny = sg.ny, nz = sg.nz;
@@ -1560,6 +1577,7 @@ __global__ void __launch_bounds__(64 * NW, D == 1 ? 4 : 3) k_mx_pair_lds(
   }
   const double sblk = block_sum(r2, sh);
   if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
+  kstamp_end(ks);
 }
 
 // The three-sweep kernel: k_mx_pair_lds's scheme one level deeper, e_j ->
@@ -1590,7 +1608,8 @@ template <bool PAD, bool FIRST, int NW>
 __global__ void __launch_bounds__(64 * NW, 3) k_mx_triple_lds(
     SweepGeo sg, SweepCoef cf, const float *__restrict__ r32, const float *__restrict__ e,
     float *__restrict__ out, float *__restrict__ eo_out, const float *__restrict__ eo_prev,
-    const double *__restrict__ Pin, double *__restrict__ Pout, JacobiState *st, MxLaunch L) {
+    const double *__restrict__ Pin, double *__restrict__ Pout, JacobiState *st, MxLaunch L,
+    unsigned long long *ks) {
   constexpr int CR = 2 * NW;                    // rows of the first sweep
   __shared__ double sh[NW];
   __shared__ int decide;
@@ -1599,6 +1618,7 @@ __global__ void __launch_bounds__(64 * NW, 3) k_mx_triple_lds(
   __shared__ float4 EJs[2][CR + 2][64];
   __shared__ float4 E1s[2][CR][64];
   __shared__ float4 E2s[2][CR][64];
+  kstamp_begin(ks);
   StopSlot cur = mx_decide(st, L, Pin, sh, &decide);
   if (!cur.done && (int64_t)L.base + L.j >= st->last) {   // iterate 2*maxit+1 reached (:58)
     cur.done = 1;
@@ -1606,7 +1626,10 @@ __global__ void __launch_bounds__(64 * NW, 3) k_mx_triple_lds(
     cur.buf = L.ibuf;
   }
   if (blockIdx.x == 0 && lin_tid() == 0) st->slot[L.seq & 1] = cur;
-  if (cur.done) return;
+  if (cur.done) {
+    kstamp_end(ks);
+    return;
+  }
   const bool res = !FIRST && (L.test || cur.near);
   const uint32_t NX = sg.NX, P = sg.P;
   const int nx = sg.nx, ny = sg.ny, nz = sg.nz;
@@ -1838,6 +1861,7 @@ __global__ void __launch_bounds__(64 * NW, 3) k_mx_triple_lds(
   }
   const double sblk = block_sum(r2, sh);
   if (lin_tid() == 0 && res) Pout[blockIdx.x] = sblk;
+  kstamp_end(ks);
 }
 
 // End of an outer step: the stop test of its last inner iterate and, when
@@ -2106,12 +2130,13 @@ int sp_gridify(picard_ctx *c, bool flux) {
 // One sweep launch (timed with events t0/t1 when given).
 template <bool PX>
 void launch_sweep(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t t1,
+                  unsigned long long *ks,
                   const SweepGeo &sg, const SweepCoef &cf, const float *r32, const float *e,
                   float *out, float *eo_out, const float *eo_prev, const double *pin, double *pout,
                   JacobiState *js, const MxLaunch &L) {
   auto fn = pad ? k_mx_sweep<PX, true> : k_mx_sweep<PX, false>;
   hipExtLaunchKernelGGL(fn, dim3(blocks), dim3(64, kSwRows), 0, s, t0, t1, 0, sg, cf, r32, e,
-                        out, eo_out, eo_prev, pin, pout, js, L);
+                        out, eo_out, eo_prev, pin, pout, js, L, ks);
 }
 
 // Resident blocks per XCD of the sweep kernel (the heaviest of the solve).
@@ -2242,6 +2267,7 @@ SweepGeo lds_sched(picard_ctx *c) {
 }
 
 void launch_pair_lds(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t t1,
+                  unsigned long long *ks,
                      const SweepGeo &sg, const SweepCoef &cf, const float *r32, const float *e,
                      float *out, float *eo_out, const float *eo_prev, const double *pin,
                      double *pout, JacobiState *js, const MxLaunch &L) {
@@ -2249,7 +2275,7 @@ void launch_pair_lds(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEven
   auto fn = L.j == 0 ? (pad ? k_mx_pair_lds<true, true, W, Dp> : k_mx_pair_lds<false, true, W, Dp>)
                      : (pad ? k_mx_pair_lds<true, false, W, Dp> : k_mx_pair_lds<false, false, W, Dp>);
   hipExtLaunchKernelGGL(fn, dim3(blocks), dim3(64, kLdsWaves), 0, s, t0, t1, 0, sg, cf, r32, e,
-                        out, eo_out, eo_prev, pin, pout, js, L);
+                        out, eo_out, eo_prev, pin, pout, js, L, ks);
 }
 
 // The three-sweep LDS kernel's schedule: tiles of at most kTriRows rows x z
@@ -2293,6 +2319,7 @@ SweepGeo tri_sched(picard_ctx *c) {
 }
 
 void launch_triple_lds(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t t1,
+                  unsigned long long *ks,
                        const SweepGeo &sg, const SweepCoef &cf, const float *r32, const float *e,
                        float *out, float *eo_out, const float *eo_prev, const double *pin,
                        double *pout, JacobiState *js, const MxLaunch &L) {
@@ -2300,12 +2327,13 @@ void launch_triple_lds(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEv
   auto fn = L.j == 0 ? (pad ? k_mx_triple_lds<true, true, W> : k_mx_triple_lds<false, true, W>)
                      : (pad ? k_mx_triple_lds<true, false, W> : k_mx_triple_lds<false, false, W>);
   hipExtLaunchKernelGGL(fn, dim3(blocks), dim3(64, kTriWaves), 0, s, t0, t1, 0, sg, cf, r32, e,
-                        out, eo_out, eo_prev, pin, pout, js, L);
+                        out, eo_out, eo_prev, pin, pout, js, L, ks);
 }
 
 // One two-sweep launch (timed with events t0/t1 when given).
 template <bool PX, bool MULTI>
 void launch_pair(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t t1,
+                  unsigned long long *ks,
                  const SweepGeo &sg, const SweepCoef &cf, const float *r32, const float *e,
                  float *out, float *eo_out, const float *eo_prev, const double *pin, double *pout,
                  JacobiState *js, const MxLaunch &L) {
@@ -2313,7 +2341,7 @@ void launch_pair(bool pad, int blocks, hipStream_t s, hipEvent_t t0, hipEvent_t 
   auto fn = L.j == 0 ? (pad ? k_mx_pair<PX, true, kPairRows, true, M> : k_mx_pair<PX, false, kPairRows, true, M>)
                      : (pad ? k_mx_pair<PX, true, kPairRows, false, M> : k_mx_pair<PX, false, kPairRows, false, M>);
   hipExtLaunchKernelGGL(fn, dim3(blocks), dim3(64, kSwRows), 0, s, t0, t1, 0, sg, cf, r32, e,
-                        out, eo_out, eo_prev, pin, pout, js, L);
+                        out, eo_out, eo_prev, pin, pout, js, L, ks);
 }
 
 // Tiles of the outer-step kernel: 64-cell row segments x 4-row groups x z
@@ -2446,8 +2474,9 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
   StopSlot fin{};
   bool stop_near = false;                   // the stop test found the stop near
   while (true) {
-    // timing: events around one launch in kTimeEvery (event records around
-    // every launch would slow the timed solve itself)
+    // timing: one launch in kTimeEvery, with dispatch-packet events around it
+    // and the kernel's own clock stamps (events around every launch would
+    // slow the timed solve itself)
     const int timed = (m + kTimeEvery - 1) / kTimeEvery;         // one launch position in 16
     if (c->timing) {
       while ((int64_t)c->tev.size() < 2 * timed) {
@@ -2455,6 +2484,7 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
         PICARD_HIP(hipEventCreate(&e));
         c->tev.push_back(e);
       }
+      PICARD_TRY(kstamp_reset(c, timed));
     }
     // outer residual of iterate `base` (its stop test happens in the next
     // launch); the stale-halo norm needs e_m and e_{m-1} of the previous outer
@@ -2515,6 +2545,7 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       // launches with and without the stop test are sampled in proportion
       const int tpos = (kTimeEvery / 2 + 2 * outer) % kTimeEvery;
       const bool tj = c->timing && j >= 1 && li % kTimeEvery == tpos && nrec < timed;
+      unsigned long long *ks = tj ? c->kst + picard_ctx::kStampWords * nrec : nullptr;
       hipEvent_t t0 = tj ? c->tev[2 * nrec] : nullptr;
       hipEvent_t t1 = tj ? c->tev[2 * nrec + 1] : nullptr;
       if (tj) {
@@ -2525,28 +2556,28 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       int blocks = swB;
       if (size == 3) {
         blocks = ptB;
-        launch_triple_lds(pad, ptB, c->stream, t0, t1, tgeo, cf, c->r32, e, out, eo_out, eo_prev,
+        launch_triple_lds(pad, ptB, c->stream, t0, t1, ks, tgeo, cf, c->r32, e, out, eo_out, eo_prev,
                           pin, pout, c->js, L);
       } else if (use_lds) {
         blocks = plB;
-        launch_pair_lds(pad, plB, c->stream, t0, t1, lgeo, cf, c->r32, e, out, eo_out, eo_prev,
+        launch_pair_lds(pad, plB, c->stream, t0, t1, ks, lgeo, cf, c->r32, e, out, eo_out, eo_prev,
                         pin, pout, c->js, L);
       } else if (size == 2) {
         blocks = prB;
         if (sg.px)
-          launch_pair<true, false>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out,
+          launch_pair<true, false>(pad, prB, c->stream, t0, t1, ks, pgeo, cf, c->r32, e, out, eo_out,
                                    eo_prev, pin, pout, c->js, L);
         else if (pgeo.ntx > 1)
-          launch_pair<false, true>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out, eo_out,
+          launch_pair<false, true>(pad, prB, c->stream, t0, t1, ks, pgeo, cf, c->r32, e, out, eo_out,
                                    eo_prev, pin, pout, c->js, L);
         else
-          launch_pair<false, false>(pad, prB, c->stream, t0, t1, pgeo, cf, c->r32, e, out,
+          launch_pair<false, false>(pad, prB, c->stream, t0, t1, ks, pgeo, cf, c->r32, e, out,
                                     eo_out, eo_prev, pin, pout, c->js, L);
       } else if (sg.px) {
-        launch_sweep<true>(pad, swB, c->stream, t0, t1, sg, cf, c->r32, e, out, eo_out, eo_prev,
+        launch_sweep<true>(pad, swB, c->stream, t0, t1, ks, sg, cf, c->r32, e, out, eo_out, eo_prev,
                            pin, pout, c->js, L);
       } else {
-        launch_sweep<false>(pad, swB, c->stream, t0, t1, sg, cf, c->r32, e, out, eo_out, eo_prev,
+        launch_sweep<false>(pad, swB, c->stream, t0, t1, ks, sg, cf, c->r32, e, out, eo_out, eo_prev,
                             pin, pout, c->js, L);
       }
       ++seq;
@@ -2582,6 +2613,7 @@ int sp_epotential(picard_ctx *c, double maxerr, int32_t maxit, int64_t *sweeps) 
       }
       c->t_launches += nrec;
       c->t_sweeps += tsw;
+      PICARD_TRY(kstamp_collect(c, nrec));
     }
     fin = c->js_host->slot[(seq - 1) & 1];
     stop_near = fin.near != 0;

@@ -692,6 +692,13 @@ class Domain:
         check("picard_timing_read", lib().picard_timing_read(self.h, ct.byref(ms), ct.byref(n)))
         return ms.value, n.value
 
+    def timing_kernel_clock(self):
+        """The timed launches by the kernels' own clock stamps: [ms], count."""
+        ms, n = ct.c_double(), ct.c_int64()
+        check("picard_timing_kernel_clock",
+              lib().picard_timing_kernel_clock(self.h, ct.byref(ms), ct.byref(n)))
+        return ms.value, n.value
+
     def timing_sweeps(self):
         """Jacobi sweeps of the launches timing_read timed."""
         n = ct.c_int64()
