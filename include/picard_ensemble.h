@@ -88,11 +88,11 @@ typedef struct {
 
 typedef struct picard_ensemble picard_ensemble;
 
-/* Tile shape of the solve kernels: cells per thread (1, 2 or 4), threads per
- * block (256), resident blocks per CU of the persistent launch (1..8).  A
- * tile is threads * cells consecutive interior cells of one simulation; the
- * autotuner tries the nine combinations of cells per thread and blocks per CU
- * in {2, 4, 8}. */
+/* Tile shape of the solve kernels: cells per thread (1, 2, 4 or 8), threads
+ * per block (256), resident blocks per CU of the persistent launch (1..8).  A
+ * tile is threads * cells consecutive interior cells of one simulation, thread
+ * t taking cells t, t + threads, ...; the autotuner tries the twelve
+ * combinations of cells per thread and blocks per CU in {2, 4, 8}. */
 typedef struct {
   int32_t cells_per_thread;
   int32_t threads;

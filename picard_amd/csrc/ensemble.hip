@@ -653,9 +653,9 @@ __global__ void __launch_bounds__(kT) k_outer(Arenas A, int cpt) {
     const int64_t X = sd.X, XY = (int64_t)sd.X * sd.Y;
     const uint32_t nx = (uint32_t)sd.nx, ny = (uint32_t)sd.ny, nz = (uint32_t)sd.nz;
     const uint32_t pl = nx * ny;
-    const uint32_t c0 = (uint32_t)tl.a + threadIdx.x * (uint32_t)cpt;
+    const uint32_t c0 = (uint32_t)tl.a + threadIdx.x;
     for (int q = 0; q < cpt; ++q) {
-      const uint32_t c = c0 + q;
+      const uint32_t c = c0 + q * kT;
       if (c >= (uint32_t)sd.cells) break;
       const uint32_t x = c % nx, y = (c / nx) % ny, z = c / pl;
       const int64_t node = (x + 1) + X * (y + 1) + XY * (z + 1);
@@ -720,18 +720,23 @@ __global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner, int jj) {
     const float *r = A.r32 + sd.cbase;
     const double ifixq = st.ifixq;
     unsigned long long units = 0;
-    // the thread's first cell, then its neighbours along the row (32-bit
-    // index arithmetic: a simulation holds fewer than 2^31 cells)
+    // the thread's cells a + q kT + tid, q < CPT: every load and store of a
+    // wave is one contiguous run (32-bit index arithmetic: a simulation holds
+    // fewer than 2^31 cells); all loads first, then the updates
     const uint32_t nx = (uint32_t)sd.nx, ny = (uint32_t)sd.ny, nz = (uint32_t)sd.nz;
     const uint32_t cells = (uint32_t)sd.cells, pl = nx * ny;
-    uint32_t c = (uint32_t)tl.a + threadIdx.x * CPT;
-    uint32_t x = c % nx, y = (c / nx) % ny, z = c / pl;
+    const uint32_t c0 = (uint32_t)tl.a + threadIdx.x;
+    float rr[CPT], ec[CPT], sum6[CPT];
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
+      const uint32_t c = c0 + q * kT;
+      rr[q] = 0.0f;
+      ec[q] = 0.0f;
+      sum6[q] = 0.0f;
       if (c < cells) {
-        const float rr = r[c];
-        float sum6 = 0.0f, ec = 0.0f;
+        rr[q] = r[c];
         if (j > 0) {
+          const uint32_t x = c % nx, y = (c / nx) % ny, z = c / pl;
           const uint32_t cn = y + 1 < ny ? c + nx : c - (ny - 1) * nx;
           const uint32_t cs = y > 0 ? c - nx : c + (ny - 1) * nx;
           const uint32_t ct = z + 1 < nz ? c + pl : c - (nz - 1) * pl;
@@ -739,21 +744,19 @@ __global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner, int jj) {
           const float eE = x + 1 < nx ? ein[c + 1] : 0.0f;
           const float eW = x > 0 ? ein[c - 1] : 0.0f;
           const float eN = ein[cn], eS = ein[cs], eT = ein[ct], eB = ein[cb];
-          ec = ein[c];
-          sum6 = ((eE + eW) + (eN + eS)) + (eT + eB);
+          ec[q] = ein[c];
+          sum6[q] = ((eE + eW) + (eN + eS)) + (eT + eB);
         }
-        eout[c] = __builtin_fmaf(sd.wf, sum6, __builtin_fmaf(sd.kf, ec, -sd.aff * rr));
-        const float res = rr - sd.i2f * (sum6 - 6.0f * ec);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      const uint32_t c = c0 + q * kT;
+      if (c < cells) {
+        eout[c] = __builtin_fmaf(sd.wf, sum6[q], __builtin_fmaf(sd.kf, ec[q], -sd.aff * rr[q]));
+        const float res = rr[q] - sd.i2f * (sum6[q] - 6.0f * ec[q]);
         const double u2 = ((double)res * (double)res) * ifixq;
         units += u2 < (double)(kStopUnits + 1) ? (unsigned long long)u2 : kStopUnits + 1;
-      }
-      ++c;
-      if (++x == nx) {
-        x = 0;
-        if (++y == ny) {
-          y = 0;
-          ++z;
-        }
       }
     }
     // wave, then block sum of the integer units, stored for the tile
@@ -1090,6 +1093,7 @@ struct picard_ensemble {
   // refinement pass
   hipGraphExec_t g_pre[2] = {nullptr, nullptr}, g_post[2] = {nullptr, nullptr};
   hipGraphExec_t g_chunk[2] = {nullptr, nullptr};
+  hipEvent_t chunk_ev[2] = {nullptr, nullptr};   // ends of the last two solve chunks enqueued
   int64_t n_host = 0;                // particles in the arena after the last step or load
   // capacities of the device work lists (entries)
   int64_t cap_node_t = 0, cap_cell_t = 0, cap_face_t = 0, cap_fred_t = 0;
@@ -1382,7 +1386,8 @@ static int launch_sweeps(picard_ensemble *e, int count, bool tune) {
     switch (e->tile.cells_per_thread) {
       case 1: launch_sweep<1>(e, tune, q); break;
       case 2: launch_sweep<2>(e, tune, q); break;
-      default: launch_sweep<4>(e, tune, q); break;
+      case 4: launch_sweep<4>(e, tune, q); break;
+      default: launch_sweep<8>(e, tune, q); break;
     }
   }
   PICARD_HIP(hipGetLastError());
@@ -1586,6 +1591,7 @@ int picard_ensemble_create(const picard_ensemble_config *cfg, picard_ensemble **
                            hipHostMallocMapped | hipHostMallocCoherent));
   PICARD_HIP(hipHostMalloc(reinterpret_cast<void **>(&e->host_err), sizeof(int32_t),
                            hipHostMallocMapped | hipHostMallocCoherent));
+  for (auto &ev : e->chunk_ev) PICARD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   std::memset(e->host_status, 0, sizeof(picard_sim_status) * S);
   *e->host_n = 0;
   *e->host_err = 0;
@@ -1620,6 +1626,8 @@ int picard_ensemble_destroy(picard_ensemble *e) {
   for (void *p : {(void *)e->host_active, (void *)e->host_status, (void *)e->host_n,
                   (void *)e->host_err})
     if (p) (void)hipHostFree(p);
+  for (hipEvent_t ev : e->chunk_ev)
+    if (ev) (void)hipEventDestroy(ev);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return PICARD_OK;
@@ -1755,18 +1763,31 @@ int picard_ensemble_step(picard_ensemble *e, picard_ensemble_stats *stats) {
   for (auto &s : e->slots)
     if (s.live) maxit = std::max<int64_t>(maxit, s.p.maxit);
   const int64_t max_chunks = (2 * maxit + 2) / kChunkSweeps + 2 * per_ref + 2;
-  int64_t chunks = 0;
-  for (;;) {
-    if (chunks >= max_chunks) {
-      set_error("ensemble step: the solve did not report its end within 2*maxit+1 sweeps");
-      return PICARD_ERR_STATE;
-    }
-    const bool outer = chunks % per_ref == 0;
+  // One chunk is kept enqueued beyond the one the host waits for, so the
+  // device never idles on the host's read of the count; the chunk enqueued
+  // after the last one needed finds no simulation solving and changes no
+  // state (its sweeps and decisions skip stopped simulations, and a
+  // refinement pass touches solving ones only).
+  int64_t chunks = 0, enqueued = 0;
+  auto enqueue = [&]() -> int {
+    const bool outer = enqueued % per_ref == 0;
     if (e->cfg.use_graphs) PICARD_HIP(hipGraphLaunch(e->g_chunk[outer ? 1 : 0], e->stream));
     else PICARD_TRY(launch_chunk(e, outer));
-    PICARD_HIP(hipStreamSynchronize(e->stream));
+    PICARD_HIP(hipEventRecord(e->chunk_ev[enqueued % 2], e->stream));
+    ++enqueued;
+    return PICARD_OK;
+  };
+  PICARD_TRY(enqueue());
+  for (;;) {
+    if (enqueued < max_chunks) PICARD_TRY(enqueue());
+    PICARD_HIP(hipEventSynchronize(e->chunk_ev[chunks % 2]));
     ++chunks;
     if (*reinterpret_cast<volatile int32_t *>(e->host_active) == 0) break;
+    if (chunks >= max_chunks) {
+      set_error("ensemble step: the solve did not report its end within 2*maxit+1 sweeps");
+      (void)hipStreamSynchronize(e->stream);
+      return PICARD_ERR_STATE;
+    }
   }
   const auto t_post = std::chrono::steady_clock::now();
   if (e->cfg.use_graphs) PICARD_HIP(hipGraphLaunch(e->g_post[cur], e->stream));
@@ -1783,7 +1804,7 @@ int picard_ensemble_step(picard_ensemble *e, picard_ensemble_stats *stats) {
   }
   out.sims = live;
   out.particles = 0;
-  out.solve_chunks = chunks;
+  out.solve_chunks = enqueued;
   for (int s = 0; s < (int)e->slots.size(); ++s) {
     if (!e->slots[s].live) continue;
     const picard_sim_status &o = e->host_status[s];
@@ -1903,9 +1924,10 @@ int picard_ensemble_particles_read(picard_ensemble *e, int32_t slot, double *rp,
 
 int picard_ensemble_set_tile(picard_ensemble *e, const picard_ensemble_tile *t) {
   ENS_CHECK(e);
-  if (!t || (t->cells_per_thread != 1 && t->cells_per_thread != 2 && t->cells_per_thread != 4) ||
+  if (!t || (t->cells_per_thread != 1 && t->cells_per_thread != 2 && t->cells_per_thread != 4 &&
+             t->cells_per_thread != 8) ||
       t->threads != kT || t->blocks_per_cu < 1 || t->blocks_per_cu > 8) {
-    set_error("picard_ensemble_set_tile: cells_per_thread 1, 2 or 4; threads 256; "
+    set_error("picard_ensemble_set_tile: cells_per_thread 1, 2, 4 or 8; threads 256; "
               "blocks_per_cu 1..8");
     return PICARD_ERR_ARG;
   }
@@ -1934,7 +1956,7 @@ int picard_ensemble_tune(picard_ensemble *e, int32_t sweeps, picard_ensemble_til
   }
   if (e->dirty) PICARD_TRY(rebuild(e));
   std::vector<picard_ensemble_tile> cand;
-  for (int cpt : {1, 2, 4})
+  for (int cpt : {1, 2, 4, 8})
     for (int bpc : {2, 4, 8}) cand.push_back({cpt, kT, bpc, 0});
   const picard_ensemble_tile keep = e->tile;
   hipEvent_t t0, t1;
