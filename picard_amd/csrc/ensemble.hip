@@ -551,9 +551,10 @@ __global__ void __launch_bounds__(kT) k_bnorm(Arenas A) {
 // ---------------------------------------------------------------------------
 // BCpotentials.f90:51-77 on the two x faces: U(face node) = 1/(4 pi eps0)
 // sum over cells C * D(offset), D of CalcDistance.f90:25-91 for x flow with y
-// and z periodic.  Face tiles {slot, face, first face node, x-plane chunk}:
-// one face node per thread (y fastest, so the D loads of a wave are
-// consecutive), partial sums per chunk, summed in chunk order afterwards.
+// and z periodic.  Face tiles {slot, face, first item, x-plane chunk}: an
+// item is kFaceNodes face nodes consecutive in y (one y row), one item per
+// thread; partial sums per chunk, summed in chunk order afterwards.
+constexpr int kFaceNodes = 8;
 
 __global__ void __launch_bounds__(kT) k_distance(Arenas A, int slot) {
   const SimDev &sd = A.sims[slot];
@@ -578,35 +579,82 @@ __global__ void __launch_bounds__(kT) k_distance(Arenas A, int slot) {
   }
 }
 
+// Each node's sum runs over x planes, then z rows, then y, in that order, one
+// FMA per cell.  For a block of kFaceNodes y cells jb..jb+R-1 of a row, the
+// item's nodes jj0..jj0+R-1 need D at |jb - jj0 + u - (R - 1)|, u < 2R - 1: a
+// window of the row that slides by R per block, so a thread loads R values of
+// D and R of C (the same for the whole wave) per R*R FMAs.
 __global__ void __launch_bounds__(kT) k_facesum(Arenas A) {
+  constexpr int R = kFaceNodes;
   const int nt = A.g->n_face_tiles;
   for (int t = blockIdx.x; t < nt; t += gridDim.x) {
     const Tile tl = A.face_tiles[t];
     const SimDev &sd = A.sims[tl.slot];
     const int face = tl.a, chunk = tl.c;
-    const int64_t fnodes = (int64_t)sd.ny * sd.nz;
-    const int64_t fn = (int64_t)tl.b + threadIdx.x;
-    const bool on = fn < fnodes;
-    const int jj = on ? (int)(fn % sd.ny) : 0, kk = on ? (int)(fn / sd.ny) : 0;
+    const int ny = sd.ny, nz = sd.nz;
+    const int gy = (ny + R - 1) / R;                  // items per y row
+    const int item = tl.b + (int)threadIdx.x;
+    const bool on = item < gy * nz;
+    const int jj0 = on ? (item % gy) * R : 0, kk = on ? item / gy : 0;
     const int i0 = chunk * kFacePlanes;
     const int i1 = i0 + kFacePlanes < sd.nx ? i0 + kFacePlanes : sd.nx;
     const int64_t X = sd.X, XY = (int64_t)sd.X * sd.Y;
-    const int64_t nb = sd.ny + 1, nbc = nb * (sd.nz + 1);
+    const int64_t nb = ny + 1, nbc = nb * (nz + 1);
     const double *Dt = A.D + sd.dbase;
-    double acc = 0.0;
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    // D index of window slot u for the block starting at jb (clamped into the
+    // row: a clamped slot only meets a node or cell beyond the row)
+    auto widx = [&](int jb, int u) {
+      int d = jb - jj0 + u - (R - 1);
+      d = d < 0 ? -d : d;
+      return d < ny ? d : ny;
+    };
     for (int ip = i0; ip < i1; ++ip) {
       const int a = face == 0 ? ip + 1 : sd.nx - ip;
-      for (int kp = 0; kp < sd.nz; ++kp) {
+      for (int kp = 0; kp < nz; ++kp) {
         const int dk = kp > kk ? kp - kk : kk - kp;
         const double *Drow = Dt + a * nbc + dk * nb;
         const double *Crow = A.C + sd.gbase + (ip + 1) + X * 1 + XY * (kp + 1);
-        for (int jp = 0; jp < sd.ny; ++jp) {
-          const int dj = jp > jj ? jp - jj : jj - jp;
-          acc = __builtin_fma(Crow[X * jp], Drow[dj], acc);
+        double w[2 * R - 1];
+#pragma unroll
+        for (int u = 0; u < 2 * R - 1; ++u) w[u] = Drow[widx(0, u)];
+        for (int jb = 0; jb < ny; jb += R) {
+          if (jb + R <= ny) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+              const double c = Crow[X * (jb + q)];
+#pragma unroll
+              for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(c, w[q - r + R - 1], acc[r]);
+            }
+          } else {
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+              if (jb + q < ny) {
+                const double c = Crow[X * (jb + q)];
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(c, w[q - r + R - 1], acc[r]);
+              }
+            }
+          }
+          if (jb + R < ny) {
+#pragma unroll
+            for (int u = 0; u < R - 1; ++u) w[u] = w[u + R];
+#pragma unroll
+            for (int u = R - 1; u < 2 * R - 1; ++u) w[u] = Drow[widx(jb + R, u)];
+          }
         }
       }
     }
-    if (on) A.fscr[sd.fbase + ((int64_t)face * sd.fchunks + chunk) * fnodes + fn] = acc;
+    if (on) {
+      const int64_t fnodes = (int64_t)ny * nz;
+      double *out = A.fscr + sd.fbase + ((int64_t)face * sd.fchunks + chunk) * fnodes +
+                    (int64_t)ny * kk;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (jj0 + r < ny) out[jj0 + r] = acc[r];
+    }
   }
 }
 
@@ -1295,11 +1343,12 @@ static int rebuild(picard_ensemble *e) {
       for (int x = 0; x < 8; ++x)
         if (m < share[x].size()) cell_t.push_back(share[x][m]);
     const int64_t fn = (int64_t)sd.ny * sd.nz;
-    for (int face = 0; face < 2; ++face)
-      for (int64_t a = 0; a < fn; a += kT) {
+    const int64_t items = (int64_t)((sd.ny + kFaceNodes - 1) / kFaceNodes) * sd.nz;
+    for (int face = 0; face < 2; ++face) {
+      for (int64_t a = 0; a < items; a += kT)
         for (int ch = 0; ch < sd.fchunks; ++ch) face_t.push_back({s, face, (int32_t)a, ch});
-        fred_t.push_back({s, face, (int32_t)a, 0});
-      }
+      for (int64_t a = 0; a < fn; a += kT) fred_t.push_back({s, face, (int32_t)a, 0});
+    }
     sd.pfirst = pfirst;
     imap.push_back({s, (int32_t)ions, 0, 0});
     pfirst += sd.inflow;
