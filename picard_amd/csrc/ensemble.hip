@@ -700,11 +700,15 @@ __global__ void __launch_bounds__(kT) k_outer(Arenas A, int cpt) {
     const float *e = A.e32[st.j % kEBufs] + sd.cbase;
     const int64_t X = sd.X, XY = (int64_t)sd.X * sd.Y;
     const uint32_t nx = (uint32_t)sd.nx, ny = (uint32_t)sd.ny, nz = (uint32_t)sd.nz;
-    const uint32_t pl = nx * ny;
+    const uint32_t pl = nx * ny, cells = (uint32_t)sd.cells;
+    // the slot's constants in registers before the first store
+    const double i2 = sd.i2, dVeps0 = sd.dVeps0;
+    const double *Cs = A.C + sd.gbase;
+    float *r32 = A.r32 + sd.cbase;
     const uint32_t c0 = (uint32_t)tl.a + threadIdx.x;
     for (int q = 0; q < cpt; ++q) {
       const uint32_t c = c0 + q * kT;
-      if (c >= (uint32_t)sd.cells) break;
+      if (c >= cells) break;
       const uint32_t x = c % nx, y = (c / nx) % ny, z = c / pl;
       const int64_t node = (x + 1) + X * (y + 1) + XY * (z + 1);
       const uint32_t yn = y + 1 < ny ? y + 1 : 0, ys = y > 0 ? y - 1 : ny - 1;
@@ -718,7 +722,7 @@ __global__ void __launch_bounds__(kT) k_outer(Arenas A, int cpt) {
       double uN = uc[nN], uS = uc[nS], uT = uc[nT], uB = uc[nB];
       if (upd) {
         uC += (double)e[c];
-        if (x + 1 < sd.nx) uE += (double)e[cE];
+        if (x + 1 < nx) uE += (double)e[cE];
         if (x > 0) uW += (double)e[cW];
         uN += (double)e[cN];
         uS += (double)e[cS];
@@ -726,9 +730,9 @@ __global__ void __launch_bounds__(kT) k_outer(Arenas A, int cpt) {
         uB += (double)e[cB];
         un[node] = uC;
       }
-      const double b = -A.C[sd.gbase + node] / sd.dVeps0;
-      const double lap = ((uE + uW - 2.0 * uC) + (uN + uS - 2.0 * uC) + (uT + uB - 2.0 * uC)) * sd.i2;
-      A.r32[sd.cbase + c] = (float)(b - lap);
+      const double b = -Cs[node] / dVeps0;
+      const double lap = ((uE + uW - 2.0 * uC) + (uN + uS - 2.0 * uC) + (uT + uB - 2.0 * uC)) * i2;
+      r32[c] = (float)(b - lap);
     }
   }
 }
@@ -767,6 +771,10 @@ __global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner, int jj) {
     float *eout = A.e32[(j + 1) % kEBufs] + sd.cbase;
     const float *r = A.r32 + sd.cbase;
     const double ifixq = st.ifixq;
+    // the slot's constants in registers before the first store (a store
+    // through eout could otherwise alias them and force a reload per cell)
+    const float wf = sd.wf, kf = sd.kf, aff = sd.aff, i2f = sd.i2f;
+    const int64_t ubase = sd.ubase;
     unsigned long long units = 0;
     // the thread's cells a + q kT + tid, q < CPT: every load and store of a
     // wave is one contiguous run (32-bit index arithmetic: a simulation holds
@@ -774,35 +782,49 @@ __global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner, int jj) {
     const uint32_t nx = (uint32_t)sd.nx, ny = (uint32_t)sd.ny, nz = (uint32_t)sd.nz;
     const uint32_t cells = (uint32_t)sd.cells, pl = nx * ny;
     const uint32_t c0 = (uint32_t)tl.a + threadIdx.x;
+    // every load unconditional (addresses clamped into the slot, values
+    // selected afterwards), all of them before the first sum, so a thread
+    // waits for memory once per tile
     float rr[CPT], ec[CPT], sum6[CPT];
+    float vE[CPT], vW[CPT], vN[CPT], vS[CPT], vT[CPT], vB[CPT];
+    uint32_t xs[CPT];
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
-      const uint32_t c = c0 + q * kT;
-      rr[q] = 0.0f;
+      const uint32_t c = min(c0 + q * kT, cells - 1);
+      const uint32_t x = c % nx, y = (c / nx) % ny, z = c / pl;
+      xs[q] = x;
+      rr[q] = r[c];
       ec[q] = 0.0f;
+      vE[q] = vW[q] = vN[q] = vS[q] = vT[q] = vB[q] = 0.0f;
+      if (j > 0) {                                     // uniform
+        const uint32_t cn = y + 1 < ny ? c + nx : c - (ny - 1) * nx;
+        const uint32_t cs = y > 0 ? c - nx : c + (ny - 1) * nx;
+        const uint32_t ct = z + 1 < nz ? c + pl : c - (nz - 1) * pl;
+        const uint32_t cb = z > 0 ? c - pl : c + (nz - 1) * pl;
+        vE[q] = ein[x + 1 < nx ? c + 1 : c];
+        vW[q] = ein[x > 0 ? c - 1 : c];
+        vN[q] = ein[cn];
+        vS[q] = ein[cs];
+        vT[q] = ein[ct];
+        vB[q] = ein[cb];
+        ec[q] = ein[c];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
       sum6[q] = 0.0f;
-      if (c < cells) {
-        rr[q] = r[c];
-        if (j > 0) {
-          const uint32_t x = c % nx, y = (c / nx) % ny, z = c / pl;
-          const uint32_t cn = y + 1 < ny ? c + nx : c - (ny - 1) * nx;
-          const uint32_t cs = y > 0 ? c - nx : c + (ny - 1) * nx;
-          const uint32_t ct = z + 1 < nz ? c + pl : c - (nz - 1) * pl;
-          const uint32_t cb = z > 0 ? c - pl : c + (nz - 1) * pl;
-          const float eE = x + 1 < nx ? ein[c + 1] : 0.0f;
-          const float eW = x > 0 ? ein[c - 1] : 0.0f;
-          const float eN = ein[cn], eS = ein[cs], eT = ein[ct], eB = ein[cb];
-          ec[q] = ein[c];
-          sum6[q] = ((eE + eW) + (eN + eS)) + (eT + eB);
-        }
+      if (j > 0) {
+        const float eE = xs[q] + 1 < nx ? vE[q] : 0.0f;
+        const float eW = xs[q] > 0 ? vW[q] : 0.0f;
+        sum6[q] = ((eE + eW) + (vN[q] + vS[q])) + (vT[q] + vB[q]);
       }
     }
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
       const uint32_t c = c0 + q * kT;
       if (c < cells) {
-        eout[c] = __builtin_fmaf(sd.wf, sum6[q], __builtin_fmaf(sd.kf, ec[q], -sd.aff * rr[q]));
-        const float res = rr[q] - sd.i2f * (sum6[q] - 6.0f * ec[q]);
+        eout[c] = __builtin_fmaf(wf, sum6[q], __builtin_fmaf(kf, ec[q], -aff * rr[q]));
+        const float res = rr[q] - i2f * (sum6[q] - 6.0f * ec[q]);
         const double u2 = ((double)res * (double)res) * ifixq;
         units += u2 < (double)(kStopUnits + 1) ? (unsigned long long)u2 : kStopUnits + 1;
       }
@@ -814,7 +836,7 @@ __global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner, int jj) {
     if (threadIdx.x == 0) {
       unsigned long long sum = 0;
       for (int w = 0; w < kT / 64; ++w) sum += wsum[w];
-      A.units[A.ucap * (j % kDecideEvery) + sd.ubase + tl.b] = sum;
+      A.units[A.ucap * (j % kDecideEvery) + ubase + tl.b] = sum;
     }
     __syncthreads();
   }
