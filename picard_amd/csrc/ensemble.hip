@@ -847,8 +847,7 @@ __global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner, int jj) {
 // j + kDecideEvery - 1 from their tiles' units, and the stop rule applied to
 // each in turn, as after every sweep.
 __global__ void __launch_bounds__(kT) k_decide(Arenas A, int inner, int jd) {
-  __shared__ unsigned long long wsum[kT / 64];
-  __shared__ int go;
+  __shared__ unsigned long long wsum[kDecideEvery][kT / 64];
   const int nl = A.g->n_live;
   const int j0 = A.g->chunk_j0 + jd;
   for (int q = blockIdx.x; q < nl; q += gridDim.x) {
@@ -856,37 +855,46 @@ __global__ void __launch_bounds__(kT) k_decide(Arenas A, int inner, int jd) {
     const SimDev &sd = A.sims[s];
     SimState &st = A.state[s];
     if (!st.solving || j0 >= inner) continue;         // block-uniform
+    // the group's iterates summed in one pass over the tiles
+    const int tiles = sd.tiles;
+    const unsigned long long *u = A.units + sd.ubase;
+    unsigned long long acc[kDecideEvery];
+#pragma unroll
+    for (int k = 0; k < kDecideEvery; ++k) acc[k] = 0;
+    for (int t = threadIdx.x; t < tiles; t += kT) {
+#pragma unroll
+      for (int k = 0; k < kDecideEvery; ++k) acc[k] += u[A.ucap * ((j0 + k) % kDecideEvery) + t];
+    }
+#pragma unroll
     for (int k = 0; k < kDecideEvery; ++k) {
-      const int j = j0 + k;
-      const unsigned long long *u = A.units + A.ucap * (j % kDecideEvery) + sd.ubase;
-      unsigned long long acc = 0;
-      for (int t = threadIdx.x; t < sd.tiles; t += kT) acc += u[t];
-      for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-      if (lane_id() == 0) wsum[threadIdx.x >> 6] = acc;
-      __syncthreads();
-      if (threadIdx.x == 0) {
+      for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_down(acc[k], off, 64);
+      if (lane_id() == 0) wsum[k][threadIdx.x >> 6] = acc[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int64_t cap = 2 * (int64_t)sd.maxit + 1;
+      for (int k = 0; k < kDecideEvery; ++k) {
         unsigned long long tot = 0;
-        for (int w = 0; w < kT / 64; ++w) tot += wsum[w];
+        for (int w = 0; w < kT / 64; ++w) tot += wsum[k][w];
+        const int j = j0 + k;
         const int64_t it = st.base + j;
-        const int64_t cap = 2 * (int64_t)sd.maxit + 1;
-        go = 0;
         if (tot <= kStopUnits) {                 // |res| <= maxerr |b| (CalcEpotential.f90:60-62)
           st.solving = 0;
           st.sweeps = it;
           st.stop_j = j;
-        } else if (it + 1 >= cap) {              // 2*maxit+1 sweeps (:67-69)
+          break;
+        }
+        if (it + 1 >= cap) {                     // 2*maxit+1 sweeps (:67-69)
           st.solving = 0;
           st.sweeps = cap;
           st.stop_j = j + 1;
           st.noconv = 1;
-        } else {
-          st.j = j + 1;
-          go = 1;
+          break;
         }
+        st.j = j + 1;
       }
-      __syncthreads();
-      if (!go) break;                            // block-uniform
     }
+    __syncthreads();
   }
 }
 
