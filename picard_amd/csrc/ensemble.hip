@@ -756,6 +756,33 @@ __global__ void k_outer_done(Arenas A) {
 
 // One FP32 Jacobi sweep e_{j+1} = J(e_j) on every solving slot's cell tiles,
 // with the stop test of iterate base + j from the same neighbour sums.
+// base[i] through a 32-bit byte offset (a slot holds fewer than 2^30 cells),
+// so the access takes the uniform base and one 32-bit offset register.
+__device__ __forceinline__ float ldf(const float *base, uint32_t i) {
+  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (i << 2));
+}
+
+__device__ __forceinline__ void stf(float *base, uint32_t i, float v) {
+  *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (i << 2)) = v;
+}
+
+// v / d for v < 2^22 (one float product, one fix-up).
+__device__ __forceinline__ uint32_t small_quot(uint32_t v, uint32_t d, float invd) {
+  uint32_t q = (uint32_t)((float)v * invd);
+  const int32_t r = (int32_t)v - (int32_t)(q * d);
+  if (r < 0) --q;
+  else if (r >= (int32_t)d) ++q;
+  return q;
+}
+
+// Residual units min(u2, 2^40 + 1) truncated to an integer: for a value below
+// 2^52, trunc(v) + 2^52 holds the integer in its low mantissa bits (NaN and
+// infinities clamp to the cap, as the comparison with the cap does).
+__device__ __forceinline__ unsigned long long units_of(double u2) {
+  const double v = __builtin_trunc(__builtin_fmin(u2, (double)(kStopUnits + 1))) + 4503599627370496.0;
+  return (unsigned long long)__double_as_longlong(v) - 0x4330000000000000ull;
+}
+
 template <int CPT, bool TUNE>
 __global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner, int jj) {
   __shared__ unsigned long long wsum[kT / 64];
@@ -778,55 +805,89 @@ __global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner, int jj) {
     unsigned long long units = 0;
     // the thread's cells a + q kT + tid, q < CPT: every load and store of a
     // wave is one contiguous run (32-bit index arithmetic: a simulation holds
-    // fewer than 2^31 cells); all loads first, then the updates
+    // fewer than 2^31 cells).  The kernel is bound by instruction issue, so
+    // the cell coordinates are not divided out per cell: the tile's first
+    // cell is split once (uniform), the thread's offset carried in with two
+    // small quotients, and each further cell steps by kT cells with two
+    // carries.
     const uint32_t nx = (uint32_t)sd.nx, ny = (uint32_t)sd.ny, nz = (uint32_t)sd.nz;
     const uint32_t cells = (uint32_t)sd.cells, pl = nx * ny;
-    const uint32_t c0 = (uint32_t)tl.a + threadIdx.x;
-    // every load unconditional (addresses clamped into the slot, values
-    // selected afterwards), all of them before the first sum, so a thread
-    // waits for memory once per tile
-    float rr[CPT], ec[CPT], sum6[CPT];
-    float vE[CPT], vW[CPT], vN[CPT], vS[CPT], vT[CPT], vB[CPT];
-    uint32_t xs[CPT];
-#pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-      const uint32_t c = min(c0 + q * kT, cells - 1);
-      const uint32_t x = c % nx, y = (c / nx) % ny, z = c / pl;
-      xs[q] = x;
-      rr[q] = r[c];
-      ec[q] = 0.0f;
-      vE[q] = vW[q] = vN[q] = vS[q] = vT[q] = vB[q] = 0.0f;
-      if (j > 0) {                                     // uniform
-        const uint32_t cn = y + 1 < ny ? c + nx : c - (ny - 1) * nx;
-        const uint32_t cs = y > 0 ? c - nx : c + (ny - 1) * nx;
-        const uint32_t ct = z + 1 < nz ? c + pl : c - (nz - 1) * pl;
-        const uint32_t cb = z > 0 ? c - pl : c + (nz - 1) * pl;
-        vE[q] = ein[x + 1 < nx ? c + 1 : c];
-        vW[q] = ein[x > 0 ? c - 1 : c];
-        vN[q] = ein[cn];
-        vS[q] = ein[cs];
-        vT[q] = ein[ct];
-        vB[q] = ein[cb];
-        ec[q] = ein[c];
-      }
+    const uint32_t a = (uint32_t)tl.a;
+    const uint32_t za = a / pl, ra = a - za * pl, ya = ra / nx, xa = ra - ya * nx;
+    const uint32_t kq = (uint32_t)kT / nx, sx = (uint32_t)kT - kq * nx, sy = kq % ny, sz = kq / ny;
+    uint32_t x = xa + threadIdx.x, y = ya, z = za;
+    {
+      const uint32_t dy = small_quot(x, nx, 1.0f / (float)nx);
+      x -= dy * nx;
+      y += dy;
+      const uint32_t dz = small_quot(y, ny, 1.0f / (float)ny);
+      y -= dz * ny;
+      z += dz;
     }
+    const uint32_t c0 = a + threadIdx.x;
+    // every load unconditional (cells beyond the slot read its last cell,
+    // values unused), all of them before the first sum, so a thread waits
+    // for memory once per tile
+    float rr[CPT], ec[CPT], sum6[CPT];
+    if (j > 0) {                                       // uniform
+      float vE[CPT], vW[CPT], vN[CPT], vS[CPT], vT[CPT], vB[CPT];
+      bool hasE[CPT], hasW[CPT];
 #pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-      sum6[q] = 0.0f;
-      if (j > 0) {
-        const float eE = xs[q] + 1 < nx ? vE[q] : 0.0f;
-        const float eW = xs[q] > 0 ? vW[q] : 0.0f;
+      for (int q = 0; q < CPT; ++q) {
+        if (q > 0) {
+          x += sx;
+          y += sy;
+          z += sz;
+          if (x >= nx) {
+            x -= nx;
+            ++y;
+          }
+          if (y >= ny) {
+            y -= ny;
+            ++z;
+          }
+        }
+        const bool in = z < nz;
+        const uint32_t c = in ? c0 + q * kT : cells - 1;
+        const uint32_t xl = in ? x : nx - 1, yl = in ? y : ny - 1, zl = in ? z : nz - 1;
+        hasE[q] = xl + 1 < nx;
+        hasW[q] = xl > 0;
+        const uint32_t cn = yl + 1 < ny ? c + nx : c - (ny - 1) * nx;
+        const uint32_t cs = yl > 0 ? c - nx : c + (ny - 1) * nx;
+        const uint32_t ct = zl + 1 < nz ? c + pl : c - (nz - 1) * pl;
+        const uint32_t cb = zl > 0 ? c - pl : c + (nz - 1) * pl;
+        rr[q] = ldf(r, c);
+        vE[q] = ldf(ein, hasE[q] ? c + 1 : c);
+        vW[q] = ldf(ein, hasW[q] ? c - 1 : c);
+        vN[q] = ldf(ein, cn);
+        vS[q] = ldf(ein, cs);
+        vT[q] = ldf(ein, ct);
+        vB[q] = ldf(ein, cb);
+        ec[q] = ldf(ein, c);
+      }
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) {
+        const float eE = hasE[q] ? vE[q] : 0.0f;
+        const float eW = hasW[q] ? vW[q] : 0.0f;
         sum6[q] = ((eE + eW) + (vN[q] + vS[q])) + (vT[q] + vB[q]);
+      }
+    } else {                                           // e_0 = 0
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) {
+        const uint32_t c = c0 + q * kT;
+        rr[q] = ldf(r, c < cells ? c : cells - 1);
+        ec[q] = 0.0f;
+        sum6[q] = 0.0f;
       }
     }
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
       const uint32_t c = c0 + q * kT;
       if (c < cells) {
-        eout[c] = __builtin_fmaf(wf, sum6[q], __builtin_fmaf(kf, ec[q], -aff * rr[q]));
+        stf(eout, c, __builtin_fmaf(wf, sum6[q], __builtin_fmaf(kf, ec[q], -aff * rr[q])));
         const float res = rr[q] - i2f * (sum6[q] - 6.0f * ec[q]);
         const double u2 = ((double)res * (double)res) * ifixq;
-        units += u2 < (double)(kStopUnits + 1) ? (unsigned long long)u2 : kStopUnits + 1;
+        units += units_of(u2);
       }
     }
     // wave, then block sum of the integer units, stored for the tile
