@@ -783,6 +783,63 @@ __device__ __forceinline__ unsigned long long units_of(double u2) {
   return (unsigned long long)__double_as_longlong(v) - 0x4330000000000000ull;
 }
 
+__device__ __forceinline__ float ldb(const float *base, uint32_t byte_off) {
+  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
+// The operands of a thread's cells c0 + q kT (coordinates x, y, z of the
+// first, stepped by (sx, sy, sz) per kT cells) for an iterate j > 0: r, e at
+// the cell and its six neighbours (periodic in y and z, zero beyond the x
+// faces), through byte offsets from the uniform bases; all loads before the
+// first sum.  FULL: every cell inside the slot; otherwise cells beyond it read
+// the slot's last cell (values unused).
+template <int CPT, bool FULL>
+__device__ __forceinline__ void sweep_gather(const float *ein, const float *r, uint32_t cells,
+                                             uint32_t nx, uint32_t ny, uint32_t nz, uint32_t x,
+                                             uint32_t y, uint32_t z, uint32_t c0, uint32_t sx,
+                                             uint32_t sy, uint32_t sz, float (&rr)[CPT],
+                                             float (&ec)[CPT], float (&sum6)[CPT]) {
+  const uint32_t pl = nx * ny;
+  const uint32_t bnx = nx * 4u, bpl = pl * 4u, bwy = (ny - 1) * bnx, bwz = (nz - 1) * bpl;
+  float vE[CPT], vW[CPT], vN[CPT], vS[CPT], vT[CPT], vB[CPT];
+  bool hasE[CPT], hasW[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    if (q > 0) {
+      x += sx;
+      y += sy;
+      z += sz;
+      if (x >= nx) {
+        x -= nx;
+        ++y;
+      }
+      if (y >= ny) {
+        y -= ny;
+        ++z;
+      }
+    }
+    const bool in = FULL || z < nz;
+    const uint32_t xl = in ? x : nx - 1, yl = in ? y : ny - 1, zl = in ? z : nz - 1;
+    const uint32_t cb = (in ? c0 + q * kT : cells - 1) * 4u;
+    hasE[q] = xl + 1 < nx;
+    hasW[q] = xl > 0;
+    rr[q] = ldb(r, cb);
+    ec[q] = ldb(ein, cb);
+    vE[q] = ldb(ein, hasE[q] ? cb + 4u : cb);
+    vW[q] = ldb(ein, hasW[q] ? cb - 4u : cb);
+    vN[q] = ldb(ein, yl + 1 < ny ? cb + bnx : cb - bwy);
+    vS[q] = ldb(ein, yl > 0 ? cb - bnx : cb + bwy);
+    vT[q] = ldb(ein, zl + 1 < nz ? cb + bpl : cb - bwz);
+    vB[q] = ldb(ein, zl > 0 ? cb - bpl : cb + bwz);
+  }
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    const float eE = hasE[q] ? vE[q] : 0.0f;
+    const float eW = hasW[q] ? vW[q] : 0.0f;
+    sum6[q] = ((eE + eW) + (vN[q] + vS[q])) + (vT[q] + vB[q]);
+  }
+}
+
 template <int CPT, bool TUNE>
 __global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner, int jj) {
   __shared__ unsigned long long wsum[kT / 64];
@@ -830,47 +887,10 @@ __global__ void __launch_bounds__(kT) k_sweep(Arenas A, int inner, int jj) {
     // for memory once per tile
     float rr[CPT], ec[CPT], sum6[CPT];
     if (j > 0) {                                       // uniform
-      float vE[CPT], vW[CPT], vN[CPT], vS[CPT], vT[CPT], vB[CPT];
-      bool hasE[CPT], hasW[CPT];
-#pragma unroll
-      for (int q = 0; q < CPT; ++q) {
-        if (q > 0) {
-          x += sx;
-          y += sy;
-          z += sz;
-          if (x >= nx) {
-            x -= nx;
-            ++y;
-          }
-          if (y >= ny) {
-            y -= ny;
-            ++z;
-          }
-        }
-        const bool in = z < nz;
-        const uint32_t c = in ? c0 + q * kT : cells - 1;
-        const uint32_t xl = in ? x : nx - 1, yl = in ? y : ny - 1, zl = in ? z : nz - 1;
-        hasE[q] = xl + 1 < nx;
-        hasW[q] = xl > 0;
-        const uint32_t cn = yl + 1 < ny ? c + nx : c - (ny - 1) * nx;
-        const uint32_t cs = yl > 0 ? c - nx : c + (ny - 1) * nx;
-        const uint32_t ct = zl + 1 < nz ? c + pl : c - (nz - 1) * pl;
-        const uint32_t cb = zl > 0 ? c - pl : c + (nz - 1) * pl;
-        rr[q] = ldf(r, c);
-        vE[q] = ldf(ein, hasE[q] ? c + 1 : c);
-        vW[q] = ldf(ein, hasW[q] ? c - 1 : c);
-        vN[q] = ldf(ein, cn);
-        vS[q] = ldf(ein, cs);
-        vT[q] = ldf(ein, ct);
-        vB[q] = ldf(ein, cb);
-        ec[q] = ldf(ein, c);
-      }
-#pragma unroll
-      for (int q = 0; q < CPT; ++q) {
-        const float eE = hasE[q] ? vE[q] : 0.0f;
-        const float eW = hasW[q] ? vW[q] : 0.0f;
-        sum6[q] = ((eE + eW) + (vN[q] + vS[q])) + (vT[q] + vB[q]);
-      }
+      if (a + (uint32_t)(CPT * kT) <= cells)           // every tile but a slot's last
+        sweep_gather<CPT, true>(ein, r, cells, nx, ny, nz, x, y, z, c0, sx, sy, sz, rr, ec, sum6);
+      else
+        sweep_gather<CPT, false>(ein, r, cells, nx, ny, nz, x, y, z, c0, sx, sy, sz, rr, ec, sum6);
     } else {                                           // e_0 = 0
 #pragma unroll
       for (int q = 0; q < CPT; ++q) {
