@@ -60,7 +60,7 @@ constexpr double kTau = 6.2831853071795864769252867665590;
 constexpr double kEC = 1.60217646e-19, kMP = 1.6726216e-27, kME = 9.1093819e-31;
 constexpr double kEPS0 = 8.854187817e-12;
 constexpr int kChunkSweeps = 64;     // sweeps per replay of the solve graph
-constexpr int kDecideEvery = 4;      // sweeps per stop decision (divides kChunkSweeps)
+constexpr int kDecideEvery = 8;      // sweeps per stop decision (divides kChunkSweeps)
 constexpr int kEBufs = kDecideEvery + 1;   // correction buffers
 constexpr int kFacePlanes = 8;       // x-planes per face-sum tile
 constexpr int kMaxSlots = 4096;
