@@ -584,8 +584,57 @@ __global__ void __launch_bounds__(kT) k_distance(Arenas A, int slot) {
 // item's nodes jj0..jj0+R-1 need D at |jb - jj0 + u - (R - 1)|, u < 2R - 1: a
 // window of the row that slides by R per block, so a thread loads R values of
 // D and R of C (the same for the whole wave) per R*R FMAs.
+constexpr int kFaceRowMax = 512;      // y rows staged in LDS (longer rows read C from memory)
+
+// The FMAs of one (x plane, z row) of cells for the thread's kFaceNodes nodes,
+// C of the row from `crow` (LDS or memory, element stride cs).
+template <typename CRow>
+__device__ __forceinline__ void facesum_row(double (&acc)[kFaceNodes], const double *Drow, CRow crow,
+                                            int ny, int jj0) {
+  constexpr int R = kFaceNodes;
+  // D index of window slot u for the block starting at jb (clamped into the
+  // row: a clamped slot only meets a node or cell beyond the row)
+  auto widx = [&](int jb, int u) {
+    int d = jb - jj0 + u - (R - 1);
+    d = d < 0 ? -d : d;
+    return d < ny ? d : ny;
+  };
+  double w[2 * R - 1];
+#pragma unroll
+  for (int u = 0; u < 2 * R - 1; ++u) w[u] = Drow[widx(0, u)];
+  for (int jb = 0; jb < ny; jb += R) {
+    if (jb + R <= ny) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const double c = crow(jb + q);
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(c, w[q - r + R - 1], acc[r]);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        if (jb + q < ny) {
+          const double c = crow(jb + q);
+#pragma unroll
+          for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(c, w[q - r + R - 1], acc[r]);
+        }
+      }
+    }
+    if (jb + R < ny) {
+#pragma unroll
+      for (int u = 0; u < R - 1; ++u) w[u] = w[u + R];
+#pragma unroll
+      for (int u = R - 1; u < 2 * R - 1; ++u) w[u] = Drow[widx(jb + R, u)];
+    }
+  }
+}
+
+// All threads of a tile walk the same (x plane, z row) sequence, so each row
+// of C is loaded once per block into LDS (double-buffered: one barrier per
+// row) and read from there by every wave.
 __global__ void __launch_bounds__(kT) k_facesum(Arenas A) {
   constexpr int R = kFaceNodes;
+  __shared__ double cs[2][kFaceRowMax];
   const int nt = A.g->n_face_tiles;
   for (int t = blockIdx.x; t < nt; t += gridDim.x) {
     const Tile tl = A.face_tiles[t];
@@ -601,50 +650,41 @@ __global__ void __launch_bounds__(kT) k_facesum(Arenas A) {
     const int64_t X = sd.X, XY = (int64_t)sd.X * sd.Y;
     const int64_t nb = ny + 1, nbc = nb * (nz + 1);
     const double *Dt = A.D + sd.dbase;
+    const double *C0 = A.C + sd.gbase;
     double acc[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = 0.0;
-    // D index of window slot u for the block starting at jb (clamped into the
-    // row: a clamped slot only meets a node or cell beyond the row)
-    auto widx = [&](int jb, int u) {
-      int d = jb - jj0 + u - (R - 1);
-      d = d < 0 ? -d : d;
-      return d < ny ? d : ny;
+    // rows in order: it = (ip - i0) nz + kp
+    auto crow_ptr = [&](int it) {
+      const int ip = i0 + it / nz, kp = it % nz;
+      return C0 + (ip + 1) + X * 1 + XY * (kp + 1);
     };
-    for (int ip = i0; ip < i1; ++ip) {
+    auto drow_ptr = [&](int it) {
+      const int ip = i0 + it / nz, kp = it % nz;
       const int a = face == 0 ? ip + 1 : sd.nx - ip;
-      for (int kp = 0; kp < nz; ++kp) {
-        const int dk = kp > kk ? kp - kk : kk - kp;
-        const double *Drow = Dt + a * nbc + dk * nb;
-        const double *Crow = A.C + sd.gbase + (ip + 1) + X * 1 + XY * (kp + 1);
-        double w[2 * R - 1];
-#pragma unroll
-        for (int u = 0; u < 2 * R - 1; ++u) w[u] = Drow[widx(0, u)];
-        for (int jb = 0; jb < ny; jb += R) {
-          if (jb + R <= ny) {
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-              const double c = Crow[X * (jb + q)];
-#pragma unroll
-              for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(c, w[q - r + R - 1], acc[r]);
-            }
-          } else {
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-              if (jb + q < ny) {
-                const double c = Crow[X * (jb + q)];
-#pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(c, w[q - r + R - 1], acc[r]);
-              }
-            }
-          }
-          if (jb + R < ny) {
-#pragma unroll
-            for (int u = 0; u < R - 1; ++u) w[u] = w[u + R];
-#pragma unroll
-            for (int u = R - 1; u < 2 * R - 1; ++u) w[u] = Drow[widx(jb + R, u)];
-          }
+      const int dk = kp > kk ? kp - kk : kk - kp;
+      return Dt + a * nbc + dk * nb;
+    };
+    const int rows = (i1 - i0) * nz;
+    if (ny <= kFaceRowMax) {                          // block-uniform
+      {
+        const double *Crow = crow_ptr(0);
+        for (int q = threadIdx.x; q < ny; q += kT) cs[0][q] = Crow[X * q];
+      }
+      __syncthreads();
+      for (int it = 0; it < rows; ++it) {
+        if (it + 1 < rows) {
+          const double *Crow = crow_ptr(it + 1);
+          for (int q = threadIdx.x; q < ny; q += kT) cs[(it + 1) & 1][q] = Crow[X * q];
         }
+        const double *cb = cs[it & 1];
+        facesum_row(acc, drow_ptr(it), [&](int q) { return cb[q]; }, ny, jj0);
+        __syncthreads();
+      }
+    } else {
+      for (int it = 0; it < rows; ++it) {
+        const double *Crow = crow_ptr(it);
+        facesum_row(acc, drow_ptr(it), [&](int q) { return Crow[X * q]; }, ny, jj0);
       }
     }
     if (on) {
