@@ -3,7 +3,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 256]
                     [--fp64-steps K64] [--fp64-warmup W64]
-                    [--form both|split|fp64] [--no-cpu-baseline]
+                    [--form both|split|fp64] [--no-cpu-baseline] [--no-server]
     python bench.py --solve-only SWEEPS [--form split|fp64]   (profiling)
     python bench.py --copy-cal                                (profiling)
     python bench.py --stream-rates                            (stream reference)
@@ -29,7 +29,11 @@ split steps, so its count is its own and the whole run stays within a few
 minutes at K = 20).  The line reports both forms, their ratio over the same
 step indices (whole step, and the solve phase alone per sweep), and the split
 form's error against the FP64 form after the first step (phi, E, densities:
-normwise relative).
+normwise relative).  The metric's second half, p50/p99 job completion latency,
+comes from the ensemble server on its standard stream (40 Poisson jobs at 2
+jobs/s, 32^3-160^3 grids, step counts scaled by 1/100), run after the forms
+and reported as `server` in the same line (`--no-server` skips it; `--server`
+runs it alone with its own options).
 
 Prints one JSON line with the roofline of the dominant kernel (the split
 form's FP32 Jacobi sweep, timed live over one launch in 16 by the kernel's
@@ -501,6 +505,9 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--form", choices=("both", "split", "fp64"), default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-server", action="store_true",
+                    help="skip the job-server leg of the default run (the metric's p50/p99 job "
+                         "latency)")
     ap.add_argument("--solve-only", type=int, default=0, metavar="SWEEPS",
                     help="profiling: one capped field solve of --form (split or fp64) only")
     ap.add_argument("--inner", type=int, default=0, metavar="M",
@@ -733,6 +740,25 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_server and args.form == "both":
+        # the metric's second half: the ensemble server's p50/p99 job
+        # completion latency on its standard stream (40 Poisson jobs at 2
+        # jobs/s, 32^3-160^3 grids, step counts scaled by 1/100), run after the
+        # single-domain forms have released their device memory
+        try:
+            r = server_bench(2.0, 40, 0.01, 32, 160, 240.0, lanes_split=80)
+            out["server"] = {
+                "workload": "Poisson arrivals at 2 jobs/s, 40 jobs, grids log-uniform "
+                            "32^3-160^3, 8-32 ppc, 200-2000 steps x 0.01, 250 GB HBM admission "
+                            "budget, two lanes (edges up to 80 and above)",
+                **{k: r[k] for k in ("p50_latency_s", "p99_latency_s", "particle_steps_per_s",
+                                     "wall_s", "jobs_done", "p50_wait_s", "p99_wait_s",
+                                     "peak_jobs", "peak_bytes")},
+                "lanes": r.get("lanes"),
+            }
+        except Exception as exc:          # the single-domain line still prints
+            log(f"server leg failed: {exc!r}")
+            out["server"] = {"error": repr(exc)}
     if not args.no_cpu_baseline:
         P = head["pushed"] / head_steps
         out["cpu_baseline"] = cpu_baseline(workload=dict(
