@@ -22,9 +22,10 @@
 // (units of the simulation's threshold^2 * 2^-40, clamped at 2^40 + 1); each
 // tile stores its integer sum, and a small launch after every kDecideEvery
 // sweeps sums each simulation's tiles (integers: the same for any tiling and
-// any mix of simulations) for each of those iterates in turn and applies the
-// stop rule of CalcEpotential.f90:60-62 (sum <= 2^40, i.e. |res| <= maxerr
-// |b|) to the first that passes, advancing or stopping that simulation.  The
+// any mix of simulations) for all of those iterates in one pass and applies
+// the stop rule of CalcEpotential.f90:60-62 (sum <= 2^40, i.e. |res| <=
+// maxerr |b|) to each in turn, stopping at the first that passes, advancing
+// or stopping that simulation.  The
 // corrections rotate over kDecideEvery + 1 buffers, so the iterate the rule
 // picks is still held when it is picked: a simulation stops on the same
 // iterate as with a decision after every sweep, and the sweeps it ran past it
