@@ -180,34 +180,80 @@ constexpr double kFluxFixInv = 1.0 / 16777216.0;
 
 // Particles are sorted by (cell, species), so a thread walks kDepChunk
 // consecutive particles and adds the fixed-point weights of each run of one
-// cell and species in registers, with one atomic per corner when the run
-// ends: integer sums, so the result equals one atomic per particle.
+// cell and species in registers, flushing one add per corner when the run
+// ends: integer sums, so the result equals one atomic per particle.  Without
+// the fluxes the adds are staged in LDS per block: the block's 4096 sorted
+// particles cover a short stretch of lower nodes from the first one's, b0, so
+// their corners fall in four node segments b0 + {0, X, XY, XY + X} + [0,
+// kDepSeg); runs whose corners lie inside go to the segments with LDS
+// atomics, the rest straight to memory, and the block then adds each nonzero
+// segment entry to memory once (a few hundred global atomics per block
+// instead of eight per run).
 constexpr int kDepChunk = 16;
+constexpr int kDepSeg = 128;                    // nodes per LDS segment
+constexpr int kDepLdsSpecies = 3;               // species the LDS segments hold
 
 template <bool FLUX>
 __global__ void __launch_bounds__(kT) k_sp_deposit(ParticlesSplit p, int64_t n, GridDims g,
                                                     unsigned long long *__restrict__ Nfix,
                                                     unsigned long long *__restrict__ Ffix,
                                                     int32_t *__restrict__ errors) {
+  __shared__ unsigned long long seg[FLUX ? 1 : 4 * kDepSeg * kDepLdsSpecies];
+  __shared__ int64_t seg_b0;
+  __shared__ int seg_on;
   const int64_t q0 = ((int64_t)blockIdx.x * kT + threadIdx.x) * kDepChunk;
-  if (q0 >= n) return;
-  const int64_t q1 = q0 + kDepChunk < n ? q0 + kDepChunk : n;
+  if (FLUX && q0 >= n) return;
+  const int64_t q1 = q0 + kDepChunk < n ? q0 + kDepChunk : n;   // empty past n
   const int64_t X = g.X, XY = (int64_t)g.X * g.Y;
+  const int S = g.S;
+  int64_t b0 = 0;
+  bool on = false;
+  if constexpr (!FLUX) {
+    if (threadIdx.x == 0) {
+      // the block's first particle sets b0 (a particle outside the grid
+      // there leaves the whole block on the direct path)
+      int ok = 0;
+      int64_t bb = 0;
+      const int64_t qf = (int64_t)blockIdx.x * kT * kDepChunk;
+      if (qf < n && S <= kDepLdsSpecies) {
+        const int i = p.c[0][qf], j = p.c[1][qf], k = p.c[2][qf];
+        if (!(i < 0 || j < 0 || k < 0 || i > g.nx || j > g.ny || k > g.nz)) {
+          ok = 1;
+          bb = (int64_t)i + X * j + XY * k;
+        }
+      }
+      seg_on = ok;
+      seg_b0 = bb;
+    }
+    for (int e = threadIdx.x; e < 4 * kDepSeg * kDepLdsSpecies; e += kT) seg[e] = 0;
+    __syncthreads();
+    on = seg_on != 0;
+    b0 = seg_b0;
+  }
   int64_t cur_base = -1;
   int cur_s = -1;
   unsigned long long W[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   unsigned long long Wf[FLUX ? 24 : 1] = {};
   auto flush = [&]() {
     if (cur_base < 0) return;
+    if (!FLUX && on && cur_base >= b0 && cur_base - b0 + 1 < kDepSeg) {
+      const int off = (int)(cur_base - b0);
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        atomicAdd(&seg[((o >> 1) * kDepSeg + off + (o & 1)) * kDepLdsSpecies + cur_s], W[o]);
+        W[o] = 0;
+      }
+      return;
+    }
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
       const int64_t node = cur_base + (o & 1) + X * ((o >> 1) & 1) + XY * (o >> 2);
-      atomicAdd(&Nfix[node * g.S + cur_s], W[o]);
+      atomicAdd(&Nfix[node * S + cur_s], W[o]);
       W[o] = 0;
       if (FLUX) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          atomicAdd(&Ffix[(node * 3 + d) * g.S + cur_s], Wf[3 * o + d]);
+          atomicAdd(&Ffix[(node * 3 + d) * S + cur_s], Wf[3 * o + d]);
           Wf[3 * o + d] = 0;
         }
       }
@@ -248,6 +294,19 @@ __global__ void __launch_bounds__(kT) k_sp_deposit(ParticlesSplit p, int64_t n, 
   }
   flush();
   if (nerr) atomicAdd(errors, nerr);
+  if constexpr (!FLUX) {
+    __syncthreads();
+    if (on) {
+      for (int e = threadIdx.x; e < 4 * kDepSeg * kDepLdsSpecies; e += kT) {
+        const unsigned long long v = seg[e];
+        if (!v) continue;
+        const int sidx = e % kDepLdsSpecies, r = e / kDepLdsSpecies;
+        const int sg_ = r / kDepSeg, off = r % kDepSeg;
+        const int64_t node = b0 + off + X * (sg_ & 1) + XY * (sg_ >> 1);
+        atomicAdd(&Nfix[node * S + sidx], v);
+      }
+    }
+  }
 }
 
 // Integer guard folds (ParticlesGridify.f90:129-285, z, y, x) and
