@@ -407,18 +407,58 @@ __global__ void k_ranges(Arenas A) {
 // (slot, cell, species) summed in registers, one atomic per corner and run.
 
 constexpr int kDepChunk = 16;
+constexpr int kDepSeg = 128;          // nodes per LDS segment of the deposit
 
+// A thread adds the fixed-point weights of each run of one (cell, species)
+// among its kDepChunk sorted particles in registers and flushes one add per
+// corner when the run ends.  The adds are staged in LDS per block chunk of
+// kT * kDepChunk particles: from the chunk's first particle (slot s0, lower
+// node b0) the corners of runs of the same slot with lower nodes in [b0, b0 +
+// kDepSeg - 1) fall in four node segments b0 + {0, X, XY, XY + X} + [0,
+// kDepSeg), added with LDS atomics; other runs go straight to memory.  The
+// block then adds each nonzero segment entry to memory once.  Integer sums:
+// the same result as one atomic per particle.
 __global__ void __launch_bounds__(kT) k_deposit(Arenas A, PartArena p) {
+  __shared__ unsigned long long seg[4 * kDepSeg * 2];
+  __shared__ int64_t seg_b0;
+  __shared__ int seg_slot;
   const int64_t n = A.g->n;
-  for (int64_t q0 = ((int64_t)blockIdx.x * kT + threadIdx.x) * kDepChunk; q0 < n;
-       q0 += (int64_t)gridDim.x * kT * kDepChunk) {
-    const int64_t q1 = q0 + kDepChunk < n ? q0 + kDepChunk : n;
+  const int64_t span = (int64_t)kT * kDepChunk;
+  for (int64_t cb = (int64_t)blockIdx.x * span; cb < n; cb += (int64_t)gridDim.x * span) {
+    if (threadIdx.x == 0) {
+      int slot0 = -1;
+      int64_t bb = 0;
+      const int sl = p.sim[cb];
+      const SimDev &sd = A.sims[sl];
+      const int i = p.c[0][cb], j = p.c[1][cb], k = p.c[2][cb];
+      if (!(i < 0 || j < 0 || k < 0 || i > sd.nx || j > sd.ny || k > sd.nz)) {
+        slot0 = sl;
+        bb = sd.gbase + i + (int64_t)sd.X * j + (int64_t)sd.X * sd.Y * k;
+      }
+      seg_slot = slot0;
+      seg_b0 = bb;
+    }
+    for (int e = threadIdx.x; e < 4 * kDepSeg * 2; e += kT) seg[e] = 0;
+    __syncthreads();
+    const int slot0 = seg_slot;
+    const int64_t b0 = seg_b0;
+    const int64_t q0 = cb + (int64_t)threadIdx.x * kDepChunk;
+    const int64_t q1 = q0 + kDepChunk < n ? q0 + kDepChunk : n;    // empty past n
     int64_t cur = -1;
-    int cur_s = 0;
+    int cur_s = 0, cur_slot = -1;
     int64_t X = 0, XY = 0;
     unsigned long long W[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     auto flush = [&]() {
       if (cur < 0) return;
+      if (cur_slot == slot0 && cur >= b0 && cur - b0 + 1 < kDepSeg) {
+        const int off = (int)(cur - b0);
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+          atomicAdd(&seg[((o >> 1) * kDepSeg + off + (o & 1)) * 2 + cur_s], W[o]);
+          W[o] = 0;
+        }
+        return;
+      }
 #pragma unroll
       for (int o = 0; o < 8; ++o) {
         const int64_t node = cur + (o & 1) + X * ((o >> 1) & 1) + XY * (o >> 2);
@@ -427,7 +467,8 @@ __global__ void __launch_bounds__(kT) k_deposit(Arenas A, PartArena p) {
       }
     };
     for (int64_t q = q0; q < q1; ++q) {
-      const SimDev &sd = A.sims[p.sim[q]];
+      const int sl = p.sim[q];
+      const SimDev &sd = A.sims[sl];
       const int i = p.c[0][q], j = p.c[1][q], k = p.c[2][q];
       if (i < 0 || j < 0 || k < 0 || i > sd.nx || j > sd.ny || k > sd.nz) continue;
       const int64_t Xs = sd.X, XYs = (int64_t)sd.X * sd.Y;
@@ -437,6 +478,7 @@ __global__ void __launch_bounds__(kT) k_deposit(Arenas A, PartArena p) {
         flush();
         cur = node;
         cur_s = s;
+        cur_slot = sl;
         X = Xs;
         XY = XYs;
       }
@@ -449,8 +491,23 @@ __global__ void __launch_bounds__(kT) k_deposit(Arenas A, PartArena p) {
       }
     }
     flush();
+    __syncthreads();
+    if (slot0 >= 0) {
+      const SimDev &s0 = A.sims[slot0];
+      const int64_t X0 = s0.X, XY0 = (int64_t)s0.X * s0.Y;
+      for (int e = threadIdx.x; e < 4 * kDepSeg * 2; e += kT) {
+        const unsigned long long v = seg[e];
+        if (!v) continue;
+        const int sidx = e & 1, r = e >> 1;
+        const int sg = r / kDepSeg, off = r % kDepSeg;
+        const int64_t node = b0 + off + X0 * (sg & 1) + XY0 * (sg >> 1);
+        atomicAdd(&A.Nfix[2 * node + sidx], v);
+      }
+    }
+    __syncthreads();                    // the segments are cleared for the next chunk
   }
 }
+
 
 // ---------------------------------------------------------------------------
 // per-node kernels over node tiles {slot, first node}: 256 nodes each
