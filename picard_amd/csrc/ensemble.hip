@@ -368,24 +368,28 @@ __global__ void __launch_bounds__(kT) k_flag(Arenas A, PartArena p) {
   if (nerr) atomicOr(&A.g->err, 2);
 }
 
+// One particle per thread over a grid sized for the arena's capacity (the
+// count is in device memory, so the captured launch cannot size itself to
+// it): blocks past the count leave at once, and every wave in range walks a
+// single key-cursor-copy chain instead of hundreds in a grid-stride loop.
 __global__ void __launch_bounds__(kT) k_scatter(Arenas A, PartArena src, PartArena dst) {
   const int64_t n = A.g->n;
-  for (int64_t b0 = (int64_t)blockIdx.x * kT; b0 < n; b0 += (int64_t)gridDim.x * kT) {
-    const int64_t q = b0 + threadIdx.x;
-    const int32_t k = q < n ? A.key[q] : -1;
-    const int32_t r = wave_add1(A.cursor, k);
-    if (k < 0 || r < 0) continue;
-    const int64_t at = (int64_t)A.hist[k] + r;
+  const int64_t b0 = (int64_t)blockIdx.x * kT;
+  if (b0 >= n) return;                                   // block-uniform
+  const int64_t q = b0 + threadIdx.x;
+  const int32_t k = q < n ? A.key[q] : -1;               // lanes past n take part in the adds
+  const int32_t r = wave_add1(A.cursor, k);
+  if (k < 0 || r < 0) return;
+  const int64_t at = (int64_t)A.hist[k] + r;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      dst.c[d][at] = src.c[d][q];
-      dst.f[d][at] = src.f[d][q];
-      dst.v[d][at] = src.v[d][q];
-    }
-    dst.sp[at] = src.sp[q];
-    dst.sim[at] = src.sim[q];
-    dst.id[at] = src.id[q];
+  for (int d = 0; d < 3; ++d) {
+    dst.c[d][at] = src.c[d][q];
+    dst.f[d][at] = src.f[d][q];
+    dst.v[d][at] = src.v[d][q];
   }
+  dst.sp[at] = src.sp[q];
+  dst.sim[at] = src.sim[q];
+  dst.id[at] = src.id[q];
 }
 
 // Each live slot's particle range after the scatter; the arena's new count.
@@ -1613,7 +1617,7 @@ static int launch_pre(picard_ensemble *e, int cur) {
   k_flag<<<gp, kT, 0, st>>>(A, X);
   PICARD_HIP(hipGetLastError());
   PICARD_TRY(scan_exclusive_stream(st, A.hist, A.hist, A.kcap, e->scan_tmp, e->scan_tmp_len));
-  k_scatter<<<gp, kT, 0, st>>>(A, X, Y);
+  k_scatter<<<(int)((e->cap + kT - 1) / kT), kT, 0, st>>>(A, X, Y);
   k_ranges<<<(kMaxSlots + kT - 1) / kT, kT, 0, st>>>(A);
   k_fill_u64<<<1024, kT, 0, st>>>(A.Nfix, 2 * e->max_nodes);
   k_deposit<<<gp, kT, 0, st>>>(A, Y);
