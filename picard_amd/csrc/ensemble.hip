@@ -8,11 +8,11 @@
 // together by (slot key range, lower node, species), so each simulation's
 // particles are contiguous after the sort, in the order of its key range.
 //
-// Work lists.  Every phase is one launch of a fixed persistent grid that
-// walks a device-resident list of (slot, tile) items: node tiles for the
-// per-node kernels, cell tiles (the autotuned shape) for the solve, face
-// tiles for the open-boundary sums, and the particle range for the particle
-// kernels.  The lists and their lengths live in device memory, so the
+// Work lists.  Every phase is one launch of a fixed grid (persistent, but for
+// the scatter's, sized for the particle arena's capacity) that walks a
+// device-resident list of (slot, tile) items: node tiles for the per-node
+// kernels, cell tiles (the autotuned shape) for the solve, face tiles for the
+// open-boundary sums, and the particle range for the particle kernels.  The lists and their lengths live in device memory, so the
 // captured graphs stay valid when simulations join or leave.
 //
 // Solve.  Mixed precision per simulation: an FP64 refinement pass forms
